@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark: kNN Gaussian-connectivity graph + Leiden on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on): 100,000 points x 50 dims
+from a 30-component isotropic Gaussian mixture (centres ~ N(0, 16 I), sd 1, proportions ~
+Dirichlet(2)), exact k = 15 neighbours, Gaussian connectivities with sigma_i = distance to the
+k-th neighbour, symmetrized by max, then Leiden (Rb, resolution 1.0, seed 42). One step =
+kNN graph build (device resident) + Leiden on that graph.
+
+Prints one JSON line. `value` = Leiden edges/s (undirected edges of the level-0 graph / Leiden
+wall time); kNN points/s is reported beside it. Other workloads: --config small (planted
+partition, 10k nodes) and --config large (4M x 30, k = 7).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "single-clustering_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "Leiden edges/s, kNN points/s on 1 MI355X; learned vs plain at equal modularity"
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def make_workload(name):
+    import workloads as W
+
+    if name == "knn_leiden":
+        x, truth = W.gaussian_mixture(100_000, 50, 30, 16.0, sd=1.0, alpha=2.0, seed=42)
+        return {"kind": "knn", "x": x, "k": 15, "truth": truth, "objective": ("rb", 1.0),
+                "workload": "knn_leiden_100k_d50_k15", "n": 100_000, "d": 50}
+    if name == "large":
+        x, truth = W.gaussian_mixture(4_000_000, 30, 200, 9.0, sd=1.0, alpha=1.0, seed=42)
+        return {"kind": "knn", "x": x, "k": 7, "truth": truth, "objective": ("rb", 1.0),
+                "workload": "knn_leiden_4M_d30_k7", "n": 4_000_000, "d": 30}
+    if name == "small":
+        ptrs, nbrs, wts, truth = W.planted_partition(seed=42)
+        return {"kind": "graph", "csr": (ptrs, nbrs, wts), "truth": truth, "objective": ("rb", 1.0),
+                "workload": "planted_partition_10k_20blocks", "n": 10_000}
+    raise SystemExit(f"unknown config {name}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="knn_leiden", choices=["knn_leiden", "small", "large"])
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see the sharding paragraph of DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import scgpu
+
+    wl = make_workload(args.config)
+    log(f"[bench] workload {wl['workload']} ready")
+    obj_name, res = wl["objective"]
+    cfg = scgpu.LeidenConfig(objective=scgpu.ObjectiveKind(obj_name, res), seed=42)
+
+    def step():
+        t0 = time.perf_counter()
+        if wl["kind"] == "knn":
+            g, kst = scgpu.knn_graph(wl["x"], wl["k"], precision=args.precision, return_stats=True)
+        else:
+            g, kst = scgpu.CSRNetwork.from_csr_parts(*wl["csr"]), None
+        t1 = time.perf_counter()
+        c = scgpu.leiden(g, cfg)
+        t2 = time.perf_counter()
+        return g, c, kst, scgpu.last_stats(), t1 - t0, t2 - t1
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    scgpu.set_profiling(True)
+    t_knn, t_leiden, lstats, kstats = [], [], [], []
+    g = c = None
+    for i in range(args.steps):
+        g, c, kst, lst, a, b = step()
+        t_knn.append(a)
+        t_leiden.append(b)
+        lstats.append(lst)
+        kstats.append(kst)
+        log(f"[bench] step {i + 1}/{args.steps}: graph {a * 1e3:.2f} ms, leiden {b * 1e3:.2f} ms, "
+            f"{c.n_clusters()} clusters")
+    scgpu.set_profiling(False)
+
+    m = g.edge_count()
+    n = g.node_count()
+    tl = float(np.mean(t_leiden))
+    tk = float(np.mean(t_knn))
+    edges_per_s = m / tl
+    q = scgpu.modularity(g, c.labels(), 1.0)
+
+    # roofline of the dominant kernel (time summed over the timed steps)
+    move_ms = float(np.mean([s["move_kernel_ms"] for s in lstats]))
+    move_bytes = float(np.mean([s["move_kernel_bytes"] for s in lstats]))
+    launches = float(np.mean([s["move_kernel_launches"] for s in lstats]))
+    leiden_roof = {
+        "kernel": "k_decide (local-move decide, all degree buckets)",
+        "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+        "achieved": (move_bytes / (move_ms / 1e3) / 1e9) if move_ms > 0 else None,
+        "kernel_ms_per_step": move_ms, "launches_per_step": launches,
+        "bytes_per_launch": move_bytes / launches if launches else None,
+    }
+    leiden_roof["frac"] = leiden_roof["achieved"] / HBM_PEAK_GBS if leiden_roof["achieved"] else None
+    leiden_roof["traffic"] = None
+    knn_roof = None
+    if wl["kind"] == "knn":
+        tile_ms = float(np.mean([s["tile_kernel_ms"] for s in kstats]))
+        flops = kstats[0]["flops"]
+        knn_roof = {"kernel": f"k_knn_tile<{args.precision}>", "bound": "mfma", "unit": "TFLOP/s",
+                    "peak": BF16_PEAK_TFLOPS, "achieved": flops / (tile_ms / 1e3) / 1e12,
+                    "kernel_ms_per_step": tile_ms, "traffic": None}
+        knn_roof["frac"] = knn_roof["achieved"] / BF16_PEAK_TFLOPS
+    dominant = leiden_roof
+    if knn_roof and knn_roof["kernel_ms_per_step"] > leiden_roof["kernel_ms_per_step"]:
+        dominant = knn_roof
+
+    # CPU baseline: the oracle's single-threaded Leiden on the same graph, bounded sample
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        ptrs, nbrs, wts = g.to_csr_parts()
+        og = O.Graph.from_csr_parts(ptrs, nbrs, wts)
+        times = []
+        t_start = time.perf_counter()
+        ol = None
+        while len(times) < 5 and (time.perf_counter() - t_start) < args.cpu_baseline_seconds:
+            t0 = time.perf_counter()
+            ol, onc = O.leiden(og, seed=42, objective=obj_name, resolution=res)
+            times.append(time.perf_counter() - t0)
+            log(f"[bench] cpu baseline rep {len(times)}: {times[-1]:.3f} s")
+        t_cpu = float(np.median(times))
+        model, ncpu = cpu_info()
+        q_cpu = O.modularity(og, ol, 1.0)
+        cpu = {"value": m / t_cpu, "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": f"{len(times)} full leiden() runs of the C oracle on the same {n}-node graph, "
+                         f"median {t_cpu:.3f} s, host {model} ({ncpu} logical CPUs visible)",
+               "modularity": q_cpu, "ari_vs_gpu": O.ari(c.labels(), ol), "n_clusters": int(onc)}
+
+    out = {
+        "metric": METRIC,
+        "value": edges_per_s,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": (tk + tl) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (delta-quality), int64 fixed-point totals, f32 weights; kNN " +
+                 ("split-bf16 MFMA + fp64 rescore" if args.precision == "bf16" else "fp32 MFMA + fp64 rescore"),
+        "data": "synthetic",
+        "config": {"workload": wl["workload"], "nodes": n, "edges": m, "leiden": "Rb resolution 1.0, seed 42",
+                   "parallelism": "single GPU"},
+        "leiden_ms": tl * 1e3,
+        "graph_build_ms": tk * 1e3,
+        "knn_points_per_s": (wl["n"] / tk) if wl["kind"] == "knn" else None,
+        "modularity": q,
+        "n_clusters": c.n_clusters(),
+        "leiden_stats": lstats[-1],
+        "roofline": dominant,
+        "roofline_other": knn_roof if dominant is leiden_roof else leiden_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+/*
+ * scgpu.h -- C ABI of the MI355X (gfx950) Leiden + kNN-graph library.
+ *
+ * Drop-in boundary for SingleRust/single-clustering v0.7.0. Each entry point names the Rust
+ * item it replaces (file:line in that crate). Plain pointers and sizes only; no exceptions cross
+ * this boundary; every fallible call returns an sc_status and fills an optional sc_error.
+ *
+ * Graph handles are immutable after construction and device-resident; concurrent sc_leiden
+ * calls on one handle are safe (each call owns its workspace and stream).
+ */
+#ifndef SCGPU_H
+#define SCGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One code per ClusteringError variant, in declaration order (src/error.rs:7-68), plus
+ * device / argument / memory failures that the CPU crate has no counterpart for. */
+typedef enum sc_status {
+    SC_OK = 0,
+    SC_NODE_INDEX_OUT_OF_RANGE = 1,      /* ClusteringError::NodeIndexOutOfRange      */
+    SC_NON_FINITE_WEIGHT = 2,            /* ClusteringError::NonFiniteWeight          */
+    SC_NEGATIVE_WEIGHT = 3,              /* ClusteringError::NegativeWeight           */
+    SC_INVALID_NODE_WEIGHT = 4,          /* ClusteringError::InvalidNodeWeight        */
+    SC_NODE_WEIGHT_LENGTH_MISMATCH = 5,  /* ClusteringError::NodeWeightLengthMismatch */
+    SC_MEMBERSHIP_LENGTH_MISMATCH = 6,   /* ClusteringError::MembershipLengthMismatch */
+    SC_INVALID_CONFIG = 7,               /* ClusteringError::InvalidConfig            */
+    SC_TOO_MANY_NODES = 8,               /* ClusteringError::TooManyNodes             */
+    SC_INVALID_CSR = 9,                  /* ClusteringError::InvalidCsr               */
+    SC_ASYMMETRIC_GRAPH = 10,            /* ClusteringError::AsymmetricGraph          */
+    SC_DEVICE_ERROR = 100,
+    SC_INVALID_ARGUMENT = 101,
+    SC_OUT_OF_MEMORY = 102
+} sc_status;
+
+/* Payload of the failing variant (error.rs:7-68); unused fields are zero. msg holds the
+ * variant's Display text (error.rs:70-113) or the device error string. */
+typedef struct sc_error {
+    int32_t code;
+    uint64_t node;
+    uint64_t n_nodes;
+    uint64_t edge_u;
+    uint64_t edge_v;
+    double weight;
+    uint64_t got;
+    uint64_t expected;
+    double degree_sum;
+    double expected_sum;
+    char msg[256];
+} sc_error;
+
+/* ObjectiveKind (leiden/objective.rs:188-199). */
+typedef enum sc_objective { SC_RB = 0, SC_CPM = 1 } sc_objective;
+
+/* LeidenConfig, field by field (leiden/mod.rs:49-90). Option<T> fields become has_x + x. */
+typedef struct sc_leiden_config {
+    int32_t objective;            /* sc_objective */
+    double resolution;
+    int32_t has_seed;             /* 0: seed from the OS (ChaCha8Rng::from_os_rng) */
+    uint64_t seed;
+    uint64_t n_iterations;
+    uint64_t max_levels;
+    double tolerance;
+    int32_t refine;
+    double refine_randomness;
+    uint64_t patience;
+    int32_t has_max_community_weight;
+    double max_community_weight;
+} sc_leiden_config;
+
+typedef struct sc_graph sc_graph;
+
+/* Per-call counters of the last sc_leiden on this thread (no reference counterpart; used by
+ * the benchmark to compute algorithmic bytes and by tests to inspect convergence). */
+typedef struct sc_leiden_stats {
+    uint64_t passes;
+    uint64_t levels;
+    uint64_t move_iterations;
+    uint64_t refine_rounds;
+    uint64_t vertex_visits;        /* local-move + refine vertex evaluations */
+    uint64_t edge_visits;          /* adjacency entries read by those evaluations */
+    uint64_t moves;
+    double algorithmic_bytes;      /* minimum unique HBM bytes of every executed pass */
+    double move_kernel_ms;         /* summed device time of the local-move decide kernels */
+    double move_kernel_bytes;      /* algorithmic bytes of those kernels */
+    uint64_t move_kernel_launches;
+    double total_ms;
+} sc_leiden_stats;
+
+/* kNN options: sigma rule and symmetrization (neighborhood/mod.rs:104-168). */
+typedef struct sc_knn_opts {
+    int32_t precision;   /* 0 = BF16 MFMA candidates + exact rescore, 1 = FP32 candidates + exact rescore */
+    int32_t mode;        /* 0 = sigma: k-th neighbour distance, union (max) symmetrization;
+                            1 = reference-compatible: sigma = upper median, lower-row emission */
+} sc_knn_opts;
+
+typedef struct sc_knn_stats {
+    uint64_t n;
+    uint64_t candidates_rescored;
+    uint64_t rows_fallback;        /* rows whose certified candidate set overflowed */
+    double tile_kernel_ms;
+    double total_ms;
+    double flops;                  /* 2 * n^2 * d (unpadded) */
+} sc_knn_stats;
+
+/* ---- library ------------------------------------------------------------------------- */
+const char *sc_version(void);
+/* Releases cached device memory; optional. */
+void sc_shutdown(void);
+/* Records device time of the hot kernels into sc_leiden_stats (adds HIP events around each
+ * local-move decide launch). Also switched on by the environment variable SCGPU_PROFILE=1. */
+void sc_set_profiling(int32_t enable);
+
+/* LeidenConfig::default (leiden/mod.rs:92-106). */
+sc_leiden_config sc_leiden_config_default(void);
+
+/* ---- graph ----------------------------------------------------------------------------- */
+/* CSRNetwork::from_csr_parts (src/network/csr_network.rs:185-256). Host arrays are borrowed
+ * for the call and copied to HBM; rows may be unsorted, duplicates are summed, zero sums are
+ * dropped; the degree-sum check (:712-723) runs. node_weights may be NULL (unit weights). */
+sc_status sc_graph_from_csr_parts(const uint64_t *node_ptrs, uint64_t n_nodes,
+                                  const uint32_t *neighbors, const float *weights, uint64_t nnz,
+                                  const double *node_weights, sc_graph **out, sc_error *err);
+
+/* Zero-copy analogue of from_csr_parts for device-resident buffers (ownership is not taken;
+ * the arrays are read during the call, and the graph keeps its own compacted copy). */
+sc_status sc_graph_from_csr_parts_device(const uint64_t *d_node_ptrs, uint64_t n_nodes,
+                                         const uint32_t *d_neighbors, const float *d_weights,
+                                         uint64_t nnz, const double *d_node_weights,
+                                         sc_graph **out, sc_error *err);
+
+/* CSRNetwork::from_edges_with_node_weights (csr_network.rs:120-167). Each undirected edge once;
+ * (v, v, w) is a self-loop. node_weights may be NULL. */
+sc_status sc_graph_from_edges(uint64_t n_nodes, const uint64_t *edge_from, const uint64_t *edge_to,
+                              const double *edge_weight, uint64_t n_edges,
+                              const double *node_weights, sc_graph **out, sc_error *err);
+
+void sc_graph_free(sc_graph *g);
+
+/* Accessors (csr_network.rs:407-508). */
+uint64_t sc_graph_node_count(const sc_graph *g);
+uint64_t sc_graph_edge_count(const sc_graph *g);
+uint64_t sc_graph_nnz(const sc_graph *g);
+double sc_graph_total_weight(const sc_graph *g);
+int32_t sc_graph_has_self_loops(const sc_graph *g);
+
+/* CSRNetwork::to_csr_parts / node_weights / strengths (csr_network.rs:484-495, :428-430):
+ * copies the device arrays back. Any output pointer may be NULL. */
+sc_status sc_graph_to_csr_parts(const sc_graph *g, uint64_t *node_ptrs, uint32_t *neighbors,
+                                float *weights, double *node_weights, double *strengths,
+                                sc_error *err);
+
+/* CSRNetwork::validate_symmetry (csr_network.rs:627-646). */
+sc_status sc_graph_validate_symmetry(const sc_graph *g, sc_error *err);
+
+/* CSRNetwork::aggregate (csr_network.rs:516-608). groups: consecutive ids in [0, n_groups)
+ * for every node (VectorGrouping::from_assignments after normalization, grouping.rs:154-172). */
+sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph **out,
+                             sc_error *err);
+
+/* ---- clustering ------------------------------------------------------------------------ */
+/* leiden (leiden/mod.rs:168-204). labels_out: n entries, consecutive from 0 (Clustering). */
+sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *labels_out,
+                    uint64_t *n_clusters_out, sc_error *err);
+
+/* Counters of the last sc_leiden on the calling thread. */
+sc_status sc_leiden_last_stats(sc_leiden_stats *out);
+
+/* modularity (leiden/objective.rs:221-244). Infallible in the crate; returns NaN on a device
+ * error or a label vector shorter than the graph cannot be detected, so callers pass n labels. */
+double sc_modularity(const sc_graph *g, const uint64_t *labels, double resolution);
+
+/* Rb::quality / Cpm::quality of Partition::from_membership(labels)
+ * (objective.rs:108-119, :171-178; partition.rs:75-112). */
+sc_status sc_quality(const sc_graph *g, const uint64_t *labels, int32_t objective,
+                     double resolution, double *out, sc_error *err);
+
+/* disconnected_community_count (tests/common/mod.rs:171-194): communities with more than one
+ * connected component using same-label edges only. */
+sc_status sc_disconnected_community_count(const sc_graph *g, const uint64_t *labels,
+                                          uint64_t *out, sc_error *err);
+
+/* ---- kNN ------------------------------------------------------------------------------- */
+/* Exact k nearest neighbours of every row (self excluded, FP64 distance, ties to the lower
+ * index). Replaces the search of knn_arrayd_adaptive (neighborhood/mod.rs:592-625), whose
+ * kd-tree path is exact and whose HNSW path is approximate. data: n x d row-major f32.
+ * idx_out: n*k, d2_out: n*k squared distances (either may be NULL). */
+sc_status sc_knn(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc_knn_opts *opts,
+                 uint32_t *idx_out, double *d2_out, sc_knn_stats *stats, sc_error *err);
+
+/* kNN + Gaussian connectivity + symmetrization + ingest, device-resident end to end: the
+ * connectivities of knn_arrayd_kiddo_gaussian (neighborhood/mod.rs:104-181) fed straight into
+ * CSRNetwork::from_csr_parts. */
+sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k,
+                       const sc_knn_opts *opts, sc_graph **out, sc_knn_stats *stats,
+                       sc_error *err);
+
+/* Gaussian connectivity of a given kNN result (neighborhood/mod.rs:104-168), as a full
+ * symmetric CSR with sorted rows. Capacity of neighbors/weights: 2*n*k. */
+sc_status sc_knn_gaussian_csr(const uint32_t *idx, const double *d2, uint64_t n, uint32_t k,
+                              int32_t mode, uint64_t *node_ptrs, uint32_t *neighbors,
+                              float *weights, uint64_t *nnz_out, sc_error *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,203 @@
+// common.hpp -- shared host/device infrastructure of libscgpu (gfx950 only).
+//
+// Error model: internal code throws ScError; every extern "C" entry point catches it and turns
+// it into an sc_status + sc_error, so no exception crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/scgpu.h"
+
+namespace scgpu {
+
+struct ScError {
+    sc_status code;
+    std::string msg;
+    sc_error payload;
+    ScError(sc_status c, std::string m) : code(c), msg(std::move(m)) {
+        std::memset(&payload, 0, sizeof(payload));
+    }
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char *what, const char *file, int line);
+
+#define HIP_OK(expr)                                                   \
+    do {                                                               \
+        hipError_t _e = (expr);                                        \
+        if (_e != hipSuccess) ::scgpu::throw_hip(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define KERNEL_OK() HIP_OK(hipGetLastError())
+
+void fill_error(sc_error *err, const ScError &e);
+
+// ---- caching device allocator -----------------------------------------------------------
+// hipMalloc is slow and synchronizing; blocks are cached by rounded size and reused. A block
+// is returned to the cache only after the owning call has synchronized its stream.
+class DevicePool {
+  public:
+    static DevicePool &get();
+    void *alloc(size_t bytes);
+    void release(void *p);
+    void trim();
+    size_t cached_bytes() const { return cached_; }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> live_;
+    size_t cached_ = 0;
+};
+
+template <typename T>
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { resize(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { reset(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    void resize(size_t n) {
+        reset();
+        n_ = n;
+        p_ = static_cast<T *>(DevicePool::get().alloc((n ? n : 1) * sizeof(T)));
+    }
+    void reset() {
+        if (p_) DevicePool::get().release(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T *get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+
+  private:
+    T *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Stream owned by the calling thread (default priority), created lazily.
+hipStream_t thread_stream();
+
+// Profiling switch: when on, the Leiden driver records events around its decide kernels.
+bool profiling_enabled();
+void set_profiling(int on);
+
+// ---- fixed point -----------------------------------------------------------------------
+// Community totals and neighbour-community weights are accumulated as int64 fixed point with
+// a per-level power-of-two scale, so atomic accumulation is order independent (deterministic)
+// while keeping ~2^-61 relative resolution of the largest possible sum.
+inline int fixed_scale_for(double max_sum) {
+    if (!(max_sum > 0.0)) return 40;
+    int e;
+    std::frexp(max_sum, &e);  // max_sum < 2^e
+    return 61 - e;
+}
+
+__device__ __forceinline__ long long to_fx(double x, int scale) {
+    return __double2ll_rn(ldexp(x, scale));
+}
+
+__device__ __forceinline__ double from_fx(long long x, int scale) {
+    return ldexp((double)x, -scale);
+}
+
+// ---- hashing / counter-based randomness ---------------------------------------------------
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+__host__ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Uniform in (0, 1) from 53 random bits, never exactly 0 or 1.
+__device__ __forceinline__ double u01_open(uint64_t bits) {
+    return ((double)(bits >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// ---- launch helpers ---------------------------------------------------------------------
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 65535u * 16u) {
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// Device-wide helpers implemented with hipcub in util.hip.
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s);
+void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s);
+void radix_sort_pairs_u64_f32(uint64_t *keys, float *vals, uint64_t n, int end_bit, hipStream_t s);
+void radix_sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s);
+void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s);
+// Deterministic sum of n doubles (fixed two-level tree).
+double device_sum_f64(const double *in, uint64_t n, hipStream_t s);
+double device_max_f64(const double *in, uint64_t n, hipStream_t s);
+
+inline int bits_for(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b) != 0) b++;
+    return b;
+}
+
+template <typename T>
+void to_device(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+void to_host(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+}  // namespace scgpu
+
+// The device-resident graph behind sc_graph (CSRNetworkData, csr_network.rs:39-60), plus the
+// per-node self-loop weight, cached so kernels never binary-search a row for it.
+struct sc_graph {
+    uint64_t n = 0;
+    uint64_t nnz = 0;
+    scgpu::DevBuf<uint64_t> ptrs;      // n + 1
+    scgpu::DevBuf<uint32_t> nbrs;      // nnz, strictly increasing per row
+    scgpu::DevBuf<float> wts;          // nnz
+    scgpu::DevBuf<double> node_w;      // n
+    scgpu::DevBuf<double> strength;    // n (self-loops twice)
+    scgpu::DevBuf<double> self_loop;   // n
+    double total_weight = 0.0;
+    uint64_t edge_count = 0;
+    bool any_self_loops = false;
+    double total_node_weight = 0.0;
+    double max_strength = 0.0;
+    uint32_t max_degree = 0;
+};
+
+namespace scgpu {
+// Builds the cached aggregates of a graph whose rows are already sorted, merged and free of
+// zero weights (CSRNetwork::finish's second half, csr_network.rs:313-351): strengths,
+// self-loops, total weight, edge count, and the degree-sum check.
+void graph_finish_sorted(sc_graph &g, hipStream_t s);
+// Full finish from arbitrary rows: sort, merge duplicates in f64, drop zero sums, compact.
+void graph_finish_unsorted(sc_graph &g, hipStream_t s);
+}  // namespace scgpu

@@ -1,0 +1,668 @@
+// knn.hip -- exact kNN on MFMA and the Gaussian connectivity graph builder.
+//
+// Replaces the neighbour search of neighborhood/mod.rs (kiddo kd-tree :51-202 / HNSW :359-578)
+// with a brute-force contraction on the matrix cores, and restates the sigma / Gaussian weight /
+// CSR steps (:104-181, :208-227) on the GPU.
+//
+// Pipeline:
+//  1. prep: f32 rows -> zero-padded split-BF16 (hi, lo) and padded f32 copies, f32 norms.
+//  2. tile: 64 query rows per workgroup (16 per wave), all points streamed through LDS in
+//     stages of 64. s = |y|^2 - 2 x.y per pair (|x|^2 is constant per row); with BF16 the dot
+//     product is xh.yh + xh.yl + xl.yh on v_mfma_f32_16x16x32_bf16 (FP32 accumulate), with FP32
+//     it is v_mfma_f32_16x16x4_f32. Each row keeps its 2k smallest s in an LDS list; a pair is
+//     looked at only when it beats the row's current 2k-th value.
+//  3. rescore: the 2k candidates get exact FP64 squared distances (sequential sum, no
+//     contraction), sorted by (d2, index). A row is certified exact when the smallest distance
+//     any non-candidate could have (2k-th approximate value minus a rigorous error bound) is
+//     strictly above the k-th exact distance; other rows are recomputed by an FP64 scan.
+//  4. gaussian: sigma_i from the k-th (or median) distance, w_ij in f64 stored as f32,
+//     symmetrized, sorted into CSR and ingested without leaving HBM.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+namespace scgpu {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int KNN_BM = 64;      // query rows per workgroup (16 per wave)
+constexpr int KNN_TN = 64;      // points per LDS stage
+constexpr int KNN_MAXK2 = 64;   // 2k candidates per row at most
+constexpr uint32_t KNONE = 0xFFFFFFFFu;
+
+__device__ __forceinline__ unsigned short bf16_bits_rn(float x) {
+    uint32_t u = __float_as_uint(x);
+    // round to nearest even (inputs are finite)
+    uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(r >> 16);
+}
+
+__device__ __forceinline__ float bf16_bits_to_f32(unsigned short b) {
+    return __uint_as_float(((uint32_t)b) << 16);
+}
+
+// One thread per point: split-BF16 copies, padded f32 copy, f32 norm, norm bounds.
+__global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, uint32_t Kpf,
+                           unsigned short *Xh, unsigned short *Xl, float *Xf, float *ny, double *nx64,
+                           unsigned int *ymax_bits) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const float *x = X + (uint64_t)j * d;
+        float s = 0.0f;
+        double s64 = 0.0;
+        for (uint32_t t = 0; t < Kp; t++) {
+            float v = t < d ? x[t] : 0.0f;
+            unsigned short h = bf16_bits_rn(v);
+            float rem = v - bf16_bits_to_f32(h);
+            unsigned short l = bf16_bits_rn(rem);
+            Xh[(uint64_t)j * Kp + t] = h;
+            Xl[(uint64_t)j * Kp + t] = l;
+            if (t < Kpf) Xf[(uint64_t)j * Kpf + t] = v;
+            s += v * v;
+            double dv = (double)v;
+            s64 = __dadd_rn(s64, __dmul_rn(dv, dv));
+        }
+        ny[j] = s;
+        nx64[j] = s64;
+        atomicMax(ymax_bits, __float_as_uint(sqrtf(s) * 1.0001f));
+    }
+}
+
+template <bool BF16>
+__global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
+                                                  const float *Xf, const float *ny, uint32_t n, uint32_t npad,
+                                                  uint32_t Kp, uint32_t Kpf, uint32_t K2, uint32_t *cand_idx,
+                                                  float *cand_s) {
+    // LDS: one stage of KNN_TN points (padded rows) + the candidate lists of the block's rows
+    extern __shared__ unsigned char smem[];
+    const uint32_t strideB = BF16 ? (Kp + 8) : (Kpf + 4);  // elements per staged point
+    unsigned short *sBh = reinterpret_cast<unsigned short *>(smem);
+    unsigned short *sBl = sBh + KNN_TN * strideB;
+    float *sBf = reinterpret_cast<float *>(smem);
+    size_t stage_bytes = BF16 ? (size_t)2 * KNN_TN * strideB * 2 : (size_t)KNN_TN * strideB * 4;
+    float *sNy = reinterpret_cast<float *>(smem + stage_bytes);
+    float *lval = sNy + KNN_TN;
+    uint32_t *lidx = reinterpret_cast<uint32_t *>(lval + KNN_BM * K2);
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const uint32_t row0 = blockIdx.x * KNN_BM;
+    const uint32_t wrow0 = row0 + wave * 16;
+
+    for (uint32_t i = tid; i < KNN_BM * K2; i += 256) {
+        lval[i] = INFINITY;
+        lidx[i] = KNONE;
+    }
+
+    // A fragments of this wave's 16 rows, kept in registers for the whole sweep
+    const uint32_t arow = wrow0 + (lane & 15);
+    bf16x8 ah[4], al[4];
+    float af[32];
+    const uint32_t nkc = BF16 ? Kp / 32 : Kpf / 4;
+    if (BF16) {
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            if ((uint32_t)kc < nkc) {
+                const uint64_t off = (uint64_t)arow * Kp + kc * 32 + 8 * (lane >> 4);
+                ah[kc] = *reinterpret_cast<const bf16x8 *>(Xh + off);
+                al[kc] = *reinterpret_cast<const bf16x8 *>(Xl + off);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int kc = 0; kc < 32; kc++)
+            if ((uint32_t)kc < nkc) af[kc] = Xf[(uint64_t)arow * Kpf + kc * 4 + (lane >> 4)];
+    }
+
+    // per-lane copies of the thresholds of its 4 rows (rows 4*(lane/16) + i of the wave)
+    float thr[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    const int rbase = wave * 16 + 4 * (lane >> 4);  // block-local row of thr[0]
+    __syncthreads();
+
+    for (uint32_t c0 = 0; c0 < npad; c0 += KNN_TN) {
+        // stage KNN_TN points into LDS (16-byte chunks)
+        if (BF16) {
+            const uint32_t chunks_per_pt = Kp / 8;
+            for (uint32_t q = tid; q < KNN_TN * chunks_per_pt * 2; q += 256) {
+                uint32_t which = q / (KNN_TN * chunks_per_pt);
+                uint32_t r = q % (KNN_TN * chunks_per_pt);
+                uint32_t p = r / chunks_per_pt, ch = r % chunks_per_pt;
+                const unsigned short *src = (which ? Xl : Xh) + (uint64_t)(c0 + p) * Kp + ch * 8;
+                unsigned short *dst = (which ? sBl : sBh) + p * strideB + ch * 8;
+                *reinterpret_cast<uint4 *>(dst) = *reinterpret_cast<const uint4 *>(src);
+            }
+        } else {
+            const uint32_t chunks_per_pt = Kpf / 4;
+            for (uint32_t q = tid; q < KNN_TN * chunks_per_pt; q += 256) {
+                uint32_t p = q / chunks_per_pt, ch = q % chunks_per_pt;
+                *reinterpret_cast<float4 *>(sBf + p * strideB + ch * 4) =
+                    *reinterpret_cast<const float4 *>(Xf + (uint64_t)(c0 + p) * Kpf + ch * 4);
+            }
+        }
+        if (tid < KNN_TN) sNy[tid] = ny[c0 + tid];
+        __syncthreads();
+
+        for (int sub = 0; sub < KNN_TN / 16; sub++) {
+            const int bcol = sub * 16 + (lane & 15);  // staged point of this lane's column
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (BF16) {
+#pragma unroll
+                for (int kc = 0; kc < 4; kc++) {
+                    if ((uint32_t)kc < nkc) {
+                        const uint32_t off = bcol * strideB + kc * 32 + 8 * (lane >> 4);
+                        bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
+                        bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bh, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bl, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kc], bh, acc, 0, 0, 0);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int kc = 0; kc < 32; kc++) {
+                    if ((uint32_t)kc < nkc) {
+                        float b = sBf[bcol * strideB + kc * 4 + (lane >> 4)];
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kc], b, acc, 0, 0, 0);
+                    }
+                }
+            }
+            const uint32_t col = c0 + bcol;
+            const float nyc = sNy[bcol];
+            float s[4];
+            bool hit[4];
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                s[i] = nyc - 2.0f * acc[i];
+                const uint32_t grow = wrow0 + 4 * (lane >> 4) + i;
+                hit[i] = col < n && grow < n && col != grow && s[i] < thr[i];
+                any |= hit[i];
+            }
+            uint64_t mask = __ballot(any);
+            if (mask) {
+                while (mask) {
+                    const int src = __ffsll((unsigned long long)mask) - 1;
+                    mask &= mask - 1;
+                    if (lane == src) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            if (!hit[i]) continue;
+                            float *vv = lval + (rbase + i) * K2;
+                            uint32_t *ii = lidx + (rbase + i) * K2;
+                            if (!(s[i] < vv[K2 - 1])) continue;
+                            int p = (int)K2 - 1;
+                            while (p > 0 && vv[p - 1] > s[i]) {
+                                vv[p] = vv[p - 1];
+                                ii[p] = ii[p - 1];
+                                p--;
+                            }
+                            vv[p] = s[i];
+                            ii[p] = col;
+                        }
+                    }
+                }
+                __threadfence_block();
+#pragma unroll
+                for (int i = 0; i < 4; i++) thr[i] = lval[(rbase + i) * K2 + K2 - 1];
+            }
+        }
+        __syncthreads();
+    }
+    // write the block's candidate lists
+    for (uint32_t i = tid; i < KNN_BM * K2; i += 256) {
+        uint32_t r = row0 + i / K2;
+        if (r < n) {
+            cand_idx[(uint64_t)r * K2 + (i % K2)] = lidx[i];
+            cand_s[(uint64_t)r * K2 + (i % K2)] = lval[i];
+        }
+    }
+}
+
+__device__ __forceinline__ double exact_d2(const float *X, uint32_t d, uint32_t i, uint32_t j) {
+    const float *a = X + (uint64_t)i * d;
+    const float *b = X + (uint64_t)j * d;
+    double s = 0.0;
+    for (uint32_t t = 0; t < d; t++) {
+        double diff = (double)a[t] - (double)b[t];
+        s = __dadd_rn(s, __dmul_rn(diff, diff));
+    }
+    return s;
+}
+
+// One thread per row: FP64 rescore of the candidates, top-k by (d2, index), certificate.
+__global__ void k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t K2,
+                              const uint32_t *cand_idx, const float *cand_s, const double *nx64, double eps_coef,
+                              double eps_const, uint32_t *idx_out, double *d2_out, uint32_t *fallback,
+                              unsigned int *n_fallback, unsigned long long *n_rescored) {
+    unsigned long long local = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double bd[KNN_MAXK2];
+        uint32_t bi[KNN_MAXK2];
+        uint32_t cnt = 0;
+        float amax = -INFINITY;
+        bool full = true;
+        for (uint32_t t = 0; t < K2; t++) {
+            uint32_t j = cand_idx[(uint64_t)i * K2 + t];
+            if (j == KNONE) {
+                full = false;
+                continue;
+            }
+            amax = fmaxf(amax, cand_s[(uint64_t)i * K2 + t]);
+            double dd = exact_d2(X, d, i, j);
+            local++;
+            // insertion by (d2, index)
+            uint32_t p = cnt;
+            while (p > 0 && (bd[p - 1] > dd || (bd[p - 1] == dd && bi[p - 1] > j))) {
+                bd[p] = bd[p - 1];
+                bi[p] = bi[p - 1];
+                p--;
+            }
+            bd[p] = dd;
+            bi[p] = j;
+            cnt++;
+        }
+        for (uint32_t t = 0; t < k; t++) {
+            idx_out[(uint64_t)i * k + t] = t < cnt ? bi[t] : KNONE;
+            d2_out[(uint64_t)i * k + t] = t < cnt ? bd[t] : INFINITY;
+        }
+        // certified when no non-candidate can reach the k-th distance (strictly)
+        bool ok = true;
+        if (full && cnt >= k) {
+            double xnorm = sqrt(nx64[i]);
+            double eps = eps_coef * xnorm + eps_const;
+            double lb = nx64[i] + (double)amax - eps;
+            ok = lb > bd[k - 1];
+        } else if (full) {
+            ok = false;
+        }
+        if (!ok) {
+            unsigned int slot = atomicAdd(n_fallback, 1u);
+            fallback[slot] = i;
+        }
+    }
+    if (local) atomicAdd(n_rescored, local);
+}
+
+// Exact FP64 scan of one row per workgroup (128 threads), for rows the certificate rejected.
+constexpr int FB_THREADS = 128;
+constexpr int FB_MAXK = 32;
+
+__global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uint32_t n, uint32_t d, uint32_t k,
+                                                             const uint32_t *rows, uint32_t *idx_out, double *d2_out) {
+    __shared__ double sd[FB_THREADS * FB_MAXK];
+    __shared__ uint32_t si[FB_THREADS * FB_MAXK];
+    __shared__ uint32_t head[FB_THREADS];
+    const uint32_t i = rows[blockIdx.x];
+    const int t = threadIdx.x;
+    double *md = sd + t * FB_MAXK;
+    uint32_t *mi = si + t * FB_MAXK;
+    uint32_t cnt = 0;
+    for (uint32_t j = t; j < n; j += FB_THREADS) {
+        if (j == i) continue;
+        double dd = exact_d2(X, d, i, j);
+        if (cnt == k && !(dd < md[k - 1])) continue;  // j increases: ties keep the lower index
+        uint32_t p = cnt < k ? cnt : k - 1;
+        while (p > 0 && md[p - 1] > dd) {
+            md[p] = md[p - 1];
+            mi[p] = mi[p - 1];
+            p--;
+        }
+        md[p] = dd;
+        mi[p] = j;
+        if (cnt < k) cnt++;
+    }
+    for (uint32_t q = cnt; q < k; q++) {
+        md[q] = INFINITY;
+        mi[q] = KNONE;
+    }
+    head[t] = 0;
+    __syncthreads();
+    if (t == 0) {
+        for (uint32_t out = 0; out < k; out++) {
+            int best = -1;
+            double bdv = INFINITY;
+            uint32_t bidx = KNONE;
+            for (int q = 0; q < FB_THREADS; q++) {
+                uint32_t h = head[q];
+                if (h >= k) continue;
+                double v = sd[q * FB_MAXK + h];
+                uint32_t ix = si[q * FB_MAXK + h];
+                if (ix == KNONE) continue;
+                if (v < bdv || (v == bdv && ix < bidx)) {
+                    bdv = v;
+                    bidx = ix;
+                    best = q;
+                }
+            }
+            if (best >= 0) head[best]++;
+            idx_out[(uint64_t)i * k + out] = bidx;
+            d2_out[(uint64_t)i * k + out] = bdv;
+        }
+    }
+}
+
+struct KnnResult {
+    DevBuf<uint32_t> idx;
+    DevBuf<double> d2;
+    DevBuf<float> X;  // device copy of the input (kept for callers that need it)
+};
+
+void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precision, uint32_t *d_idx,
+                double *d_d2, sc_knn_stats *st, hipStream_t s) {
+    if (k == 0 || k > FB_MAXK) throw ScError(SC_INVALID_ARGUMENT, "k must be in [1, 32]");
+    if (d == 0 || d > 128) throw ScError(SC_INVALID_ARGUMENT, "d must be in [1, 128]");
+    if (n < 2) throw ScError(SC_INVALID_ARGUMENT, "need at least 2 points");
+    const bool bf16 = precision == 0;
+    const uint32_t Kp = (d + 31) / 32 * 32;
+    const uint32_t Kpf = (d + 3) / 4 * 4;
+    const uint32_t npad = (n + KNN_BM - 1) / KNN_BM * KNN_BM;
+    uint32_t K2 = std::min<uint32_t>(2 * k, KNN_MAXK2);
+    if (K2 > n - 1) K2 = std::max<uint32_t>(k, n - 1);
+    if (K2 < k) K2 = k;
+    DevBuf<unsigned short> Xh((size_t)npad * Kp), Xl((size_t)npad * Kp);
+    DevBuf<float> Xf((size_t)npad * Kpf), ny(npad);
+    DevBuf<double> nx64(npad);
+    DevBuf<unsigned int> ymax(1);
+    HIP_OK(hipMemsetAsync(Xh.get(), 0, (size_t)npad * Kp * 2, s));
+    HIP_OK(hipMemsetAsync(Xl.get(), 0, (size_t)npad * Kp * 2, s));
+    HIP_OK(hipMemsetAsync(Xf.get(), 0, (size_t)npad * Kpf * 4, s));
+    HIP_OK(hipMemsetAsync(ny.get(), 0, (size_t)npad * 4, s));
+    HIP_OK(hipMemsetAsync(ymax.get(), 0, sizeof(unsigned int), s));
+    hipEvent_t e0, e1, e2, e3;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventCreate(&e2));
+    HIP_OK(hipEventCreate(&e3));
+    HIP_OK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, n, d, Kp, Kpf, Xh.get(),
+                       Xl.get(), Xf.get(), ny.get(), nx64.get(), ymax.get());
+    KERNEL_OK();
+    DevBuf<uint32_t> cidx((size_t)n * K2);
+    DevBuf<float> cs((size_t)n * K2);
+    const uint32_t strideB = bf16 ? (Kp + 8) : (Kpf + 4);
+    size_t stage_bytes = bf16 ? (size_t)2 * KNN_TN * strideB * 2 : (size_t)KNN_TN * strideB * 4;
+    size_t lds = stage_bytes + KNN_TN * 4 + (size_t)KNN_BM * K2 * 8;
+    if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d/k");
+    HIP_OK(hipEventRecord(e1, s));
+    if (bf16)
+        hipLaunchKernelGGL(k_knn_tile<true>, dim3(npad / KNN_BM), dim3(256), lds, s, Xh.get(), Xl.get(), Xf.get(),
+                           ny.get(), n, npad, Kp, Kpf, K2, cidx.get(), cs.get());
+    else
+        hipLaunchKernelGGL(k_knn_tile<false>, dim3(npad / KNN_BM), dim3(256), lds, s, Xh.get(), Xl.get(), Xf.get(),
+                           ny.get(), n, npad, Kp, Kpf, K2, cidx.get(), cs.get());
+    KERNEL_OK();
+    HIP_OK(hipEventRecord(e2, s));
+    unsigned int ymax_bits = 0;
+    to_host(&ymax_bits, ymax.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    float ymaxf;
+    std::memcpy(&ymaxf, &ymax_bits, 4);
+    double Y = (double)ymaxf;
+    const double u24 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -16);
+    // |s_approx - s| <= norm error + 2 |dot error|; see the header comment
+    double dot_coef = bf16 ? (3.01 * u16 + 3.03 * (3.0 * Kp) * u24) : (2.02 * (double)Kpf * u24);
+    double eps_coef = 2.0 * (2.0 * dot_coef * Y) + std::ldexp(1.0, -20) * 2.0 * Y;
+    double eps_const = 2.0 * (1.01 * d * u24 * Y * Y) + std::ldexp(1.0, -20) * Y * Y;
+    DevBuf<uint32_t> fb(n);
+    DevBuf<unsigned int> nfb(1);
+    DevBuf<unsigned long long> nres(1);
+    HIP_OK(hipMemsetAsync(nfb.get(), 0, sizeof(unsigned int), s));
+    HIP_OK(hipMemsetAsync(nres.get(), 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_knn_rescore, dim3(grid_for(n, 128, 8192)), dim3(128), 0, s, d_X, n, d, k, K2, cidx.get(),
+                       cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fb.get(), nfb.get(), nres.get());
+    KERNEL_OK();
+    unsigned int nf = 0;
+    unsigned long long nr = 0;
+    to_host(&nf, nfb.get(), 1, s);
+    to_host(&nr, nres.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (nf) {
+        hipLaunchKernelGGL(k_knn_fallback, dim3(nf), dim3(FB_THREADS), 0, s, d_X, n, d, k, fb.get(), d_idx, d_d2);
+        KERNEL_OK();
+    }
+    HIP_OK(hipEventRecord(e3, s));
+    HIP_OK(hipEventSynchronize(e3));
+    float t_tile = 0.0f, t_all = 0.0f;
+    HIP_OK(hipEventElapsedTime(&t_tile, e1, e2));
+    HIP_OK(hipEventElapsedTime(&t_all, e0, e3));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(e3);
+    if (st) {
+        st->n = n;
+        st->candidates_rescored = nr;
+        st->rows_fallback = nf;
+        st->tile_kernel_ms = t_tile;
+        st->total_ms = t_all;
+        st->flops = 2.0 * (double)n * (double)n * (double)d;
+    }
+}
+
+// ---- Gaussian connectivity (neighborhood/mod.rs:104-168) --------------------------------------
+__global__ void k_sigma(const double *d2, uint32_t n, uint32_t k, int mode, double *sigma) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (mode == 0) {
+            double dk = d2[(uint64_t)i * k + k - 1];
+            sigma[i] = isfinite(dk) ? sqrt(dk) : 1.0;
+        } else {
+            // upper median of the nonzero squared distances (self at 0 is filtered, :108-121)
+            double v[FB_MAXK];
+            uint32_t c = 0;
+            for (uint32_t t = 0; t < k; t++) {
+                double x = d2[(uint64_t)i * k + t];
+                if (x > 0.0 && isfinite(x)) v[c++] = x;
+            }
+            if (c == 0) {
+                sigma[i] = 1.0;
+                continue;
+            }
+            for (uint32_t a = 1; a < c; a++) {
+                double x = v[a];
+                int b = (int)a - 1;
+                while (b >= 0 && v[b] > x) {
+                    v[b + 1] = v[b];
+                    b--;
+                }
+                v[b + 1] = x;
+            }
+            sigma[i] = sqrt(v[c / 2]);
+        }
+    }
+}
+
+__device__ __forceinline__ float gauss_w(double si, double sj, double d2) {
+    double si2 = si * si, sj2 = sj * sj;
+    double den = si2 + sj2;
+    if (!(den > 0.0)) return 0.0f;
+    double num = 2.0 * si * sj;
+    return (float)(sqrt(num / den) * exp(-d2 / den));
+}
+
+// Emits (row << 32 | col, w) entries. mode 0: both directions of the unordered pair, weight from
+// the canonical (min, max) order; duplicates from the two endpoints' lists collapse later.
+// mode 1: only pairs i <= j from row i's list with w > 1e-14, both directions (:131-168).
+__global__ void k_gauss_emit(const uint32_t *idx, const double *d2, const double *sigma, uint32_t n, uint32_t k,
+                             int mode, uint64_t *keys, float *vals) {
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < (uint64_t)n * k;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t i = (uint32_t)(q / k);
+        uint32_t j = idx[q];
+        uint64_t o = 2 * q;
+        keys[o] = ~0ull;
+        keys[o + 1] = ~0ull;
+        vals[o] = 0.0f;
+        vals[o + 1] = 0.0f;
+        if (j == KNONE) continue;
+        if (mode == 0) {
+            uint32_t a = i < j ? i : j, b = i < j ? j : i;
+            float w = gauss_w(sigma[a], sigma[b], d2[q]);
+            if (!(w > 0.0f)) continue;
+            keys[o] = ((uint64_t)a << 32) | b;
+            vals[o] = w;
+            keys[o + 1] = ((uint64_t)b << 32) | a;
+            vals[o + 1] = w;
+        } else {
+            if (!(i <= j)) continue;
+            float w = gauss_w(sigma[i], sigma[j], d2[q]);
+            if (!(w > 1e-14f)) continue;
+            keys[o] = ((uint64_t)i << 32) | j;
+            vals[o] = w;
+            if (i != j) {
+                keys[o + 1] = ((uint64_t)j << 32) | i;
+                vals[o + 1] = w;
+            }
+        }
+    }
+}
+
+__global__ void k_dedup_heads(const uint64_t *keys, uint64_t m, uint32_t *head) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x)
+        head[e] = (keys[e] != ~0ull && (e == 0 || keys[e] != keys[e - 1])) ? 1u : 0u;
+}
+
+// symmetrize by max: equal keys carry equal weights here, the max keeps that explicit
+__global__ void k_dedup_emit(const uint64_t *keys, const float *vals, const uint32_t *head, const uint32_t *pos,
+                             uint64_t m, uint32_t *nbrs, float *wts, unsigned long long *row_count) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
+        if (!head[e]) continue;
+        uint64_t key = keys[e];
+        float w = vals[e];
+        for (uint64_t f = e + 1; f < m && keys[f] == key; f++) w = fmaxf(w, vals[f]);
+        uint32_t o = pos[e];
+        nbrs[o] = (uint32_t)(key & 0xFFFFFFFFull);
+        wts[o] = w;
+        atomicAdd(&row_count[key >> 32], 1ull);
+    }
+}
+
+std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const double *d_d2, uint32_t n, uint32_t k,
+                                                int mode, hipStream_t s) {
+    DevBuf<double> sigma(n);
+    hipLaunchKernelGGL(k_sigma, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d_d2, n, k, mode, sigma.get());
+    KERNEL_OK();
+    uint64_t m = 2ull * n * k;
+    DevBuf<uint64_t> keys(m);
+    DevBuf<float> vals(m);
+    hipLaunchKernelGGL(k_gauss_emit, dim3(grid_for((uint64_t)n * k, 256, 8192)), dim3(256), 0, s, d_idx, d_d2,
+                       sigma.get(), n, k, mode, keys.get(), vals.get());
+    KERNEL_OK();
+    radix_sort_pairs_u64_f32(keys.get(), vals.get(), m, 64, s);
+    DevBuf<uint32_t> head(m), pos(m);
+    hipLaunchKernelGGL(k_dedup_heads, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, keys.get(), m, head.get());
+    KERNEL_OK();
+    exclusive_scan_u32(head.get(), pos.get(), m, s);
+    uint32_t lp = 0, lh = 0;
+    to_host(&lp, pos.get() + m - 1, 1, s);
+    to_host(&lh, head.get() + m - 1, 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    uint64_t nnz = (uint64_t)lp + lh;
+    auto g = std::make_unique<sc_graph>();
+    g->n = n;
+    g->nnz = nnz;
+    g->nbrs.resize(nnz);
+    g->wts.resize(nnz);
+    DevBuf<unsigned long long> rc(n + 1);
+    HIP_OK(hipMemsetAsync(rc.get(), 0, (n + 1) * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_dedup_emit, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, keys.get(), vals.get(), head.get(),
+                       pos.get(), m, g->nbrs.get(), g->wts.get(), rc.get());
+    KERNEL_OK();
+    g->ptrs.resize(n + 1);
+    exclusive_scan_u64(reinterpret_cast<uint64_t *>(rc.get()), g->ptrs.get(), n + 1, s);
+    g->node_w.resize(n);
+    std::vector<double> ones(n, 1.0);
+    to_device(g->node_w.get(), ones.data(), n, s);
+    graph_finish_sorted(*g, s);
+    HIP_OK(hipStreamSynchronize(s));
+    return g;
+}
+
+}  // namespace scgpu
+
+using namespace scgpu;
+
+static int knn_precision(const sc_knn_opts *o) { return o ? o->precision : 0; }
+static int knn_mode(const sc_knn_opts *o) { return o ? o->mode : 0; }
+
+extern "C" {
+
+sc_status sc_knn(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc_knn_opts *opts, uint32_t *idx_out,
+                 double *d2_out, sc_knn_stats *stats, sc_error *err) {
+    try {
+        if (!data) throw ScError(SC_INVALID_ARGUMENT, "null data");
+        if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        hipStream_t s = thread_stream();
+        DevBuf<float> X((size_t)n * d);
+        to_device(X.get(), data, (size_t)n * d, s);
+        DevBuf<uint32_t> idx((size_t)n * k);
+        DevBuf<double> d2((size_t)n * k);
+        knn_device(X.get(), (uint32_t)n, d, k, knn_precision(opts), idx.get(), d2.get(), stats, s);
+        if (idx_out) to_host(idx_out, idx.get(), (size_t)n * k, s);
+        if (d2_out) to_host(d2_out, d2.get(), (size_t)n * k, s);
+        HIP_OK(hipStreamSynchronize(s));
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc_knn_opts *opts, sc_graph **out,
+                       sc_knn_stats *stats, sc_error *err) {
+    try {
+        if (!data || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        *out = nullptr;
+        hipStream_t s = thread_stream();
+        DevBuf<float> X((size_t)n * d);
+        to_device(X.get(), data, (size_t)n * d, s);
+        DevBuf<uint32_t> idx((size_t)n * k);
+        DevBuf<double> d2((size_t)n * k);
+        knn_device(X.get(), (uint32_t)n, d, k, knn_precision(opts), idx.get(), d2.get(), stats, s);
+        std::unique_ptr<sc_graph> g = gaussian_graph_device(idx.get(), d2.get(), (uint32_t)n, k, knn_mode(opts), s);
+        *out = g.release();
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+sc_status sc_knn_gaussian_csr(const uint32_t *idx, const double *d2, uint64_t n, uint32_t k, int32_t mode,
+                              uint64_t *node_ptrs, uint32_t *neighbors, float *weights, uint64_t *nnz_out,
+                              sc_error *err) {
+    try {
+        if (!idx || !d2 || !node_ptrs || !neighbors || !weights || !nnz_out)
+            throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (k == 0 || k > 32) throw ScError(SC_INVALID_ARGUMENT, "k must be in [1, 32]");
+        for (uint64_t q = 0; q < n * k; q++)
+            if (idx[q] != KNONE && (uint64_t)idx[q] >= n) {
+                ScError e(SC_NODE_INDEX_OUT_OF_RANGE, "neighbour index " + std::to_string(idx[q]) +
+                                                          " is out of range for " + std::to_string(n) + " points");
+                e.payload.node = idx[q];
+                e.payload.n_nodes = n;
+                throw e;
+            }
+        hipStream_t s = thread_stream();
+        DevBuf<uint32_t> di((size_t)n * k);
+        DevBuf<double> dd((size_t)n * k);
+        to_device(di.get(), idx, (size_t)n * k, s);
+        to_device(dd.get(), d2, (size_t)n * k, s);
+        std::unique_ptr<sc_graph> g = gaussian_graph_device(di.get(), dd.get(), (uint32_t)n, k, mode, s);
+        to_host(node_ptrs, g->ptrs.get(), n + 1, s);
+        to_host(neighbors, g->nbrs.get(), g->nnz, s);
+        to_host(weights, g->wts.get(), g->nnz, s);
+        HIP_OK(hipStreamSynchronize(s));
+        *nnz_out = g->nnz;
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+}  // extern "C"

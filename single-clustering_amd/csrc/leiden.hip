@@ -1,0 +1,1234 @@
+// leiden.hip -- the Leiden core on gfx950: local moving, refinement, aggregation, driver.
+//
+// Restates leiden/{local_move.rs:39-122, refine.rs:29-153, mod.rs:168-289} and
+// network/csr_network.rs:516-608 (aggregate) for a GPU:
+//
+//  * Local moving runs synchronous sub-sweeps. Each iteration splits the active vertices into P
+//    pseudo-random subsets; within a subset every vertex picks its best community from one
+//    snapshot (decide), then all moves are applied (apply) and neighbours outside the new
+//    community are re-activated (activate) -- the reference's queue re-push rule
+//    (local_move.rs:114-118). A singleton may join another singleton only if the target id is
+//    lower, which removes swap oscillation. Iterations stop when nothing moves.
+//  * Neighbour-community weights are accumulated per vertex in an LDS open-addressing hash
+//    (global memory for very high degree), vertex groups of G lanes sized by degree bucket.
+//    Weights and community totals are int64 fixed point: atomic accumulation is exact and order
+//    independent, so a fixed seed gives bit-identical labels.
+//  * Delta-quality is the reference's formula in f64 (objective.rs:91-106, :158-169), ties go
+//    to the lower community id, acceptance needs gain > 1e-12 * max(m, 1) (local_move.rs:22-28).
+//  * Refinement (only singletons move, moves stay inside the working community,
+//    refine.rs:9-11) runs in rounds. A community that is targeted in a round does not move in
+//    that round (so every join attaches to a stable community and connectivity is kept), except
+//    a mutual pair, where the higher id joins the lower when nobody else targets it. The
+//    exp(gain/theta) draw of refine.rs:110-153 is taken with the Gumbel-max trick, which has the
+//    same distribution and does not depend on candidate order.
+//  * Aggregation sums every coarse edge in the reference's order (members ascending, rows
+//    ascending) via a stable radix sort by (group, neighbour group), so coarse weights are
+//    bit-identical to the CPU path.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <random>
+#include <vector>
+
+namespace scgpu {
+
+double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, double gamma, hipStream_t s);
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
+constexpr int NUM_BUCKETS = 7;
+constexpr int MOVE_SUBSWEEPS = 4;
+constexpr int MAX_MOVE_ITERS = 100;
+constexpr int MAX_REFINE_ROUNDS = 64;
+
+thread_local sc_leiden_stats t_stats;
+
+struct DecideArgs {
+    // level graph
+    uint32_t n;
+    const uint64_t *ptrs;
+    const uint32_t *nbrs;
+    const float *wts;
+    const double *strength;
+    const double *node_w;
+    const double *self_loop;
+    // vertices of this bucket
+    const uint32_t *list;
+    uint32_t list_len;
+    // partition state (working partition for local moving, refined one for refinement)
+    const uint32_t *memb;
+    const uint32_t *constraint;
+    const long long *Kfx;
+    const long long *Wfx;
+    const uint32_t *csize;
+    uint8_t *active;
+    uint32_t *target;
+    uint32_t *tround;
+    uint32_t *movers;
+    unsigned int *n_movers;
+    uint32_t *targeted_cnt;
+    // objective
+    int objective;
+    double gamma;
+    double four_m;
+    double threshold;
+    int sK, sW, sw;
+    int has_max;
+    double max_w;
+    // randomness / scheduling
+    uint64_t seed;
+    uint32_t P;
+    uint32_t sub;
+    uint32_t round;
+    double theta;
+    // global hash tables for the highest-degree bucket
+    uint32_t *gkeys;
+    long long *gvals;
+    const uint64_t *goff;
+    // counters
+    unsigned long long *stats;  // [vertices, edges, candidates]
+};
+
+__device__ __forceinline__ double delta_insert(const DecideArgs &a, double w_c, double loop, double k,
+                                               double na, double Kc, double Wc) {
+    if (a.objective == 0) {
+        if (a.four_m == 0.0) return 0.0;
+        return (w_c + loop) - a.gamma * k * (2.0 * Kc + k) / a.four_m;
+    }
+    return (w_c + loop) - a.gamma * na * (2.0 * Wc + na - 1.0) / 2.0;
+}
+
+__device__ __forceinline__ uint32_t subset_of(uint64_t seed, uint32_t v, uint32_t P) {
+    return (uint32_t)((mix64(seed ^ ((uint64_t)v * 0x9E3779B97F4A7C15ull)) >> 33) % P);
+}
+
+__device__ __forceinline__ double gumbel(uint64_t seed, uint32_t round, uint32_t v, uint32_t c) {
+    uint64_t h = mix64(seed ^ mix64(((uint64_t)v << 32) ^ c) ^ ((uint64_t)round * 0xD1B54A32D192ED03ull));
+    double u = u01_open(h);
+    return -log(-log(u));
+}
+
+template <typename KeyPtr, typename ValPtr>
+__device__ __forceinline__ void table_insert(KeyPtr keys, ValPtr vals, uint32_t mask, uint32_t c, long long w) {
+    uint32_t h = hash32(c) & mask;
+    while (true) {
+        uint32_t prev = atomicCAS(&keys[h], EMPTY_KEY, c);
+        if (prev == EMPTY_KEY || prev == c) {
+            atomicAdd((unsigned long long *)&vals[h], (unsigned long long)w);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// argmax over (score, id): larger score wins, ties to the smaller id
+__device__ __forceinline__ void better(double &bs, uint32_t &bc, double s, uint32_t c) {
+    if (s > bs || (s == bs && c < bc)) { bs = s; bc = c; }
+}
+
+template <int G>
+__device__ __forceinline__ void group_reduce_wave(double &bs, uint32_t &bc, long long &wown) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        double os = __shfl_xor(bs, off, G);
+        uint32_t oc = __shfl_xor(bc, off, G);
+        long long ow = __shfl_xor(wown, off, G);
+        better(bs, bc, os, oc);
+        wown += ow;
+    }
+}
+
+template <int BS, int G, int T, bool REFINE, bool GTAB>
+__global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
+    constexpr int NG = BS / G;
+    constexpr int TS = GTAB ? 1 : NG * T;
+    __shared__ uint32_t s_keys[TS];
+    __shared__ long long s_vals[TS];
+    __shared__ double r_s[BS / 64];
+    __shared__ uint32_t r_c[BS / 64];
+    __shared__ long long r_w[BS / 64];
+    __shared__ unsigned long long s_stats[3];
+    if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
+
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    unsigned long long st_v = 0, st_e = 0, st_c = 0;
+
+    for (uint64_t base = (uint64_t)blockIdx.x * NG; base < a.list_len; base += (uint64_t)gridDim.x * NG) {
+        const uint64_t idx = base + grp;
+        bool valid = idx < a.list_len;
+        uint32_t v = valid ? a.list[idx] : 0;
+        if (valid) {
+            valid = a.active[v] != 0 && subset_of(a.seed, v, a.P) == a.sub;
+            if (REFINE) valid = valid && a.memb[v] == v && a.csize[v] == 1;
+        }
+        uint32_t *keys;
+        long long *vals;
+        uint32_t tsize;
+        if (GTAB) {
+            uint64_t o = valid ? a.goff[idx] : 0;
+            tsize = valid ? (uint32_t)(a.goff[idx + 1] - o) : 0;
+            keys = a.gkeys + o;
+            vals = a.gvals + o;
+        } else {
+            keys = s_keys + grp * T;
+            vals = s_vals + grp * T;
+            tsize = T;
+        }
+        for (uint32_t s = lane; s < tsize; s += G) {
+            keys[s] = EMPTY_KEY;
+            vals[s] = 0;
+        }
+        __syncthreads();
+        if (GTAB) __threadfence_block();
+
+        uint64_t beg = 0, end = 0;
+        uint32_t own = 0, cv = 0;
+        if (valid) {
+            beg = a.ptrs[v];
+            end = a.ptrs[v + 1];
+            own = a.memb[v];
+            if (REFINE) cv = a.constraint[v];
+        }
+        const uint32_t mask = tsize - 1;
+        for (uint64_t e = beg + lane; e < end; e += G) {
+            uint32_t u = a.nbrs[e];
+            if (u == v) continue;  // self-loops are handled by the partition (partition.rs:430)
+            if (REFINE && a.constraint[u] != cv) continue;
+            uint32_t c = a.memb[u];
+            long long wf = to_fx((double)a.wts[e], a.sw);
+            table_insert(keys, vals, mask, c, wf);
+        }
+        __syncthreads();
+        if (GTAB) __threadfence_block();
+
+        double bs = -INFINITY;
+        uint32_t bc = NONE;
+        long long wown = 0;
+        double k = 0.0, na = 0.0, loop = 0.0, rbase = 0.0;
+        if (valid) {
+            k = a.strength[v];
+            na = a.node_w[v];
+            loop = a.self_loop[v];
+            // refinement: the node leaves its singleton, which becomes empty (refine.rs:71-74)
+            if (REFINE) rbase = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
+            for (uint32_t s = lane; s < tsize; s += G) {
+                uint32_t c = keys[s];
+                if (c == EMPTY_KEY) continue;
+                long long wfx = vals[s];
+                if (c == own) {
+                    wown = wfx;
+                    continue;
+                }
+                st_c++;
+                double Wc = from_fx(a.Wfx[c], a.sW);
+                if (a.has_max && !(Wc + na <= a.max_w)) continue;
+                double Kc = a.objective == 0 ? from_fx(a.Kfx[c], a.sK) : 0.0;
+                double gain = delta_insert(a, from_fx(wfx, a.sw), loop, k, na, Kc, Wc);
+                if (REFINE) {
+                    double rel = gain - rbase;
+                    if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
+                    double score = a.theta > 0.0 ? rel / a.theta + gumbel(a.seed, a.round, v, c) : rel;
+                    better(bs, bc, score, c);
+                } else {
+                    better(bs, bc, gain, c);
+                }
+            }
+        }
+        // group-wide reduction
+        if (G <= 64) {
+            group_reduce_wave<(G <= 64 ? G : 64)>(bs, bc, wown);
+        } else {
+            group_reduce_wave<64>(bs, bc, wown);
+            const int wave = threadIdx.x / 64;
+            if ((threadIdx.x & 63) == 0) {
+                r_s[wave] = bs;
+                r_c[wave] = bc;
+                r_w[wave] = wown;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                bs = r_s[0];
+                bc = r_c[0];
+                wown = r_w[0];
+                for (int w = 1; w < BS / 64; w++) {
+                    better(bs, bc, r_s[w], r_c[w]);
+                    wown += r_w[w];
+                }
+            }
+        }
+        if (valid && lane == 0) {
+            a.active[v] = 0;
+            st_v++;
+            st_e += end - beg;
+            if (!REFINE) {
+                long long kfx = a.objective == 0 ? to_fx(k, a.sK) : 0;
+                long long wfxv = to_fx(na, a.sW);
+                double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, a.sK) : 0.0;
+                double Wown = from_fx(a.Wfx[own] - wfxv, a.sW);
+                double base = delta_insert(a, from_fx(wown, a.sw), loop, k, na, Kown, Wown);
+                // the empty community: this vertex's own id, when no one holds it
+                if (a.csize[v] == 0 && (!a.has_max || na <= a.max_w)) {
+                    double ge = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
+                    better(bs, bc, ge, v);
+                }
+                bool accept = bc != NONE && bc != own && bs - base > a.threshold;
+                if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
+                if (accept) {
+                    a.target[v] = bc;
+                    unsigned int slot = atomicAdd(a.n_movers, 1u);
+                    a.movers[slot] = v;
+                }
+            } else if (bc != NONE) {
+                a.target[v] = bc;
+                a.tround[v] = a.round;
+                atomicAdd(&a.targeted_cnt[bc], 1u);
+                unsigned int slot = atomicAdd(a.n_movers, 1u);
+                a.movers[slot] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0 && st_v) {
+        atomicAdd(&s_stats[0], st_v);
+        atomicAdd(&s_stats[1], st_e);
+    }
+    if (st_c) atomicAdd(&s_stats[2], st_c);
+    __syncthreads();
+    if (threadIdx.x < 3 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+struct ApplyArgs {
+    const uint32_t *movers;
+    const unsigned int *n_movers;
+    uint32_t *memb;
+    const uint32_t *target;
+    const uint32_t *tround;
+    const uint32_t *targeted_cnt;
+    const uint32_t *csize_snapshot;  // unused placeholder for symmetry with decide
+    long long *Kfx;
+    long long *Wfx;
+    uint32_t *csize;
+    const double *strength;
+    const double *node_w;
+    int sK, sW;
+    uint32_t round;
+    uint8_t *active;
+    uint32_t *moved;
+    unsigned int *n_moved;
+};
+
+// Local moving: every decided mover moves (targets were checked against the snapshot).
+__global__ void k_apply_move(ApplyArgs a) {
+    unsigned int cnt = *a.n_movers;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        uint32_t v = a.movers[i];
+        uint32_t t = a.target[v];
+        uint32_t old = a.memb[v];
+        a.memb[v] = t;
+        long long kf = to_fx(a.strength[v], a.sK), wf = to_fx(a.node_w[v], a.sW);
+        atomicAdd((unsigned long long *)&a.Kfx[t], (unsigned long long)kf);
+        atomicAdd((unsigned long long *)&a.Kfx[old], (unsigned long long)(-kf));
+        atomicAdd((unsigned long long *)&a.Wfx[t], (unsigned long long)wf);
+        atomicAdd((unsigned long long *)&a.Wfx[old], (unsigned long long)(-wf));
+        atomicAdd(&a.csize[t], 1u);
+        atomicSub(&a.csize[old], 1u);
+    }
+}
+
+// Refinement: a targeted singleton stays and absorbs its joiners, except the higher id of a
+// mutual pair that nobody else targets.
+__global__ void k_apply_refine(ApplyArgs a) {
+    unsigned int cnt = *a.n_movers;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        uint32_t v = a.movers[i];
+        uint32_t t = a.target[v];
+        uint32_t tc = a.targeted_cnt[v];
+        bool go = tc == 0 ||
+                  (tc == 1 && t < v && a.tround[t] == a.round && a.target[t] == v);
+        if (!go) {
+            a.active[v] = 1;  // blocked: look again next round
+            continue;
+        }
+        a.memb[v] = t;
+        long long kf = to_fx(a.strength[v], a.sK), wf = to_fx(a.node_w[v], a.sW);
+        atomicAdd((unsigned long long *)&a.Kfx[t], (unsigned long long)kf);
+        atomicAdd((unsigned long long *)&a.Kfx[v], (unsigned long long)(-kf));
+        atomicAdd((unsigned long long *)&a.Wfx[t], (unsigned long long)wf);
+        atomicAdd((unsigned long long *)&a.Wfx[v], (unsigned long long)(-wf));
+        atomicAdd(&a.csize[t], 1u);
+        atomicSub(&a.csize[v], 1u);
+        unsigned int slot = atomicAdd(a.n_moved, 1u);
+        a.moved[slot] = v;
+    }
+}
+
+// Re-activation after moves. Local moving: neighbours not in the mover's new community
+// (local_move.rs:114-118). Refinement: neighbours inside the same working community.
+template <int G, bool REFINE>
+__global__ void __launch_bounds__(256) k_activate(const uint32_t *list, const unsigned int *count,
+                                                  const uint64_t *ptrs, const uint32_t *nbrs,
+                                                  const uint32_t *memb, const uint32_t *constraint,
+                                                  uint8_t *active) {
+    constexpr int NG = 256 / G;
+    unsigned int cnt = *count;
+    const int lane = threadIdx.x % G;
+    for (unsigned int i = blockIdx.x * NG + threadIdx.x / G; i < cnt; i += gridDim.x * NG) {
+        uint32_t v = list[i];
+        uint32_t mv = REFINE ? constraint[v] : memb[v];
+        for (uint64_t e = ptrs[v] + lane; e < ptrs[v + 1]; e += G) {
+            uint32_t u = nbrs[e];
+            if (u == v) continue;
+            if (REFINE ? (constraint[u] == mv) : (memb[u] != mv)) active[u] = 1;
+        }
+    }
+}
+
+__global__ void k_init_totals(const uint32_t *memb, const double *strength, const double *node_w, uint32_t n,
+                              int sK, int sW, long long *Kfx, long long *Wfx, uint32_t *csize) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        uint32_t c = memb[v];
+        atomicAdd((unsigned long long *)&Kfx[c], (unsigned long long)to_fx(strength[v], sK));
+        atomicAdd((unsigned long long *)&Wfx[c], (unsigned long long)to_fx(node_w[v], sW));
+        atomicAdd(&csize[c], 1u);
+    }
+}
+
+__global__ void k_iota(uint32_t *p, uint32_t n) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) p[v] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t x) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = x;
+}
+
+// ---- degree buckets ------------------------------------------------------------------------
+__device__ __forceinline__ int bucket_of(uint64_t d) {
+    if (d <= 8) return 0;
+    if (d <= 16) return 1;
+    if (d <= 32) return 2;
+    if (d <= 128) return 3;
+    if (d <= 1024) return 4;
+    if (d <= 2048) return 5;
+    return 6;
+}
+
+__global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, uint32_t *counts) {
+    __shared__ uint32_t sc[NUM_BUCKETS];
+    if (threadIdx.x < NUM_BUCKETS) sc[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        atomicAdd(&sc[bucket_of(ptrs[v + 1] - ptrs[v])], 1u);
+    __syncthreads();
+    if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
+}
+
+__global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, const uint32_t *starts, uint32_t *cursor,
+                              uint32_t *lists) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        int b = bucket_of(ptrs[v + 1] - ptrs[v]);
+        uint32_t slot = atomicAdd(&cursor[b], 1u);
+        lists[starts[b] + slot] = v;
+    }
+}
+
+__global__ void k_gtab_sizes(const uint32_t *list, uint32_t len, const uint64_t *ptrs, uint64_t *sizes) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= len; i += gridDim.x * blockDim.x) {
+        if (i == len) { sizes[i] = 0; continue; }
+        uint32_t v = list[i];
+        uint64_t d = ptrs[v + 1] - ptrs[v];
+        uint64_t t = 1;
+        while (t < 2 * d) t <<= 1;
+        sizes[i] = t;
+    }
+}
+
+struct Buckets {
+    DevBuf<uint32_t> lists;
+    uint32_t start[NUM_BUCKETS + 1] = {0};
+    DevBuf<uint64_t> goff;
+    DevBuf<uint32_t> gkeys;
+    DevBuf<long long> gvals;
+};
+
+static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
+    uint32_t n = (uint32_t)L.n;
+    DevBuf<uint32_t> cnt(2 * NUM_BUCKETS + 2);
+    HIP_OK(hipMemsetAsync(cnt.get(), 0, (2 * NUM_BUCKETS + 2) * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_bucket_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, cnt.get());
+    KERNEL_OK();
+    uint32_t h[NUM_BUCKETS];
+    to_host(h, cnt.get(), NUM_BUCKETS, s);
+    HIP_OK(hipStreamSynchronize(s));
+    b.start[0] = 0;
+    for (int i = 0; i < NUM_BUCKETS; i++) b.start[i + 1] = b.start[i] + h[i];
+    DevBuf<uint32_t> starts(NUM_BUCKETS + 1);
+    to_device(starts.get(), b.start, NUM_BUCKETS + 1, s);
+    b.lists.resize(n);
+    hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, starts.get(),
+                       cnt.get() + NUM_BUCKETS, b.lists.get());
+    KERNEL_OK();
+    uint32_t nb6 = h[6];
+    if (nb6) {
+        b.goff.resize(nb6 + 1);
+        DevBuf<uint64_t> sizes(nb6 + 1);
+        hipLaunchKernelGGL(k_gtab_sizes, dim3(grid_for(nb6 + 1, 256, 1024)), dim3(256), 0, s,
+                           b.lists.get() + b.start[6], nb6, L.ptrs.get(), sizes.get());
+        KERNEL_OK();
+        exclusive_scan_u64(sizes.get(), b.goff.get(), nb6 + 1, s);
+        uint64_t total = 0;
+        to_host(&total, b.goff.get() + nb6, 1, s);
+        HIP_OK(hipStreamSynchronize(s));
+        b.gkeys.resize(total);
+        b.gvals.resize(total);
+    }
+}
+
+// ---- profiling ---------------------------------------------------------------------------
+struct KernelTimer {
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    size_t used = 0;
+    double total_ms = 0.0;
+    uint64_t launches = 0;
+    bool on = false;
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(hipStream_t s, hipEvent_t &a) {
+        if (!on) return;
+        a = next();
+        HIP_OK(hipEventRecord(a, s));
+    }
+    void end(hipStream_t s, hipEvent_t a) {
+        if (!on) return;
+        hipEvent_t b = next();
+        HIP_OK(hipEventRecord(b, s));
+        pending.emplace_back(a, b);
+        launches++;
+    }
+    // call after a stream synchronize
+    void harvest() {
+        for (auto &p : pending) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, p.first, p.second));
+            total_ms += ms;
+        }
+        pending.clear();
+        used = 0;
+    }
+    ~KernelTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// ---- level state -----------------------------------------------------------------------------
+struct LevelState {
+    DevBuf<long long> Kfx, Wfx;
+    DevBuf<uint32_t> csize;
+    DevBuf<uint8_t> active;
+    DevBuf<uint32_t> target, tround, movers, moved, targeted;
+    DevBuf<unsigned int> counters;           // [0] movers, [1] moved
+    DevBuf<unsigned long long> stat;         // vertices, edges, candidates
+};
+
+struct RunCtx {
+    const sc_leiden_config *cfg;
+    hipStream_t s;
+    KernelTimer timer;
+    sc_leiden_stats st{};
+    double move_bytes = 0.0;
+};
+
+template <int BS, int G, int T, bool REFINE, bool GTAB>
+static void launch_decide(const DecideArgs &a, hipStream_t s) {
+    constexpr int NG = BS / G;
+    unsigned grid = grid_for(a.list_len, NG, 8192);
+    hipLaunchKernelGGL((k_decide<BS, G, T, REFINE, GTAB>), dim3(grid), dim3(BS), 0, s, a);
+    KERNEL_OK();
+}
+
+template <bool REFINE>
+static void decide_all(DecideArgs a, const Buckets &b, hipStream_t s, KernelTimer *timer) {
+    for (int bk = 0; bk < NUM_BUCKETS; bk++) {
+        uint32_t len = b.start[bk + 1] - b.start[bk];
+        if (!len) continue;
+        a.list = b.lists.get() + b.start[bk];
+        a.list_len = len;
+        hipEvent_t ev = nullptr;
+        if (timer) timer->begin(s, ev);
+        switch (bk) {
+            case 0: launch_decide<256, 8, 16, REFINE, false>(a, s); break;
+            case 1: launch_decide<256, 16, 32, REFINE, false>(a, s); break;
+            case 2: launch_decide<256, 32, 64, REFINE, false>(a, s); break;
+            case 3: launch_decide<256, 64, 256, REFINE, false>(a, s); break;
+            case 4: launch_decide<64, 64, 2048, REFINE, false>(a, s); break;
+            case 5: launch_decide<256, 256, 4096, REFINE, false>(a, s); break;
+            default:
+                a.gkeys = b.gkeys.get();
+                a.gvals = b.gvals.get();
+                a.goff = b.goff.get();
+                launch_decide<256, 256, 1, REFINE, true>(a, s);
+                break;
+        }
+        if (timer) timer->end(s, ev);
+    }
+}
+
+static DecideArgs base_args(const sc_graph &L, LevelState &ls, const sc_leiden_config &cfg, int sK, int sW,
+                            int sw) {
+    DecideArgs a{};
+    a.n = (uint32_t)L.n;
+    a.ptrs = L.ptrs.get();
+    a.nbrs = L.nbrs.get();
+    a.wts = L.wts.get();
+    a.strength = L.strength.get();
+    a.node_w = L.node_w.get();
+    a.self_loop = L.self_loop.get();
+    a.Kfx = ls.Kfx.get();
+    a.Wfx = ls.Wfx.get();
+    a.csize = ls.csize.get();
+    a.active = ls.active.get();
+    a.target = ls.target.get();
+    a.tround = ls.tround.get();
+    a.movers = ls.movers.get();
+    a.n_movers = ls.counters.get();
+    a.targeted_cnt = ls.targeted.get();
+    a.objective = cfg.objective;
+    a.gamma = cfg.resolution;
+    a.four_m = 4.0 * L.total_weight;
+    a.threshold = 1e-12 * std::max(std::fabs(L.total_weight), 1.0);
+    a.sK = sK;
+    a.sW = sW;
+    a.sw = sw;
+    a.has_max = cfg.has_max_community_weight;
+    a.max_w = cfg.max_community_weight;
+    a.P = 1;
+    a.stats = ls.stat.get();
+    return a;
+}
+
+static void alloc_level(LevelState &ls, uint32_t n) {
+    ls.Kfx.resize(n);
+    ls.Wfx.resize(n);
+    ls.csize.resize(n);
+    ls.active.resize(n);
+    ls.target.resize(n);
+    ls.tround.resize(n);
+    ls.movers.resize(n);
+    ls.moved.resize(n);
+    ls.targeted.resize(n);
+    ls.counters.resize(4);
+    ls.stat.resize(4);
+}
+
+static void init_totals(const sc_graph &L, const uint32_t *memb, LevelState &ls, int sK, int sW, hipStream_t s) {
+    uint32_t n = (uint32_t)L.n;
+    HIP_OK(hipMemsetAsync(ls.Kfx.get(), 0, n * sizeof(long long), s));
+    HIP_OK(hipMemsetAsync(ls.Wfx.get(), 0, n * sizeof(long long), s));
+    HIP_OK(hipMemsetAsync(ls.csize.get(), 0, n * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_init_totals, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, memb, L.strength.get(),
+                       L.node_w.get(), n, sK, sW, ls.Kfx.get(), ls.Wfx.get(), ls.csize.get());
+    KERNEL_OK();
+}
+
+static void harvest_stats(LevelState &ls, RunCtx &ctx, bool move_phase) {
+    unsigned long long h[3];
+    to_host(h, ls.stat.get(), 3, ctx.s);
+    HIP_OK(hipStreamSynchronize(ctx.s));
+    HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 3 * sizeof(unsigned long long), ctx.s));
+    ctx.st.vertex_visits += h[0];
+    ctx.st.edge_visits += h[1];
+    // minimum unique bytes: neighbour id + weight + membership per entry, 32 B per vertex, one
+    // 8-byte community total per candidate scored
+    double bytes = 12.0 * (double)h[1] + 32.0 * (double)h[0] + 8.0 * (double)h[2];
+    ctx.st.algorithmic_bytes += bytes;
+    if (move_phase) ctx.move_bytes += bytes;
+}
+
+// Local moving on one level (local_move.rs:39-122), synchronous sub-sweeps.
+static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const Buckets &b, RunCtx &ctx,
+                       uint64_t level_seed, int sK, int sW, int sw) {
+    uint32_t n = (uint32_t)L.n;
+    hipStream_t s = ctx.s;
+    init_totals(L, memb, ls, sK, sW, s);
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
+    KERNEL_OK();
+    HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
+    DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
+    a.memb = memb;
+    a.P = MOVE_SUBSWEEPS;
+    ApplyArgs ap{};
+    ap.movers = ls.movers.get();
+    ap.n_movers = ls.counters.get();
+    ap.memb = memb;
+    ap.target = ls.target.get();
+    ap.Kfx = ls.Kfx.get();
+    ap.Wfx = ls.Wfx.get();
+    ap.csize = ls.csize.get();
+    ap.strength = L.strength.get();
+    ap.node_w = L.node_w.get();
+    ap.sK = sK;
+    ap.sW = sW;
+    DevBuf<unsigned int> iter_moves(MOVE_SUBSWEEPS);
+    for (int it = 0; it < MAX_MOVE_ITERS; it++) {
+        a.seed = mix64(level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
+        for (uint32_t sub = 0; sub < (uint32_t)MOVE_SUBSWEEPS; sub++) {
+            a.sub = sub;
+            a.n_movers = iter_moves.get() + sub;
+            ap.n_movers = iter_moves.get() + sub;
+            HIP_OK(hipMemsetAsync(iter_moves.get() + sub, 0, sizeof(unsigned int), s));
+            decide_all<false>(a, b, s, ctx.timer.on ? &ctx.timer : nullptr);
+            hipLaunchKernelGGL(k_apply_move, dim3(1024), dim3(256), 0, s, ap);
+            KERNEL_OK();
+            hipLaunchKernelGGL((k_activate<16, false>), dim3(1024), dim3(256), 0, s, ls.movers.get(),
+                               iter_moves.get() + sub, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)memb,
+                               (const uint32_t *)nullptr, ls.active.get());
+            KERNEL_OK();
+        }
+        unsigned int mv[MOVE_SUBSWEEPS];
+        to_host(mv, iter_moves.get(), MOVE_SUBSWEEPS, s);
+        HIP_OK(hipStreamSynchronize(s));
+        if (ctx.timer.on) ctx.timer.harvest();
+        uint64_t moves = 0;
+        for (int i = 0; i < MOVE_SUBSWEEPS; i++) moves += mv[i];
+        ctx.st.move_iterations++;
+        ctx.st.moves += moves;
+        if (moves == 0) break;
+    }
+    harvest_stats(ls, ctx, true);
+}
+
+// Refinement on one level (refine.rs:29-103); returns the refined membership in rmemb.
+static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, LevelState &ls, const Buckets &b,
+                   RunCtx &ctx, uint64_t level_seed, int sK, int sW, int sw) {
+    uint32_t n = (uint32_t)L.n;
+    hipStream_t s = ctx.s;
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rmemb, n);
+    KERNEL_OK();
+    init_totals(L, rmemb, ls, sK, sW, s);
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
+    KERNEL_OK();
+    HIP_OK(hipMemsetAsync(ls.tround.get(), 0xFF, n * sizeof(uint32_t), s));
+    DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
+    a.memb = rmemb;
+    a.constraint = memb;
+    a.P = 1;
+    a.sub = 0;
+    double mean_w = L.edge_count > 0 ? L.total_weight / (double)L.edge_count : 1.0;
+    a.theta = ctx.cfg->refine_randomness * mean_w;  // refine.rs:44-50
+    a.seed = mix64(level_seed ^ 0xA0761D6478BD642Full);
+    ApplyArgs ap{};
+    ap.movers = ls.movers.get();
+    ap.n_movers = ls.counters.get();
+    ap.memb = rmemb;
+    ap.target = ls.target.get();
+    ap.tround = ls.tround.get();
+    ap.targeted_cnt = ls.targeted.get();
+    ap.Kfx = ls.Kfx.get();
+    ap.Wfx = ls.Wfx.get();
+    ap.csize = ls.csize.get();
+    ap.strength = L.strength.get();
+    ap.node_w = L.node_w.get();
+    ap.sK = sK;
+    ap.sW = sW;
+    ap.active = ls.active.get();
+    ap.moved = ls.moved.get();
+    ap.n_moved = ls.counters.get() + 1;
+    for (uint32_t r = 0; r < (uint32_t)MAX_REFINE_ROUNDS; r++) {
+        a.round = r;
+        ap.round = r;
+        HIP_OK(hipMemsetAsync(ls.counters.get(), 0, 2 * sizeof(unsigned int), s));
+        HIP_OK(hipMemsetAsync(ls.targeted.get(), 0, n * sizeof(uint32_t), s));
+        decide_all<true>(a, b, s, nullptr);
+        hipLaunchKernelGGL(k_apply_refine, dim3(1024), dim3(256), 0, s, ap);
+        KERNEL_OK();
+        hipLaunchKernelGGL((k_activate<16, true>), dim3(1024), dim3(256), 0, s, ls.moved.get(),
+                           ls.counters.get() + 1, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)rmemb, memb,
+                           ls.active.get());
+        KERNEL_OK();
+        unsigned int c[2];
+        to_host(c, ls.counters.get(), 2, s);
+        HIP_OK(hipStreamSynchronize(s));
+        ctx.st.refine_rounds++;
+        if (c[1] == 0) break;
+    }
+    harvest_stats(ls, ctx, false);
+}
+
+// ---- renumbering -----------------------------------------------------------------------------
+__global__ void k_used_flags(const uint32_t *ids, uint32_t n, uint32_t *flags) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        flags[ids[v]] = 1u;
+}
+
+__global__ void k_remap(const uint32_t *ids, uint32_t n, const uint32_t *newid, uint32_t *out) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        out[v] = newid[ids[v]];
+}
+
+// Ascending compaction of the ids in use (Partition::renumber, partition.rs:265-296).
+// ids are < id_space. Writes compacted ids to out (may alias ids) and returns their count.
+static uint32_t renumber(const uint32_t *ids, uint32_t n, uint32_t id_space, uint32_t *out, hipStream_t s) {
+    DevBuf<uint32_t> flags(id_space + 1), newid(id_space + 1);
+    HIP_OK(hipMemsetAsync(flags.get(), 0, (id_space + 1) * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_used_flags, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, n, flags.get());
+    KERNEL_OK();
+    exclusive_scan_u32(flags.get(), newid.get(), id_space + 1, s);
+    uint32_t count = 0;
+    to_host(&count, newid.get() + id_space, 1, s);
+    hipLaunchKernelGGL(k_remap, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, n, newid.get(), out);
+    KERNEL_OK();
+    HIP_OK(hipStreamSynchronize(s));
+    return count;
+}
+
+// ---- aggregation (csr_network.rs:516-608) ----------------------------------------------------
+__global__ void k_pairs_for_sort(const uint32_t *groups, uint32_t n, uint32_t *keys, uint32_t *vals) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        keys[v] = groups[v];
+        vals[v] = v;
+    }
+}
+
+__global__ void k_group_starts(const uint32_t *sorted_groups, uint32_t n, uint32_t ng, uint64_t *starts) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+        if (i == n) { starts[ng] = n; continue; }
+        if (i == 0 || sorted_groups[i] != sorted_groups[i - 1]) starts[sorted_groups[i]] = i;
+    }
+}
+
+// Coarse node weight and summed self-loop weight of each group, members in ascending order.
+__global__ void k_group_sums(const uint64_t *starts, const uint32_t *members, uint32_t ng, const double *node_w,
+                             const double *self_loop, double *cnw, double *cloops) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+        double a = 0.0, l = 0.0;
+        for (uint64_t j = starts[g]; j < starts[g + 1]; j++) {
+            uint32_t v = members[j];
+            a += node_w[v];
+            l += self_loop[v];
+        }
+        cnw[g] = a;
+        cloops[g] = l;
+    }
+}
+
+__global__ void k_member_degrees(const uint32_t *members, uint32_t n, const uint64_t *ptrs, uint64_t *deg) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j <= n; j += gridDim.x * blockDim.x) {
+        if (j == n) { deg[j] = 0; continue; }
+        uint32_t v = members[j];
+        deg[j] = ptrs[v + 1] - ptrs[v];
+    }
+}
+
+// Fine entries in member order; key (g, h) for h >= g, a sentinel above every key otherwise.
+template <int G>
+__global__ void __launch_bounds__(256) k_emit_fine(const uint32_t *members, const uint32_t *sorted_groups, uint32_t n,
+                                                   const uint64_t *moff, const uint64_t *ptrs, const uint32_t *nbrs,
+                                                   const float *wts, const uint32_t *groups, int hbits,
+                                                   uint64_t sentinel, uint64_t *keys, float *vals,
+                                                   unsigned long long *n_valid) {
+    constexpr int NG = 256 / G;
+    const int lane = threadIdx.x % G;
+    unsigned long long local = 0;
+    for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
+        uint32_t v = members[j];
+        uint64_t g = sorted_groups[j];
+        uint64_t o = moff[j], lo = ptrs[v], hi = ptrs[v + 1];
+        for (uint64_t e = lo + lane; e < hi; e += G) {
+            uint64_t h = groups[nbrs[e]];
+            uint64_t out = o + (e - lo);
+            if (h >= g) {
+                keys[out] = (g << hbits) | h;
+                local++;
+            } else {
+                keys[out] = sentinel;
+            }
+            vals[out] = wts[e];
+        }
+    }
+    if (local) atomicAdd(n_valid, local);
+}
+
+__global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// One thread per coarse pair: sequential f64 sum in the sorted (= reference) order.
+__global__ void k_pair_sums(const uint64_t *keys, const float *vals, const uint32_t *head, const uint32_t *run,
+                            uint64_t n, int hbits, const double *cloops, uint32_t *pa, uint32_t *pb, float *pw,
+                            uint32_t *keep) {
+    const uint64_t hmask = (1ull << hbits) - 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (!head[i]) continue;
+        uint64_t key = keys[i];
+        double acc = 0.0;
+        for (uint64_t j = i; j < n && keys[j] == key; j++) acc += (double)vals[j];
+        uint32_t r = run[i];
+        uint32_t g = (uint32_t)(key >> hbits), h = (uint32_t)(key & hmask);
+        pa[r] = g;
+        pb[r] = h;
+        keep[r] = acc > 0.0 ? (g == h ? 1u : 2u) : 0u;
+        pw[r] = (g == h) ? (float)(0.5 * (acc + cloops[g])) : (float)acc;
+    }
+}
+
+__global__ void k_pair_emit(const uint32_t *pa, const uint32_t *pb, const float *pw, const uint32_t *keep,
+                            const uint32_t *off, uint32_t np, int cbits, uint64_t *keys, float *vals) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x) {
+        uint32_t k = keep[r];
+        if (!k) continue;
+        uint32_t o = off[r];
+        uint64_t a = pa[r], b = pb[r];
+        keys[o] = (a << cbits) | b;
+        vals[o] = pw[r];
+        if (k == 2) {
+            keys[o + 1] = (b << cbits) | a;
+            vals[o + 1] = pw[r];
+        }
+    }
+}
+
+__global__ void k_csr_from_keys(const uint64_t *keys, const float *vals, uint64_t n, int cbits, uint32_t *nbrs,
+                                float *wts, unsigned long long *row_count) {
+    const uint64_t cmask = (1ull << cbits) - 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = keys[i];
+        nbrs[i] = (uint32_t)(k & cmask);
+        wts[i] = vals[i];
+        atomicAdd(&row_count[k >> cbits], 1ull);
+    }
+}
+
+std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *groups, uint32_t ng, hipStream_t s) {
+    uint32_t n = (uint32_t)L.n;
+    auto out = std::make_unique<sc_graph>();
+    out->n = ng;
+    // members grouped by group, ascending vertex within a group (stable sort)
+    DevBuf<uint32_t> sg(n), members(n);
+    hipLaunchKernelGGL(k_pairs_for_sort, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups, n, sg.get(), members.get());
+    KERNEL_OK();
+    radix_sort_pairs_u32_u32(sg.get(), members.get(), n, std::max(1, bits_for(ng)), s);
+    DevBuf<uint64_t> starts(ng + 1);
+    hipLaunchKernelGGL(k_group_starts, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, sg.get(), n, ng, starts.get());
+    KERNEL_OK();
+    out->node_w.resize(ng);
+    DevBuf<double> cloops(ng);
+    hipLaunchKernelGGL(k_group_sums, dim3(grid_for(ng, 256, 4096)), dim3(256), 0, s, starts.get(), members.get(), ng,
+                       L.node_w.get(), L.self_loop.get(), out->node_w.get(), cloops.get());
+    KERNEL_OK();
+    // fine entries in member order
+    DevBuf<uint64_t> deg(n + 1), moff(n + 1);
+    hipLaunchKernelGGL(k_member_degrees, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, members.get(), n,
+                       L.ptrs.get(), deg.get());
+    KERNEL_OK();
+    exclusive_scan_u64(deg.get(), moff.get(), n + 1, s);
+    uint64_t nnz = L.nnz;
+    const int hbits = std::max(1, bits_for(ng));
+    const uint64_t sentinel = (uint64_t)ng << hbits;
+    const int kbits = std::min(64, hbits + bits_for(ng) + 1);
+    DevBuf<uint64_t> keys(nnz);
+    DevBuf<float> vals(nnz);
+    DevBuf<unsigned long long> nvalid(1);
+    HIP_OK(hipMemsetAsync(nvalid.get(), 0, sizeof(unsigned long long), s));
+    if (nnz) {
+        hipLaunchKernelGGL((k_emit_fine<16>), dim3(grid_for(n, 16, 8192)), dim3(256), 0, s, members.get(), sg.get(), n,
+                           moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, hbits, sentinel, keys.get(),
+                           vals.get(), nvalid.get());
+        KERNEL_OK();
+        radix_sort_pairs_u64_f32(keys.get(), vals.get(), nnz, kbits, s);
+    }
+    unsigned long long nv = 0;
+    to_host(&nv, nvalid.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    uint64_t npairs_max = nv;
+    DevBuf<uint32_t> head(nv + 1), run(nv + 1);
+    uint32_t np = 0;
+    if (nv) {
+        hipLaunchKernelGGL(k_key_heads, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, keys.get(), (uint64_t)nv, head.get());
+        KERNEL_OK();
+        exclusive_scan_u32(head.get(), run.get(), nv, s);
+        uint32_t lr = 0, lh = 0;
+        to_host(&lr, run.get() + nv - 1, 1, s);
+        to_host(&lh, head.get() + nv - 1, 1, s);
+        HIP_OK(hipStreamSynchronize(s));
+        np = lr + lh;
+    }
+    (void)npairs_max;
+    DevBuf<uint32_t> pa(np + 1), pb(np + 1), keep(np + 1), off(np + 1);
+    DevBuf<float> pw(np + 1);
+    uint64_t total_entries = 0;
+    if (np) {
+        hipLaunchKernelGGL(k_pair_sums, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, keys.get(), vals.get(), head.get(),
+                           run.get(), (uint64_t)nv, hbits, cloops.get(), pa.get(), pb.get(), pw.get(), keep.get());
+        KERNEL_OK();
+        exclusive_scan_u32(keep.get(), off.get(), np, s);
+        uint32_t lo = 0, lk = 0;
+        to_host(&lo, off.get() + np - 1, 1, s);
+        to_host(&lk, keep.get() + np - 1, 1, s);
+        HIP_OK(hipStreamSynchronize(s));
+        total_entries = (uint64_t)lo + lk;
+    }
+    const int cbits = std::max(1, bits_for(ng));
+    DevBuf<uint64_t> ckeys(total_entries);
+    DevBuf<float> cvals(total_entries);
+    if (np) {
+        hipLaunchKernelGGL(k_pair_emit, dim3(grid_for(np, 256, 4096)), dim3(256), 0, s, pa.get(), pb.get(), pw.get(),
+                           keep.get(), off.get(), np, cbits, ckeys.get(), cvals.get());
+        KERNEL_OK();
+        radix_sort_pairs_u64_f32(ckeys.get(), cvals.get(), total_entries, std::min(64, 2 * cbits), s);
+    }
+    out->nnz = total_entries;
+    out->nbrs.resize(total_entries);
+    out->wts.resize(total_entries);
+    DevBuf<unsigned long long> rc(ng + 1);
+    HIP_OK(hipMemsetAsync(rc.get(), 0, (ng + 1) * sizeof(unsigned long long), s));
+    if (total_entries) {
+        hipLaunchKernelGGL(k_csr_from_keys, dim3(grid_for(total_entries, 256, 8192)), dim3(256), 0, s, ckeys.get(),
+                           cvals.get(), total_entries, cbits, out->nbrs.get(), out->wts.get(), rc.get());
+        KERNEL_OK();
+    }
+    out->ptrs.resize(ng + 1);
+    exclusive_scan_u64(reinterpret_cast<uint64_t *>(rc.get()), out->ptrs.get(), ng + 1, s);
+    graph_finish_sorted(*out, s);
+    return out;
+}
+
+// ---- driver (leiden/mod.rs:168-289) -----------------------------------------------------------
+__global__ void k_scatter_coarse_memb(const uint32_t *groups, const uint32_t *memb, uint32_t n, uint32_t *cm) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        cm[groups[v]] = memb[v];
+}
+
+__global__ void k_gather_u32(const uint32_t *map, uint32_t *x, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = map[x[i]];
+}
+
+__global__ void k_u32_to_u64(const uint32_t *in, uint64_t *out, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[i];
+}
+
+// One pass (run_once, mod.rs:214-289). labels (n0, consecutive ids) are read and overwritten.
+static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_seed, RunCtx &ctx) {
+    hipStream_t s = ctx.s;
+    const sc_leiden_config &cfg = *ctx.cfg;
+    uint32_t n0 = (uint32_t)G0.n;
+    const sc_graph *level = &G0;
+    std::unique_ptr<sc_graph> owned;
+    DevBuf<uint32_t> memb(n0), rmemb(n0), node_at(n0), groups(n0);
+    HIP_OK(hipMemcpyAsync(memb.get(), labels, n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, node_at.get(), n0);
+    KERNEL_OK();
+    LevelState ls;
+    alloc_level(ls, n0);
+    for (uint64_t lvl = 0; lvl < cfg.max_levels; lvl++) {
+        const sc_graph &L = *level;
+        uint32_t n = (uint32_t)L.n;
+        ctx.st.levels++;
+        int sK = fixed_scale_for(2.0 * L.total_weight);
+        int sW = fixed_scale_for(L.total_node_weight);
+        int sw = fixed_scale_for(L.max_strength);
+        uint64_t level_seed = mix64(pass_seed ^ (0x9E3779B97F4A7C15ull * (lvl + 1)));
+        Buckets b;
+        build_buckets(L, b, s);
+        local_move(L, memb.get(), ls, b, ctx, level_seed, sK, sW, sw);
+        uint32_t ng;
+        if (cfg.refine) {
+            refine(L, memb.get(), rmemb.get(), ls, b, ctx, level_seed, sK, sW, sw);
+            ng = renumber(rmemb.get(), n, n, groups.get(), s);
+        } else {
+            ng = renumber(memb.get(), n, n, groups.get(), s);
+        }
+        if (ng >= n) break;  // no shrink, nothing left to gain (mod.rs:263-265)
+        std::unique_ptr<sc_graph> coarse = aggregate_device(L, groups.get(), ng, s);
+        // several coarse nodes can share a working community (mod.rs:269-272); compact its ids
+        hipLaunchKernelGGL(k_scatter_coarse_memb, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups.get(),
+                           memb.get(), n, rmemb.get());
+        KERNEL_OK();
+        renumber(rmemb.get(), ng, n, memb.get(), s);
+        hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, groups.get(), node_at.get(), n0);
+        KERNEL_OK();
+        owned = std::move(coarse);
+        level = owned.get();
+    }
+    uint32_t n = (uint32_t)level->n;
+    uint32_t nc = renumber(memb.get(), n, n, rmemb.get(), s);
+    // labels[v] = final community of v's node at the last level
+    HIP_OK(hipMemcpyAsync(labels, node_at.get(), n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, rmemb.get(), labels, n0);
+    KERNEL_OK();
+    HIP_OK(hipStreamSynchronize(s));
+    return nc;
+}
+
+static double labels_quality(const sc_graph &G0, const uint32_t *labels, const sc_leiden_config &cfg, hipStream_t s) {
+    uint32_t n = (uint32_t)G0.n;
+    DevBuf<uint64_t> l64(n);
+    hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, labels, l64.get(), n);
+    KERNEL_OK();
+    return quality_device(G0, l64.get(), cfg.objective == SC_RB ? 1 : 2, cfg.resolution, s);
+}
+
+static void validate_config(const sc_leiden_config &c) {
+    auto bad = [](const std::string &m) { throw ScError(SC_INVALID_CONFIG, "invalid configuration: " + m); };
+    if (c.objective != SC_RB && c.objective != SC_CPM) bad("unknown objective");
+    if (!std::isfinite(c.resolution) || c.resolution < 0.0)
+        bad("resolution must be finite and non-negative, got " + std::to_string(c.resolution));
+    if (c.n_iterations == 0) bad("n_iterations must be at least 1");
+    if (c.max_levels == 0) bad("max_levels must be at least 1");
+    if (c.patience == 0) bad("patience must be at least 1");
+    if (!std::isfinite(c.refine_randomness) || c.refine_randomness < 0.0)
+        bad("refine_randomness must be finite and non-negative, got " + std::to_string(c.refine_randomness));
+    if (c.has_max_community_weight && (!std::isfinite(c.max_community_weight) || c.max_community_weight <= 0.0))
+        bad("max_community_weight must be finite and positive, got " + std::to_string(c.max_community_weight));
+}
+
+void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_labels, uint64_t *n_clusters,
+                   hipStream_t s, sc_leiden_stats *stats_out) {
+    validate_config(cfg);
+    RunCtx ctx;
+    ctx.cfg = &cfg;
+    ctx.s = s;
+    ctx.timer.on = profiling_enabled();
+    hipEvent_t t0, t1;
+    HIP_OK(hipEventCreate(&t0));
+    HIP_OK(hipEventCreate(&t1));
+    HIP_OK(hipEventRecord(t0, s));
+    uint32_t n0 = (uint32_t)G0.n;
+    uint64_t seed = cfg.seed;
+    if (!cfg.has_seed) {
+        std::random_device rd;
+        seed = ((uint64_t)rd() << 32) ^ rd();
+    }
+    uint64_t nc = 0;
+    if (n0 > 0) {
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, d_labels, n0);
+        KERNEL_OK();
+        double q_prev = labels_quality(G0, d_labels, cfg, s);
+        uint64_t stagnant = 0;
+        double scale = std::max(std::fabs(G0.total_weight), 1.0);
+        for (uint64_t pass = 0; pass < cfg.n_iterations; pass++) {
+            ctx.st.passes++;
+            uint64_t pass_seed = mix64(seed + 0x9E3779B97F4A7C15ull * (pass + 1));
+            nc = run_once(G0, d_labels, pass_seed, ctx);
+            double q = labels_quality(G0, d_labels, cfg, s);
+            double improvement = q - q_prev;
+            q_prev = q;
+            if (improvement <= cfg.tolerance * scale) {
+                if (++stagnant >= cfg.patience) break;
+            } else {
+                stagnant = 0;
+            }
+        }
+    }
+    HIP_OK(hipEventRecord(t1, s));
+    HIP_OK(hipEventSynchronize(t1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    ctx.st.total_ms = ms;
+    ctx.st.move_kernel_ms = ctx.timer.total_ms;
+    ctx.st.move_kernel_launches = ctx.timer.launches;
+    ctx.st.move_kernel_bytes = ctx.move_bytes;
+    *n_clusters = nc;
+    if (stats_out) *stats_out = ctx.st;
+}
+
+}  // namespace scgpu
+
+using namespace scgpu;
+
+extern "C" {
+
+sc_leiden_config sc_leiden_config_default(void) {
+    sc_leiden_config c;
+    std::memset(&c, 0, sizeof(c));
+    c.objective = SC_RB;
+    c.resolution = 1.0;
+    c.has_seed = 1;
+    c.seed = 42;
+    c.n_iterations = 2;
+    c.max_levels = 64;
+    c.tolerance = 1e-9;
+    c.refine = 1;
+    c.refine_randomness = 0.01;
+    c.patience = 2;
+    c.has_max_community_weight = 0;
+    c.max_community_weight = 0.0;
+    return c;
+}
+
+sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *labels_out, uint64_t *n_clusters_out,
+                    sc_error *err) {
+    try {
+        if (!g || !cfg || !n_clusters_out || (!labels_out && g->n)) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
+        hipStream_t s = thread_stream();
+        uint32_t n = (uint32_t)g->n;
+        DevBuf<uint32_t> labels(n);
+        DevBuf<uint64_t> l64(n);
+        sc_leiden_stats st{};
+        leiden_device(*g, *cfg, labels.get(), n_clusters_out, s, &st);
+        if (n) {
+            hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, labels.get(), l64.get(), n);
+            KERNEL_OK();
+            to_host(labels_out, l64.get(), n, s);
+        }
+        HIP_OK(hipStreamSynchronize(s));
+        t_stats = st;
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+sc_status sc_leiden_last_stats(sc_leiden_stats *out) {
+    if (!out) return SC_INVALID_ARGUMENT;
+    *out = t_stats;
+    return SC_OK;
+}
+
+sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph **out, sc_error *err) {
+    try {
+        if (!g || !out || (!groups && g->n)) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        uint64_t n = g->n;
+        uint64_t maxg = 0;
+        for (uint64_t v = 0; v < n; v++) maxg = std::max(maxg, groups[v]);
+        if (n && maxg >= 0xFFFFFFFFull) throw ScError(SC_INVALID_ARGUMENT, "group ids must be below 2^32 - 1");
+        // VectorGrouping::from_assignments normalizes ids to consecutive, ascending
+        std::vector<uint32_t> g32(n);
+        for (uint64_t v = 0; v < n; v++) g32[v] = (uint32_t)groups[v];
+        hipStream_t s = thread_stream();
+        DevBuf<uint32_t> d(n), dn(n);
+        to_device(d.get(), g32.data(), n, s);
+        uint32_t ng = n ? renumber(d.get(), (uint32_t)n, (uint32_t)(maxg + 1), dn.get(), s) : 0;
+        std::unique_ptr<sc_graph> c;
+        if (n) {
+            c = aggregate_device(*g, dn.get(), ng, s);
+        } else {
+            c = std::make_unique<sc_graph>();
+            c->ptrs.resize(1);
+            HIP_OK(hipMemsetAsync(c->ptrs.get(), 0, sizeof(uint64_t), s));
+        }
+        HIP_OK(hipStreamSynchronize(s));
+        *out = c.release();
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+// quality.hip -- modularity, Rb/Cpm quality and the community-connectivity check.
+//
+// modularity: leiden/objective.rs:221-244. Rb::quality :108-119, Cpm::quality :171-178 over
+// Partition::from_membership (partition.rs:75-112). disconnected_community_count:
+// tests/common/mod.rs:171-194.
+//
+// Per-community sums follow the reference's accumulation order exactly: members in ascending
+// vertex order, each member's row in ascending neighbour order, one f64 chain per community.
+// Members are grouped by a stable radix sort of (label, vertex); one thread walks each
+// community. The final sum over communities is a fixed-shape tree (deterministic).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace scgpu {
+
+__global__ void k_labels_u32(const uint64_t *labels, uint64_t n, uint32_t *keys, uint32_t *vals,
+                             unsigned long long *max_label) {
+    unsigned long long local = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t l = labels[v];
+        keys[v] = (uint32_t)l;
+        vals[v] = (uint32_t)v;
+        if (l > local) local = l;
+    }
+    atomicMax(max_label, local);
+}
+
+__global__ void k_segment_heads(const uint32_t *keys, uint64_t n, uint32_t *head) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// One thread per community (segment of the label-sorted vertex list).
+// mode 0: modularity term 2*in/2m - gamma*(K/2m)^2; mode 1: Rb quality term; mode 2: Cpm.
+__global__ void k_community_terms(const uint32_t *sorted_labels, const uint32_t *members,
+                                  const uint32_t *head, uint64_t n, const uint32_t *labels32,
+                                  const uint64_t *ptrs, const uint32_t *nbrs, const float *wts,
+                                  const double *strength, const double *node_w, int mode,
+                                  double gamma, double m, double *term) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (!head[i]) {
+            term[i] = 0.0;
+            continue;
+        }
+        uint32_t c = sorted_labels[i];
+        double internal = 0.0, st = 0.0, nw = 0.0;
+        for (uint64_t j = i; j < n && sorted_labels[j] == c; j++) {
+            uint64_t v = members[j];
+            st += strength[v];
+            nw += node_w[v];
+            for (uint64_t e = ptrs[v]; e < ptrs[v + 1]; e++) {
+                uint64_t u = nbrs[e];
+                if (labels32[u] == c && (u == v || v < u)) internal += (double)wts[e];
+            }
+        }
+        double t;
+        if (mode == 0) {
+            double two_m = 2.0 * m;
+            double f = st / two_m;
+            t = 2.0 * internal / two_m - gamma * (f * f);
+        } else if (mode == 1) {
+            double four_m = 4.0 * m;
+            t = (four_m == 0.0) ? 0.0 : internal - gamma * st * st / four_m;
+        } else {
+            t = internal - gamma * nw * (nw - 1.0) / 2.0;
+        }
+        term[i] = t;
+    }
+}
+
+struct SortedLabels {
+    DevBuf<uint32_t> keys, members, labels32, head;
+};
+
+static void sort_labels(const sc_graph &g, const uint64_t *d_labels, SortedLabels &sl, hipStream_t s) {
+    uint64_t n = g.n;
+    sl.keys.resize(n);
+    sl.members.resize(n);
+    sl.labels32.resize(n);
+    sl.head.resize(n);
+    DevBuf<unsigned long long> mx(1);
+    HIP_OK(hipMemsetAsync(mx.get(), 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_labels_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d_labels, n,
+                       sl.keys.get(), sl.members.get(), mx.get());
+    KERNEL_OK();
+    unsigned long long max_label = 0;
+    to_host(&max_label, mx.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (max_label >= 0xFFFFFFFFull)
+        throw ScError(SC_INVALID_ARGUMENT, "labels must be below 2^32 - 1");
+    HIP_OK(hipMemcpyAsync(sl.labels32.get(), sl.keys.get(), n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    radix_sort_pairs_u32_u32(sl.keys.get(), sl.members.get(), n, std::max(1, bits_for(max_label)), s);
+    hipLaunchKernelGGL(k_segment_heads, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sl.keys.get(), n,
+                       sl.head.get());
+    KERNEL_OK();
+}
+
+double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, double gamma, hipStream_t s) {
+    uint64_t n = g.n;
+    if (n == 0) return 0.0;
+    if (mode == 0 && g.total_weight == 0.0) return 0.0;
+    SortedLabels sl;
+    sort_labels(g, d_labels, sl, s);
+    DevBuf<double> term(n);
+    hipLaunchKernelGGL(k_community_terms, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, sl.keys.get(),
+                       sl.members.get(), sl.head.get(), n, sl.labels32.get(), g.ptrs.get(), g.nbrs.get(),
+                       g.wts.get(), g.strength.get(), g.node_w.get(), mode, gamma, g.total_weight,
+                       term.get());
+    KERNEL_OK();
+    return device_sum_f64(term.get(), n, s);
+}
+
+// ---- connected components restricted to same-label edges --------------------------------
+// Parents always point to a smaller id, so following them terminates at a root.
+__device__ __forceinline__ uint32_t cc_find(const uint32_t *comp, uint32_t v) {
+    const volatile uint32_t *c = comp;
+    uint32_t p = c[v];
+    while (p != v) {
+        v = p;
+        p = c[v];
+    }
+    return p;
+}
+
+__global__ void k_cc_init(uint32_t *comp, uint64_t n) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        comp[v] = (uint32_t)v;
+}
+
+// Hook: for every same-label edge join the two roots, the larger root pointing to the smaller.
+__global__ void k_cc_hook(const uint64_t *ptrs, const uint32_t *nbrs, const uint32_t *labels,
+                          uint64_t n, uint32_t *comp, unsigned int *changed) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lv = labels[v];
+        for (uint64_t e = ptrs[v]; e < ptrs[v + 1]; e++) {
+            uint32_t u = nbrs[e];
+            if (u == v || labels[u] != lv) continue;
+            uint32_t a = cc_find(comp, (uint32_t)v), b = cc_find(comp, u);
+            while (a != b) {
+                uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+                // hook only while hi is still a root; otherwise re-find and retry
+                uint32_t old = atomicCAS(&comp[hi], hi, lo);
+                if (old == hi) {
+                    *changed = 1u;
+                    break;
+                }
+                a = cc_find(comp, hi);
+                b = cc_find(comp, lo);
+            }
+        }
+    }
+}
+
+__global__ void k_cc_compress(uint32_t *comp, uint64_t n) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        comp[v] = cc_find(comp, (uint32_t)v);
+}
+
+__global__ void k_cc_count(const uint32_t *comp, const uint32_t *labels, uint64_t n, uint32_t *ncomp) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        if (comp[v] == v) atomicAdd(&ncomp[labels[v]], 1u);
+}
+
+__global__ void k_cc_bad(const uint32_t *ncomp, uint64_t nl, unsigned long long *bad) {
+    unsigned long long local = 0;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nl;
+         c += (uint64_t)gridDim.x * blockDim.x)
+        if (ncomp[c] > 1) local++;
+    if (local) atomicAdd(bad, local);
+}
+
+uint64_t disconnected_count_device(const sc_graph &g, const uint32_t *d_labels32, uint64_t n_labels,
+                                   hipStream_t s) {
+    uint64_t n = g.n;
+    if (n == 0) return 0;
+    DevBuf<uint32_t> comp(n);
+    DevBuf<unsigned int> changed(1);
+    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, comp.get(), n);
+    KERNEL_OK();
+    for (int it = 0; it < 1000; it++) {
+        HIP_OK(hipMemsetAsync(changed.get(), 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_cc_hook, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, g.ptrs.get(), g.nbrs.get(),
+                           d_labels32, n, comp.get(), changed.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, comp.get(), n);
+        KERNEL_OK();
+        unsigned int ch = 0;
+        to_host(&ch, changed.get(), 1, s);
+        HIP_OK(hipStreamSynchronize(s));
+        if (!ch) break;
+    }
+    DevBuf<uint32_t> ncomp(n_labels);
+    HIP_OK(hipMemsetAsync(ncomp.get(), 0, n_labels * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_cc_count, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, comp.get(), d_labels32, n,
+                       ncomp.get());
+    KERNEL_OK();
+    DevBuf<unsigned long long> bad(1);
+    HIP_OK(hipMemsetAsync(bad.get(), 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_cc_bad, dim3(grid_for(n_labels, 256, 4096)), dim3(256), 0, s, ncomp.get(), n_labels,
+                       bad.get());
+    KERNEL_OK();
+    unsigned long long out = 0;
+    to_host(&out, bad.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    return out;
+}
+
+}  // namespace scgpu
+
+using namespace scgpu;
+
+extern "C" {
+
+double sc_modularity(const sc_graph *g, const uint64_t *labels, double resolution) {
+    try {
+        if (!g || (!labels && g->n)) return NAN;
+        hipStream_t s = thread_stream();
+        DevBuf<uint64_t> d(g->n);
+        to_device(d.get(), labels, g->n, s);
+        double q = quality_device(*g, d.get(), 0, resolution, s);
+        HIP_OK(hipStreamSynchronize(s));
+        return q;
+    } catch (const ScError &) {
+        return NAN;
+    }
+}
+
+sc_status sc_quality(const sc_graph *g, const uint64_t *labels, int32_t objective, double resolution,
+                     double *out, sc_error *err) {
+    try {
+        if (!g || !out || (!labels && g->n)) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (objective != SC_RB && objective != SC_CPM) throw ScError(SC_INVALID_ARGUMENT, "unknown objective");
+        hipStream_t s = thread_stream();
+        DevBuf<uint64_t> d(g->n);
+        to_device(d.get(), labels, g->n, s);
+        *out = quality_device(*g, d.get(), objective == SC_RB ? 1 : 2, resolution, s);
+        HIP_OK(hipStreamSynchronize(s));
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+sc_status sc_disconnected_community_count(const sc_graph *g, const uint64_t *labels, uint64_t *out,
+                                          sc_error *err) {
+    try {
+        if (!g || !out || (!labels && g->n)) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        uint64_t n = g->n;
+        uint64_t maxl = 0;
+        for (uint64_t v = 0; v < n; v++) maxl = std::max(maxl, labels[v]);
+        if (n && maxl >= 0xFFFFFFFFull) throw ScError(SC_INVALID_ARGUMENT, "labels must be below 2^32 - 1");
+        std::vector<uint32_t> l32(n);
+        for (uint64_t v = 0; v < n; v++) l32[v] = (uint32_t)labels[v];
+        hipStream_t s = thread_stream();
+        DevBuf<uint32_t> d(n);
+        to_device(d.get(), l32.data(), n, s);
+        *out = disconnected_count_device(*g, d.get(), n ? maxl + 1 : 0, s);
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// util.hip -- allocator, streams, error plumbing and device-wide primitives.
+#include "common.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <atomic>
+#include <cstdlib>
+
+namespace scgpu {
+
+void throw_hip(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "HIP error %d (%s) at %s:%d in %s", (int)e,
+                  hipGetErrorString(e), file, line, what);
+    sc_status code = (e == hipErrorOutOfMemory) ? SC_OUT_OF_MEMORY : SC_DEVICE_ERROR;
+    throw ScError(code, buf);
+}
+
+void fill_error(sc_error *err, const ScError &e) {
+    if (!err) return;
+    *err = e.payload;
+    err->code = (int32_t)e.code;
+    std::snprintf(err->msg, sizeof(err->msg), "%s", e.msg.c_str());
+}
+
+// One pool per host thread: every block is recycled on the stream of the thread that freed it,
+// so stream order alone protects reuse (work of another thread never sees a recycled block).
+DevicePool &DevicePool::get() {
+    thread_local DevicePool *pool = new DevicePool();  // never destroyed: outlives teardown
+    return *pool;
+}
+
+static size_t round_size(size_t b) {
+    if (b <= 4096) return 4096;
+    if (b <= (size_t(1) << 20)) {
+        size_t r = 4096;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t mb = size_t(1) << 20;
+    return (b + mb - 1) / mb * mb;
+}
+
+void *DevicePool::alloc(size_t bytes) {
+    size_t sz = round_size(bytes);
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(sz);
+        // reuse a cached block when it is not more than 25% larger than needed
+        if (it != free_.end() && it->first <= sz + sz / 4) {
+            void *p = it->second;
+            size_t have = it->first;
+            free_.erase(it);
+            cached_ -= have;
+            live_[p] = have;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        HIP_OK(hipMalloc(&p, sz));
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    live_[p] = sz;
+    return p;
+}
+
+void DevicePool::release(void *p) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    free_.emplace(it->second, p);
+    cached_ += it->second;
+    live_.erase(it);
+}
+
+void DevicePool::trim() {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached_ = 0;
+}
+
+hipStream_t thread_stream() {
+    thread_local hipStream_t s = nullptr;
+    if (!s) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+static std::atomic<int> g_profiling{-1};
+
+bool profiling_enabled() {
+    int v = g_profiling.load();
+    if (v < 0) {
+        const char *e = std::getenv("SCGPU_PROFILE");
+        v = (e && e[0] == '1') ? 1 : 0;
+        g_profiling.store(v);
+    }
+    return v == 1;
+}
+
+void set_profiling(int on) { g_profiling.store(on ? 1 : 0); }
+
+// ---- hipcub wrappers ----------------------------------------------------------------------
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s) {
+    if (n == 0) return;
+    size_t tmp = 0;
+    HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    DevBuf<unsigned char> t(tmp);
+    HIP_OK(hipcub::DeviceScan::ExclusiveSum(t.get(), tmp, in, out, (int)n, s));
+}
+
+void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s) {
+    if (n == 0) return;
+    size_t tmp = 0;
+    HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    DevBuf<unsigned char> t(tmp);
+    HIP_OK(hipcub::DeviceScan::ExclusiveSum(t.get(), tmp, in, out, (int)n, s));
+}
+
+template <typename K, typename V>
+static void sort_pairs(K *keys, V *vals, uint64_t n, int end_bit, hipStream_t s) {
+    if (n <= 1) return;
+    DevBuf<K> k2(n);
+    DevBuf<V> v2(n);
+    hipcub::DoubleBuffer<K> kb(keys, k2.get());
+    hipcub::DoubleBuffer<V> vb(vals, v2.get());
+    size_t tmp = 0;
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)n, 0, end_bit, s));
+    DevBuf<unsigned char> t(tmp);
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, kb, vb, (int)n, 0, end_bit, s));
+    if (kb.Current() != keys)
+        HIP_OK(hipMemcpyAsync(keys, kb.Current(), n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if (vb.Current() != vals)
+        HIP_OK(hipMemcpyAsync(vals, vb.Current(), n * sizeof(V), hipMemcpyDeviceToDevice, s));
+    // the temporaries return to the pool; keep them alive until the copies above are queued
+    // (stream order guarantees later users of the recycled blocks run after the copies)
+}
+
+void radix_sort_pairs_u64_f32(uint64_t *keys, float *vals, uint64_t n, int end_bit, hipStream_t s) {
+    sort_pairs(keys, vals, n, end_bit, s);
+}
+void radix_sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s) {
+    sort_pairs(keys, vals, n, end_bit, s);
+}
+void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s) {
+    sort_pairs(keys, vals, n, end_bit, s);
+}
+
+// ---- deterministic reductions -----------------------------------------------------------
+// Fixed shape: 1024 blocks x 256 threads of strided partial sums, then one block. The
+// association order depends only on n, so results are bit-reproducible.
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+template <bool MAX>
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_stage(const double *in, uint64_t n, double *out) {
+    __shared__ double sh[RED_THREADS];
+    double acc = MAX ? -INFINITY : 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * RED_THREADS) {
+        double x = in[i];
+        acc = MAX ? fmax(acc, x) : acc + x;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double o = sh[threadIdx.x + w];
+            sh[threadIdx.x] = MAX ? fmax(sh[threadIdx.x], o) : sh[threadIdx.x] + o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+template <bool MAX>
+static double reduce_f64(const double *in, uint64_t n, hipStream_t s) {
+    if (n == 0) return MAX ? -INFINITY : 0.0;
+    DevBuf<double> part(RED_BLOCKS + 1);
+    hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, in, n,
+                       part.get());
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(1), dim3(RED_THREADS), 0, s, part.get(),
+                       (uint64_t)RED_BLOCKS, part.get() + RED_BLOCKS);
+    KERNEL_OK();
+    double r = 0.0;
+    HIP_OK(hipMemcpyAsync(&r, part.get() + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return r;
+}
+
+double device_sum_f64(const double *in, uint64_t n, hipStream_t s) { return reduce_f64<false>(in, n, s); }
+double device_max_f64(const double *in, uint64_t n, hipStream_t s) { return reduce_f64<true>(in, n, s); }
+
+}  // namespace scgpu
+
+extern "C" {
+
+const char *sc_version(void) { return "scgpu 0.1.0 (gfx950)"; }
+
+void sc_set_profiling(int32_t enable) { scgpu::set_profiling(enable); }
+
+void sc_shutdown(void) {
+    (void)hipDeviceSynchronize();
+    scgpu::DevicePool::get().trim();
+}
+
+}

@@ -1,0 +1,151 @@
+"""GPU parity: ingest, aggregation, modularity/quality and the connectivity check through the
+C ABI, against the CPU oracle (bit-exact for integer/structure work, f32 weights bit-exact,
+f64 quality values within 1e-12 relative)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import scgpu
+import workloads as W
+from helpers import load_fixture, reference_test_graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def both_from_edges(n, edges):
+    return scgpu.CSRNetwork.from_edges(n, edges), O.Graph.from_edges(n, edges)
+
+
+def assert_same_graph(g, og):
+    gp, gn, gw = g.to_csr_parts()
+    op, on, ow = og.csr()
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gn, on)
+    assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32))  # f32 bit-exact
+    assert np.array_equal(g.strengths(), og.strengths())          # row-ordered f64 sums
+    assert np.array_equal(g.node_weights(), og.node_weights())
+    assert g.edge_count() == og.edge_count()
+    assert g.has_self_loops() == og.has_self_loops()
+    assert g.total_weight() == pytest.approx(og.total_weight(), rel=1e-12, abs=0)
+
+
+def test_reference_kats_through_the_abi():
+    g = scgpu.CSRNetwork.from_edges(3, [(0, 1, 1.0), (1, 2, 1.0), (0, 2, 1.0), (0, 0, 3.0)])
+    assert list(g.strengths()) == [8.0, 2.0, 2.0] and g.total_weight() == 6.0
+    g = scgpu.CSRNetwork.from_edges(2, [(0, 1, 1.0), (0, 1, 2.0)])
+    assert g.total_weight() == 3.0 and g.edge_count() == 1
+    g = scgpu.CSRNetwork.from_csr_parts([0, 3, 4], [1, 1, 0, 0], [1.0, 2.0, 0.0, 3.0])
+    p, n, w = g.to_csr_parts()
+    assert list(p) == [0, 1, 2] and list(n) == [1, 0] and list(w) == [3.0, 3.0]
+
+
+@pytest.mark.parametrize("args,variant", [
+    (([0, 5], [0], [1.0]), "InvalidCsr"),
+    (([0, 1], [9], [1.0]), "NodeIndexOutOfRange"),
+    (([0, 2, 1], [0, 1], [1.0, 1.0]), "InvalidCsr"),
+    (([0, 1], [0], [-1.0]), "NegativeWeight"),
+    (([0, 1], [0], [float("nan")]), "NonFiniteWeight"),
+    (([0, 1, 1], [1], [1.0]), "AsymmetricGraph"),
+    (([0, 1, 2], [1, 0], [1.0, 5.0]), "AsymmetricGraph"),
+])
+def test_malformed_input_is_rejected(args, variant):
+    with pytest.raises(scgpu.ClusteringError) as e:
+        scgpu.CSRNetwork.from_csr_parts(*args)
+    assert e.value.variant == variant
+    if variant == "NodeIndexOutOfRange":
+        assert (e.value.node, e.value.n_nodes) == (9, 1)
+    if variant == "AsymmetricGraph":
+        assert "symmetrised" in e.value.message
+
+
+def test_validate_symmetry():
+    g = scgpu.CSRNetwork.from_csr_parts([0, 2, 3, 4], [1, 2, 0, 0], [1.0, 3.0, 3.0, 1.0])
+    with pytest.raises(scgpu.ClusteringError):
+        g.validate_symmetry()
+    scgpu.CSRNetwork.from_edges(3, [(0, 1, 1.0), (1, 2, 1.0), (0, 0, 2.0)]).validate_symmetry()
+
+
+def test_ingest_matches_oracle_on_reference_graphs():
+    for name, n, edges in reference_test_graphs():
+        g, og = both_from_edges(n, edges)
+        assert_same_graph(g, og)
+
+
+def test_ingest_unsorted_duplicates_zeros_random():
+    rng = np.random.default_rng(11)
+    for trial in range(10):
+        n = int(rng.integers(1, 400))
+        m = int(rng.integers(0, 4 * n))
+        a = rng.integers(0, n, size=m)
+        b = rng.integers(0, n, size=m)
+        w = np.where(rng.random(m) < 0.1, 0.0, rng.random(m) * 3).astype(np.float32)
+        # symmetric entry list with duplicates and zeros, rows shuffled
+        rows = np.concatenate([a, b])
+        cols = np.concatenate([b, a])
+        vals = np.concatenate([w, w])
+        perm = rng.permutation(len(rows))
+        rows, cols, vals = rows[perm], cols[perm], vals[perm]
+        order = np.argsort(rows, kind="stable")
+        rows, cols, vals = rows[order], cols[order], vals[order]
+        ptrs = np.zeros(n + 1, np.uint64)
+        np.add.at(ptrs, rows + 1, 1)
+        ptrs = np.cumsum(ptrs).astype(np.uint64)
+        try:
+            og = O.Graph.from_csr_parts(ptrs, cols, vals)
+        except O.OracleError as e:
+            with pytest.raises(scgpu.ClusteringError) as ge:
+                scgpu.CSRNetwork.from_csr_parts(ptrs, cols, vals)
+            assert ge.value.code == e.code
+            continue
+        g = scgpu.CSRNetwork.from_csr_parts(ptrs, cols, vals)
+        assert_same_graph(g, og)
+
+
+def test_aggregate_bit_exact_vs_oracle():
+    rng = np.random.default_rng(3)
+    cases = list(reference_test_graphs())
+    for i in range(8):
+        n = int(rng.integers(20, 300))
+        edges = [(int(a), int(b), float(0.05 + 4 * rng.random())) for a, b in
+                 zip(rng.integers(0, n, 3 * n), rng.integers(0, n, 3 * n))]
+        cases.append((f"random{i}", n, edges))
+    for name, n, edges in cases:
+        if n == 0:
+            continue
+        g, og = both_from_edges(n, edges)
+        for k in (1, max(1, n // 7), max(1, n // 2), n):
+            groups = rng.integers(0, k, size=n)
+            _, groups = np.unique(groups, return_inverse=True)
+            assert_same_graph(g.aggregate(groups), og.aggregate(groups))
+
+
+def test_modularity_and_quality_on_fixture_memberships():
+    graphs, cases = load_fixture()
+    built = {k: (scgpu.CSRNetwork.from_edges(n, e), O.Graph.from_edges(n, e)) for k, (n, e) in graphs.items()}
+    for c in cases:
+        g, og = built[c["graph"]]
+        memb = np.asarray(c["reference_membership"], dtype=np.uint64)
+        m_gpu = scgpu.modularity(g, memb, 1.0)
+        assert m_gpu == pytest.approx(O.modularity(og, memb, 1.0), rel=1e-12, abs=1e-15)
+        assert abs(m_gpu - c["reference_modularity"]) <= 1e-7
+        obj = scgpu.ObjectiveKind(c["objective"], c["resolution"])
+        q_gpu = scgpu.quality(g, memb, obj)
+        assert q_gpu == pytest.approx(O.quality(og, memb, c["objective"], c["resolution"]), rel=1e-12, abs=1e-12)
+        assert abs(2 * q_gpu - c["reference_quality"]) <= 1e-6 * max(abs(c["reference_quality"]), 1.0)
+
+
+def test_disconnected_count_matches_oracle():
+    rng = np.random.default_rng(8)
+    for name, n, edges in reference_test_graphs():
+        if n == 0:
+            continue
+        g, og = both_from_edges(n, edges)
+        for k in (1, 2, 3, max(1, n // 4)):
+            labels = rng.integers(0, k, size=n).astype(np.uint64)
+            assert scgpu.disconnected_community_count(g, labels) == O.disconnected_community_count(og, labels)
+    ptrs, nbrs, wts, truth = W.planted_partition(n=3000, blocks=10, p_in=0.01, p_out=0.0005, seed=2)
+    g = scgpu.CSRNetwork.from_csr_parts(ptrs, nbrs, wts)
+    og = O.Graph.from_csr_parts(ptrs, nbrs, wts)
+    for k in (2, 10, 100):
+        labels = rng.integers(0, k, size=3000).astype(np.uint64)
+        assert scgpu.disconnected_community_count(g, labels) == O.disconnected_community_count(og, labels)

@@ -1,0 +1,328 @@
+"""The CPU oracle pinned against the reference's golden vectors, KATs and closed forms.
+
+Every check here exercises oracle/oracle.c only (no GPU). Cited reference tests:
+csr_network.rs:748-995, objective.rs:251-367, partition.rs:574-682, tests/reference.rs,
+tests/known_answers.rs, tests/invariants.rs, tests/optimality.rs.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import workloads as W
+from helpers import load_fixture, reference_test_graphs, ring_of_cliques
+
+
+# ---- ChaCha block function: RFC 7539 vectors -------------------------------------------
+def test_chacha20_block_rfc7539_2_3_2():
+    key = np.frombuffer(bytes(range(32)), dtype="<u4")
+    out = O.chacha_block(key, (0x09000000 << 32) | 1, 0x4A000000, 20)
+    expect = [0xE4E7F110, 0x15593BD1, 0x1FDD0F50, 0xC47120A3, 0xC7F4D1C7, 0x0368C033, 0x9AAA2204, 0x4E6CD4C3,
+              0x466482D2, 0x09AA9F07, 0x05D7C214, 0xA2028BD9, 0xD19C12B5, 0xB94E16DE, 0xE883D0CB, 0x4E3C50A2]
+    assert list(out) == expect
+
+
+def test_chacha20_zero_key_keystream():
+    out = O.chacha_block(np.zeros(8, np.uint32), 0, 0, 20)
+    ks = bytes(out.astype("<u4").tobytes())
+    assert ks.hex().startswith("76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7")
+
+
+def test_chacha8_zero_key_keystream():
+    out = O.chacha_block(np.zeros(8, np.uint32), 0, 0, 8)
+    ks = bytes(out.astype("<u4").tobytes())
+    assert ks.hex().startswith("3e00ef2f895f40d67f5bb8e81f09a5a12c840ec3ce9a7f3b181be188ef711a1e")
+
+
+# ---- CSRNetwork KATs (csr_network.rs:743-995) ----------------------------------------------
+def triangle_with_loop():
+    return O.Graph.from_edges(3, [(0, 1, 1.0), (1, 2, 1.0), (0, 2, 1.0), (0, 0, 3.0)])
+
+
+def test_self_loop_counts_twice_toward_strength():
+    g = triangle_with_loop()
+    assert list(g.strengths()) == [8.0, 2.0, 2.0]
+    assert g.total_weight() == 6.0
+    assert g.self_loop_weight(0) == 3.0 and g.self_loop_weight(1) == 0.0
+
+
+def test_parallel_edges_are_merged():
+    g = O.Graph.from_edges(2, [(0, 1, 1.0), (0, 1, 2.0)])
+    ptrs, nbrs, wts = g.csr()
+    assert list(ptrs) == [0, 1, 2] and list(wts) == [3.0, 3.0]
+    assert g.total_weight() == 3.0 and g.edge_count() == 1
+
+
+def test_from_csr_parts_matches_from_edges():
+    via_edges = O.Graph.from_edges(4, [(0, 1, 1.5), (1, 2, 2.0), (0, 2, 0.5), (0, 0, 3.0), (3, 3, 1.0)])
+    via_csr = O.Graph.from_csr_parts([0, 3, 5, 7, 8], [0, 1, 2, 0, 2, 0, 1, 3],
+                                     [3.0, 1.5, 0.5, 1.5, 2.0, 0.5, 2.0, 1.0])
+    for a, b in zip(via_edges.csr(), via_csr.csr()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(via_edges.strengths(), via_csr.strengths())
+    assert via_edges.edge_count() == via_csr.edge_count()
+
+
+def test_from_csr_parts_sorts_merges_and_drops_zero():
+    g = O.Graph.from_csr_parts([0, 3, 4], [1, 1, 0, 0], [1.0, 2.0, 0.0, 3.0])
+    ptrs, nbrs, wts = g.csr()
+    assert list(ptrs) == [0, 1, 2] and list(nbrs) == [1, 0] and list(wts) == [3.0, 3.0]
+
+
+@pytest.mark.parametrize("args,code", [
+    (([0, 5], [0], [1.0]), 9),
+    (([0, 1], [9], [1.0]), 1),
+    (([0, 2, 1], [0, 1], [1.0, 1.0]), 9),
+    (([0, 1], [0], [-1.0]), 3),
+    (([0, 1], [0], [float("nan")]), 2),
+    (([0, 1, 1], [1], [1.0]), 10),          # asymmetric: 0 lists 1, 1 does not list 0
+    (([0, 1, 2], [1, 0], [1.0, 5.0]), 10),  # both present, weights differ
+])
+def test_malformed_input_is_rejected(args, code):
+    with pytest.raises(O.OracleError) as e:
+        O.Graph.from_csr_parts(*args)
+    assert e.value.code == code
+
+
+def test_validate_symmetry_catches_balanced_asymmetry():
+    g = O.Graph.from_csr_parts([0, 2, 3, 4], [1, 2, 0, 0], [1.0, 3.0, 3.0, 1.0])
+    assert not g.validate_symmetry()
+    assert triangle_with_loop().validate_symmetry()
+
+
+def brute_aggregate(ptrs, nbrs, wts, nw, groups):
+    """Aggregation by definition: coarse (g,h) weight = sum of fine entries, loops once."""
+    ng = int(groups.max()) + 1
+    acc = {}
+    loops = np.zeros(ng)
+    for v in range(len(ptrs) - 1):
+        for e in range(ptrs[v], ptrs[v + 1]):
+            u = nbrs[e]
+            g, h = groups[v], groups[u]
+            if u == v:
+                loops[g] += wts[e]
+            if h < g:
+                continue
+            acc[(g, h)] = acc.get((g, h), 0.0) + float(wts[e])
+    cw = np.zeros(ng)
+    np.add.at(cw, groups, nw)
+    return acc, loops, cw
+
+
+def test_aggregate_matches_definition_and_conserves():
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        n = int(rng.integers(2, 60))
+        edges = [(int(a), int(b), float(0.05 + 4 * rng.random())) for a in range(n) for b in range(a, n)
+                 if rng.random() < 0.15]
+        if not edges:
+            continue
+        g = O.Graph.from_edges(n, edges)
+        groups = rng.integers(0, max(1, n // 3), size=n)
+        _, groups = np.unique(groups, return_inverse=True)
+        c = g.aggregate(groups)
+        ptrs, nbrs, wts = g.csr()
+        acc, loops, cw = brute_aggregate(ptrs, nbrs, wts, g.node_weights(), groups)
+        cp, cn, cwt = c.csr()
+        got = {}
+        for a in range(c.n):
+            for e in range(cp[a], cp[a + 1]):
+                got[(a, int(cn[e]))] = float(cwt[e])
+        for (a, b), w in acc.items():
+            expect = np.float32(0.5 * (w + loops[a])) if a == b else np.float32(w)
+            assert got[(a, b)] == pytest.approx(float(expect), rel=1e-6)
+            assert got[(b, a)] == got[(a, b)]
+        assert np.allclose(c.node_weights(), cw)
+        assert abs(c.total_weight() - g.total_weight()) <= 1e-6 * max(g.total_weight(), 1)
+        assert abs(c.strengths().sum() - 2 * c.total_weight()) <= 1e-9 * max(c.total_weight(), 1)
+
+
+# ---- quality / modularity (objective.rs, tests/reference.rs:163-209) -------------------------
+def test_karate_optimum_modularity():
+    g = O.Graph.from_edges(34, W.karate_edges())
+    q = O.modularity(g, W.KARATE_OPTIMUM)
+    assert abs(q - 0.4198) < 1e-3
+    for gamma in [0.5, 1.0, 2.0, 4.0]:
+        assert abs(O.quality(g, W.KARATE_OPTIMUM, "rb", gamma) / g.total_weight()
+                   - O.modularity(g, W.KARATE_OPTIMUM, gamma)) < 1e-12
+
+
+def test_modularity_closed_form_by_brute_force():
+    """Textbook Q = (1/2m) sum_ij [A_ij - gamma k_i k_j / 2m] delta(c_i, c_j) on dense matrices."""
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        n = 12
+        A = np.triu((rng.random((n, n)) < 0.3) * rng.random((n, n)), 1)
+        A = A.astype(np.float32).astype(np.float64)
+        A = A + A.T
+        edges = [(i, j, A[i, j]) for i in range(n) for j in range(i + 1, n) if A[i, j] > 0]
+        if not edges:
+            continue
+        g = O.Graph.from_edges(n, edges)
+        labels = rng.integers(0, 4, size=n)
+        k = A.sum(1)
+        m2 = k.sum()
+        same = labels[:, None] == labels[None, :]
+        for gamma in [0.5, 1.0, 3.0]:
+            q = ((A - gamma * np.outer(k, k) / m2) * same).sum() / m2
+            assert O.modularity(g, labels, gamma) == pytest.approx(q, abs=1e-12)
+
+
+def test_fixture_quality_definitions_match_leidenalg():
+    graphs, cases = load_fixture()
+    built = {k: O.Graph.from_edges(n, e) for k, (n, e) in graphs.items()}
+    assert len(cases) >= 100
+    for c in cases:
+        g = built[c["graph"]]
+        q = 2.0 * O.quality(g, c["reference_membership"], c["objective"], c["resolution"])
+        assert abs(q - c["reference_quality"]) <= 1e-6 * max(abs(c["reference_quality"]), 1.0), c["name"]
+        m = O.modularity(g, c["reference_membership"], 1.0)
+        assert abs(m - c["reference_modularity"]) <= 1e-7, c["name"]
+
+
+def test_delta_matches_quality_change_via_local_move_gain():
+    """local_move's reported gain equals the objective change (local_move.rs:533-548)."""
+    g = O.Graph.from_edges(34, W.karate_edges())
+    for obj, res in [("rb", 1.0), ("rb", 0.37), ("cpm", 0.1)]:
+        start = np.arange(34)
+        before = O.quality(g, start, obj, res)
+        memb, gain = O.local_move_once(g, start, obj, res, seed=7)
+        after = O.quality(g, memb, obj, res)
+        assert abs(after - before - gain) < 1e-9
+
+
+# ---- Leiden (known_answers.rs, invariants.rs, reference.rs) --------------------------------
+def test_karate_known_answer():
+    g = O.Graph.from_edges(34, W.karate_edges())
+    labels, nc = O.leiden(g)
+    assert nc == 4
+    assert abs(O.modularity(g, labels) - 0.4198) < 1e-3
+
+
+def test_invariants_on_reference_test_graphs():
+    for name, n, edges in reference_test_graphs():
+        g = O.Graph.from_edges(n, edges)
+        for seed in range(3):
+            for obj, res in [("rb", 0.5), ("rb", 1.0), ("rb", 3.0), ("cpm", 0.05)]:
+                labels, nc = O.leiden(g, objective=obj, resolution=res, seed=seed)
+                assert O.disconnected_community_count(g, labels) == 0, name
+                if n:
+                    assert sorted(set(labels.tolist())) == list(range(nc)), name
+        if n:
+            again = [O.leiden(g, seed=7)[0] for _ in range(2)]
+            assert np.array_equal(again[0], again[1])
+            for gamma in [0.5, 1.0, 2.0]:
+                start = O.quality(g, np.arange(n), "rb", gamma)
+                labels, _ = O.leiden(g, resolution=gamma)
+                assert O.quality(g, labels, "rb", gamma) >= start - 1e-9
+
+
+def test_cpm_ring_of_cliques():
+    n, edges = ring_of_cliques()
+    g = O.Graph.from_edges(n, edges)
+    _, nc = O.leiden(g, objective="cpm", resolution=0.1)
+    assert nc == 20
+
+
+def test_fixture_optimizer_statistics():
+    """tests/reference.rs:220-386: no case far below leidenalg, mean >= -0.5%, counts track."""
+    graphs, cases = load_fixture()
+    built = {k: O.Graph.from_edges(n, e) for k, (n, e) in graphs.items()}
+    rels = []
+    for c in cases:
+        g = built[c["graph"]]
+        runs = [O.leiden(g, objective=c["objective"], resolution=c["resolution"], seed=42 + s, n_iterations=5)
+                for s in range(2)]
+        quals = [2.0 * O.quality(g, l, c["objective"], c["resolution"]) for l, _ in runs]
+        ref = c["reference_quality"]
+        rels.append((quals[0] - ref) / max(abs(ref), 1e-9))
+        # no case grossly short (best of 2 seeds, 8% slack) and counts track leidenalg
+        assert max(quals) >= ref - 0.08 * max(abs(ref), 1.0), c["name"]
+        best = runs[int(np.argmax(quals))]
+        assert abs(best[1] - c["reference_n_clusters"]) <= 2 + c["reference_n_clusters"] // 5, c["name"]
+    assert np.mean(rels) > -0.005
+
+
+def restricted_growth(n):
+    a = [0] * n
+    def rec(i, mx):
+        if i == n:
+            yield list(a)
+            return
+        for v in range(mx + 2):
+            a[i] = v
+            yield from rec(i + 1, max(mx, v))
+    a[0] = 0
+    yield from rec(1, 0)
+
+
+def test_partition_enumerator_bell_numbers():
+    for n, bell in [(1, 1), (2, 2), (3, 5), (4, 15), (5, 52), (6, 203), (7, 877)]:
+        assert sum(1 for _ in restricted_growth(n)) == bell
+
+
+def test_exhaustive_optimality_small_graphs():
+    """tests/optimality.rs:113-191 at n = 6..8: >= 90% optimal, worst gap < 0.05."""
+    rng = np.random.default_rng(17)
+    checked = optimal = 0
+    worst = 0.0
+    for n in (6, 7, 8):
+        parts = np.array(list(restricted_growth(n)), dtype=np.uint64)
+        for i in range(6):
+            p = 0.15 + 0.5 * i / 6
+            edges = [(a, b, 1.0) for a in range(n) for b in range(a + 1, n) if rng.random() < p]
+            if not edges:
+                continue
+            g = O.Graph.from_edges(n, edges)
+            for gamma in (0.5, 1.0, 2.0):
+                best = max(O.modularity(g, pp, gamma) for pp in parts)
+                labels, _ = O.leiden(g, resolution=gamma)
+                ours = O.modularity(g, labels, gamma)
+                assert ours <= best + 1e-9
+                checked += 1
+                if ours >= best - 1e-9:
+                    optimal += 1
+                else:
+                    worst = max(worst, best - ours)
+    assert optimal / checked >= 0.9
+    assert worst < 0.05
+
+
+# ---- kNN oracle -----------------------------------------------------------------------------
+def test_knn_bruteforce_matches_numpy():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(300, 7)).astype(np.float32)
+    x[10] = x[11]  # an exact duplicate: tie broken by index
+    idx, d2 = O.knn_bruteforce(x, 5)
+    xd = x.astype(np.float64)
+    D = ((xd[:, None, :] - xd[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(D, np.inf)
+    order = np.lexsort((np.broadcast_to(np.arange(300), D.shape), D), axis=1)[:, :5]
+    assert np.array_equal(idx, order)
+    assert np.allclose(d2, np.take_along_axis(D, order, 1), rtol=1e-12, atol=0)
+    assert idx[10, 0] == 11 and d2[10, 0] == 0.0
+
+
+def test_gaussian_weights_closed_form():
+    idx = np.array([[1], [0]], dtype=np.uint32)
+    d2 = np.array([[4.0], [4.0]])
+    ptrs, nbrs, wts = O.knn_gaussian(idx, d2, mode=0)
+    # sigma_0 = sigma_1 = 2: w = sqrt(2*4/8) * exp(-4/8)
+    assert list(ptrs) == [0, 1, 2] and list(nbrs) == [1, 0]
+    assert wts[0] == np.float32(math.exp(-0.5)) and wts[1] == wts[0]
+
+
+def test_gaussian_union_symmetrization():
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(200, 5)).astype(np.float32)
+    idx, d2 = O.knn_bruteforce(x, 4)
+    ptrs, nbrs, wts = O.knn_gaussian(idx, d2, mode=0)
+    pairs = set()
+    for i in range(200):
+        for j in idx[i]:
+            pairs.add((min(i, int(j)), max(i, int(j))))
+    assert ptrs[-1] == 2 * len(pairs)
+    g = O.Graph.from_csr_parts(ptrs, nbrs, wts)
+    assert g.validate_symmetry()
