@@ -88,6 +88,9 @@ typedef struct sc_leiden_stats {
     double move_kernel_bytes;      /* algorithmic bytes of those kernels */
     uint64_t move_kernel_launches;
     double total_ms;
+    /* host wall time per phase: [0] degree buckets, [1] local moving, [2] refinement,
+     * [3] renumbering + coarse membership, [4] aggregation, [5] quality evaluation */
+    double phase_ms[6];
 } sc_leiden_stats;
 
 /* kNN options: sigma rule and symmetrization (neighborhood/mod.rs:104-168). */
@@ -197,6 +200,11 @@ sc_status sc_knn(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc
 sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k,
                        const sc_knn_opts *opts, sc_graph **out, sc_knn_stats *stats,
                        sc_error *err);
+
+/* Same as sc_knn_graph for points already resident in device memory (d_data: n x d f32). */
+sc_status sc_knn_graph_device(const float *d_data, uint64_t n, uint32_t d, uint32_t k,
+                              const sc_knn_opts *opts, sc_graph **out, sc_knn_stats *stats,
+                              sc_error *err);
 
 /* Gaussian connectivity of a given kNN result (neighborhood/mod.rs:104-168), as a full
  * symmetric CSR with sorted rows. Capacity of neighbors/weights: 2*n*k. */

@@ -25,9 +25,15 @@ FLAGS = [
 ]
 
 
+# knn.hip's FP64 rescore must round multiply and add separately (bit-equal distances with the
+# CPU restatement); clang's __dmul_rn/__dadd_rn are plain operators, so contraction is turned
+# off for the whole translation unit.
+EXTRA = {"knn.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src):
     obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")

@@ -5,13 +5,15 @@
 // CSR steps (:104-181, :208-227) on the GPU.
 //
 // Pipeline:
-//  1. prep: f32 rows -> zero-padded split-BF16 (hi, lo) and padded f32 copies, f32 norms.
-//  2. tile: 64 query rows per workgroup (16 per wave), all points streamed through LDS in
-//     stages of 64. s = |y|^2 - 2 x.y per pair (|x|^2 is constant per row); with BF16 the dot
-//     product is xh.yh + xh.yl + xl.yh on v_mfma_f32_16x16x32_bf16 (FP32 accumulate), with FP32
-//     it is v_mfma_f32_16x16x4_f32. Each row keeps its 2k smallest s in an LDS list; a pair is
-//     looked at only when it beats the row's current 2k-th value.
-//  3. rescore: the 2k candidates get exact FP64 squared distances (sequential sum, no
+//  1. prep: f32 rows -> zero-padded split-BF16 (hi, lo) copies and a padded f32 copy, f32 norms.
+//  2. tile: 128 query rows per workgroup (two 16-row tiles per wave), points streamed through
+//     LDS in stages of 64; the next stage is prefetched into registers while the current one is
+//     on the matrix cores. s = |y|^2 - 2 x.y per pair (|x|^2 is constant per row); with BF16 the
+//     dot product is xh.yh + xh.yl + xl.yh on v_mfma_f32_16x16x32_bf16 (FP32 accumulate), with
+//     FP32 it is v_mfma_f32_16x16x4_f32. Each row keeps its 2k smallest s in an LDS list; a pair
+//     touches the list only when it beats the row's current 2k-th value. Insertions run one
+//     leader lane per row per round, so different rows insert in parallel.
+//  3. rescore: the candidates get exact FP64 squared distances (sequential sum, no
 //     contraction), sorted by (d2, index). A row is certified exact when the smallest distance
 //     any non-candidate could have (2k-th approximate value minus a rigorous error bound) is
 //     strictly above the k-th exact distance; other rows are recomputed by an FP64 scan.
@@ -28,10 +30,13 @@ namespace scgpu {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int KNN_BM = 64;      // query rows per workgroup (16 per wave)
-constexpr int KNN_TN = 64;      // points per LDS stage
-constexpr int KNN_MAXK2 = 64;   // 2k candidates per row at most
+constexpr int KNN_RT = 2;                    // 16-row tiles per wave
+constexpr int KNN_BM = 64 * KNN_RT;          // query rows per workgroup
+constexpr int KNN_TN = 64;                   // points per LDS stage
+constexpr int KNN_MAXK2 = 64;                // 2k candidates per row and split at most
+constexpr int KNN_MAXCAND = 128;             // candidates per row over all splits
 constexpr uint32_t KNONE = 0xFFFFFFFFu;
 
 __device__ __forceinline__ unsigned short bf16_bits_rn(float x) {
@@ -45,10 +50,9 @@ __device__ __forceinline__ float bf16_bits_to_f32(unsigned short b) {
     return __uint_as_float(((uint32_t)b) << 16);
 }
 
-// One thread per point: split-BF16 copies, padded f32 copy, f32 norm, norm bounds.
-__global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, uint32_t Kpf,
-                           unsigned short *Xh, unsigned short *Xl, float *Xf, float *ny, double *nx64,
-                           unsigned int *ymax_bits) {
+// One thread per point: split-BF16 copies, padded f32 copy, f32 norm, exact FP64 norm, max norm.
+__global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, unsigned short *Xh,
+                           unsigned short *Xl, float *Xf, float *ny, double *nx64, unsigned int *ymax_bits) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const float *x = X + (uint64_t)j * d;
         float s = 0.0f;
@@ -60,7 +64,7 @@ __global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, 
             unsigned short l = bf16_bits_rn(rem);
             Xh[(uint64_t)j * Kp + t] = h;
             Xl[(uint64_t)j * Kp + t] = l;
-            if (t < Kpf) Xf[(uint64_t)j * Kpf + t] = v;
+            Xf[(uint64_t)j * Kp + t] = v;
             s += v * v;
             double dv = (double)v;
             s64 = __dadd_rn(s64, __dmul_rn(dv, dv));
@@ -71,157 +75,216 @@ __global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, 
     }
 }
 
-template <bool BF16>
+// Sorted insertion of (s, col) into one row's candidate list (ascending, K2 entries).
+__device__ __forceinline__ void list_insert(float *vv, uint32_t *ii, uint32_t K2, float s, uint32_t col) {
+    if (!(s < vv[K2 - 1])) return;
+    int p = (int)K2 - 1;
+    while (p > 0 && vv[p - 1] > s) {
+        vv[p] = vv[p - 1];
+        ii[p] = ii[p - 1];
+        p--;
+    }
+    vv[p] = s;
+    ii[p] = col;
+}
+
+// NKC: K chunks of 32 (K padded to 32 * NKC). grid.x = row blocks, grid.y = column splits.
+template <bool BF16, int NKC>
 __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, const float *ny, uint32_t n, uint32_t npad,
-                                                  uint32_t Kp, uint32_t Kpf, uint32_t K2, uint32_t *cand_idx,
+                                                  uint32_t K2, uint32_t cols_per_split, uint32_t *cand_idx,
                                                   float *cand_s) {
-    // LDS: one stage of KNN_TN points (padded rows) + the candidate lists of the block's rows
+    constexpr int KP = 32 * NKC;
+    constexpr int STRIDE = BF16 ? (KP + 8) : (KP + 4);           // staged point stride (elements)
+    constexpr int STAGE_BYTES = BF16 ? 2 * KNN_TN * STRIDE * 2 : KNN_TN * STRIDE * 4;
+    constexpr int CHUNKS = BF16 ? 2 * KNN_TN * KP / 8 : KNN_TN * KP / 4;   // 16-byte chunks per stage
+    constexpr int CPT = CHUNKS / 256;                             // chunks per thread
+    static_assert(CHUNKS % 256 == 0, "stage must split evenly over the workgroup");
+
     extern __shared__ unsigned char smem[];
-    const uint32_t strideB = BF16 ? (Kp + 8) : (Kpf + 4);  // elements per staged point
     unsigned short *sBh = reinterpret_cast<unsigned short *>(smem);
-    unsigned short *sBl = sBh + KNN_TN * strideB;
+    unsigned short *sBl = sBh + KNN_TN * STRIDE;
     float *sBf = reinterpret_cast<float *>(smem);
-    size_t stage_bytes = BF16 ? (size_t)2 * KNN_TN * strideB * 2 : (size_t)KNN_TN * strideB * 4;
-    float *sNy = reinterpret_cast<float *>(smem + stage_bytes);
+    float *sNy = reinterpret_cast<float *>(smem + STAGE_BYTES);
     float *lval = sNy + KNN_TN;
     uint32_t *lidx = reinterpret_cast<uint32_t *>(lval + KNN_BM * K2);
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
+    const int quarter = lane >> 4;
     const uint32_t row0 = blockIdx.x * KNN_BM;
-    const uint32_t wrow0 = row0 + wave * 16;
+    const uint32_t split = blockIdx.y;
+    const uint32_t cbeg = split * cols_per_split;
+    const uint32_t cend = min(npad, cbeg + cols_per_split);
 
-    for (uint32_t i = tid; i < KNN_BM * K2; i += 256) {
+    for (uint32_t i = tid; i < (uint32_t)KNN_BM * K2; i += 256) {
         lval[i] = INFINITY;
         lidx[i] = KNONE;
     }
 
-    // A fragments of this wave's 16 rows, kept in registers for the whole sweep
-    const uint32_t arow = wrow0 + (lane & 15);
-    bf16x8 ah[4], al[4];
-    float af[32];
-    const uint32_t nkc = BF16 ? Kp / 32 : Kpf / 4;
-    if (BF16) {
+    // A fragments of this wave's rows, in registers for the whole sweep
+    bf16x8 ah[KNN_RT][NKC], al[KNN_RT][NKC];
+    float af[KNN_RT][8 * NKC];
 #pragma unroll
-        for (int kc = 0; kc < 4; kc++) {
-            if ((uint32_t)kc < nkc) {
-                const uint64_t off = (uint64_t)arow * Kp + kc * 32 + 8 * (lane >> 4);
-                ah[kc] = *reinterpret_cast<const bf16x8 *>(Xh + off);
-                al[kc] = *reinterpret_cast<const bf16x8 *>(Xl + off);
-            }
-        }
-    } else {
+    for (int rt = 0; rt < KNN_RT; rt++) {
+        const uint32_t arow = row0 + wave * 16 * KNN_RT + rt * 16 + (lane & 15);
 #pragma unroll
-        for (int kc = 0; kc < 32; kc++)
-            if ((uint32_t)kc < nkc) af[kc] = Xf[(uint64_t)arow * Kpf + kc * 4 + (lane >> 4)];
-    }
-
-    // per-lane copies of the thresholds of its 4 rows (rows 4*(lane/16) + i of the wave)
-    float thr[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-    const int rbase = wave * 16 + 4 * (lane >> 4);  // block-local row of thr[0]
-    __syncthreads();
-
-    for (uint32_t c0 = 0; c0 < npad; c0 += KNN_TN) {
-        // stage KNN_TN points into LDS (16-byte chunks)
-        if (BF16) {
-            const uint32_t chunks_per_pt = Kp / 8;
-            for (uint32_t q = tid; q < KNN_TN * chunks_per_pt * 2; q += 256) {
-                uint32_t which = q / (KNN_TN * chunks_per_pt);
-                uint32_t r = q % (KNN_TN * chunks_per_pt);
-                uint32_t p = r / chunks_per_pt, ch = r % chunks_per_pt;
-                const unsigned short *src = (which ? Xl : Xh) + (uint64_t)(c0 + p) * Kp + ch * 8;
-                unsigned short *dst = (which ? sBl : sBh) + p * strideB + ch * 8;
-                *reinterpret_cast<uint4 *>(dst) = *reinterpret_cast<const uint4 *>(src);
-            }
-        } else {
-            const uint32_t chunks_per_pt = Kpf / 4;
-            for (uint32_t q = tid; q < KNN_TN * chunks_per_pt; q += 256) {
-                uint32_t p = q / chunks_per_pt, ch = q % chunks_per_pt;
-                *reinterpret_cast<float4 *>(sBf + p * strideB + ch * 4) =
-                    *reinterpret_cast<const float4 *>(Xf + (uint64_t)(c0 + p) * Kpf + ch * 4);
-            }
-        }
-        if (tid < KNN_TN) sNy[tid] = ny[c0 + tid];
-        __syncthreads();
-
-        for (int sub = 0; sub < KNN_TN / 16; sub++) {
-            const int bcol = sub * 16 + (lane & 15);  // staged point of this lane's column
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int kc = 0; kc < NKC; kc++) {
             if (BF16) {
-#pragma unroll
-                for (int kc = 0; kc < 4; kc++) {
-                    if ((uint32_t)kc < nkc) {
-                        const uint32_t off = bcol * strideB + kc * 32 + 8 * (lane >> 4);
-                        bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
-                        bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bh, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bl, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kc], bh, acc, 0, 0, 0);
-                    }
-                }
+                const uint64_t off = (uint64_t)arow * KP + kc * 32 + 8 * quarter;
+                ah[rt][kc] = *reinterpret_cast<const bf16x8 *>(Xh + off);
+                al[rt][kc] = *reinterpret_cast<const bf16x8 *>(Xl + off);
             } else {
 #pragma unroll
-                for (int kc = 0; kc < 32; kc++) {
-                    if ((uint32_t)kc < nkc) {
-                        float b = sBf[bcol * strideB + kc * 4 + (lane >> 4)];
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kc], b, acc, 0, 0, 0);
+                for (int q = 0; q < 8; q++) af[rt][kc * 8 + q] = Xf[(uint64_t)arow * KP + kc * 32 + q * 4 + quarter];
+            }
+        }
+    }
+
+    // per-thread staging slots: chunk q = tid + 256 * c of the stage
+    uint32_t src_off[CPT], dst_off[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const uint32_t q = tid + 256 * c;
+        if (BF16) {
+            constexpr uint32_t per_pt = KP / 8, per_arr = KNN_TN * KP / 8;
+            const uint32_t which = q / per_arr, r = q % per_arr, p = r / per_pt, ch = r % per_pt;
+            src_off[c] = which * 0x80000000u | (p * KP + ch * 8);  // element offset within the stage, array bit
+            dst_off[c] = which * (KNN_TN * STRIDE) + p * STRIDE + ch * 8;
+        } else {
+            constexpr uint32_t per_pt = KP / 4;
+            const uint32_t p = q / per_pt, ch = q % per_pt;
+            src_off[c] = p * KP + ch * 4;
+            dst_off[c] = p * STRIDE + ch * 4;
+        }
+    }
+    u32x4 pre[CPT];
+    float ny_pre = 0.0f;
+    // loads stage C0 of the points into pre[] / ny_pre (macro keeps the offset arrays in VGPRs)
+#define KNN_PREFETCH(C0)                                                                                   \
+    do {                                                                                                   \
+        _Pragma("unroll") for (int c = 0; c < CPT; c++) {                                                  \
+            if (BF16) {                                                                                    \
+                const unsigned short *base_ = (src_off[c] & 0x80000000u) ? Xl : Xh;                        \
+                pre[c] = *reinterpret_cast<const u32x4 *>(base_ + (uint64_t)(C0) * KP + (src_off[c] & 0x7FFFFFFFu)); \
+            } else {                                                                                       \
+                pre[c] = *reinterpret_cast<const u32x4 *>(Xf + (uint64_t)(C0) * KP + src_off[c]);          \
+            }                                                                                              \
+        }                                                                                                  \
+        if (tid < KNN_TN) ny_pre = ny[(C0) + tid];                                                         \
+    } while (0)
+
+    // per-lane part of the staged-point offset: column (lane & 15), K slice of this quarter
+    const uint32_t lane_base = (uint32_t)(lane & 15) * STRIDE + (BF16 ? 8u * quarter : (uint32_t)quarter);
+    float thr[KNN_RT][4];
+#pragma unroll
+    for (int rt = 0; rt < KNN_RT; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) thr[rt][i] = INFINITY;
+
+    if (cbeg < cend) KNN_PREFETCH(cbeg);
+    for (uint32_t c0 = cbeg; c0 < cend; c0 += KNN_TN) {
+        __syncthreads();  // previous stage fully consumed (and list init visible)
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            if (BF16) *reinterpret_cast<u32x4 *>(sBh + dst_off[c]) = pre[c];
+            else *reinterpret_cast<u32x4 *>(sBf + dst_off[c]) = pre[c];
+        }
+        if (tid < KNN_TN) sNy[tid] = ny_pre;
+        if (c0 + KNN_TN < cend) KNN_PREFETCH(c0 + KNN_TN);  // lands while this stage computes
+        __syncthreads();
+
+#pragma unroll 1
+        for (int sub = 0; sub < KNN_TN / 16; sub++) {
+            const int bcol = sub * 16 + (lane & 15);
+            const uint32_t lane_off = (uint32_t)sub * (16u * STRIDE) + lane_base;
+            f32x4 acc[KNN_RT];
+#pragma unroll
+            for (int rt = 0; rt < KNN_RT; rt++) acc[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < NKC; kc++) {
+                if (BF16) {
+                    const uint32_t off = lane_off + kc * 32;
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
+#pragma unroll
+                    for (int rt = 0; rt < KNN_RT; rt++) {
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][kc], bh, acc[rt], 0, 0, 0);
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][kc], bl, acc[rt], 0, 0, 0);
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt][kc], bh, acc[rt], 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float b = sBf[lane_off + kc * 32 + q * 4];
+#pragma unroll
+                        for (int rt = 0; rt < KNN_RT; rt++)
+                            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rt][kc * 8 + q], b, acc[rt], 0, 0, 0);
                     }
                 }
             }
             const uint32_t col = c0 + bcol;
             const float nyc = sNy[bcol];
-            float s[4];
-            bool hit[4];
+            float s[KNN_RT][4];
+            bool hit[KNN_RT][4];
             bool any = false;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                s[i] = nyc - 2.0f * acc[i];
-                const uint32_t grow = wrow0 + 4 * (lane >> 4) + i;
-                hit[i] = col < n && grow < n && col != grow && s[i] < thr[i];
-                any |= hit[i];
+            for (int rt = 0; rt < KNN_RT; rt++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    s[rt][i] = nyc - 2.0f * acc[rt][i];
+                    const uint32_t grow = row0 + wave * 16 * KNN_RT + rt * 16 + 4 * quarter + i;
+                    hit[rt][i] = col < n && grow < n && col != grow && s[rt][i] < thr[rt][i];
+                    any |= hit[rt][i];
+                }
             }
-            uint64_t mask = __ballot(any);
-            if (mask) {
-                while (mask) {
-                    const int src = __ffsll((unsigned long long)mask) - 1;
-                    mask &= mask - 1;
-                    if (lane == src) {
+            if (__ballot(any)) {
+                // rounds: per (tile, row) one leader lane per quarter inserts; quarters and rows
+                // of different (tile, i) never share a list, so the inserts of a round are disjoint
+                while (__ballot(any)) {
+                    any = false;
+#pragma unroll
+                    for (int rt = 0; rt < KNN_RT; rt++) {
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            if (!hit[i]) continue;
-                            float *vv = lval + (rbase + i) * K2;
-                            uint32_t *ii = lidx + (rbase + i) * K2;
-                            if (!(s[i] < vv[K2 - 1])) continue;
-                            int p = (int)K2 - 1;
-                            while (p > 0 && vv[p - 1] > s[i]) {
-                                vv[p] = vv[p - 1];
-                                ii[p] = ii[p - 1];
-                                p--;
+                            const uint64_t m = __ballot(hit[rt][i]);
+                            const uint32_t qm = (uint32_t)(m >> (16 * quarter)) & 0xFFFFu;
+                            const bool leader = hit[rt][i] && ((qm & ((1u << (lane & 15)) - 1u)) == 0u);
+                            if (leader) {
+                                const int r = wave * 16 * KNN_RT + rt * 16 + 4 * quarter + i;
+                                list_insert(lval + r * K2, lidx + r * K2, K2, s[rt][i], col);
+                                hit[rt][i] = false;
                             }
-                            vv[p] = s[i];
-                            ii[p] = col;
+                            any |= hit[rt][i];
                         }
                     }
                 }
                 __threadfence_block();
 #pragma unroll
-                for (int i = 0; i < 4; i++) thr[i] = lval[(rbase + i) * K2 + K2 - 1];
+                for (int rt = 0; rt < KNN_RT; rt++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        thr[rt][i] = lval[(wave * 16 * KNN_RT + rt * 16 + 4 * quarter + i) * K2 + K2 - 1];
             }
         }
-        __syncthreads();
     }
-    // write the block's candidate lists
-    for (uint32_t i = tid; i < KNN_BM * K2; i += 256) {
+#undef KNN_PREFETCH
+    __syncthreads();
+    // write this split's candidate lists: cand[(row * splits + split) * K2 + t]
+    const uint32_t splits = gridDim.y;
+    for (uint32_t i = tid; i < (uint32_t)KNN_BM * K2; i += 256) {
         uint32_t r = row0 + i / K2;
         if (r < n) {
-            cand_idx[(uint64_t)r * K2 + (i % K2)] = lidx[i];
-            cand_s[(uint64_t)r * K2 + (i % K2)] = lval[i];
+            const uint64_t o = ((uint64_t)r * splits + split) * K2 + (i % K2);
+            cand_idx[o] = lidx[i];
+            cand_s[o] = lval[i];
         }
     }
 }
 
+// Sequential FP64 sum of squared differences (this file is built without contraction, so the
+// multiply and the add round separately, as in the CPU restatement).
 __device__ __forceinline__ double exact_d2(const float *X, uint32_t d, uint32_t i, uint32_t j) {
     const float *a = X + (uint64_t)i * d;
     const float *b = X + (uint64_t)j * d;
@@ -233,51 +296,62 @@ __device__ __forceinline__ double exact_d2(const float *X, uint32_t d, uint32_t 
     return s;
 }
 
-// One thread per row: FP64 rescore of the candidates, top-k by (d2, index), certificate.
-__global__ void k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t K2,
+// One thread per row: FP64 rescore of the candidates of all splits, top-k by (d2, index),
+// certificate.
+__global__ void k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t K2, uint32_t splits,
                               const uint32_t *cand_idx, const float *cand_s, const double *nx64, double eps_coef,
                               double eps_const, uint32_t *idx_out, double *d2_out, uint32_t *fallback,
                               unsigned int *n_fallback, unsigned long long *n_rescored) {
     unsigned long long local = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double bd[KNN_MAXK2];
-        uint32_t bi[KNN_MAXK2];
+        double bd[KNN_MAXCAND];
+        uint32_t bi[KNN_MAXCAND];
         uint32_t cnt = 0;
-        float amax = -INFINITY;
-        bool full = true;
-        for (uint32_t t = 0; t < K2; t++) {
-            uint32_t j = cand_idx[(uint64_t)i * K2 + t];
-            if (j == KNONE) {
-                full = false;
-                continue;
+        float amin = INFINITY;  // smallest list maximum over the splits whose list is full
+        bool any_full = false;
+        for (uint32_t sp = 0; sp < splits; sp++) {
+            const uint64_t base = ((uint64_t)i * splits + sp) * K2;
+            bool full = true;
+            float amax = -INFINITY;
+            for (uint32_t t = 0; t < K2; t++) {
+                uint32_t j = cand_idx[base + t];
+                if (j == KNONE) {
+                    full = false;
+                    continue;
+                }
+                amax = fmaxf(amax, cand_s[base + t]);
+                double dd = exact_d2(X, d, i, j);
+                local++;
+                uint32_t p = cnt;
+                while (p > 0 && (bd[p - 1] > dd || (bd[p - 1] == dd && bi[p - 1] > j))) {
+                    bd[p] = bd[p - 1];
+                    bi[p] = bi[p - 1];
+                    p--;
+                }
+                bd[p] = dd;
+                bi[p] = j;
+                cnt++;
             }
-            amax = fmaxf(amax, cand_s[(uint64_t)i * K2 + t]);
-            double dd = exact_d2(X, d, i, j);
-            local++;
-            // insertion by (d2, index)
-            uint32_t p = cnt;
-            while (p > 0 && (bd[p - 1] > dd || (bd[p - 1] == dd && bi[p - 1] > j))) {
-                bd[p] = bd[p - 1];
-                bi[p] = bi[p - 1];
-                p--;
+            if (full) {
+                any_full = true;
+                amin = fminf(amin, amax);
             }
-            bd[p] = dd;
-            bi[p] = j;
-            cnt++;
         }
         for (uint32_t t = 0; t < k; t++) {
             idx_out[(uint64_t)i * k + t] = t < cnt ? bi[t] : KNONE;
             d2_out[(uint64_t)i * k + t] = t < cnt ? bd[t] : INFINITY;
         }
-        // certified when no non-candidate can reach the k-th distance (strictly)
+        // a split whose list never filled holds all of its points, so only full splits can hide
+        // a closer point; certified when none of them can reach the k-th distance (strictly)
         bool ok = true;
-        if (full && cnt >= k) {
-            double xnorm = sqrt(nx64[i]);
-            double eps = eps_coef * xnorm + eps_const;
-            double lb = nx64[i] + (double)amax - eps;
-            ok = lb > bd[k - 1];
-        } else if (full) {
-            ok = false;
+        if (any_full) {
+            if (cnt < k) {
+                ok = false;
+            } else {
+                double eps = eps_coef * sqrt(nx64[i]) + eps_const;
+                double lb = nx64[i] + (double)amin - eps;
+                ok = lb > bd[k - 1];
+            }
         }
         if (!ok) {
             unsigned int slot = atomicAdd(n_fallback, 1u);
@@ -345,11 +419,23 @@ __global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uin
     }
 }
 
-struct KnnResult {
-    DevBuf<uint32_t> idx;
-    DevBuf<double> d2;
-    DevBuf<float> X;  // device copy of the input (kept for callers that need it)
-};
+template <bool BF16, int NKC>
+static size_t tile_lds_bytes(uint32_t K2) {
+    constexpr int KP = 32 * NKC;
+    constexpr int STRIDE = BF16 ? (KP + 8) : (KP + 4);
+    constexpr size_t STAGE = BF16 ? (size_t)2 * KNN_TN * STRIDE * 2 : (size_t)KNN_TN * STRIDE * 4;
+    return STAGE + KNN_TN * 4 + (size_t)KNN_BM * K2 * 8;
+}
+
+template <bool BF16, int NKC>
+static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
+                        const float *ny, uint32_t n, uint32_t npad, uint32_t K2, uint32_t cps, uint32_t *cidx,
+                        float *cs, hipStream_t s) {
+    size_t lds = tile_lds_bytes<BF16, NKC>(K2);
+    if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d/k");
+    hipLaunchKernelGGL((k_knn_tile<BF16, NKC>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, K2, cps, cidx, cs);
+    KERNEL_OK();
+}
 
 void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precision, uint32_t *d_idx,
                 double *d_d2, sc_knn_stats *st, hipStream_t s) {
@@ -357,19 +443,26 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     if (d == 0 || d > 128) throw ScError(SC_INVALID_ARGUMENT, "d must be in [1, 128]");
     if (n < 2) throw ScError(SC_INVALID_ARGUMENT, "need at least 2 points");
     const bool bf16 = precision == 0;
-    const uint32_t Kp = (d + 31) / 32 * 32;
-    const uint32_t Kpf = (d + 3) / 4 * 4;
+    const uint32_t nkc = (d + 31) / 32;
+    const uint32_t Kp = 32 * nkc;
     const uint32_t npad = (n + KNN_BM - 1) / KNN_BM * KNN_BM;
     uint32_t K2 = std::min<uint32_t>(2 * k, KNN_MAXK2);
     if (K2 > n - 1) K2 = std::max<uint32_t>(k, n - 1);
     if (K2 < k) K2 = k;
+    // column splits so that small inputs still fill the chip; every split is whole stages
+    const uint32_t row_blocks = npad / KNN_BM;
+    uint32_t splits = std::max<uint32_t>(1, std::min<uint32_t>(8, (1024 + row_blocks - 1) / row_blocks));
+    while (splits > 1 && (uint64_t)splits * K2 > KNN_MAXCAND) splits--;
+    uint32_t cps = (npad + splits - 1) / splits;
+    cps = (cps + KNN_TN - 1) / KNN_TN * KNN_TN;
+    splits = (npad + cps - 1) / cps;
     DevBuf<unsigned short> Xh((size_t)npad * Kp), Xl((size_t)npad * Kp);
-    DevBuf<float> Xf((size_t)npad * Kpf), ny(npad);
+    DevBuf<float> Xf((size_t)npad * Kp), ny(npad);
     DevBuf<double> nx64(npad);
     DevBuf<unsigned int> ymax(1);
     HIP_OK(hipMemsetAsync(Xh.get(), 0, (size_t)npad * Kp * 2, s));
     HIP_OK(hipMemsetAsync(Xl.get(), 0, (size_t)npad * Kp * 2, s));
-    HIP_OK(hipMemsetAsync(Xf.get(), 0, (size_t)npad * Kpf * 4, s));
+    HIP_OK(hipMemsetAsync(Xf.get(), 0, (size_t)npad * Kp * 4, s));
     HIP_OK(hipMemsetAsync(ny.get(), 0, (size_t)npad * 4, s));
     HIP_OK(hipMemsetAsync(ymax.get(), 0, sizeof(unsigned int), s));
     hipEvent_t e0, e1, e2, e3;
@@ -378,23 +471,25 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     HIP_OK(hipEventCreate(&e2));
     HIP_OK(hipEventCreate(&e3));
     HIP_OK(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, n, d, Kp, Kpf, Xh.get(),
-                       Xl.get(), Xf.get(), ny.get(), nx64.get(), ymax.get());
+    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, n, d, Kp, Xh.get(), Xl.get(),
+                       Xf.get(), ny.get(), nx64.get(), ymax.get());
     KERNEL_OK();
-    DevBuf<uint32_t> cidx((size_t)n * K2);
-    DevBuf<float> cs((size_t)n * K2);
-    const uint32_t strideB = bf16 ? (Kp + 8) : (Kpf + 4);
-    size_t stage_bytes = bf16 ? (size_t)2 * KNN_TN * strideB * 2 : (size_t)KNN_TN * strideB * 4;
-    size_t lds = stage_bytes + KNN_TN * 4 + (size_t)KNN_BM * K2 * 8;
-    if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d/k");
+    DevBuf<uint32_t> cidx((size_t)n * splits * K2);
+    DevBuf<float> cs((size_t)n * splits * K2);
     HIP_OK(hipEventRecord(e1, s));
-    if (bf16)
-        hipLaunchKernelGGL(k_knn_tile<true>, dim3(npad / KNN_BM), dim3(256), lds, s, Xh.get(), Xl.get(), Xf.get(),
-                           ny.get(), n, npad, Kp, Kpf, K2, cidx.get(), cs.get());
-    else
-        hipLaunchKernelGGL(k_knn_tile<false>, dim3(npad / KNN_BM), dim3(256), lds, s, Xh.get(), Xl.get(), Xf.get(),
-                           ny.get(), n, npad, Kp, Kpf, K2, cidx.get(), cs.get());
-    KERNEL_OK();
+    dim3 grid(row_blocks, splits);
+    const unsigned short *xh = Xh.get(), *xl = Xl.get();
+    const float *xf = Xf.get();
+    switch (nkc * 2 + (bf16 ? 0 : 1)) {
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+    }
     HIP_OK(hipEventRecord(e2, s));
     unsigned int ymax_bits = 0;
     to_host(&ymax_bits, ymax.get(), 1, s);
@@ -403,8 +498,8 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     std::memcpy(&ymaxf, &ymax_bits, 4);
     double Y = (double)ymaxf;
     const double u24 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -16);
-    // |s_approx - s| <= norm error + 2 |dot error|; see the header comment
-    double dot_coef = bf16 ? (3.01 * u16 + 3.03 * (3.0 * Kp) * u24) : (2.02 * (double)Kpf * u24);
+    // |s_approx - s| <= norm error + 2 |dot error|, doubled for safety; see the header comment
+    double dot_coef = bf16 ? (3.01 * u16 + 3.03 * (3.0 * Kp) * u24) : (2.02 * (double)Kp * u24);
     double eps_coef = 2.0 * (2.0 * dot_coef * Y) + std::ldexp(1.0, -20) * 2.0 * Y;
     double eps_const = 2.0 * (1.01 * d * u24 * Y * Y) + std::ldexp(1.0, -20) * Y * Y;
     DevBuf<uint32_t> fb(n);
@@ -412,8 +507,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     DevBuf<unsigned long long> nres(1);
     HIP_OK(hipMemsetAsync(nfb.get(), 0, sizeof(unsigned int), s));
     HIP_OK(hipMemsetAsync(nres.get(), 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_knn_rescore, dim3(grid_for(n, 128, 8192)), dim3(128), 0, s, d_X, n, d, k, K2, cidx.get(),
-                       cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fb.get(), nfb.get(), nres.get());
+    hipLaunchKernelGGL(k_knn_rescore, dim3(grid_for(n, 64, 8192)), dim3(64), 0, s, d_X, n, d, k, K2, splits,
+                       cidx.get(), cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fb.get(), nfb.get(),
+                       nres.get());
     KERNEL_OK();
     unsigned int nf = 0;
     unsigned long long nr = 0;
@@ -623,6 +719,25 @@ sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k, co
         DevBuf<uint32_t> idx((size_t)n * k);
         DevBuf<double> d2((size_t)n * k);
         knn_device(X.get(), (uint32_t)n, d, k, knn_precision(opts), idx.get(), d2.get(), stats, s);
+        std::unique_ptr<sc_graph> g = gaussian_graph_device(idx.get(), d2.get(), (uint32_t)n, k, knn_mode(opts), s);
+        *out = g.release();
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+sc_status sc_knn_graph_device(const float *d_data, uint64_t n, uint32_t d, uint32_t k, const sc_knn_opts *opts,
+                              sc_graph **out, sc_knn_stats *stats, sc_error *err) {
+    try {
+        if (!d_data || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        *out = nullptr;
+        hipStream_t s = thread_stream();
+        DevBuf<uint32_t> idx((size_t)n * k);
+        DevBuf<double> d2((size_t)n * k);
+        knn_device(d_data, (uint32_t)n, d, k, knn_precision(opts), idx.get(), d2.get(), stats, s);
         std::unique_ptr<sc_graph> g = gaussian_graph_device(idx.get(), d2.get(), (uint32_t)n, k, knn_mode(opts), s);
         *out = g.release();
         return SC_OK;

@@ -31,6 +31,7 @@
 #include <memory>
 #include <random>
 #include <vector>
+#include <chrono>
 
 namespace scgpu {
 
@@ -318,16 +319,21 @@ struct ApplyArgs {
     uint8_t *active;
     uint32_t *moved;
     unsigned int *n_moved;
+    const uint32_t *claim;  // with a community-weight cap: lowest mover id per target, else null
 };
 
 // Local moving: every decided mover moves (targets were checked against the snapshot).
 __global__ void k_apply_move(ApplyArgs a) {
     unsigned int cnt = *a.n_movers;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        uint32_t v = a.movers[i];
-        uint32_t t = a.target[v];
-        uint32_t old = a.memb[v];
-        a.memb[v] = t;
+        const uint64_t v = a.movers[i];
+        const uint64_t t = a.target[v];
+        if (a.claim && a.claim[t] != (uint32_t)v) {
+            a.active[v] = 1;  // another vertex claimed the capped community first: retry later
+            continue;
+        }
+        const uint64_t old = a.memb[v];
+        a.memb[v] = (uint32_t)t;
         long long kf = to_fx(a.strength[v], a.sK), wf = to_fx(a.node_w[v], a.sW);
         atomicAdd((unsigned long long *)&a.Kfx[t], (unsigned long long)kf);
         atomicAdd((unsigned long long *)&a.Kfx[old], (unsigned long long)(-kf));
@@ -340,14 +346,20 @@ __global__ void k_apply_move(ApplyArgs a) {
 
 // Refinement: a targeted singleton stays and absorbs its joiners, except the higher id of a
 // mutual pair that nobody else targets.
+// Indices are widened to 64 bits where they are loaded and the condition is evaluated without
+// short-circuit branches: every operand is in bounds (t is a community id < n), and a
+// branchy form of this test was compiled into address arithmetic that, on the tc == 0 path,
+// took the high half of a 64-bit index from a stale register.
 __global__ void k_apply_refine(ApplyArgs a) {
     unsigned int cnt = *a.n_movers;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        uint32_t v = a.movers[i];
-        uint32_t t = a.target[v];
-        uint32_t tc = a.targeted_cnt[v];
-        bool go = tc == 0 ||
-                  (tc == 1 && t < v && a.tround[t] == a.round && a.target[t] == v);
+        const uint64_t v = a.movers[i];
+        const uint64_t t = a.target[v];
+        const uint32_t tc = a.targeted_cnt[v];
+        const uint32_t tr = a.tround[t];
+        const uint64_t tt = a.target[t];
+        const uint32_t cl = a.claim ? a.claim[t] : (uint32_t)v;
+        const bool go = ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == a.round) & (tt == v))) & (cl == (uint32_t)v);
         if (!go) {
             a.active[v] = 1;  // blocked: look again next round
             continue;
@@ -362,6 +374,17 @@ __global__ void k_apply_refine(ApplyArgs a) {
         atomicSub(&a.csize[v], 1u);
         unsigned int slot = atomicAdd(a.n_moved, 1u);
         a.moved[slot] = v;
+    }
+}
+
+// With a community-weight cap, each target community admits one joiner per sub-sweep (the
+// lowest vertex id), so the snapshot check W[c] + w_v <= cap stays true after the moves.
+__global__ void k_claim(const uint32_t *movers, const unsigned int *count, const uint32_t *target, uint32_t *claim) {
+    unsigned int cnt = *count;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint64_t v = movers[i];
+        const uint64_t t = target[v];
+        atomicMin(&claim[t], (uint32_t)v);
     }
 }
 
@@ -535,7 +558,7 @@ struct LevelState {
     DevBuf<long long> Kfx, Wfx;
     DevBuf<uint32_t> csize;
     DevBuf<uint8_t> active;
-    DevBuf<uint32_t> target, tround, movers, moved, targeted;
+    DevBuf<uint32_t> target, tround, movers, moved, targeted, claim;
     DevBuf<unsigned int> counters;           // [0] movers, [1] moved
     DevBuf<unsigned long long> stat;         // vertices, edges, candidates
 };
@@ -626,6 +649,7 @@ static void alloc_level(LevelState &ls, uint32_t n) {
     ls.movers.resize(n);
     ls.moved.resize(n);
     ls.targeted.resize(n);
+    ls.claim.resize(n);
     ls.counters.resize(4);
     ls.stat.resize(4);
 }
@@ -678,6 +702,9 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     ap.node_w = L.node_w.get();
     ap.sK = sK;
     ap.sW = sW;
+    ap.active = ls.active.get();
+    const bool capped = ctx.cfg->has_max_community_weight != 0;
+    ap.claim = capped ? ls.claim.get() : nullptr;
     DevBuf<unsigned int> iter_moves(MOVE_SUBSWEEPS);
     for (int it = 0; it < MAX_MOVE_ITERS; it++) {
         a.seed = mix64(level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
@@ -687,6 +714,12 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
             ap.n_movers = iter_moves.get() + sub;
             HIP_OK(hipMemsetAsync(iter_moves.get() + sub, 0, sizeof(unsigned int), s));
             decide_all<false>(a, b, s, ctx.timer.on ? &ctx.timer : nullptr);
+            if (capped) {
+                HIP_OK(hipMemsetAsync(ls.claim.get(), 0xFF, n * sizeof(uint32_t), s));
+                hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), iter_moves.get() + sub,
+                                   ls.target.get(), ls.claim.get());
+                KERNEL_OK();
+            }
             hipLaunchKernelGGL(k_apply_move, dim3(1024), dim3(256), 0, s, ap);
             KERNEL_OK();
             hipLaunchKernelGGL((k_activate<16, false>), dim3(1024), dim3(256), 0, s, ls.movers.get(),
@@ -743,12 +776,20 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     ap.active = ls.active.get();
     ap.moved = ls.moved.get();
     ap.n_moved = ls.counters.get() + 1;
+    const bool capped = ctx.cfg->has_max_community_weight != 0;
+    ap.claim = capped ? ls.claim.get() : nullptr;
     for (uint32_t r = 0; r < (uint32_t)MAX_REFINE_ROUNDS; r++) {
         a.round = r;
         ap.round = r;
         HIP_OK(hipMemsetAsync(ls.counters.get(), 0, 2 * sizeof(unsigned int), s));
         HIP_OK(hipMemsetAsync(ls.targeted.get(), 0, n * sizeof(uint32_t), s));
         decide_all<true>(a, b, s, nullptr);
+        if (capped) {
+            HIP_OK(hipMemsetAsync(ls.claim.get(), 0xFF, n * sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), ls.counters.get(),
+                               ls.target.get(), ls.claim.get());
+            KERNEL_OK();
+        }
         hipLaunchKernelGGL(k_apply_refine, dim3(1024), dim3(256), 0, s, ap);
         KERNEL_OK();
         hipLaunchKernelGGL((k_activate<16, true>), dim3(1024), dim3(256), 0, s, ls.moved.get(),
@@ -1004,6 +1045,21 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
 }
 
 // ---- driver (leiden/mod.rs:168-289) -----------------------------------------------------------
+// Host wall time per phase; each lap synchronizes the stream so device work is attributed to
+// the phase that queued it.
+struct PhaseClock {
+    RunCtx &ctx;
+    hipStream_t s;
+    std::chrono::steady_clock::time_point t;
+    PhaseClock(RunCtx &c, hipStream_t st) : ctx(c), s(st), t(std::chrono::steady_clock::now()) {}
+    void lap(int phase) {
+        HIP_OK(hipStreamSynchronize(s));
+        auto now = std::chrono::steady_clock::now();
+        ctx.st.phase_ms[phase] += std::chrono::duration<double, std::milli>(now - t).count();
+        t = now;
+    }
+};
+
 __global__ void k_scatter_coarse_memb(const uint32_t *groups, const uint32_t *memb, uint32_t n, uint32_t *cm) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
         cm[groups[v]] = memb[v];
@@ -1038,18 +1094,24 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         int sW = fixed_scale_for(L.total_node_weight);
         int sw = fixed_scale_for(L.max_strength);
         uint64_t level_seed = mix64(pass_seed ^ (0x9E3779B97F4A7C15ull * (lvl + 1)));
+        PhaseClock pc(ctx, s);
         Buckets b;
         build_buckets(L, b, s);
+        pc.lap(0);
         local_move(L, memb.get(), ls, b, ctx, level_seed, sK, sW, sw);
+        pc.lap(1);
         uint32_t ng;
         if (cfg.refine) {
             refine(L, memb.get(), rmemb.get(), ls, b, ctx, level_seed, sK, sW, sw);
+            pc.lap(2);
             ng = renumber(rmemb.get(), n, n, groups.get(), s);
         } else {
             ng = renumber(memb.get(), n, n, groups.get(), s);
         }
+        pc.lap(3);
         if (ng >= n) break;  // no shrink, nothing left to gain (mod.rs:263-265)
         std::unique_ptr<sc_graph> coarse = aggregate_device(L, groups.get(), ng, s);
+        pc.lap(4);
         // several coarse nodes can share a working community (mod.rs:269-272); compact its ids
         hipLaunchKernelGGL(k_scatter_coarse_memb, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups.get(),
                            memb.get(), n, rmemb.get());
@@ -1057,6 +1119,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         renumber(rmemb.get(), ng, n, memb.get(), s);
         hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, groups.get(), node_at.get(), n0);
         KERNEL_OK();
+        pc.lap(3);
         owned = std::move(coarse);
         level = owned.get();
     }
@@ -1113,14 +1176,18 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
     if (n0 > 0) {
         hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, d_labels, n0);
         KERNEL_OK();
+        PhaseClock pq(ctx, s);
         double q_prev = labels_quality(G0, d_labels, cfg, s);
+        pq.lap(5);
         uint64_t stagnant = 0;
         double scale = std::max(std::fabs(G0.total_weight), 1.0);
         for (uint64_t pass = 0; pass < cfg.n_iterations; pass++) {
             ctx.st.passes++;
             uint64_t pass_seed = mix64(seed + 0x9E3779B97F4A7C15ull * (pass + 1));
             nc = run_once(G0, d_labels, pass_seed, ctx);
+            PhaseClock pq2(ctx, s);
             double q = labels_quality(G0, d_labels, cfg, s);
+            pq2.lap(5);
             double improvement = q - q_prev;
             q_prev = q;
             if (improvement <= cfg.tolerance * scale) {

@@ -4,10 +4,12 @@
 // Partition::from_membership (partition.rs:75-112). disconnected_community_count:
 // tests/common/mod.rs:171-194.
 //
-// Per-community sums follow the reference's accumulation order exactly: members in ascending
-// vertex order, each member's row in ascending neighbour order, one f64 chain per community.
-// Members are grouped by a stable radix sort of (label, vertex); one thread walks each
-// community. The final sum over communities is a fixed-shape tree (deterministic).
+// Each vertex sums its own internal contribution in row order (neighbours u with the same label
+// and u == v or v < u: every edge once, loops once), exactly the reference's inner loop. The
+// per-community totals (internal weight, strength, node weight) are accumulated as int64 fixed
+// point with power-of-two scales chosen so no sum exceeds 2^61: integer atomics are exact and
+// associative, so the result does not depend on scheduling (deterministic) and carries a
+// relative error below n * 2^-61. The sum over communities is a fixed-shape tree.
 #include "common.hpp"
 
 #include <algorithm>
@@ -15,49 +17,48 @@
 
 namespace scgpu {
 
-__global__ void k_labels_u32(const uint64_t *labels, uint64_t n, uint32_t *keys, uint32_t *vals,
-                             unsigned long long *max_label) {
+__global__ void k_max_label(const uint64_t *labels, uint64_t n, unsigned long long *max_label) {
     unsigned long long local = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t l = labels[v];
-        keys[v] = (uint32_t)l;
-        vals[v] = (uint32_t)v;
         if (l > local) local = l;
     }
     atomicMax(max_label, local);
 }
 
-__global__ void k_segment_heads(const uint32_t *keys, uint64_t n, uint32_t *head) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+// G lanes per vertex stride its row; the lane partial sums combine in a fixed shuffle tree.
+template <int G>
+__global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, uint64_t n, const uint64_t *ptrs,
+                                                      const uint32_t *nbrs, const float *wts, const double *strength,
+                                                      const double *node_w, int s_in, int sK, int sW,
+                                                      long long *in_fx, long long *K_fx, long long *W_fx) {
+    constexpr int NG = 256 / G;
+    const int lane = threadIdx.x % G;
+    for (uint64_t v = blockIdx.x * (uint64_t)NG + threadIdx.x / G; v < n; v += (uint64_t)gridDim.x * NG) {
+        const uint64_t c = labels[v];
+        double acc = 0.0;
+        for (uint64_t e = ptrs[v] + lane; e < ptrs[v + 1]; e += G) {
+            uint64_t u = nbrs[e];
+            if (labels[u] == c && (u == v || v < u)) acc += (double)wts[e];
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+        if (lane == 0) {
+            if (acc != 0.0) atomicAdd((unsigned long long *)&in_fx[c], (unsigned long long)to_fx(acc, s_in));
+            atomicAdd((unsigned long long *)&K_fx[c], (unsigned long long)to_fx(strength[v], sK));
+            atomicAdd((unsigned long long *)&W_fx[c], (unsigned long long)to_fx(node_w[v], sW));
+        }
+    }
 }
 
-// One thread per community (segment of the label-sorted vertex list).
 // mode 0: modularity term 2*in/2m - gamma*(K/2m)^2; mode 1: Rb quality term; mode 2: Cpm.
-__global__ void k_community_terms(const uint32_t *sorted_labels, const uint32_t *members,
-                                  const uint32_t *head, uint64_t n, const uint32_t *labels32,
-                                  const uint64_t *ptrs, const uint32_t *nbrs, const float *wts,
-                                  const double *strength, const double *node_w, int mode,
-                                  double gamma, double m, double *term) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        if (!head[i]) {
-            term[i] = 0.0;
-            continue;
-        }
-        uint32_t c = sorted_labels[i];
-        double internal = 0.0, st = 0.0, nw = 0.0;
-        for (uint64_t j = i; j < n && sorted_labels[j] == c; j++) {
-            uint64_t v = members[j];
-            st += strength[v];
-            nw += node_w[v];
-            for (uint64_t e = ptrs[v]; e < ptrs[v + 1]; e++) {
-                uint64_t u = nbrs[e];
-                if (labels32[u] == c && (u == v || v < u)) internal += (double)wts[e];
-            }
-        }
+__global__ void k_community_terms(const long long *in_fx, const long long *K_fx, const long long *W_fx,
+                                  uint64_t nl, int s_in, int sK, int sW, int mode, double gamma, double m,
+                                  double *term) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nl;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        double internal = from_fx(in_fx[c], s_in), st = from_fx(K_fx[c], sK), nw = from_fx(W_fx[c], sW);
         double t;
         if (mode == 0) {
             double two_m = 2.0 * m;
@@ -69,50 +70,37 @@ __global__ void k_community_terms(const uint32_t *sorted_labels, const uint32_t 
         } else {
             t = internal - gamma * nw * (nw - 1.0) / 2.0;
         }
-        term[i] = t;
+        term[c] = t;
     }
-}
-
-struct SortedLabels {
-    DevBuf<uint32_t> keys, members, labels32, head;
-};
-
-static void sort_labels(const sc_graph &g, const uint64_t *d_labels, SortedLabels &sl, hipStream_t s) {
-    uint64_t n = g.n;
-    sl.keys.resize(n);
-    sl.members.resize(n);
-    sl.labels32.resize(n);
-    sl.head.resize(n);
-    DevBuf<unsigned long long> mx(1);
-    HIP_OK(hipMemsetAsync(mx.get(), 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_labels_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d_labels, n,
-                       sl.keys.get(), sl.members.get(), mx.get());
-    KERNEL_OK();
-    unsigned long long max_label = 0;
-    to_host(&max_label, mx.get(), 1, s);
-    HIP_OK(hipStreamSynchronize(s));
-    if (max_label >= 0xFFFFFFFFull)
-        throw ScError(SC_INVALID_ARGUMENT, "labels must be below 2^32 - 1");
-    HIP_OK(hipMemcpyAsync(sl.labels32.get(), sl.keys.get(), n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    radix_sort_pairs_u32_u32(sl.keys.get(), sl.members.get(), n, std::max(1, bits_for(max_label)), s);
-    hipLaunchKernelGGL(k_segment_heads, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sl.keys.get(), n,
-                       sl.head.get());
-    KERNEL_OK();
 }
 
 double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, double gamma, hipStream_t s) {
     uint64_t n = g.n;
     if (n == 0) return 0.0;
     if (mode == 0 && g.total_weight == 0.0) return 0.0;
-    SortedLabels sl;
-    sort_labels(g, d_labels, sl, s);
-    DevBuf<double> term(n);
-    hipLaunchKernelGGL(k_community_terms, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, sl.keys.get(),
-                       sl.members.get(), sl.head.get(), n, sl.labels32.get(), g.ptrs.get(), g.nbrs.get(),
-                       g.wts.get(), g.strength.get(), g.node_w.get(), mode, gamma, g.total_weight,
-                       term.get());
+    DevBuf<unsigned long long> mx(1);
+    HIP_OK(hipMemsetAsync(mx.get(), 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_label, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d_labels, n, mx.get());
     KERNEL_OK();
-    return device_sum_f64(term.get(), n, s);
+    unsigned long long max_label = 0;
+    to_host(&max_label, mx.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (max_label >= 0xFFFFFFFFull) throw ScError(SC_INVALID_ARGUMENT, "labels must be below 2^32 - 1");
+    const uint64_t nl = max_label + 1;
+    const int s_in = fixed_scale_for(g.total_weight + 1e-300);
+    const int sK = fixed_scale_for(2.0 * g.total_weight + 1e-300);
+    const int sW = fixed_scale_for(g.total_node_weight + 1e-300);
+    DevBuf<long long> acc(3 * nl);
+    HIP_OK(hipMemsetAsync(acc.get(), 0, 3 * nl * sizeof(long long), s));
+    hipLaunchKernelGGL((k_vertex_terms<8>), dim3(grid_for(n, 32, 16384)), dim3(256), 0, s, d_labels, n, g.ptrs.get(),
+                       g.nbrs.get(), g.wts.get(), g.strength.get(), g.node_w.get(), s_in, sK, sW, acc.get(),
+                       acc.get() + nl, acc.get() + 2 * nl);
+    KERNEL_OK();
+    DevBuf<double> term(nl);
+    hipLaunchKernelGGL(k_community_terms, dim3(grid_for(nl, 256, 4096)), dim3(256), 0, s, acc.get(), acc.get() + nl,
+                       acc.get() + 2 * nl, nl, s_in, sK, sW, mode, gamma, g.total_weight, term.get());
+    KERNEL_OK();
+    return device_sum_f64(term.get(), nl, s);
 }
 
 // ---- connected components restricted to same-label edges --------------------------------

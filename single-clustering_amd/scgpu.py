@@ -69,6 +69,7 @@ class ScLeidenStats(C.Structure):
         ("move_kernel_bytes", C.c_double),
         ("move_kernel_launches", C.c_uint64),
         ("total_ms", C.c_double),
+        ("phase_ms", C.c_double * 6),
     ]
 
 
@@ -118,6 +119,8 @@ SIGNATURES = {
                 C.POINTER(ScKnnStats), C.POINTER(ScError)], C.c_int),
     "sc_knn_graph": ([f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(ScKnnOpts), C.POINTER(_GP),
                       C.POINTER(ScKnnStats), C.POINTER(ScError)], C.c_int),
+    "sc_knn_graph_device": ([C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(ScKnnOpts),
+                             C.POINTER(_GP), C.POINTER(ScKnnStats), C.POINTER(ScError)], C.c_int),
     "sc_knn_gaussian_csr": ([u32p, f64p, C.c_uint64, C.c_uint32, C.c_int32, u64p, u32p, f32p, u64p,
                              C.POINTER(ScError)], C.c_int),
 }
@@ -415,7 +418,10 @@ def leiden(graph, config=None):
 def last_stats():
     st = ScLeidenStats()
     _lib.sc_leiden_last_stats(C.byref(st))
-    return {k: getattr(st, k) for k, _ in ScLeidenStats._fields_}
+    out = {k: getattr(st, k) for k, _ in ScLeidenStats._fields_}
+    out["phase_ms"] = dict(zip(["buckets", "local_move", "refine", "renumber", "aggregate", "quality"],
+                               list(st.phase_ms)))
+    return out
 
 
 def modularity(graph, labels, resolution=1.0):
