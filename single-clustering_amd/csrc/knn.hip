@@ -32,10 +32,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int KNN_RT = 2;                    // 16-row tiles per wave
-constexpr int KNN_BM = 64 * KNN_RT;          // query rows per workgroup
-constexpr int KNN_TN = 64;                   // points per LDS stage
-constexpr int KNN_MAXK2 = 64;                // 2k candidates per row and split at most
+constexpr int KNN_BM = 256;                  // query rows per workgroup (64 per wave, one per lane)
+constexpr int KNN_TN = 64;                   // column splits are whole multiples of this many points
+constexpr int KNN_MAXK2 = 64;                // candidates per row and split at most
 constexpr int KNN_MAXCAND = 128;             // candidates per row over all splits
 constexpr uint32_t KNONE = 0xFFFFFFFFu;
 
@@ -50,85 +49,221 @@ __device__ __forceinline__ float bf16_bits_to_f32(unsigned short b) {
     return __uint_as_float(((uint32_t)b) << 16);
 }
 
-// One thread per point: split-BF16 copies, padded f32 copy, f32 norm, exact FP64 norm, max norm.
-__global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, unsigned short *Xh,
-                           unsigned short *Xl, float *Xf, float *ny, double *nx64, unsigned int *ymax_bits) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const float *x = X + (uint64_t)j * d;
-        float s = 0.0f;
-        double s64 = 0.0;
-        for (uint32_t t = 0; t < Kp; t++) {
-            float v = t < d ? x[t] : 0.0f;
-            unsigned short h = bf16_bits_rn(v);
-            float rem = v - bf16_bits_to_f32(h);
-            unsigned short l = bf16_bits_rn(rem);
-            Xh[(uint64_t)j * Kp + t] = h;
-            Xl[(uint64_t)j * Kp + t] = l;
-            Xf[(uint64_t)j * Kp + t] = v;
-            s += v * v;
-            double dv = (double)v;
-            s64 = __dadd_rn(s64, __dmul_rn(dv, dv));
+// One thread per element of the padded npad x Kp layout: split-BF16 copies and the f32 copy
+// (coalesced; padding rows and columns are zero).
+__global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, uint64_t total, unsigned short *Xh,
+                           unsigned short *Xl, float *Xf) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(e / Kp), t = (uint32_t)(e % Kp);
+        const float v = (j < n && t < d) ? X[(uint64_t)j * d + t] : 0.0f;
+        const unsigned short h = bf16_bits_rn(v);
+        Xh[e] = h;
+        Xl[e] = bf16_bits_rn(v - bf16_bits_to_f32(h));
+        Xf[e] = v;
+    }
+}
+
+// One thread per padded point: f32 norm (+inf for padding points, so that they never pass a
+// candidate threshold), exact FP64 norm, and the largest norm (one atomic per wave).
+__global__ void k_knn_norms(const float *X, uint32_t n, uint32_t d, uint32_t npad, float *ny, double *nx64,
+                            unsigned int *ymax_bits) {
+    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < npad; j0 += gridDim.x * blockDim.x) {
+        const uint32_t j = j0 + threadIdx.x;
+        float m = 0.0f;
+        if (j < n) {
+            const float *x = X + (uint64_t)j * d;
+            float s = 0.0f;
+            double s64 = 0.0;
+            for (uint32_t t = 0; t < d; t++) {
+                const float v = x[t];
+                s += v * v;
+                const double dv = (double)v;
+                s64 = __dadd_rn(s64, __dmul_rn(dv, dv));
+            }
+            ny[j] = s;
+            nx64[j] = s64;
+            m = sqrtf(s) * 1.0001f;
+        } else if (j < npad) {
+            ny[j] = INFINITY;
+            nx64[j] = 0.0;
         }
-        ny[j] = s;
-        nx64[j] = s64;
-        atomicMax(ymax_bits, __float_as_uint(sqrtf(s) * 1.0001f));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(ymax_bits, __float_as_uint(m));
     }
 }
 
-// Sorted insertion of (s, col) into one row's candidate list (ascending, K2 entries).
-__device__ __forceinline__ void list_insert(float *vv, uint32_t *ii, uint32_t K2, float s, uint32_t col) {
-    if (!(s < vv[K2 - 1])) return;
-    int p = (int)K2 - 1;
-    while (p > 0 && vv[p - 1] > s) {
-        vv[p] = vv[p - 1];
-        ii[p] = ii[p - 1];
-        p--;
+// KM candidates kept per row and split (16, 32 or 64). NKC: K chunks of 32 (K padded to 32 * NKC).
+// grid.x = row blocks of KNN_BM rows, grid.y = column splits.
+//
+// Each wave owns 64 query rows (four 16-row MFMA tiles). Hits are found in the MFMA output
+// layout itself: every lane compares its 16 values (column lane & 15, rows 4 * (lane >> 4) + i of
+// each row tile) with a register copy of those rows' thresholds, and appends a hit to its row's
+// LDS buffer at a slot taken from the compare ballot. Lane r of the wave keeps row r's KM smallest values sorted in
+// registers; when any buffer could overflow on the next sub-tile the wave merges every buffer
+// into its list with a branch-free compare-exchange network and republishes the thresholds.
+template <int KM>
+__device__ __forceinline__ void list_insert_net(float (&lv)[KM], uint32_t (&li)[KM], float x, uint32_t xi) {
+#pragma unroll
+    for (int p = 0; p < KM; p++) {
+        const bool sw = x < lv[p];
+        const float t = lv[p];
+        const uint32_t ti = li[p];
+        lv[p] = sw ? x : t;
+        li[p] = sw ? xi : ti;
+        x = sw ? t : x;
+        xi = sw ? ti : xi;
     }
-    vv[p] = s;
-    ii[p] = col;
 }
 
-// NKC: K chunks of 32 (K padded to 32 * NKC). grid.x = row blocks, grid.y = column splits.
+template <int NKC> struct KnnTileShape {
+    static constexpr int TN = NKC <= 2 ? 64 : 32;      // points per LDS stage
+    static constexpr int BCAP = 20;                     // buffered hits per row between merges
+};
+
 template <bool BF16, int NKC>
+constexpr size_t knn_tile_lds_bytes() {
+    constexpr int KP = 32 * NKC;
+    constexpr int TN = KnnTileShape<NKC>::TN;
+    constexpr size_t STRIDE = BF16 ? (KP + 8) : (KP + 4);
+    constexpr size_t STAGE = BF16 ? 2 * TN * STRIDE * 2 : TN * STRIDE * 4;
+    return STAGE + TN * 4 + KNN_BM * 4 * 2 + (size_t)KNN_BM * KnnTileShape<NKC>::BCAP * 8;
+}
+
+// Row-owner merge: lane r folds row r's buffered hits into its sorted list, clears the buffer
+// and republishes the row's threshold; then every lane reloads the 16 thresholds it compares
+// against in the MFMA layout.
+template <int KM, int BCAP>
+__device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[KM], float (&thr_r)[4][4],
+                                               uint32_t (&cnt_r)[4][4], uint32_t *sCnt, float *sThr, uint2 *sBuf,
+                                               int wave, int lane) {
+    const int rl = wave * 64 + lane;
+    const int q = lane >> 4;
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) sCnt[wave * 64 + rt * 16 + 4 * q + i] = cnt_r[rt][i];
+    }
+    __threadfence_block();
+    const uint32_t c = sCnt[rl];
+    for (uint32_t p = 0; __any(p < c); p++) {
+        const uint2 e = p < c ? sBuf[rl * BCAP + p] : make_uint2(__float_as_uint(INFINITY), KNONE);
+        list_insert_net<KM>(lv, li, __uint_as_float(e.x), e.y);
+    }
+    sThr[rl] = lv[KM - 1];
+    __threadfence_block();
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) {
+        const f32x4 t = *reinterpret_cast<const f32x4 *>(sThr + wave * 64 + rt * 16 + 4 * q);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            thr_r[rt][i] = t[i];
+            cnt_r[rt][i] = 0;
+        }
+    }
+}
+
+// MFMA products of one 16-column sub-tile against the wave's four row tiles.
+template <bool BF16, int NKC, int STRIDE, int TN>
+__device__ __forceinline__ void knn_subtile_mfma(const unsigned short *sBh, const float *sBf, uint32_t lane_off,
+                                                 const bf16x8 (&ah)[4][NKC], const bf16x8 (&al)[4][NKC],
+                                                 const float (&af)[4][8 * NKC], f32x4 (&acc)[4]) {
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kc = 0; kc < NKC; kc++) {
+        if (BF16) {
+            const uint32_t off = lane_off + kc * 32;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBh + TN * STRIDE + off);
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][kc], bh, acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][kc], bl, acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt][kc], bh, acc[rt], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float b = sBf[lane_off + kc * 32 + q * 4];
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++)
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rt][kc * 8 + q], b, acc[rt], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// Threshold test of one sub-tile's values in the MFMA layout. A hit's buffer slot is its row's
+// count plus the number of hits of lower lanes of the same row (16-lane group) in the ballot.
+// Returns true when some row could overflow its buffer on the next sub-tile.
+template <int BCAP>
+__device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float nyc, uint32_t col, uint32_t row0,
+                                                 uint32_t rl_base, uint64_t qmask, const float (&thr_r)[4][4],
+                                                 uint32_t (&cnt_r)[4][4], uint2 *sBuf) {
+    bool need = false;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float sv = __builtin_fmaf(-2.0f, acc[rt][i], nyc);  // nyc - 2 acc, one rounding
+            bool hit = sv < thr_r[rt][i];
+            if (__ballot(hit)) {
+                const uint32_t rl = rl_base + rt * 16 + i;
+                hit = hit && col != row0 + rl;
+                const uint64_t gm = __ballot(hit) & qmask;
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(gm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)gm, 0u));
+                if (hit) sBuf[rl * BCAP + cnt_r[rt][i] + below] = make_uint2(__float_as_uint(sv), col);
+                cnt_r[rt][i] += (uint32_t)__popcll(gm);
+                need |= cnt_r[rt][i] > (uint32_t)(BCAP - 16);
+            }
+        }
+    return need;
+}
+
+template <bool BF16, int NKC, int KM>
 __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, const float *ny, uint32_t n, uint32_t npad,
-                                                  uint32_t K2, uint32_t cols_per_split, uint32_t *cand_idx,
-                                                  float *cand_s) {
+                                                  uint32_t cols_per_split, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
-    constexpr int STRIDE = BF16 ? (KP + 8) : (KP + 4);           // staged point stride (elements)
-    constexpr int STAGE_BYTES = BF16 ? 2 * KNN_TN * STRIDE * 2 : KNN_TN * STRIDE * 4;
-    constexpr int CHUNKS = BF16 ? 2 * KNN_TN * KP / 8 : KNN_TN * KP / 4;   // 16-byte chunks per stage
+    constexpr int TN = KnnTileShape<NKC>::TN;
+    constexpr int BCAP = KnnTileShape<NKC>::BCAP;
+    constexpr int STRIDE = BF16 ? (KP + 8) : (KP + 4);            // staged point stride (elements)
+    constexpr int STAGE_BYTES = BF16 ? 2 * TN * STRIDE * 2 : TN * STRIDE * 4;
+    constexpr int CHUNKS = BF16 ? 2 * TN * KP / 8 : TN * KP / 4;  // 16-byte chunks per stage
     constexpr int CPT = CHUNKS / 256;                             // chunks per thread
     static_assert(CHUNKS % 256 == 0, "stage must split evenly over the workgroup");
+    static_assert(STAGE_BYTES % 16 == 0, "threshold array must stay 16-byte aligned");
 
     extern __shared__ unsigned char smem[];
     unsigned short *sBh = reinterpret_cast<unsigned short *>(smem);
-    unsigned short *sBl = sBh + KNN_TN * STRIDE;
     float *sBf = reinterpret_cast<float *>(smem);
     float *sNy = reinterpret_cast<float *>(smem + STAGE_BYTES);
-    float *lval = sNy + KNN_TN;
-    uint32_t *lidx = reinterpret_cast<uint32_t *>(lval + KNN_BM * K2);
+    float *sThr = sNy + TN;                                       // per block row
+    uint32_t *sCnt = reinterpret_cast<uint32_t *>(sThr + KNN_BM); // per block row
+    uint2 *sBuf = reinterpret_cast<uint2 *>(sCnt + KNN_BM);       // [row][BCAP] (s bits, column)
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
     const int quarter = lane >> 4;
     const uint32_t row0 = blockIdx.x * KNN_BM;
+    const uint32_t wrow0 = row0 + wave * 64;
     const uint32_t split = blockIdx.y;
     const uint32_t cbeg = split * cols_per_split;
     const uint32_t cend = min(npad, cbeg + cols_per_split);
 
-    for (uint32_t i = tid; i < (uint32_t)KNN_BM * K2; i += 256) {
-        lval[i] = INFINITY;
-        lidx[i] = KNONE;
-    }
+    sThr[tid] = INFINITY;
 
-    // A fragments of this wave's rows, in registers for the whole sweep
-    bf16x8 ah[KNN_RT][NKC], al[KNN_RT][NKC];
-    float af[KNN_RT][8 * NKC];
+    // A fragments of the wave's four row tiles, in registers for the whole sweep
+    bf16x8 ah[4][NKC], al[4][NKC];
+    float af[4][8 * NKC];
 #pragma unroll
-    for (int rt = 0; rt < KNN_RT; rt++) {
-        const uint32_t arow = row0 + wave * 16 * KNN_RT + rt * 16 + (lane & 15);
+    for (int rt = 0; rt < 4; rt++) {
+        const uint32_t arow = wrow0 + rt * 16 + (lane & 15);
 #pragma unroll
         for (int kc = 0; kc < NKC; kc++) {
             if (BF16) {
@@ -148,10 +283,10 @@ __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, cons
     for (int c = 0; c < CPT; c++) {
         const uint32_t q = tid + 256 * c;
         if (BF16) {
-            constexpr uint32_t per_pt = KP / 8, per_arr = KNN_TN * KP / 8;
+            constexpr uint32_t per_pt = KP / 8, per_arr = TN * KP / 8;
             const uint32_t which = q / per_arr, r = q % per_arr, p = r / per_pt, ch = r % per_pt;
             src_off[c] = which * 0x80000000u | (p * KP + ch * 8);  // element offset within the stage, array bit
-            dst_off[c] = which * (KNN_TN * STRIDE) + p * STRIDE + ch * 8;
+            dst_off[c] = which * (TN * STRIDE) + p * STRIDE + ch * 8;
         } else {
             constexpr uint32_t per_pt = KP / 4;
             const uint32_t p = q / per_pt, ch = q % per_pt;
@@ -161,7 +296,7 @@ __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, cons
     }
     u32x4 pre[CPT];
     float ny_pre = 0.0f;
-    // loads stage C0 of the points into pre[] / ny_pre (macro keeps the offset arrays in VGPRs)
+    // loads stage C0 of the points into pre[] / ny_pre (a macro keeps the offsets in VGPRs)
 #define KNN_PREFETCH(C0)                                                                                   \
     do {                                                                                                   \
         _Pragma("unroll") for (int c = 0; c < CPT; c++) {                                                  \
@@ -172,113 +307,71 @@ __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, cons
                 pre[c] = *reinterpret_cast<const u32x4 *>(Xf + (uint64_t)(C0) * KP + src_off[c]);          \
             }                                                                                              \
         }                                                                                                  \
-        if (tid < KNN_TN) ny_pre = ny[(C0) + tid];                                                         \
+        if (tid < TN) ny_pre = ny[(C0) + tid];                                                             \
     } while (0)
 
-    // per-lane part of the staged-point offset: column (lane & 15), K slice of this quarter
-    const uint32_t lane_base = (uint32_t)(lane & 15) * STRIDE + (BF16 ? 8u * quarter : (uint32_t)quarter);
-    float thr[KNN_RT][4];
+    float lv[KM];
+    uint32_t li[KM];
 #pragma unroll
-    for (int rt = 0; rt < KNN_RT; rt++)
+    for (int p = 0; p < KM; p++) {
+        lv[p] = INFINITY;
+        li[p] = KNONE;
+    }
+    float thr_r[4][4];
+    uint32_t cnt_r[4][4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) thr[rt][i] = INFINITY;
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            thr_r[rt][i] = INFINITY;
+            cnt_r[rt][i] = 0;
+        }
 
+    const uint32_t lane_base = (uint32_t)(lane & 15) * STRIDE + (BF16 ? 8u * quarter : (uint32_t)quarter);
+    // block-local row of this lane's value i of row tile rt, minus 16 rt + i
+    const uint32_t rl_base = wave * 64 + 4 * quarter;
+    const uint64_t qmask = 0xFFFFull << (16 * quarter);
     if (cbeg < cend) KNN_PREFETCH(cbeg);
-    for (uint32_t c0 = cbeg; c0 < cend; c0 += KNN_TN) {
-        __syncthreads();  // previous stage fully consumed (and list init visible)
+    for (uint32_t c0 = cbeg; c0 < cend; c0 += TN) {
+        __syncthreads();  // previous stage fully consumed
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (BF16) *reinterpret_cast<u32x4 *>(sBh + dst_off[c]) = pre[c];
             else *reinterpret_cast<u32x4 *>(sBf + dst_off[c]) = pre[c];
         }
-        if (tid < KNN_TN) sNy[tid] = ny_pre;
-        if (c0 + KNN_TN < cend) KNN_PREFETCH(c0 + KNN_TN);  // lands while this stage computes
+        if (tid < TN) sNy[tid] = ny_pre;
+        if (c0 + TN < cend) KNN_PREFETCH(c0 + TN);  // lands while this stage computes
         __syncthreads();
 
-#pragma unroll 1
-        for (int sub = 0; sub < KNN_TN / 16; sub++) {
-            const int bcol = sub * 16 + (lane & 15);
-            const uint32_t lane_off = (uint32_t)sub * (16u * STRIDE) + lane_base;
-            f32x4 acc[KNN_RT];
+        // two accumulator sets: the products of sub-tile t + 1 are issued before the threshold
+        // test of sub-tile t, so the matrix cores and the vector unit overlap
+        f32x4 acc0[4], acc1[4];
+        knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, lane_base, ah, al, af, acc0);
 #pragma unroll
-            for (int rt = 0; rt < KNN_RT; rt++) acc[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kc = 0; kc < NKC; kc++) {
-                if (BF16) {
-                    const uint32_t off = lane_off + kc * 32;
-                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
-                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
-#pragma unroll
-                    for (int rt = 0; rt < KNN_RT; rt++) {
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][kc], bh, acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][kc], bl, acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt][kc], bh, acc[rt], 0, 0, 0);
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const float b = sBf[lane_off + kc * 32 + q * 4];
-#pragma unroll
-                        for (int rt = 0; rt < KNN_RT; rt++)
-                            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rt][kc * 8 + q], b, acc[rt], 0, 0, 0);
-                    }
-                }
-            }
-            const uint32_t col = c0 + bcol;
-            const float nyc = sNy[bcol];
-            float s[KNN_RT][4];
-            bool hit[KNN_RT][4];
-            bool any = false;
-#pragma unroll
-            for (int rt = 0; rt < KNN_RT; rt++) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    s[rt][i] = nyc - 2.0f * acc[rt][i];
-                    const uint32_t grow = row0 + wave * 16 * KNN_RT + rt * 16 + 4 * quarter + i;
-                    hit[rt][i] = col < n && grow < n && col != grow && s[rt][i] < thr[rt][i];
-                    any |= hit[rt][i];
-                }
-            }
-            if (__ballot(any)) {
-                // rounds: per (tile, row) one leader lane per quarter inserts; quarters and rows
-                // of different (tile, i) never share a list, so the inserts of a round are disjoint
-                while (__ballot(any)) {
-                    any = false;
-#pragma unroll
-                    for (int rt = 0; rt < KNN_RT; rt++) {
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const uint64_t m = __ballot(hit[rt][i]);
-                            const uint32_t qm = (uint32_t)(m >> (16 * quarter)) & 0xFFFFu;
-                            const bool leader = hit[rt][i] && ((qm & ((1u << (lane & 15)) - 1u)) == 0u);
-                            if (leader) {
-                                const int r = wave * 16 * KNN_RT + rt * 16 + 4 * quarter + i;
-                                list_insert(lval + r * K2, lidx + r * K2, K2, s[rt][i], col);
-                                hit[rt][i] = false;
-                            }
-                            any |= hit[rt][i];
-                        }
-                    }
-                }
-                __threadfence_block();
-#pragma unroll
-                for (int rt = 0; rt < KNN_RT; rt++)
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        thr[rt][i] = lval[(wave * 16 * KNN_RT + rt * 16 + 4 * quarter + i) * K2 + K2 - 1];
-            }
+        for (int sub = 0; sub < TN / 16; sub += 2) {
+            knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 1) * (16u * STRIDE) + lane_base, ah,
+                                                    al, af, acc1);
+            bool need = knn_subtile_hits<BCAP>(acc0, sNy[sub * 16 + (lane & 15)], c0 + sub * 16 + (lane & 15), row0,
+                                               rl_base, qmask, thr_r, cnt_r, sBuf);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt_r, sCnt, sThr, sBuf, wave, lane);
+            if (sub + 2 < TN / 16)
+                knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
+                                                        ah, al, af, acc0);
+            need = knn_subtile_hits<BCAP>(acc1, sNy[(sub + 1) * 16 + (lane & 15)], c0 + (sub + 1) * 16 + (lane & 15),
+                                          row0, rl_base, qmask, thr_r, cnt_r, sBuf);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt_r, sCnt, sThr, sBuf, wave, lane);
         }
     }
 #undef KNN_PREFETCH
-    __syncthreads();
-    // write this split's candidate lists: cand[(row * splits + split) * K2 + t]
-    const uint32_t splits = gridDim.y;
-    for (uint32_t i = tid; i < (uint32_t)KNN_BM * K2; i += 256) {
-        uint32_t r = row0 + i / K2;
-        if (r < n) {
-            const uint64_t o = ((uint64_t)r * splits + split) * K2 + (i % K2);
-            cand_idx[o] = lidx[i];
-            cand_s[o] = lval[i];
+    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt_r, sCnt, sThr, sBuf, wave, lane);
+    // write this split's candidate list: cand[(row * splits + split) * KM + t]
+    const uint32_t my_row = wrow0 + lane;
+    if (my_row < n) {
+        const uint64_t o = ((uint64_t)my_row * gridDim.y + split) * KM;
+#pragma unroll
+        for (int p = 0; p < KM; p++) {
+            cand_idx[o + p] = li[p];
+            cand_s[o + p] = lv[p];
         }
     }
 }
@@ -419,22 +512,25 @@ __global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uin
     }
 }
 
-template <bool BF16, int NKC>
-static size_t tile_lds_bytes(uint32_t K2) {
-    constexpr int KP = 32 * NKC;
-    constexpr int STRIDE = BF16 ? (KP + 8) : (KP + 4);
-    constexpr size_t STAGE = BF16 ? (size_t)2 * KNN_TN * STRIDE * 2 : (size_t)KNN_TN * STRIDE * 4;
-    return STAGE + KNN_TN * 4 + (size_t)KNN_BM * K2 * 8;
+template <bool BF16, int NKC, int KM>
+static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
+                           const float *ny, uint32_t n, uint32_t npad, uint32_t cps, uint32_t *cidx, float *cs,
+                           hipStream_t s) {
+    constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
+    if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
+    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs);
+    KERNEL_OK();
 }
 
 template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
-                        const float *ny, uint32_t n, uint32_t npad, uint32_t K2, uint32_t cps, uint32_t *cidx,
+                        const float *ny, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, uint32_t *cidx,
                         float *cs, hipStream_t s) {
-    size_t lds = tile_lds_bytes<BF16, NKC>(K2);
-    if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d/k");
-    hipLaunchKernelGGL((k_knn_tile<BF16, NKC>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, K2, cps, cidx, cs);
-    KERNEL_OK();
+    switch (KM) {
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs, s); break;
+    }
 }
 
 void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precision, uint32_t *d_idx,
@@ -446,9 +542,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const uint32_t nkc = (d + 31) / 32;
     const uint32_t Kp = 32 * nkc;
     const uint32_t npad = (n + KNN_BM - 1) / KNN_BM * KNN_BM;
-    uint32_t K2 = std::min<uint32_t>(2 * k, KNN_MAXK2);
-    if (K2 > n - 1) K2 = std::max<uint32_t>(k, n - 1);
-    if (K2 < k) K2 = k;
+    // candidates kept per row and split: the power of two in {16, 32, 64} at or above 2k
+    uint32_t K2 = 16;
+    while (K2 < 2 * k && K2 < (uint32_t)KNN_MAXK2) K2 *= 2;
     // column splits so that small inputs still fill the chip; every split is whole stages
     const uint32_t row_blocks = npad / KNN_BM;
     uint32_t splits = std::max<uint32_t>(1, std::min<uint32_t>(8, (1024 + row_blocks - 1) / row_blocks));
@@ -460,10 +556,6 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     DevBuf<float> Xf((size_t)npad * Kp), ny(npad);
     DevBuf<double> nx64(npad);
     DevBuf<unsigned int> ymax(1);
-    HIP_OK(hipMemsetAsync(Xh.get(), 0, (size_t)npad * Kp * 2, s));
-    HIP_OK(hipMemsetAsync(Xl.get(), 0, (size_t)npad * Kp * 2, s));
-    HIP_OK(hipMemsetAsync(Xf.get(), 0, (size_t)npad * Kp * 4, s));
-    HIP_OK(hipMemsetAsync(ny.get(), 0, (size_t)npad * 4, s));
     HIP_OK(hipMemsetAsync(ymax.get(), 0, sizeof(unsigned int), s));
     hipEvent_t e0, e1, e2, e3;
     HIP_OK(hipEventCreate(&e0));
@@ -471,8 +563,12 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     HIP_OK(hipEventCreate(&e2));
     HIP_OK(hipEventCreate(&e3));
     HIP_OK(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, n, d, Kp, Xh.get(), Xl.get(),
-                       Xf.get(), ny.get(), nx64.get(), ymax.get());
+    const uint64_t total = (uint64_t)npad * Kp;
+    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_X, n, d, Kp, total, Xh.get(),
+                       Xl.get(), Xf.get());
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(npad, 256, 8192)), dim3(256), 0, s, d_X, n, d, npad, ny.get(),
+                       nx64.get(), ymax.get());
     KERNEL_OK();
     DevBuf<uint32_t> cidx((size_t)n * splits * K2);
     DevBuf<float> cs((size_t)n * splits * K2);

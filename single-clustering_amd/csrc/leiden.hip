@@ -449,12 +449,26 @@ __global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, uint32_t *count
     if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
 }
 
+// Block-local counting, one global reservation per bucket per block, then placement.
 __global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, const uint32_t *starts, uint32_t *cursor,
                               uint32_t *lists) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        int b = bucket_of(ptrs[v + 1] - ptrs[v]);
-        uint32_t slot = atomicAdd(&cursor[b], 1u);
-        lists[starts[b] + slot] = v;
+    __shared__ uint32_t cnt[NUM_BUCKETS], base[NUM_BUCKETS];
+    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gridDim.x * blockDim.x) {
+        if (threadIdx.x < NUM_BUCKETS) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t v = v0 + threadIdx.x;
+        int b = 0;
+        uint32_t local = 0;
+        if (v < n) {
+            b = bucket_of(ptrs[v + 1] - ptrs[v]);
+            local = atomicAdd(&cnt[b], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < NUM_BUCKETS && cnt[threadIdx.x])
+            base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+        if (v < n) lists[starts[b] + base[b] + local] = v;
+        __syncthreads();
     }
 }
 
@@ -904,22 +918,78 @@ __global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
         head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-// One thread per coarse pair: sequential f64 sum in the sorted (= reference) order.
-__global__ void k_pair_sums(const uint64_t *keys, const float *vals, const uint32_t *head, const uint32_t *run,
-                            uint64_t n, int hbits, const double *cloops, uint32_t *pa, uint32_t *pb, float *pw,
-                            uint32_t *keep) {
+// Coarse pair sums. The reference adds each pair's fine weights sequentially in f64 in the
+// sorted (member, row) order (csr_network.rs:548-564). When every term is a multiple of the
+// smallest f32 ulp m among them and the total is below 2^52 * m, every partial sum is exactly
+// representable in f64, so any association order yields the sequential result bit for bit;
+// long segments use a wave-parallel tree under that check and keep the sequential chain
+// otherwise. Short segments are summed sequentially by one thread.
+constexpr uint32_t PAIR_SHORT = 64;
+
+__device__ __forceinline__ void emit_pair(uint64_t key, double acc, uint32_t r, int hbits, const double *cloops,
+                                          uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep) {
     const uint64_t hmask = (1ull << hbits) - 1;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        if (!head[i]) continue;
-        uint64_t key = keys[i];
+    uint32_t g = (uint32_t)(key >> hbits), h = (uint32_t)(key & hmask);
+    pa[r] = g;
+    pb[r] = h;
+    keep[r] = acc > 0.0 ? (g == h ? 1u : 2u) : 0u;
+    pw[r] = (g == h) ? (float)(0.5 * (acc + cloops[g])) : (float)acc;
+}
+
+__global__ void k_seg_starts(const uint32_t *head, const uint32_t *run, uint64_t n, uint32_t np, uint64_t *start) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i == n) start[np] = n;
+        else if (head[i]) start[run[i]] = i;
+    }
+}
+
+__global__ void k_pair_sums_short(const uint64_t *keys, const float *vals, const uint64_t *start, uint32_t np,
+                                  int hbits, const double *cloops, uint32_t *pa, uint32_t *pb, float *pw,
+                                  uint32_t *keep, uint32_t *long_list, unsigned int *n_long) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x) {
+        const uint64_t b = start[r], e = start[r + 1];
+        if (e - b > PAIR_SHORT) {
+            long_list[atomicAdd(n_long, 1u)] = r;
+            continue;
+        }
         double acc = 0.0;
-        for (uint64_t j = i; j < n && keys[j] == key; j++) acc += (double)vals[j];
-        uint32_t r = run[i];
-        uint32_t g = (uint32_t)(key >> hbits), h = (uint32_t)(key & hmask);
-        pa[r] = g;
-        pb[r] = h;
-        keep[r] = acc > 0.0 ? (g == h ? 1u : 2u) : 0u;
-        pw[r] = (g == h) ? (float)(0.5 * (acc + cloops[g])) : (float)acc;
+        for (uint64_t j = b; j < e; j++) acc += (double)vals[j];
+        emit_pair(keys[b], acc, r, hbits, cloops, pa, pb, pw, keep);
+    }
+}
+
+// One wave per long segment.
+__global__ void __launch_bounds__(256) k_pair_sums_long(const uint64_t *keys, const float *vals, const uint64_t *start,
+                                                        const uint32_t *long_list, const unsigned int *n_long,
+                                                        int hbits, const double *cloops, uint32_t *pa, uint32_t *pb,
+                                                        float *pw, uint32_t *keep) {
+    const unsigned int cnt = *n_long;
+    const int lane = threadIdx.x & 63;
+    for (unsigned int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < cnt; q += gridDim.x * 4) {
+        const uint32_t r = long_list[q];
+        const uint64_t b = start[r], e = start[r + 1];
+        double acc = 0.0;
+        int min_e = 1000;
+        for (uint64_t j = b + lane; j < e; j += 64) {
+            float w = vals[j];
+            acc += (double)w;
+            int ex;
+            (void)frexpf(w, &ex);
+            min_e = min(min_e, w != 0.0f ? max(ex - 24, -149) : 1000);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            acc += __shfl_xor(acc, off, 64);
+            min_e = min(min_e, __shfl_xor(min_e, off, 64));
+        }
+        const bool exact = acc < ldexp(1.0, 52 + min_e);
+        if (lane == 0) {
+            if (!exact) {
+                acc = 0.0;
+                for (uint64_t j = b; j < e; j++) acc += (double)vals[j];
+            }
+            emit_pair(keys[b], acc, r, hbits, cloops, pa, pb, pw, keep);
+        }
     }
 }
 
@@ -1009,8 +1079,20 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     DevBuf<float> pw(np + 1);
     uint64_t total_entries = 0;
     if (np) {
-        hipLaunchKernelGGL(k_pair_sums, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, keys.get(), vals.get(), head.get(),
-                           run.get(), (uint64_t)nv, hbits, cloops.get(), pa.get(), pb.get(), pw.get(), keep.get());
+        DevBuf<uint64_t> seg(np + 1);
+        DevBuf<uint32_t> long_list(np);
+        DevBuf<unsigned int> n_long(1);
+        HIP_OK(hipMemsetAsync(n_long.get(), 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_seg_starts, dim3(grid_for(nv + 1, 256, 8192)), dim3(256), 0, s, head.get(), run.get(),
+                           (uint64_t)nv, np, seg.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_pair_sums_short, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, keys.get(), vals.get(),
+                           seg.get(), np, hbits, cloops.get(), pa.get(), pb.get(), pw.get(), keep.get(),
+                           long_list.get(), n_long.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_pair_sums_long, dim3(1024), dim3(256), 0, s, keys.get(), vals.get(), seg.get(),
+                           long_list.get(), n_long.get(), hbits, cloops.get(), pa.get(), pb.get(), pw.get(),
+                           keep.get());
         KERNEL_OK();
         exclusive_scan_u32(keep.get(), off.get(), np, s);
         uint32_t lo = 0, lk = 0;
