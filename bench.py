@@ -190,6 +190,7 @@ def main():
         "modularity": q,
         "n_clusters": c.n_clusters(),
         "leiden_stats": lstats[-1],
+        "knn_stats": kstats[-1],
         "roofline": dominant,
         "roofline_other": knn_roof if dominant is leiden_roof else leiden_roof,
         "cpu_baseline": cpu,

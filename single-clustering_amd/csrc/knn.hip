@@ -136,7 +136,7 @@ constexpr size_t knn_tile_lds_bytes() {
 // against in the MFMA layout.
 template <int KM, int BCAP>
 __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[KM], float (&thr_r)[4][4],
-                                               uint32_t (&cnt_r)[4][4], uint32_t *sCnt, float *sThr, uint2 *sBuf,
+                                               uint32_t (&cnt4)[4], uint32_t *sCnt, float *sThr, uint2 *sBuf,
                                                int wave, int lane) {
     const int rl = wave * 64 + lane;
     const int q = lane >> 4;
@@ -144,7 +144,7 @@ __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[K
 #pragma unroll
         for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) sCnt[wave * 64 + rt * 16 + 4 * q + i] = cnt_r[rt][i];
+            for (int i = 0; i < 4; i++) sCnt[wave * 64 + rt * 16 + 4 * q + i] = (cnt4[rt] >> (8 * i)) & 0xFFu;
     }
     __threadfence_block();
     const uint32_t c = sCnt[rl];
@@ -158,10 +158,8 @@ __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[K
     for (int rt = 0; rt < 4; rt++) {
         const f32x4 t = *reinterpret_cast<const f32x4 *>(sThr + wave * 64 + rt * 16 + 4 * q);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            thr_r[rt][i] = t[i];
-            cnt_r[rt][i] = 0;
-        }
+        for (int i = 0; i < 4; i++) thr_r[rt][i] = t[i];
+        cnt4[rt] = 0;
     }
 }
 
@@ -198,11 +196,12 @@ __device__ __forceinline__ void knn_subtile_mfma(const unsigned short *sBh, cons
 
 // Threshold test of one sub-tile's values in the MFMA layout. A hit's buffer slot is its row's
 // count plus the number of hits of lower lanes of the same row (16-lane group) in the ballot.
+// Byte i of cnt4[rt] counts the buffered hits of row 16 rt + i of this lane's quarter.
 // Returns true when some row could overflow its buffer on the next sub-tile.
 template <int BCAP>
 __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float nyc, uint32_t col, uint32_t row0,
                                                  uint32_t rl_base, uint64_t qmask, const float (&thr_r)[4][4],
-                                                 uint32_t (&cnt_r)[4][4], uint2 *sBuf) {
+                                                 uint32_t (&cnt4)[4], uint2 *sBuf) {
     bool need = false;
 #pragma unroll
     for (int rt = 0; rt < 4; rt++)
@@ -216,16 +215,18 @@ __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float ny
                 const uint64_t gm = __ballot(hit) & qmask;
                 const uint32_t below =
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(gm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)gm, 0u));
-                if (hit) sBuf[rl * BCAP + cnt_r[rt][i] + below] = make_uint2(__float_as_uint(sv), col);
-                cnt_r[rt][i] += (uint32_t)__popcll(gm);
-                need |= cnt_r[rt][i] > (uint32_t)(BCAP - 16);
+                const uint32_t c = (cnt4[rt] >> (8 * i)) & 0xFFu;
+                const uint32_t add = (uint32_t)__popcll(gm);
+                if (hit) sBuf[rl * BCAP + c + below] = make_uint2(__float_as_uint(sv), col);
+                cnt4[rt] += add << (8 * i);
+                need |= c + add > (uint32_t)(BCAP - 16);
             }
         }
     return need;
 }
 
 template <bool BF16, int NKC, int KM>
-__global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
+__global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, const float *ny, uint32_t n, uint32_t npad,
                                                   uint32_t cols_per_split, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
@@ -318,14 +319,13 @@ __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, cons
         li[p] = KNONE;
     }
     float thr_r[4][4];
-    uint32_t cnt_r[4][4];
+    uint32_t cnt4[4];
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++)
+    for (int rt = 0; rt < 4; rt++) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            thr_r[rt][i] = INFINITY;
-            cnt_r[rt][i] = 0;
-        }
+        for (int i = 0; i < 4; i++) thr_r[rt][i] = INFINITY;
+        cnt4[rt] = 0;
+    }
 
     const uint32_t lane_base = (uint32_t)(lane & 15) * STRIDE + (BF16 ? 8u * quarter : (uint32_t)quarter);
     // block-local row of this lane's value i of row tile rt, minus 16 rt + i
@@ -352,18 +352,18 @@ __global__ void __launch_bounds__(256) k_knn_tile(const unsigned short *Xh, cons
             knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 1) * (16u * STRIDE) + lane_base, ah,
                                                     al, af, acc1);
             bool need = knn_subtile_hits<BCAP>(acc0, sNy[sub * 16 + (lane & 15)], c0 + sub * 16 + (lane & 15), row0,
-                                               rl_base, qmask, thr_r, cnt_r, sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt_r, sCnt, sThr, sBuf, wave, lane);
+                                               rl_base, qmask, thr_r, cnt4, sBuf);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane);
             if (sub + 2 < TN / 16)
                 knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
                                                         ah, al, af, acc0);
             need = knn_subtile_hits<BCAP>(acc1, sNy[(sub + 1) * 16 + (lane & 15)], c0 + (sub + 1) * 16 + (lane & 15),
-                                          row0, rl_base, qmask, thr_r, cnt_r, sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt_r, sCnt, sThr, sBuf, wave, lane);
+                                          row0, rl_base, qmask, thr_r, cnt4, sBuf);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane);
         }
     }
 #undef KNN_PREFETCH
-    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt_r, sCnt, sThr, sBuf, wave, lane);
+    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane);
     // write this split's candidate list: cand[(row * splits + split) * KM + t]
     const uint32_t my_row = wrow0 + lane;
     if (my_row < n) {
@@ -389,69 +389,116 @@ __device__ __forceinline__ double exact_d2(const float *X, uint32_t d, uint32_t 
     return s;
 }
 
-// One thread per row: FP64 rescore of the candidates of all splits, top-k by (d2, index),
-// certificate.
-__global__ void k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t K2, uint32_t splits,
-                              const uint32_t *cand_idx, const float *cand_s, const double *nx64, double eps_coef,
-                              double eps_const, uint32_t *idx_out, double *d2_out, uint32_t *fallback,
-                              unsigned int *n_fallback, unsigned long long *n_rescored) {
+// One wave per row: FP64 rescore of the candidates of all splits, top-k by (d2, index), and the
+// certificate. With |s_approx - s| <= eps for every candidate (s = |y|^2 - 2 x.y), the k-th
+// smallest exact value among the candidates is at most S_k + eps, where S_k is the k-th smallest
+// approximate value; a candidate above S_k + 2 eps can therefore not be among the k nearest
+// candidates and is not rescored.
+constexpr int RS_WAVES = 4;
+
+__global__ void __launch_bounds__(64 * RS_WAVES)
+k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, uint32_t splits,
+              const uint32_t *cand_idx, const float *cand_s, const double *nx64, double eps_coef, double eps_const,
+              uint32_t *idx_out, double *d2_out, uint32_t *fallback, unsigned int *n_fallback,
+              unsigned long long *n_rescored) {
+    __shared__ float sS[RS_WAVES][KNN_MAXCAND];
+    __shared__ uint32_t sI[RS_WAVES][KNN_MAXCAND];
+    __shared__ double sD[RS_WAVES][KNN_MAXCAND];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t total = KM * splits;  // <= KNN_MAXCAND
     unsigned long long local = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double bd[KNN_MAXCAND];
-        uint32_t bi[KNN_MAXCAND];
-        uint32_t cnt = 0;
-        float amin = INFINITY;  // smallest list maximum over the splits whose list is full
-        bool any_full = false;
-        for (uint32_t sp = 0; sp < splits; sp++) {
-            const uint64_t base = ((uint64_t)i * splits + sp) * K2;
-            bool full = true;
-            float amax = -INFINITY;
-            for (uint32_t t = 0; t < K2; t++) {
-                uint32_t j = cand_idx[base + t];
-                if (j == KNONE) {
-                    full = false;
-                    continue;
-                }
-                amax = fmaxf(amax, cand_s[base + t]);
-                double dd = exact_d2(X, d, i, j);
-                local++;
-                uint32_t p = cnt;
-                while (p > 0 && (bd[p - 1] > dd || (bd[p - 1] == dd && bi[p - 1] > j))) {
-                    bd[p] = bd[p - 1];
-                    bi[p] = bi[p - 1];
-                    p--;
-                }
-                bd[p] = dd;
-                bi[p] = j;
-                cnt++;
-            }
-            if (full) {
-                any_full = true;
-                amin = fminf(amin, amax);
-            }
+    for (uint32_t i = blockIdx.x * RS_WAVES + wave; i < n; i += gridDim.x * RS_WAVES) {
+        const uint64_t base = (uint64_t)i * total;
+        float sv[2];
+        uint32_t jv[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t e = lane + 64 * h;
+            jv[h] = e < total ? cand_idx[base + e] : KNONE;
+            sv[h] = jv[h] != KNONE ? cand_s[base + e] : INFINITY;
+            sS[wave][e] = sv[h];
+            sI[wave][e] = jv[h];
         }
-        for (uint32_t t = 0; t < k; t++) {
-            idx_out[(uint64_t)i * k + t] = t < cnt ? bi[t] : KNONE;
-            d2_out[(uint64_t)i * k + t] = t < cnt ? bd[t] : INFINITY;
+        // a split whose list is full (its last entry is a point) may hide closer points; the
+        // smallest last value over those splits bounds every point not in the lists
+        float last = INFINITY;
+        bool full = false;
+        if ((uint32_t)lane < splits) {
+            const uint32_t jl = cand_idx[base + (uint64_t)lane * KM + KM - 1];
+            full = jl != KNONE;
+            if (full) last = cand_s[base + (uint64_t)lane * KM + KM - 1];
         }
-        // a split whose list never filled holds all of its points, so only full splits can hide
-        // a closer point; certified when none of them can reach the k-th distance (strictly)
+        const bool any_full = __any(full);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) last = fminf(last, __shfl_xor(last, o));
+        const float amin = last;
+        const uint32_t cnt = (uint32_t)__popcll(__ballot(jv[0] != KNONE)) + (uint32_t)__popcll(__ballot(jv[1] != KNONE));
+        __threadfence_block();
+
+        // S_k: the approximate value of rank k - 1 (ties by list position)
+        const double eps = eps_coef * sqrt(nx64[i]) + eps_const;
+        double cut = INFINITY;
+        if (cnt > k) {
+            uint32_t rank[2] = {0u, 0u};
+            for (uint32_t e = 0; e < total; e++) {
+                const float v = sS[wave][e];
+#pragma unroll
+                for (int h = 0; h < 2; h++) rank[h] += (v < sv[h] || (v == sv[h] && e < (uint32_t)lane + 64u * h)) ? 1u : 0u;
+            }
+            float sk = INFINITY;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint64_t m = __ballot(jv[h] != KNONE && rank[h] == k - 1);
+                if (m) sk = __shfl(sv[h], (int)__ffsll((unsigned long long)m) - 1);
+            }
+            cut = (double)sk + 2.0 * eps;
+        }
+
+        // exact distances of the candidates that can still be among the k nearest
+        double dd[2];
+        bool pass[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            pass[h] = jv[h] != KNONE && (double)sv[h] <= cut;
+            dd[h] = pass[h] ? exact_d2(X, d, i, jv[h]) : INFINITY;
+            sD[wave][lane + 64 * h] = dd[h];
+        }
+        const uint32_t P = (uint32_t)__popcll(__ballot(pass[0])) + (uint32_t)__popcll(__ballot(pass[1]));
+        local += lane == 0 ? P : 0;
+        __threadfence_block();
+
+        // rank by (d2, index) among the rescored candidates; the first k are the answer
+        double kth = INFINITY;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t r = 0;
+            if (pass[h]) {
+                for (uint32_t e = 0; e < total; e++) {
+                    const double v = sD[wave][e];
+                    r += (v < dd[h] || (v == dd[h] && sI[wave][e] < jv[h])) ? 1u : 0u;
+                }
+                if (r < k) {
+                    idx_out[(uint64_t)i * k + r] = jv[h];
+                    d2_out[(uint64_t)i * k + r] = dd[h];
+                }
+            }
+            const uint64_t m = __ballot(pass[h] && r == k - 1);
+            if (m) kth = __shfl(dd[h], (int)__ffsll((unsigned long long)m) - 1);
+        }
+        if ((uint32_t)lane >= P && (uint32_t)lane < k) {
+            idx_out[(uint64_t)i * k + lane] = KNONE;
+            d2_out[(uint64_t)i * k + lane] = INFINITY;
+        }
+        // certified when no full split can hide a point at or below the k-th distance
         bool ok = true;
-        if (any_full) {
-            if (cnt < k) {
-                ok = false;
-            } else {
-                double eps = eps_coef * sqrt(nx64[i]) + eps_const;
-                double lb = nx64[i] + (double)amin - eps;
-                ok = lb > bd[k - 1];
-            }
-        }
-        if (!ok) {
-            unsigned int slot = atomicAdd(n_fallback, 1u);
+        if (any_full) ok = cnt >= k && (nx64[i] + (double)amin - eps > kth);
+        if (!ok && lane == 0) {
+            const unsigned int slot = atomicAdd(n_fallback, 1u);
             fallback[slot] = i;
         }
+        __threadfence_block();  // this row's LDS reads are done before the next row's writes
     }
-    if (local) atomicAdd(n_rescored, local);
+    if (lane == 0 && local) atomicAdd(n_rescored, local);
 }
 
 // Exact FP64 scan of one row per workgroup (128 threads), for rows the certificate rejected.
@@ -542,13 +589,27 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const uint32_t nkc = (d + 31) / 32;
     const uint32_t Kp = 32 * nkc;
     const uint32_t npad = (n + KNN_BM - 1) / KNN_BM * KNN_BM;
-    // candidates kept per row and split: the power of two in {16, 32, 64} at or above 2k
+    // candidates kept per row and split: the smallest of 16, 32, 64 above k (the certificate
+    // needs one candidate beyond the k-th)
     uint32_t K2 = 16;
-    while (K2 < 2 * k && K2 < (uint32_t)KNN_MAXK2) K2 *= 2;
-    // column splits so that small inputs still fill the chip; every split is whole stages
+    while (K2 < k + 1 && K2 < (uint32_t)KNN_MAXK2) K2 *= 2;
+    // column splits: the count whose last wave of workgroups is fullest (time ~ waves / splits),
+    // with a small charge per split for the extra candidates it sends to the rescore
     const uint32_t row_blocks = npad / KNN_BM;
-    uint32_t splits = std::max<uint32_t>(1, std::min<uint32_t>(8, (1024 + row_blocks - 1) / row_blocks));
-    while (splits > 1 && (uint64_t)splits * K2 > KNN_MAXCAND) splits--;
+    int dev = 0, cus = 256;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t slots = (uint32_t)cus * ((K2 == 16 && nkc <= 2) ? 2u : 1u);  // workgroups resident at once
+    uint32_t splits = 1;
+    double best = 1e30;
+    for (uint32_t sp = 1; sp <= 8 && sp * K2 <= (uint32_t)KNN_MAXCAND; sp++) {
+        const double waves = (double)((row_blocks * sp + slots - 1) / slots);
+        const double cost = waves / sp + 0.02 * sp;
+        if (cost < best) {
+            best = cost;
+            splits = sp;
+        }
+    }
     uint32_t cps = (npad + splits - 1) / splits;
     cps = (cps + KNN_TN - 1) / KNN_TN * KNN_TN;
     splits = (npad + cps - 1) / cps;
@@ -603,7 +664,8 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     DevBuf<unsigned long long> nres(1);
     HIP_OK(hipMemsetAsync(nfb.get(), 0, sizeof(unsigned int), s));
     HIP_OK(hipMemsetAsync(nres.get(), 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_knn_rescore, dim3(grid_for(n, 64, 8192)), dim3(64), 0, s, d_X, n, d, k, K2, splits,
+    hipLaunchKernelGGL(k_knn_rescore, dim3(std::min<uint32_t>((n + RS_WAVES - 1) / RS_WAVES, 16384)),
+                       dim3(64 * RS_WAVES), 0, s, d_X, n, d, k, K2, splits,
                        cidx.get(), cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fb.get(), nfb.get(),
                        nres.get());
     KERNEL_OK();
