@@ -116,6 +116,11 @@ void sc_shutdown(void);
 /* Records device time of the hot kernels into sc_leiden_stats (adds HIP events around each
  * local-move decide launch). Also switched on by the environment variable SCGPU_PROFILE=1. */
 void sc_set_profiling(int32_t enable);
+/* No counterpart in the reference. Selects how local moving and refinement are launched:
+ * 1 (default) = one cooperative persistent launch per phase of a level, with grid barriers
+ * between sub-sweeps; 0 = one launch per kernel of each sub-sweep. Both give identical labels.
+ * The environment variable SCGPU_PERSISTENT=0 selects 0 at start-up. */
+sc_status sc_set_sweep_mode(int32_t mode);
 
 /* LeidenConfig::default (leiden/mod.rs:92-106). */
 sc_leiden_config sc_leiden_config_default(void);

@@ -27,11 +27,15 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
 #include <random>
 #include <vector>
 #include <chrono>
+#include <cstdlib>
+#include <cstdio>
+#include <string>
 
 namespace scgpu {
 
@@ -141,6 +145,191 @@ __device__ __forceinline__ void group_reduce_wave(double &bs, uint32_t &bc, long
     }
 }
 
+// Per-step values of a decide pass (the multi-launch path takes them from DecideArgs).
+struct StepParams {
+    uint64_t seed;
+    uint32_t sub;
+    uint32_t round;
+    uint32_t step;
+    unsigned int *n_movers;
+    uint32_t *mstep;  // step stamp of accepted movers (persistent local moving), else null
+};
+
+struct DecideStats {
+    unsigned long long v = 0, e = 0, c = 0;
+};
+
+// One unit of a decide pass: NG = BS / G vertices of one degree bucket, G lanes each. blist is
+// the bucket's vertex list; sel, when given, holds the positions (within blist) of this step's
+// vertices, which are then already known to be active and in the step's subset. base is the
+// index (into sel, or blist) of the unit's first vertex. Every thread of the block calls this.
+template <int BS, int G, int T, bool REFINE, bool GTAB>
+__device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParams &sp, const uint32_t *blist,
+                                            const uint32_t *sel, uint32_t len, uint64_t base, uint32_t *s_keys,
+                                            long long *s_vals, double *r_s, uint32_t *r_c, long long *r_w,
+                                            DecideStats &ds) {
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const uint64_t idx = base + grp;
+    bool valid = idx < len;
+    uint32_t pos = valid ? (sel ? sel[idx] : (uint32_t)idx) : 0u;
+    uint32_t v = valid ? blist[pos] : 0u;
+    if (valid && !sel) {
+        valid = a.active[v] != 0 && subset_of(sp.seed, v, a.P) == sp.sub;
+        if (REFINE) valid = valid && a.memb[v] == v && a.csize[v] == 1;
+    }
+    uint32_t *keys;
+    long long *vals;
+    uint32_t tsize;
+    if (GTAB) {
+        uint64_t o = valid ? a.goff[pos] : 0;
+        tsize = valid ? (uint32_t)(a.goff[pos + 1] - o) : 0;
+        keys = a.gkeys + o;
+        vals = a.gvals + o;
+    } else {
+        keys = s_keys + grp * T;
+        vals = s_vals + grp * T;
+        tsize = T;
+    }
+    for (uint32_t s = lane; s < tsize; s += G) {
+        keys[s] = EMPTY_KEY;
+        vals[s] = 0;
+    }
+    __syncthreads();
+    if (GTAB) __threadfence_block();
+
+    uint64_t beg = 0, end = 0;
+    uint32_t own = 0, cv = 0;
+    if (valid) {
+        beg = a.ptrs[v];
+        end = a.ptrs[v + 1];
+        own = a.memb[v];
+        if (REFINE) cv = a.constraint[v];
+    }
+    const uint32_t mask = tsize - 1;
+    for (uint64_t e = beg + lane; e < end; e += G) {
+        uint32_t u = a.nbrs[e];
+        if (u == v) continue;  // self-loops are handled by the partition (partition.rs:430)
+        if (REFINE && a.constraint[u] != cv) continue;
+        uint32_t c = a.memb[u];
+        long long wf = to_fx((double)a.wts[e], a.sw);
+        table_insert(keys, vals, mask, c, wf);
+    }
+    __syncthreads();
+    if (GTAB) __threadfence_block();
+
+    double bs = -INFINITY;
+    uint32_t bc = NONE;
+    long long wown = 0;
+    double k = 0.0, na = 0.0, loop = 0.0, rbase = 0.0;
+    if (valid) {
+        k = a.strength[v];
+        na = a.node_w[v];
+        loop = a.self_loop[v];
+        // refinement: the node leaves its singleton, which becomes empty (refine.rs:71-74)
+        if (REFINE) rbase = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
+        for (uint32_t s = lane; s < tsize; s += G) {
+            uint32_t c = keys[s];
+            if (c == EMPTY_KEY) continue;
+            long long wfx = vals[s];
+            if (c == own) {
+                wown = wfx;
+                continue;
+            }
+            ds.c++;
+            double Wc = from_fx(a.Wfx[c], a.sW);
+            if (a.has_max && !(Wc + na <= a.max_w)) continue;
+            double Kc = a.objective == 0 ? from_fx(a.Kfx[c], a.sK) : 0.0;
+            double gain = delta_insert(a, from_fx(wfx, a.sw), loop, k, na, Kc, Wc);
+            if (REFINE) {
+                double rel = gain - rbase;
+                if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
+                double score = a.theta > 0.0 ? rel / a.theta + gumbel(sp.seed, sp.round, v, c) : rel;
+                better(bs, bc, score, c);
+            } else {
+                better(bs, bc, gain, c);
+            }
+        }
+    }
+    // group-wide reduction
+    if (G <= 64) {
+        group_reduce_wave<(G <= 64 ? G : 64)>(bs, bc, wown);
+    } else {
+        static_assert(G <= 64 || G == BS, "groups wider than a wave span the whole block");
+        group_reduce_wave<64>(bs, bc, wown);
+        const int wave = threadIdx.x / 64;
+        if ((threadIdx.x & 63) == 0) {
+            r_s[wave] = bs;
+            r_c[wave] = bc;
+            r_w[wave] = wown;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            bs = r_s[0];
+            bc = r_c[0];
+            wown = r_w[0];
+            for (int w = 1; w < BS / 64; w++) {
+                better(bs, bc, r_s[w], r_c[w]);
+                wown += r_w[w];
+            }
+        }
+    }
+    bool push = false;
+    if (valid && lane == 0) {
+        a.active[v] = 0;
+        ds.v++;
+        ds.e += end - beg;
+        if (!REFINE) {
+            long long kfx = a.objective == 0 ? to_fx(k, a.sK) : 0;
+            long long wfxv = to_fx(na, a.sW);
+            double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, a.sK) : 0.0;
+            double Wown = from_fx(a.Wfx[own] - wfxv, a.sW);
+            double base_q = delta_insert(a, from_fx(wown, a.sw), loop, k, na, Kown, Wown);
+            // the empty community: this vertex's own id, when no one holds it
+            if (a.csize[v] == 0 && (!a.has_max || na <= a.max_w)) {
+                double ge = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
+                better(bs, bc, ge, v);
+            }
+            bool accept = bc != NONE && bc != own && bs - base_q > a.threshold;
+            if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
+            if (accept) {
+                a.target[v] = bc;
+                if (sp.mstep) sp.mstep[v] = sp.step;
+                push = true;
+            }
+        } else if (bc != NONE) {
+            a.target[v] = bc;
+            a.tround[v] = sp.round;
+            atomicAdd(&a.targeted_cnt[bc], 1u);
+            push = true;
+        }
+    }
+    // append the wave's movers with one counter update
+    const uint64_t pm = __ballot(push);
+    if (pm) {
+        const int leader = __ffsll((unsigned long long)pm) - 1;
+        unsigned int slot0 = 0;
+        if ((int)(threadIdx.x & 63) == leader) slot0 = atomicAdd(sp.n_movers, (unsigned int)__popcll(pm));
+        slot0 = __shfl(slot0, leader);
+        if (push)
+            a.movers[slot0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void flush_decide_stats(const DecideStats &ds, unsigned long long *s_stats,
+                                                   unsigned long long *stats) {
+    if (ds.v) {
+        atomicAdd(&s_stats[0], ds.v);
+        atomicAdd(&s_stats[1], ds.e);
+    }
+    if (ds.c) atomicAdd(&s_stats[2], ds.c);
+    __syncthreads();
+    if (threadIdx.x < 3 && s_stats[threadIdx.x]) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+// Multi-launch decide: one launch per degree bucket, grid-stride over the bucket's vertices.
 template <int BS, int G, int T, bool REFINE, bool GTAB>
 __global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
     constexpr int NG = BS / G;
@@ -152,153 +341,12 @@ __global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
     __shared__ long long r_w[BS / 64];
     __shared__ unsigned long long s_stats[3];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
-
-    const int lane = threadIdx.x % G;
-    const int grp = threadIdx.x / G;
-    unsigned long long st_v = 0, st_e = 0, st_c = 0;
-
-    for (uint64_t base = (uint64_t)blockIdx.x * NG; base < a.list_len; base += (uint64_t)gridDim.x * NG) {
-        const uint64_t idx = base + grp;
-        bool valid = idx < a.list_len;
-        uint32_t v = valid ? a.list[idx] : 0;
-        if (valid) {
-            valid = a.active[v] != 0 && subset_of(a.seed, v, a.P) == a.sub;
-            if (REFINE) valid = valid && a.memb[v] == v && a.csize[v] == 1;
-        }
-        uint32_t *keys;
-        long long *vals;
-        uint32_t tsize;
-        if (GTAB) {
-            uint64_t o = valid ? a.goff[idx] : 0;
-            tsize = valid ? (uint32_t)(a.goff[idx + 1] - o) : 0;
-            keys = a.gkeys + o;
-            vals = a.gvals + o;
-        } else {
-            keys = s_keys + grp * T;
-            vals = s_vals + grp * T;
-            tsize = T;
-        }
-        for (uint32_t s = lane; s < tsize; s += G) {
-            keys[s] = EMPTY_KEY;
-            vals[s] = 0;
-        }
-        __syncthreads();
-        if (GTAB) __threadfence_block();
-
-        uint64_t beg = 0, end = 0;
-        uint32_t own = 0, cv = 0;
-        if (valid) {
-            beg = a.ptrs[v];
-            end = a.ptrs[v + 1];
-            own = a.memb[v];
-            if (REFINE) cv = a.constraint[v];
-        }
-        const uint32_t mask = tsize - 1;
-        for (uint64_t e = beg + lane; e < end; e += G) {
-            uint32_t u = a.nbrs[e];
-            if (u == v) continue;  // self-loops are handled by the partition (partition.rs:430)
-            if (REFINE && a.constraint[u] != cv) continue;
-            uint32_t c = a.memb[u];
-            long long wf = to_fx((double)a.wts[e], a.sw);
-            table_insert(keys, vals, mask, c, wf);
-        }
-        __syncthreads();
-        if (GTAB) __threadfence_block();
-
-        double bs = -INFINITY;
-        uint32_t bc = NONE;
-        long long wown = 0;
-        double k = 0.0, na = 0.0, loop = 0.0, rbase = 0.0;
-        if (valid) {
-            k = a.strength[v];
-            na = a.node_w[v];
-            loop = a.self_loop[v];
-            // refinement: the node leaves its singleton, which becomes empty (refine.rs:71-74)
-            if (REFINE) rbase = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
-            for (uint32_t s = lane; s < tsize; s += G) {
-                uint32_t c = keys[s];
-                if (c == EMPTY_KEY) continue;
-                long long wfx = vals[s];
-                if (c == own) {
-                    wown = wfx;
-                    continue;
-                }
-                st_c++;
-                double Wc = from_fx(a.Wfx[c], a.sW);
-                if (a.has_max && !(Wc + na <= a.max_w)) continue;
-                double Kc = a.objective == 0 ? from_fx(a.Kfx[c], a.sK) : 0.0;
-                double gain = delta_insert(a, from_fx(wfx, a.sw), loop, k, na, Kc, Wc);
-                if (REFINE) {
-                    double rel = gain - rbase;
-                    if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
-                    double score = a.theta > 0.0 ? rel / a.theta + gumbel(a.seed, a.round, v, c) : rel;
-                    better(bs, bc, score, c);
-                } else {
-                    better(bs, bc, gain, c);
-                }
-            }
-        }
-        // group-wide reduction
-        if (G <= 64) {
-            group_reduce_wave<(G <= 64 ? G : 64)>(bs, bc, wown);
-        } else {
-            group_reduce_wave<64>(bs, bc, wown);
-            const int wave = threadIdx.x / 64;
-            if ((threadIdx.x & 63) == 0) {
-                r_s[wave] = bs;
-                r_c[wave] = bc;
-                r_w[wave] = wown;
-            }
-            __syncthreads();
-            if (lane == 0) {
-                bs = r_s[0];
-                bc = r_c[0];
-                wown = r_w[0];
-                for (int w = 1; w < BS / 64; w++) {
-                    better(bs, bc, r_s[w], r_c[w]);
-                    wown += r_w[w];
-                }
-            }
-        }
-        if (valid && lane == 0) {
-            a.active[v] = 0;
-            st_v++;
-            st_e += end - beg;
-            if (!REFINE) {
-                long long kfx = a.objective == 0 ? to_fx(k, a.sK) : 0;
-                long long wfxv = to_fx(na, a.sW);
-                double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, a.sK) : 0.0;
-                double Wown = from_fx(a.Wfx[own] - wfxv, a.sW);
-                double base = delta_insert(a, from_fx(wown, a.sw), loop, k, na, Kown, Wown);
-                // the empty community: this vertex's own id, when no one holds it
-                if (a.csize[v] == 0 && (!a.has_max || na <= a.max_w)) {
-                    double ge = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
-                    better(bs, bc, ge, v);
-                }
-                bool accept = bc != NONE && bc != own && bs - base > a.threshold;
-                if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
-                if (accept) {
-                    a.target[v] = bc;
-                    unsigned int slot = atomicAdd(a.n_movers, 1u);
-                    a.movers[slot] = v;
-                }
-            } else if (bc != NONE) {
-                a.target[v] = bc;
-                a.tround[v] = a.round;
-                atomicAdd(&a.targeted_cnt[bc], 1u);
-                unsigned int slot = atomicAdd(a.n_movers, 1u);
-                a.movers[slot] = v;
-            }
-        }
-        __syncthreads();
-    }
-    if (lane == 0 && st_v) {
-        atomicAdd(&s_stats[0], st_v);
-        atomicAdd(&s_stats[1], st_e);
-    }
-    if (st_c) atomicAdd(&s_stats[2], st_c);
-    __syncthreads();
-    if (threadIdx.x < 3 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+    StepParams sp{a.seed, a.sub, a.round, 0u, a.n_movers, nullptr};
+    DecideStats ds;
+    for (uint64_t base = (uint64_t)blockIdx.x * NG; base < a.list_len; base += (uint64_t)gridDim.x * NG)
+        decide_unit<BS, G, T, REFINE, GTAB>(a, sp, a.list, nullptr, a.list_len, base, s_keys, s_vals, r_s, r_c,
+                                            r_w, ds);
+    flush_decide_stats(ds, s_stats, a.stats);
 }
 
 struct ApplyArgs {
@@ -389,17 +437,20 @@ __global__ void k_claim(const uint32_t *movers, const unsigned int *count, const
 }
 
 // Re-activation after moves. Local moving: neighbours not in the mover's new community
-// (local_move.rs:114-118). Refinement: neighbours inside the same working community.
+// (local_move.rs:114-118). Refinement: neighbours inside the same working community. With a
+// weight cap (claim != null) a local mover whose claim lost did not move and activates nothing.
 template <int G, bool REFINE>
 __global__ void __launch_bounds__(256) k_activate(const uint32_t *list, const unsigned int *count,
                                                   const uint64_t *ptrs, const uint32_t *nbrs,
                                                   const uint32_t *memb, const uint32_t *constraint,
-                                                  uint8_t *active) {
+                                                  uint8_t *active, const uint32_t *target = nullptr,
+                                                  const uint32_t *claim = nullptr) {
     constexpr int NG = 256 / G;
     unsigned int cnt = *count;
     const int lane = threadIdx.x % G;
     for (unsigned int i = blockIdx.x * NG + threadIdx.x / G; i < cnt; i += gridDim.x * NG) {
         uint32_t v = list[i];
+        if (claim && claim[target[v]] != v) continue;
         uint32_t mv = REFINE ? constraint[v] : memb[v];
         for (uint64_t e = ptrs[v] + lane; e < ptrs[v + 1]; e += G) {
             uint32_t u = nbrs[e];
@@ -524,6 +575,399 @@ static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
     }
 }
 
+// ---- persistent level sweeps -------------------------------------------------------------------
+// One cooperative launch runs a whole local-moving (or refinement) phase of a level: every
+// sub-sweep is compact -> decide -> apply (+ activate), separated by grid barriers, and the
+// convergence test reads device counters, so nothing returns to the host between sub-sweeps.
+constexpr int SWEEP_BS = 256;
+constexpr int SWEEP_TAB = 4096;  // LDS hash entries per workgroup (the largest bucket layout)
+constexpr int SWEEP_ACT_G = 16;  // lanes per mover when re-activating neighbours
+// A barrier wait longer than this (ticks of the 100 MHz real-time clock) is reported as an error
+// instead of spinning on; every workgroup then leaves the kernel.
+constexpr long long SWEEP_TIMEOUT_TICKS = 400000000ll;
+
+struct BarrierCtl;
+
+struct SweepArgs {
+    DecideArgs d;
+    const uint32_t *lists;  // vertices grouped by degree bucket
+    uint32_t bstart[NUM_BUCKETS + 1];
+    uint32_t *work;         // compacted positions of one step, same bucket layout as lists
+    unsigned int *wcnt;     // [max_steps][NUM_BUCKETS] compacted counts
+    unsigned int *mcnt;     // [max_steps] deciders that chose to move
+    unsigned int *ccnt;     // [max_steps] vertices that moved (refinement)
+    uint32_t *mstep;        // step stamp of accepted movers (local moving)
+    uint32_t *claim;        // lowest mover id per capped target, or null without a weight cap
+    BarrierCtl *bc;         // grid barrier words (zeroed before the launch)
+    unsigned int *ctl;      // [0] steps run, [2] iterations, [3] moves
+    // writable views of d.memb, d.Kfx, d.Wfx, d.csize
+    uint32_t *memb;
+    long long *Kfx, *Wfx;
+    uint32_t *csize;
+    uint32_t max_iters;
+    uint32_t max_steps;
+    uint64_t level_seed;
+    long long *trace;       // optional: real-time stamps after each barrier (workgroup 0)
+    uint32_t trace_cap;
+};
+
+__device__ __forceinline__ unsigned int ld_acquire_ctr(const unsigned int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide barrier for co-resident workgroups (cooperative launch). Each XCD has its own L2:
+// every wave first waits until its own stores (and atomics) have reached L2, then the last
+// workgroup of each XCD to arrive writes that L2 back (one agent-scope release per XCD) before
+// arriving at the root. After the release every workgroup performs its own agent-scope acquire
+// (an invalidate issued by one workgroup does not make the others' loads fresh). A wait longer
+// than the timeout sets the error flag and every workgroup returns false.
+struct BarrierCtl {
+    // separate 128-byte lines for the words that many workgroups touch
+    unsigned int root, pad0[31];
+    unsigned int release, pad1[31];
+    unsigned int err, pad2[31];
+    unsigned int joined, pad3[31];
+    unsigned int xarrive[8][32];
+    unsigned int xblocks[8];
+};
+
+struct BarrierState {
+    unsigned int xcc, nb, nx, epoch;
+};
+
+__device__ __forceinline__ unsigned int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u; }
+
+__device__ __forceinline__ bool spin_until(const unsigned int *p, unsigned int v, unsigned int *err, long long t0) {
+    while (ld_acquire_ctr(p) < v) {
+        if (ld_acquire_ctr(err) != 0u) return false;
+        if (wall_clock64() - t0 > SWEEP_TIMEOUT_TICKS) {
+            atomicExch(err, 1u);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// Registers the workgroup with its XCD; all workgroups must call it once before any barrier.
+__device__ __forceinline__ bool barrier_join(BarrierCtl *bc, BarrierState &bs, int *s_flag) {
+    __shared__ unsigned int s_x, s_nb, s_nx;
+    if (threadIdx.x == 0) {
+        const unsigned int x = xcc_id();
+        atomicAdd(&bc->xblocks[x], 1u);
+        atomicAdd(&bc->joined, 1u);
+        int bad = spin_until(&bc->joined, gridDim.x, &bc->err, wall_clock64()) ? 0 : 1;
+        unsigned int nx = 0;
+        for (int k = 0; k < 8; k++) nx += ld_acquire_ctr(&bc->xblocks[k]) ? 1u : 0u;
+        s_x = x;
+        s_nb = ld_acquire_ctr(&bc->xblocks[x]);
+        s_nx = nx;
+        *s_flag = bad;
+    }
+    __syncthreads();
+    bs.xcc = s_x;
+    bs.nb = s_nb;
+    bs.nx = s_nx;
+    bs.epoch = 0;
+    return *s_flag == 0;
+}
+
+__device__ __forceinline__ bool grid_barrier(BarrierCtl *bc, BarrierState &bs, int *s_flag,
+                                             long long *trace = nullptr, uint32_t trace_cap = 0) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores and atomics are in L2
+    __syncthreads();
+    bs.epoch++;
+    if (threadIdx.x == 0) {
+        const unsigned int e = bs.epoch, x = bs.xcc;
+        const long long t0 = wall_clock64();
+        int bad = 0;
+        const unsigned int old = atomicAdd(&bc->xarrive[x][0], 1u);
+        const bool leader = old + 1 == bs.nb * e;
+        if (leader) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const unsigned int r = atomicAdd(&bc->root, 1u);
+            if (r + 1 == bs.nx * e) atomicExch(&bc->release, e);
+        }
+        if (!spin_until(&bc->release, e, &bc->err, t0)) bad = 1;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // the invalidate has completed before the block reads
+        *s_flag = bad;
+        if (trace && blockIdx.x == 0 && e - 1 < trace_cap) trace[e - 1] = wall_clock64();
+    }
+    __syncthreads();
+    return *s_flag == 0;
+}
+
+__device__ __forceinline__ int bucket_of_pos(const SweepArgs &A, uint32_t i) {
+    int b = 0;
+#pragma unroll
+    for (int k = 1; k < NUM_BUCKETS; k++) b += i >= A.bstart[k] ? 1 : 0;
+    return b;
+}
+
+// Positions (within their bucket) of the vertices that decide in this step, wave-aggregated
+// appends into work[bstart[b] + ...].
+template <bool REFINE>
+__device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t step) {
+    const DecideArgs &a = A.d;
+    const uint32_t n = a.n;
+    const int lane = threadIdx.x & 63;
+    unsigned int *cnt = A.wcnt + (uint64_t)step * NUM_BUCKETS;
+    for (uint32_t i0 = blockIdx.x * SWEEP_BS; i0 < n; i0 += gridDim.x * SWEEP_BS) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool take = false;
+        int b = 0;
+        if (i < n) {
+            const uint32_t v = A.lists[i];
+            b = bucket_of_pos(A, i);
+            take = a.active[v] != 0 && subset_of(seed, v, a.P) == sub;
+            if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
+        }
+        uint64_t pending = __ballot(take);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int bl = __shfl(b, leader);
+            const bool mine = take && b == bl;
+            const uint64_t m = __ballot(mine);
+            unsigned int basei = 0;
+            if (lane == leader) basei = atomicAdd(&cnt[bl], (unsigned int)__popcll(m));
+            basei = __shfl(basei, leader);
+            if (mine) {
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                A.work[A.bstart[bl] + basei + below] = i - A.bstart[bl];
+            }
+            pending &= ~m;
+        }
+    }
+}
+
+template <int B>
+struct BucketShape;
+// Narrow groups keep many vertices in flight per workgroup (each lane walks a few edges).
+template <> struct BucketShape<0> { static constexpr int G = 4, T = 16; };
+template <> struct BucketShape<1> { static constexpr int G = 4, T = 32; };
+template <> struct BucketShape<2> { static constexpr int G = 8, T = 64; };
+template <> struct BucketShape<3> { static constexpr int G = 16, T = 256; };
+template <> struct BucketShape<4> { static constexpr int G = 256, T = 2048; };
+template <> struct BucketShape<5> { static constexpr int G = 256, T = 4096; };
+template <> struct BucketShape<6> { static constexpr int G = 256, T = 1; };
+
+template <int B, bool REFINE>
+__device__ __forceinline__ void sweep_decide_unit(const SweepArgs &A, const StepParams &sp, uint32_t cntb,
+                                                  uint64_t u, uint32_t *s_keys, long long *s_vals, double *r_s,
+                                                  uint32_t *r_c, long long *r_w, DecideStats &ds) {
+    constexpr int G = BucketShape<B>::G, T = BucketShape<B>::T, NG = SWEEP_BS / G;
+    static_assert(B == 6 || NG * T <= SWEEP_TAB, "bucket table does not fit");
+    decide_unit<SWEEP_BS, G, T, REFINE, B == 6>(A.d, sp, A.lists + A.bstart[B], A.work + A.bstart[B], cntb,
+                                                 u * NG, s_keys, s_vals, r_s, r_c, r_w, ds);
+}
+
+template <bool REFINE>
+__device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParams &sp, uint32_t *s_keys,
+                                             long long *s_vals, double *r_s, uint32_t *r_c, long long *r_w,
+                                             DecideStats &ds) {
+    constexpr int NGs[NUM_BUCKETS] = {SWEEP_BS / BucketShape<0>::G, SWEEP_BS / BucketShape<1>::G,
+                                      SWEEP_BS / BucketShape<2>::G, SWEEP_BS / BucketShape<3>::G,
+                                      SWEEP_BS / BucketShape<4>::G, SWEEP_BS / BucketShape<5>::G,
+                                      SWEEP_BS / BucketShape<6>::G};
+    uint32_t cnt[NUM_BUCKETS];
+    uint64_t ustart[NUM_BUCKETS + 1];
+    ustart[0] = 0;
+#pragma unroll
+    for (int b = 0; b < NUM_BUCKETS; b++) {
+        cnt[b] = ld_acquire_ctr(A.wcnt + (uint64_t)sp.step * NUM_BUCKETS + b);
+        ustart[b + 1] = ustart[b] + (cnt[b] + NGs[b] - 1) / NGs[b];
+    }
+    for (uint64_t u = blockIdx.x; u < ustart[NUM_BUCKETS]; u += gridDim.x) {
+        if (u < ustart[1]) sweep_decide_unit<0, REFINE>(A, sp, cnt[0], u, s_keys, s_vals, r_s, r_c, r_w, ds);
+        else if (u < ustart[2]) sweep_decide_unit<1, REFINE>(A, sp, cnt[1], u - ustart[1], s_keys, s_vals, r_s, r_c, r_w, ds);
+        else if (u < ustart[3]) sweep_decide_unit<2, REFINE>(A, sp, cnt[2], u - ustart[2], s_keys, s_vals, r_s, r_c, r_w, ds);
+        else if (u < ustart[4]) sweep_decide_unit<3, REFINE>(A, sp, cnt[3], u - ustart[3], s_keys, s_vals, r_s, r_c, r_w, ds);
+        else if (u < ustart[5]) sweep_decide_unit<4, REFINE>(A, sp, cnt[4], u - ustart[4], s_keys, s_vals, r_s, r_c, r_w, ds);
+        else if (u < ustart[6]) sweep_decide_unit<5, REFINE>(A, sp, cnt[5], u - ustart[5], s_keys, s_vals, r_s, r_c, r_w, ds);
+        else sweep_decide_unit<6, REFINE>(A, sp, cnt[6], u - ustart[6], s_keys, s_vals, r_s, r_c, r_w, ds);
+    }
+}
+
+__device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint64_t from, uint64_t to) {
+    const long long kf = to_fx(A.d.strength[v], A.d.sK), wf = to_fx(A.d.node_w[v], A.d.sW);
+    atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
+    atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)(-kf));
+    atomicAdd((unsigned long long *)&A.Wfx[to], (unsigned long long)wf);
+    atomicAdd((unsigned long long *)&A.Wfx[from], (unsigned long long)(-wf));
+    atomicAdd(&A.csize[to], 1u);
+    atomicSub(&A.csize[from], 1u);
+}
+
+__global__ void k_stamp_clock(long long *out) { *out = wall_clock64(); }
+
+template <bool REFINE>
+__global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
+    __shared__ uint32_t s_keys[SWEEP_TAB];
+    __shared__ long long s_vals[SWEEP_TAB];
+    __shared__ double r_s[SWEEP_BS / 64];
+    __shared__ uint32_t r_c[SWEEP_BS / 64];
+    __shared__ long long r_w[SWEEP_BS / 64];
+    __shared__ unsigned long long s_stats[3];
+    __shared__ int s_flag;
+    const DecideArgs &a = A.d;
+    const uint32_t n = a.n;
+    const uint64_t gtid = (uint64_t)blockIdx.x * SWEEP_BS + threadIdx.x;
+    const uint64_t gsize = (uint64_t)gridDim.x * SWEEP_BS;
+    const bool capped = A.claim != nullptr;
+    if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
+    DecideStats ds;
+    BarrierState bst;
+    if (!barrier_join(A.bc, bst, &s_flag)) return;
+
+    // prologue: state of the level (refinement starts from singletons)
+    for (uint64_t v = gtid; v < n; v += gsize) {
+        A.Kfx[v] = 0;
+        A.Wfx[v] = 0;
+        A.csize[v] = 0;
+        a.active[v] = 1;
+        if (REFINE) {
+            A.memb[v] = (uint32_t)v;
+            a.tround[v] = NONE;
+            a.targeted_cnt[v] = 0;
+        } else {
+            A.mstep[v] = NONE;
+        }
+        if (capped) A.claim[v] = NONE;
+    }
+    for (uint64_t i = gtid; i < (uint64_t)A.max_steps * (NUM_BUCKETS + 2); i += gsize) {
+        if (i < (uint64_t)A.max_steps * NUM_BUCKETS) A.wcnt[i] = 0;
+        else if (i < (uint64_t)A.max_steps * (NUM_BUCKETS + 1)) A.mcnt[i - (uint64_t)A.max_steps * NUM_BUCKETS] = 0;
+        else A.ccnt[i - (uint64_t)A.max_steps * (NUM_BUCKETS + 1)] = 0;
+    }
+    if (!grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap)) return;
+    for (uint64_t v = gtid; v < n; v += gsize) {
+        const uint64_t c = REFINE ? v : a.memb[v];
+        atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)to_fx(a.strength[v], a.sK));
+        atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)to_fx(a.node_w[v], a.sW));
+        atomicAdd(&A.csize[c], 1u);
+    }
+    if (!grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap)) return;
+
+    const int lane = threadIdx.x % SWEEP_ACT_G;
+    const uint32_t gslot = threadIdx.x / SWEEP_ACT_G;
+    constexpr uint32_t NGA = SWEEP_BS / SWEEP_ACT_G;
+    uint32_t step = 0, it = 0;
+    bool ok = true;
+    for (; it < A.max_iters && ok; it++) {
+        const uint64_t seed = REFINE ? a.seed : mix64(A.level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
+        const uint32_t P = REFINE ? 1u : a.P;
+        for (uint32_t sub = 0; sub < P && ok; sub++, step++) {
+            // compact; clear the claims and join counts the previous step left behind
+            if (step > 0 && (capped || REFINE)) {
+                const unsigned int pm = ld_acquire_ctr(A.mcnt + step - 1);
+                for (uint64_t i = gtid; i < pm; i += gsize) {
+                    const uint32_t t = a.target[a.movers[i]];
+                    if (capped) A.claim[t] = NONE;
+                    if (REFINE) a.targeted_cnt[t] = 0;
+                }
+            }
+            sweep_compact<REFINE>(A, seed, sub, step);
+            if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
+
+            StepParams sp{seed, sub, it, step, A.mcnt + step, REFINE ? nullptr : A.mstep};
+            sweep_decide<REFINE>(A, sp, s_keys, s_vals, r_s, r_c, r_w, ds);
+            if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
+
+            const unsigned int nm = ld_acquire_ctr(A.mcnt + step);
+            if (capped) {
+                for (uint64_t i = gtid; i < nm; i += gsize) {
+                    const uint64_t v = a.movers[i];
+                    atomicMin(&A.claim[a.target[v]], (uint32_t)v);
+                }
+                if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
+            }
+            // apply; re-activation is fused with it where it cannot race with the moves
+            for (uint64_t i = (uint64_t)blockIdx.x * NGA + gslot; i < nm; i += (uint64_t)gridDim.x * NGA) {
+                const uint64_t v = a.movers[i];
+                const uint64_t t = a.target[v];
+                bool go;
+                if (REFINE) {
+                    const uint32_t tc = a.targeted_cnt[v];
+                    const uint32_t tr = a.tround[t];
+                    const uint64_t tt = a.target[t];
+                    const uint32_t cl = capped ? A.claim[t] : (uint32_t)v;
+                    go = ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == it) & (tt == v))) & (cl == (uint32_t)v);
+                } else {
+                    go = !capped || A.claim[t] == (uint32_t)v;
+                }
+                if (lane == 0) {
+                    if (!go) {
+                        a.active[v] = 1;  // blocked: look again next step
+                    } else {
+                        const uint64_t from = REFINE ? v : a.memb[v];
+                        A.memb[v] = (uint32_t)t;
+                        move_totals(A, v, from, t);
+                    }
+                }
+                if (REFINE) {
+                    const uint64_t mm = __ballot(go && lane == 0);
+                    if (mm && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)mm) - 1)
+                        atomicAdd(A.ccnt + step, (unsigned int)__popcll(mm));
+                }
+                if (go && (REFINE || !capped)) {
+                    // refinement: neighbours in the same working community (refine.rs); local
+                    // moving: neighbours whose community after this step differs from t
+                    // (local_move.rs:114-118) -- a mover's is its target, read from the stamp
+                    const uint32_t mv = REFINE ? a.constraint[v] : 0u;
+                    for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
+                        const uint32_t u = a.nbrs[e];
+                        if (u == v) continue;
+                        bool act;
+                        if (REFINE) {
+                            act = a.constraint[u] == mv;
+                        } else {
+                            const uint32_t cu = A.mstep[u] == step ? a.target[u] : a.memb[u];
+                            act = cu != (uint32_t)t;
+                        }
+                        if (act) a.active[u] = 1;
+                    }
+                }
+            }
+            if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
+            if (!REFINE && capped) {
+                // local moving with a cap: activate after all moves landed
+                for (uint64_t i = (uint64_t)blockIdx.x * NGA + gslot; i < nm; i += (uint64_t)gridDim.x * NGA) {
+                    const uint64_t v = a.movers[i];
+                    const uint32_t t = a.target[v];
+                    if (A.claim[t] != (uint32_t)v) continue;
+                    for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
+                        const uint32_t u = a.nbrs[e];
+                        if (u != v && a.memb[u] != t) a.active[u] = 1;
+                    }
+                }
+                if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
+            }
+        }
+        if (!ok) break;
+        // convergence, read identically by every workgroup after the barrier
+        unsigned int moved = 0;
+        if (REFINE) {
+            moved = ld_acquire_ctr(A.ccnt + step - 1);
+        } else {
+            for (uint32_t s2 = step - P; s2 < step; s2++) moved += ld_acquire_ctr(A.mcnt + s2);
+        }
+        if (moved == 0) {
+            it++;
+            break;
+        }
+    }
+    flush_decide_stats(ds, s_stats, a.stats);
+    if (gtid == 0) {
+        unsigned int total = 0;
+        for (uint32_t s2 = 0; s2 < step; s2++) total += ld_acquire_ctr((REFINE ? A.ccnt : A.mcnt) + s2);
+        A.ctl[0] = step;
+        A.ctl[2] = it;
+        A.ctl[3] = total;
+    }
+}
+
 // ---- profiling ---------------------------------------------------------------------------
 struct KernelTimer {
     std::vector<hipEvent_t> pool;
@@ -575,6 +1019,10 @@ struct LevelState {
     DevBuf<uint32_t> target, tround, movers, moved, targeted, claim;
     DevBuf<unsigned int> counters;           // [0] movers, [1] moved
     DevBuf<unsigned long long> stat;         // vertices, edges, candidates
+    // persistent sweeps: compacted step lists, mover stamps, step counters + barrier + control
+    DevBuf<uint32_t> work, mstep;
+    DevBuf<unsigned int> sweep;
+    DevBuf<unsigned char> sweep_ctl;
 };
 
 struct RunCtx {
@@ -666,6 +1114,123 @@ static void alloc_level(LevelState &ls, uint32_t n) {
     ls.claim.resize(n);
     ls.counters.resize(4);
     ls.stat.resize(4);
+    ls.work.resize(n);
+    ls.mstep.resize(n);
+    ls.sweep.resize((size_t)MAX_MOVE_ITERS * MOVE_SUBSWEEPS * (NUM_BUCKETS + 2) + 8);
+    ls.sweep_ctl.resize(sizeof(BarrierCtl));
+}
+
+// Sweep mode: 1 = one cooperative launch per phase (default), 0 = one launch per sub-sweep
+// kernel. SCGPU_PERSISTENT=0 in the environment selects 0 at start-up.
+static std::atomic<int> g_sweep_mode{-1};
+
+static bool persistent_sweeps() {
+    int m = g_sweep_mode.load(std::memory_order_relaxed);
+    if (m < 0) {
+        const char *e = std::getenv("SCGPU_PERSISTENT");
+        m = (e && e[0] == '0') ? 0 : 1;
+        g_sweep_mode.store(m, std::memory_order_relaxed);
+    }
+    return m == 1;
+}
+
+// Workgroups of a sweep launch: all must be resident at once (cooperative launch); small
+// levels use fewer, which makes each grid barrier cheaper.
+template <bool REFINE>
+static uint32_t sweep_grid(uint32_t n, uint64_t nnz) {
+    static thread_local int resident[64] = {0};
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    int &r = resident[dev & 63];
+    if (!r) {
+        int per = 0, cus = 0;
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep<REFINE>, SWEEP_BS, 0));
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        r = std::max(1, per) * std::max(1, cus);
+    }
+    static const long long forced = [] {
+        const char *e = std::getenv("SCGPU_SWEEP_BLOCKS");
+        return e ? std::atoll(e) : 0ll;
+    }();
+    const uint64_t by_work = std::max<uint64_t>((n + 63) / 64, (nnz + 2047) / 2048);
+    const uint32_t want = forced > 0 ? (uint32_t)forced : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8, by_work), 1u << 20);
+    return std::min<uint32_t>((uint32_t)r, want);
+}
+
+static bool sweep_trace_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("SCGPU_SWEEP_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
+// One cooperative launch for the whole phase; returns {steps, iterations, moves}.
+template <bool REFINE>
+static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, RunCtx &ctx, uint64_t level_seed,
+                      uint32_t *memb_w, uint64_t nnz, unsigned int out[4]) {
+    hipStream_t s = ctx.s;
+    const uint32_t n = d.n;
+    SweepArgs A{};
+    A.d = d;
+    A.lists = b.lists.get();
+    for (int i = 0; i <= NUM_BUCKETS; i++) A.bstart[i] = b.start[i];
+    A.work = ls.work.get();
+    A.max_iters = REFINE ? MAX_REFINE_ROUNDS : MAX_MOVE_ITERS;
+    A.max_steps = REFINE ? MAX_REFINE_ROUNDS : MAX_MOVE_ITERS * MOVE_SUBSWEEPS;
+    unsigned int *base = ls.sweep.get();
+    A.wcnt = base;
+    A.mcnt = base + (size_t)A.max_steps * NUM_BUCKETS;
+    A.ccnt = A.mcnt + A.max_steps;
+    A.ctl = A.ccnt + A.max_steps;
+    A.bc = reinterpret_cast<BarrierCtl *>(ls.sweep_ctl.get());
+    A.mstep = ls.mstep.get();
+    A.claim = ctx.cfg->has_max_community_weight ? ls.claim.get() : nullptr;
+    A.level_seed = level_seed;
+    A.memb = memb_w;
+    A.Kfx = ls.Kfx.get();
+    A.Wfx = ls.Wfx.get();
+    A.csize = ls.csize.get();
+    HIP_OK(hipMemsetAsync(ls.sweep_ctl.get(), 0, sizeof(BarrierCtl), s));
+    HIP_OK(hipMemsetAsync(A.ctl, 0, 4 * sizeof(unsigned int), s));
+    const uint32_t grid = sweep_grid<REFINE>(n, nnz);
+    constexpr uint32_t TRACE_CAP = 2048;
+    DevBuf<long long> trace;
+    if (sweep_trace_on()) {
+        trace.resize(TRACE_CAP + 1);
+        A.trace = trace.get() + 1;
+        A.trace_cap = TRACE_CAP;
+        hipLaunchKernelGGL(k_stamp_clock, dim3(1), dim3(1), 0, s, trace.get());
+        KERNEL_OK();
+    }
+    void *args[] = {&A};
+    hipEvent_t ev = nullptr;
+    if (ctx.timer.on && !REFINE) ctx.timer.begin(s, ev);
+    HIP_OK(hipLaunchCooperativeKernel((const void *)k_sweep<REFINE>, dim3(grid), dim3(SWEEP_BS), args, 0, s));
+    if (ctx.timer.on && !REFINE) ctx.timer.end(s, ev);
+    unsigned int ctl[4], berr = 0;
+    to_host(ctl, A.ctl, 4, s);
+    to_host(&berr, &A.bc->err, 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (ctx.timer.on) ctx.timer.harvest();
+    if (berr) throw ScError(SC_DEVICE_ERROR, "Leiden sweep: a grid barrier timed out");
+    for (int i = 0; i < 4; i++) out[i] = ctl[i];
+    if (sweep_trace_on()) {
+        // barriers: 2 prologue, then per step 3 (4 with a cap); print the phase times in us
+        const uint32_t per = A.claim ? 4u : 3u;
+        const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per);
+        std::vector<long long> h(nb + 1);
+        to_host(h.data(), trace.get(), nb + 1, s);
+        HIP_OK(hipStreamSynchronize(s));
+        std::string line = std::string(REFINE ? "[sweep refine]" : "[sweep move]") + " n=" + std::to_string(n) +
+                           " grid=" + std::to_string(grid) + " steps=" + std::to_string(ctl[0]) + " us:";
+        for (uint32_t i = 1; i <= nb; i++) {
+            char buf[32];
+            std::snprintf(buf, sizeof(buf), " %.1f", (double)(h[i] - h[i - 1]) / 100.0);
+            line += buf;
+        }
+        std::fprintf(stderr, "%s\n", line.c_str());
+    }
 }
 
 static void init_totals(const sc_graph &L, const uint32_t *memb, LevelState &ls, int sK, int sW, hipStream_t s) {
@@ -697,13 +1262,26 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
                        uint64_t level_seed, int sK, int sW, int sw) {
     uint32_t n = (uint32_t)L.n;
     hipStream_t s = ctx.s;
-    init_totals(L, memb, ls, sK, sW, s);
-    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
-    KERNEL_OK();
     HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
     DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
     a.memb = memb;
     a.P = MOVE_SUBSWEEPS;
+    if (b.start[6] < b.start[7]) {
+        a.gkeys = b.gkeys.get();
+        a.gvals = b.gvals.get();
+        a.goff = b.goff.get();
+    }
+    if (persistent_sweeps()) {
+        unsigned int out[4];
+        run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz, out);
+        ctx.st.move_iterations += out[2];
+        ctx.st.moves += out[3];
+        harvest_stats(ls, ctx, true);
+        return;
+    }
+    init_totals(L, memb, ls, sK, sW, s);
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
+    KERNEL_OK();
     ApplyArgs ap{};
     ap.movers = ls.movers.get();
     ap.n_movers = ls.counters.get();
@@ -738,7 +1316,8 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
             KERNEL_OK();
             hipLaunchKernelGGL((k_activate<16, false>), dim3(1024), dim3(256), 0, s, ls.movers.get(),
                                iter_moves.get() + sub, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)memb,
-                               (const uint32_t *)nullptr, ls.active.get());
+                               (const uint32_t *)nullptr, ls.active.get(), (const uint32_t *)ls.target.get(),
+                               (const uint32_t *)(capped ? ls.claim.get() : nullptr));
             KERNEL_OK();
         }
         unsigned int mv[MOVE_SUBSWEEPS];
@@ -759,12 +1338,6 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
                    RunCtx &ctx, uint64_t level_seed, int sK, int sW, int sw) {
     uint32_t n = (uint32_t)L.n;
     hipStream_t s = ctx.s;
-    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rmemb, n);
-    KERNEL_OK();
-    init_totals(L, rmemb, ls, sK, sW, s);
-    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
-    KERNEL_OK();
-    HIP_OK(hipMemsetAsync(ls.tround.get(), 0xFF, n * sizeof(uint32_t), s));
     DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
     a.memb = rmemb;
     a.constraint = memb;
@@ -773,6 +1346,24 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     double mean_w = L.edge_count > 0 ? L.total_weight / (double)L.edge_count : 1.0;
     a.theta = ctx.cfg->refine_randomness * mean_w;  // refine.rs:44-50
     a.seed = mix64(level_seed ^ 0xA0761D6478BD642Full);
+    if (b.start[6] < b.start[7]) {
+        a.gkeys = b.gkeys.get();
+        a.gvals = b.gvals.get();
+        a.goff = b.goff.get();
+    }
+    if (persistent_sweeps()) {
+        unsigned int out[4];
+        run_sweep<true>(a, b, ls, ctx, level_seed, rmemb, L.nnz, out);
+        ctx.st.refine_rounds += out[2];
+        harvest_stats(ls, ctx, false);
+        return;
+    }
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rmemb, n);
+    KERNEL_OK();
+    init_totals(L, rmemb, ls, sK, sW, s);
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
+    KERNEL_OK();
+    HIP_OK(hipMemsetAsync(ls.tround.get(), 0xFF, n * sizeof(uint32_t), s));
     ApplyArgs ap{};
     ap.movers = ls.movers.get();
     ap.n_movers = ls.counters.get();
@@ -1340,6 +1931,12 @@ sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *la
         fill_error(err, e);
         return e.code;
     }
+}
+
+sc_status sc_set_sweep_mode(int32_t mode) {
+    if (mode != 0 && mode != 1) return SC_INVALID_ARGUMENT;
+    g_sweep_mode.store(mode, std::memory_order_relaxed);
+    return SC_OK;
 }
 
 sc_status sc_leiden_last_stats(sc_leiden_stats *out) {

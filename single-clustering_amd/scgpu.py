@@ -94,6 +94,7 @@ SIGNATURES = {
     "sc_version": ([], C.c_char_p),
     "sc_shutdown": ([], None),
     "sc_set_profiling": ([C.c_int32], None),
+    "sc_set_sweep_mode": ([C.c_int32], C.c_int),
     "sc_leiden_config_default": ([], ScLeidenConfig),
     "sc_graph_from_csr_parts": ([u64p, C.c_uint64, u32p, f32p, C.c_uint64, f64p, C.POINTER(_GP),
                                  C.POINTER(ScError)], C.c_int),
@@ -198,6 +199,13 @@ def version():
 
 def set_profiling(on):
     _lib.sc_set_profiling(1 if on else 0)
+
+
+def set_sweep_mode(persistent):
+    """True: one cooperative launch per local-moving / refinement phase; False: one launch per
+    sub-sweep kernel. Labels are identical either way."""
+    if _lib.sc_set_sweep_mode(1 if persistent else 0) != 0:
+        raise ValueError("invalid sweep mode")
 
 
 @dataclass(frozen=True)

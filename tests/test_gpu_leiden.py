@@ -156,3 +156,24 @@ def test_planted_partition_small_config_vs_oracle():
     # disagree (ARI is stated, not bounded); quality and connectivity are the bar
     assert q_gpu >= q_cpu - 0.005 * abs(q_cpu)
     assert O.disconnected_community_count(og, c.labels()) == 0
+
+
+def test_persistent_sweeps_match_multi_launch():
+    """The cooperative one-launch-per-phase sweeps and the one-launch-per-kernel sweeps run the
+    same synchronous algorithm, so labels must be identical. 40k points give ~600 workgroups,
+    many per XCD, which exercises the grid barrier's cross-XCD visibility."""
+    x, _ = W.gaussian_mixture(40_000, 24, 20, 9.0, sd=1.0, alpha=2.0, seed=3)
+    g = scgpu.knn_graph(x, 10)
+    cfgs = [scgpu.LeidenConfig(seed=42), scgpu.LeidenConfig(seed=7, objective=scgpu.ObjectiveKind.Cpm(0.05)),
+            scgpu.LeidenConfig(seed=11, max_community_weight=400.0)]
+    try:
+        for cfg in cfgs:
+            scgpu.set_sweep_mode(True)
+            p = [scgpu.leiden(g, cfg).labels() for _ in range(3)]
+            scgpu.set_sweep_mode(False)
+            m = scgpu.leiden(g, cfg).labels()
+            for r in p[1:]:
+                assert np.array_equal(p[0], r), "persistent sweeps are not deterministic"
+            assert np.array_equal(p[0], m), f"persistent and multi-launch labels differ for {cfg}"
+    finally:
+        scgpu.set_sweep_mode(True)
