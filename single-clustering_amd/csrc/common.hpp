@@ -156,6 +156,23 @@ void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int en
 // Deterministic sum of n doubles (fixed two-level tree).
 double device_sum_f64(const double *in, uint64_t n, hipStream_t s);
 double device_max_f64(const double *in, uint64_t n, hipStream_t s);
+// The same four fixed-shape reductions as device_sum_f64 / device_max_f64 (identical
+// association order) in one launch pair and one synchronisation: out[i] = sum (or max, when
+// is_max[i]) of in[i][0..n). Null inputs give 0.
+void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_t n, double out[4], hipStream_t s);
+
+// Exponent of the lowest set bit of x (x = m * 2^q with m odd), or a large value for 0. A sum
+// of terms that are all multiples of 2^q, whose absolute values add up to less than 2^(53+q),
+// has every partial sum exactly representable in f64: any association order then yields the
+// sequential result bit for bit.
+__device__ __forceinline__ int f64_quantum(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x) & 0x7FFFFFFFFFFFFFFFull;
+    if (b == 0ull) return 4096;
+    const int ex = (int)(b >> 52);
+    const unsigned long long frac = b & 0xFFFFFFFFFFFFFull;
+    const unsigned long long mant = ex ? (frac | (1ull << 52)) : frac;
+    return (ex ? ex : 1) - 1075 + __builtin_ctzll(mant);
+}
 
 inline int bits_for(uint64_t x) {
     int b = 0;

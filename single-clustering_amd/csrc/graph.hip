@@ -4,6 +4,8 @@
 // finish :262-351, check_degree_sum :712-723, validate_symmetry :627-646) as HIP kernels.
 #include "common.hpp"
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 
@@ -161,30 +163,84 @@ __global__ void k_row_clean(const uint64_t *ptrs, const uint32_t *nbrs, const fl
     }
 }
 
-// Row-ordered sums: strength in ascending-neighbour order (self-loop twice), the self-loop
-// weight, and this row's share (u >= v) of total weight and edge count.
-__global__ void k_row_finish(const uint64_t *ptrs, const uint32_t *nbrs, const float *wts,
-                             uint64_t n, double *strength, double *self_loop, double *row_total,
-                             unsigned long long *edge_count, unsigned int *any_loop) {
+// Row statistics of a sorted CSR, G lanes per row: strength (self-loop counted twice), the
+// self-loop weight, the row's share (u >= v) of total weight and edge count, the largest degree
+// and whether any loop exists. The reference sums each row sequentially in ascending-neighbour
+// order in f64; the lanes' tree sum equals that whenever the exactness bound of f64_quantum
+// holds (f32 weights of similar magnitude always meet it), otherwise lane 0 redoes the row
+// sequentially.
+template <int G>
+__global__ void __launch_bounds__(256) k_row_stats(const uint64_t *ptrs, const uint32_t *nbrs, const float *wts,
+                                                   uint64_t n, double *strength, double *self_loop,
+                                                   double *row_total, unsigned long long *edge_count,
+                                                   unsigned int *flags) {
+    constexpr int NG = 256 / G;
+    const int lane = threadIdx.x % G;
     unsigned long long local_edges = 0;
-    bool loop_seen = false;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
-         v += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t lo = ptrs[v], hi = ptrs[v + 1];
-        double st = 0.0, tot = 0.0, loop = 0.0;
-        for (uint64_t e = lo; e < hi; e++) {
-            uint64_t u = nbrs[e];
-            double w = (double)wts[e];
-            if (u == v) { st += 2.0 * w; loop = w; loop_seen = true; }
-            else st += w;
-            if (v <= u) { tot += w; local_edges++; }
+    unsigned int max_deg = 0, loop_seen = 0;
+    for (uint64_t v0 = (uint64_t)blockIdx.x * NG; v0 < n; v0 += (uint64_t)gridDim.x * NG) {
+        const uint64_t v = v0 + threadIdx.x / G;
+        uint64_t lo = 0, hi = 0;
+        if (v < n) {
+            lo = ptrs[v];
+            hi = ptrs[v + 1];
         }
-        strength[v] = st;
-        self_loop[v] = loop;
-        row_total[v] = tot;
+        double st = 0.0, tot = 0.0, loop = 0.0, sabs = 0.0;
+        int q = 4096;
+        for (uint64_t e = lo + lane; e < hi; e += G) {
+            const uint64_t u = nbrs[e];
+            const double w = (double)wts[e];
+            q = min(q, f64_quantum(w));
+            sabs += 2.0 * fabs(w);
+            if (u == v) {
+                st += 2.0 * w;
+                loop = w;
+                loop_seen = 1u;
+            } else {
+                st += w;
+            }
+            if (v <= u) {
+                tot += w;
+                local_edges++;
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            st += __shfl_xor(st, off, G);
+            tot += __shfl_xor(tot, off, G);
+            loop += __shfl_xor(loop, off, G);
+            sabs += __shfl_xor(sabs, off, G);
+            q = min(q, __shfl_xor(q, off, G));
+        }
+        if (v < n && lane == 0) {
+            if (!(sabs < ldexp(1.0, min(53 + q, 1000)))) {
+                st = 0.0;
+                tot = 0.0;
+                for (uint64_t e = lo; e < hi; e++) {
+                    const uint64_t u = nbrs[e];
+                    const double w = (double)wts[e];
+                    st += u == v ? 2.0 * w : w;
+                    if (v <= u) tot += w;
+                }
+            }
+            strength[v] = st;
+            self_loop[v] = loop;
+            row_total[v] = tot;
+            max_deg = max(max_deg, (unsigned int)min<uint64_t>(hi - lo, 0xFFFFFFFFull));
+        }
     }
-    if (local_edges) atomicAdd(edge_count, local_edges);
-    if (loop_seen) atomicOr(any_loop, 1u);
+    // one atomic per wave for each counter
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        local_edges += __shfl_xor(local_edges, off);
+        max_deg = max(max_deg, __shfl_xor(max_deg, off));
+        loop_seen |= __shfl_xor(loop_seen, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (local_edges) atomicAdd(edge_count, local_edges);
+        if (loop_seen) atomicOr(&flags[0], 1u);
+        if (max_deg) atomicMax(&flags[2], max_deg);
+    }
 }
 
 void graph_finish_sorted(sc_graph &g, hipStream_t s) {
@@ -192,32 +248,32 @@ void graph_finish_sorted(sc_graph &g, hipStream_t s) {
     g.strength.resize(n);
     g.self_loop.resize(n);
     DevBuf<double> row_total(n);
-    DevBuf<unsigned long long> cnt(1);
-    // flags: [0] any self-loop, [1] dirty row (unused here), [2] max degree
-    DevBuf<unsigned int> flags(3);
-    HIP_OK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long), s));
-    HIP_OK(hipMemsetAsync(flags.get(), 0, 3 * sizeof(unsigned int), s));
+    // [0..1] edge count, then flags: [2] any self-loop, [3] unused, [4] max degree
+    DevBuf<unsigned int> cnt(6);
+    HIP_OK(hipMemsetAsync(cnt.get(), 0, 6 * sizeof(unsigned int), s));
+    unsigned long long *ec_d = reinterpret_cast<unsigned long long *>(cnt.get());
+    unsigned int *flags = cnt.get() + 2;
     if (n) {
-        hipLaunchKernelGGL(k_row_finish, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, g.ptrs.get(),
-                           g.nbrs.get(), g.wts.get(), n, g.strength.get(), g.self_loop.get(),
-                           row_total.get(), cnt.get(), flags.get());
-        KERNEL_OK();
-        hipLaunchKernelGGL(k_row_clean, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, g.ptrs.get(),
-                           g.nbrs.get(), g.wts.get(), n, flags.get() + 1, flags.get() + 2);
+        hipLaunchKernelGGL((k_row_stats<16>), dim3(grid_for(n, 16, 8192)), dim3(256), 0, s, g.ptrs.get(),
+                           g.nbrs.get(), g.wts.get(), n, g.strength.get(), g.self_loop.get(), row_total.get(), ec_d,
+                           flags);
         KERNEL_OK();
     }
+    unsigned int h[6] = {0, 0, 0, 0, 0, 0};
+    to_host(h, cnt.get(), 6, s);
+    const double *in[4] = {row_total.get(), g.strength.get(), g.strength.get(), g.node_w.get()};
+    const bool is_max[4] = {false, false, true, false};
+    double r[4];
+    device_reduce4_f64(in, is_max, n, r, s);  // synchronises the stream
     unsigned long long ec = 0;
-    unsigned int fl[3] = {0, 0, 0};
-    to_host(&ec, cnt.get(), 1, s);
-    to_host(fl, flags.get(), 3, s);
-    HIP_OK(hipStreamSynchronize(s));
+    std::memcpy(&ec, h, sizeof(ec));
     g.edge_count = ec;
-    g.any_self_loops = fl[0] != 0;
-    g.max_degree = fl[2];
-    g.total_weight = device_sum_f64(row_total.get(), n, s);
-    double deg_sum = device_sum_f64(g.strength.get(), n, s);
-    g.max_strength = n ? device_max_f64(g.strength.get(), n, s) : 0.0;
-    g.total_node_weight = device_sum_f64(g.node_w.get(), n, s);
+    g.any_self_loops = h[2] != 0;
+    g.max_degree = h[4];
+    g.total_weight = r[0];
+    const double deg_sum = r[1];
+    g.max_strength = n ? r[2] : 0.0;
+    g.total_node_weight = r[3];
     double expected = 2.0 * g.total_weight;
     double tol = 1e-6 * std::max(std::fabs(expected), 1.0);
     if (std::fabs(deg_sum - expected) > tol) {

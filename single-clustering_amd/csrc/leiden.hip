@@ -1453,17 +1453,54 @@ __global__ void k_group_starts(const uint32_t *sorted_groups, uint32_t n, uint32
 }
 
 // Coarse node weight and summed self-loop weight of each group, members in ascending order.
-__global__ void k_group_sums(const uint64_t *starts, const uint32_t *members, uint32_t ng, const double *node_w,
-                             const double *self_loop, double *cnw, double *cloops) {
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
-        double a = 0.0, l = 0.0;
-        for (uint64_t j = starts[g]; j < starts[g + 1]; j++) {
-            uint32_t v = members[j];
-            a += node_w[v];
-            l += self_loop[v];
+// Per group, in ascending member order (csr_network.rs:520-523): node weight and self-loop
+// sums. 16 lanes per group; the lanes' tree sum is the sequential sum whenever the f64_quantum
+// bound holds, otherwise lane 0 sums sequentially.
+__global__ void __launch_bounds__(256) k_group_sums(const uint64_t *starts, const uint32_t *members, uint32_t ng,
+                                                    const double *node_w, const double *self_loop, double *cnw,
+                                                    double *cloops) {
+    constexpr int G = 16, NG = 256 / G;
+    const int lane = threadIdx.x % G;
+    for (uint32_t g0 = blockIdx.x * NG; g0 < ng; g0 += gridDim.x * NG) {
+        const uint32_t g = g0 + threadIdx.x / G;
+        uint64_t b = 0, e = 0;
+        if (g < ng) {
+            b = starts[g];
+            e = starts[g + 1];
         }
-        cnw[g] = a;
-        cloops[g] = l;
+        double a = 0.0, l = 0.0, aa = 0.0, la = 0.0;
+        int qa = 4096, ql = 4096;
+        for (uint64_t j = b + lane; j < e; j += G) {
+            const uint32_t v = members[j];
+            const double x = node_w[v], y = self_loop[v];
+            a += x;
+            l += y;
+            aa += fabs(x);
+            la += fabs(y);
+            qa = min(qa, f64_quantum(x));
+            ql = min(ql, f64_quantum(y));
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, G);
+            l += __shfl_xor(l, off, G);
+            aa += __shfl_xor(aa, off, G);
+            la += __shfl_xor(la, off, G);
+            qa = min(qa, __shfl_xor(qa, off, G));
+            ql = min(ql, __shfl_xor(ql, off, G));
+        }
+        if (g < ng && lane == 0) {
+            if (!(aa < ldexp(1.0, min(53 + qa, 1000)))) {
+                a = 0.0;
+                for (uint64_t j = b; j < e; j++) a += node_w[members[j]];
+            }
+            if (!(la < ldexp(1.0, min(53 + ql, 1000)))) {
+                l = 0.0;
+                for (uint64_t j = b; j < e; j++) l += self_loop[members[j]];
+            }
+            cnw[g] = a;
+            cloops[g] = l;
+        }
     }
 }
 
@@ -1501,7 +1538,9 @@ __global__ void __launch_bounds__(256) k_emit_fine(const uint32_t *members, cons
             vals[out] = wts[e];
         }
     }
-    if (local) atomicAdd(n_valid, local);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(n_valid, local);
 }
 
 __global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
@@ -1510,11 +1549,9 @@ __global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
 }
 
 // Coarse pair sums. The reference adds each pair's fine weights sequentially in f64 in the
-// sorted (member, row) order (csr_network.rs:548-564). When every term is a multiple of the
-// smallest f32 ulp m among them and the total is below 2^52 * m, every partial sum is exactly
-// representable in f64, so any association order yields the sequential result bit for bit;
-// long segments use a wave-parallel tree under that check and keep the sequential chain
-// otherwise. Short segments are summed sequentially by one thread.
+// sorted (member, row) order (csr_network.rs:548-564). Long segments use a wave-parallel tree,
+// which equals the sequential chain whenever the f64_quantum bound holds, and keep the
+// sequential chain otherwise. Short segments are summed sequentially by one thread.
 constexpr uint32_t PAIR_SHORT = 64;
 
 __device__ __forceinline__ void emit_pair(uint64_t key, double acc, uint32_t r, int hbits, const double *cloops,
@@ -1559,21 +1596,21 @@ __global__ void __launch_bounds__(256) k_pair_sums_long(const uint64_t *keys, co
     for (unsigned int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < cnt; q += gridDim.x * 4) {
         const uint32_t r = long_list[q];
         const uint64_t b = start[r], e = start[r + 1];
-        double acc = 0.0;
-        int min_e = 1000;
+        double acc = 0.0, sabs = 0.0;
+        int qz = 4096;
         for (uint64_t j = b + lane; j < e; j += 64) {
-            float w = vals[j];
-            acc += (double)w;
-            int ex;
-            (void)frexpf(w, &ex);
-            min_e = min(min_e, w != 0.0f ? max(ex - 24, -149) : 1000);
+            const double w = (double)vals[j];
+            acc += w;
+            sabs += fabs(w);
+            qz = min(qz, f64_quantum(w));
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             acc += __shfl_xor(acc, off, 64);
-            min_e = min(min_e, __shfl_xor(min_e, off, 64));
+            sabs += __shfl_xor(sabs, off, 64);
+            qz = min(qz, __shfl_xor(qz, off, 64));
         }
-        const bool exact = acc < ldexp(1.0, 52 + min_e);
+        const bool exact = sabs < ldexp(1.0, min(53 + qz, 1000));
         if (lane == 0) {
             if (!exact) {
                 acc = 0.0;
@@ -1601,13 +1638,25 @@ __global__ void k_pair_emit(const uint32_t *pa, const uint32_t *pb, const float 
 }
 
 __global__ void k_csr_from_keys(const uint64_t *keys, const float *vals, uint64_t n, int cbits, uint32_t *nbrs,
-                                float *wts, unsigned long long *row_count) {
+                                float *wts) {
     const uint64_t cmask = (1ull << cbits) - 1;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t k = keys[i];
         nbrs[i] = (uint32_t)(k & cmask);
         wts[i] = vals[i];
-        atomicAdd(&row_count[k >> cbits], 1ull);
+    }
+}
+
+// Row offsets of key-sorted entries: ptrs[r] = first index whose row (key >> cbits) is >= r.
+__global__ void k_ptrs_from_keys(const uint64_t *keys, uint64_t n, int cbits, uint32_t rows, uint64_t *ptrs) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r <= rows; r += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if ((keys[mid] >> cbits) < r) lo = mid + 1;
+            else hi = mid;
+        }
+        ptrs[r] = lo;
     }
 }
 
@@ -1625,7 +1674,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     KERNEL_OK();
     out->node_w.resize(ng);
     DevBuf<double> cloops(ng);
-    hipLaunchKernelGGL(k_group_sums, dim3(grid_for(ng, 256, 4096)), dim3(256), 0, s, starts.get(), members.get(), ng,
+    hipLaunchKernelGGL(k_group_sums, dim3(grid_for(ng, 16, 4096)), dim3(256), 0, s, starts.get(), members.get(), ng,
                        L.node_w.get(), L.self_loop.get(), out->node_w.get(), cloops.get());
     KERNEL_OK();
     // fine entries in member order
@@ -1704,15 +1753,15 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     out->nnz = total_entries;
     out->nbrs.resize(total_entries);
     out->wts.resize(total_entries);
-    DevBuf<unsigned long long> rc(ng + 1);
-    HIP_OK(hipMemsetAsync(rc.get(), 0, (ng + 1) * sizeof(unsigned long long), s));
     if (total_entries) {
         hipLaunchKernelGGL(k_csr_from_keys, dim3(grid_for(total_entries, 256, 8192)), dim3(256), 0, s, ckeys.get(),
-                           cvals.get(), total_entries, cbits, out->nbrs.get(), out->wts.get(), rc.get());
+                           cvals.get(), total_entries, cbits, out->nbrs.get(), out->wts.get());
         KERNEL_OK();
     }
     out->ptrs.resize(ng + 1);
-    exclusive_scan_u64(reinterpret_cast<uint64_t *>(rc.get()), out->ptrs.get(), ng + 1, s);
+    hipLaunchKernelGGL(k_ptrs_from_keys, dim3(grid_for((uint64_t)ng + 1, 256, 4096)), dim3(256), 0, s, ckeys.get(),
+                       total_entries, cbits, ng, out->ptrs.get());
+    KERNEL_OK();
     graph_finish_sorted(*out, s);
     return out;
 }

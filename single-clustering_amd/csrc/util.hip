@@ -193,6 +193,58 @@ static double reduce_f64(const double *in, uint64_t n, hipStream_t s) {
     return r;
 }
 
+struct Reduce4Args {
+    const double *in[4];
+    int is_max[4];
+};
+
+// blockIdx.y selects the array; per array the same strided partials and tree as k_reduce_stage.
+__global__ void __launch_bounds__(RED_THREADS) k_reduce4_stage(Reduce4Args a, uint64_t n, double *out, int final) {
+    __shared__ double sh[RED_THREADS];
+    const int y = blockIdx.y;
+    const bool mx = a.is_max[y] != 0;
+    const double *in = final ? out + (size_t)y * RED_BLOCKS : a.in[y];
+    const uint64_t len = final ? (uint64_t)RED_BLOCKS : n;
+    double acc = mx ? -INFINITY : 0.0;
+    if (in) {
+        for (uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < len;
+             i += (uint64_t)gridDim.x * RED_THREADS) {
+            double x = in[i];
+            acc = mx ? fmax(acc, x) : acc + x;
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double o = sh[threadIdx.x + w];
+            sh[threadIdx.x] = mx ? fmax(sh[threadIdx.x], o) : sh[threadIdx.x] + o;
+        }
+        __syncthreads();
+    }
+    // partials at out[y * RED_BLOCKS + block], results at out[4 * RED_BLOCKS + y]
+    if (threadIdx.x == 0) out[final ? (size_t)4 * RED_BLOCKS + y : (size_t)y * RED_BLOCKS + blockIdx.x] = sh[0];
+}
+
+void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_t n, double out[4], hipStream_t s) {
+    if (n == 0) {
+        for (int i = 0; i < 4; i++) out[i] = is_max[i] ? -INFINITY : 0.0;
+        return;
+    }
+    Reduce4Args a;
+    for (int i = 0; i < 4; i++) {
+        a.in[i] = in[i];
+        a.is_max[i] = is_max[i] ? 1 : 0;
+    }
+    DevBuf<double> part(4 * RED_BLOCKS + 4);
+    hipLaunchKernelGGL(k_reduce4_stage, dim3(RED_BLOCKS, 4), dim3(RED_THREADS), 0, s, a, n, part.get(), 0);
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_reduce4_stage, dim3(1, 4), dim3(RED_THREADS), 0, s, a, n, part.get(), 1);
+    KERNEL_OK();
+    HIP_OK(hipMemcpyAsync(out, part.get() + (size_t)4 * RED_BLOCKS, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
 double device_sum_f64(const double *in, uint64_t n, hipStream_t s) { return reduce_f64<false>(in, n, s); }
 double device_max_f64(const double *in, uint64_t n, hipStream_t s) { return reduce_f64<true>(in, n, s); }
 
