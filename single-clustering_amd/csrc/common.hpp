@@ -161,6 +161,29 @@ double device_max_f64(const double *in, uint64_t n, hipStream_t s);
 // is_max[i]) of in[i][0..n). Null inputs give 0.
 void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_t n, double out[4], hipStream_t s);
 
+// Workgroup reductions (blockDim.x a multiple of 64, at most 1024); every thread must call them.
+// The result is valid in thread 0.
+template <typename T, bool MAX>
+__device__ __forceinline__ T block_reduce(T v) {
+    __shared__ T sh[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const T o = __shfl_xor(v, off);
+        v = MAX ? (o > v ? o : v) : v + o;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / 64; w++) v = MAX ? (sh[w] > v ? sh[w] : v) : v + sh[w];
+    }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T block_sum(T v) { return block_reduce<T, false>(v); }
+template <typename T>
+__device__ __forceinline__ T block_max(T v) { return block_reduce<T, true>(v); }
+
 // Exponent of the lowest set bit of x (x = m * 2^q with m odd), or a large value for 0. A sum
 // of terms that are all multiples of 2^q, whose absolute values add up to less than 2^(53+q),
 // has every partial sum exactly representable in f64: any association order then yields the

@@ -150,6 +150,7 @@ static void validate_csr_device(const uint64_t *d_ptrs, uint64_t n, const uint32
 // Per row: is it strictly increasing with no zero weight? Rows that are need no sort/merge.
 __global__ void k_row_clean(const uint64_t *ptrs, const uint32_t *nbrs, const float *wts,
                             uint64_t n, unsigned int *dirty, unsigned int *max_deg) {
+    unsigned int md = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t lo = ptrs[v], hi = ptrs[v + 1];
@@ -159,8 +160,10 @@ __global__ void k_row_clean(const uint64_t *ptrs, const uint32_t *nbrs, const fl
         }
         if (bad) atomicOr(dirty, 1u);
         uint64_t d = hi - lo;
-        atomicMax(max_deg, (unsigned int)(d > 0xFFFFFFFFull ? 0xFFFFFFFFu : d));
+        md = max(md, (unsigned int)(d > 0xFFFFFFFFull ? 0xFFFFFFFFu : d));
     }
+    md = block_max(md);
+    if (threadIdx.x == 0 && md) atomicMax(max_deg, md);
 }
 
 // Row statistics of a sorted CSR, G lanes per row: strength (self-loop counted twice), the
@@ -229,14 +232,11 @@ __global__ void __launch_bounds__(256) k_row_stats(const uint64_t *ptrs, const u
             max_deg = max(max_deg, (unsigned int)min<uint64_t>(hi - lo, 0xFFFFFFFFull));
         }
     }
-    // one atomic per wave for each counter
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        local_edges += __shfl_xor(local_edges, off);
-        max_deg = max(max_deg, __shfl_xor(max_deg, off));
-        loop_seen |= __shfl_xor(loop_seen, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
+    // one atomic per workgroup for each counter (same-address atomics serialize)
+    local_edges = block_sum(local_edges);
+    max_deg = block_max(max_deg);
+    loop_seen = block_max(loop_seen);
+    if (threadIdx.x == 0) {
         if (local_edges) atomicAdd(edge_count, local_edges);
         if (loop_seen) atomicOr(&flags[0], 1u);
         if (max_deg) atomicMax(&flags[2], max_deg);
@@ -254,7 +254,7 @@ void graph_finish_sorted(sc_graph &g, hipStream_t s) {
     unsigned long long *ec_d = reinterpret_cast<unsigned long long *>(cnt.get());
     unsigned int *flags = cnt.get() + 2;
     if (n) {
-        hipLaunchKernelGGL((k_row_stats<16>), dim3(grid_for(n, 16, 8192)), dim3(256), 0, s, g.ptrs.get(),
+        hipLaunchKernelGGL((k_row_stats<16>), dim3(grid_for(n, 16, 1024)), dim3(256), 0, s, g.ptrs.get(),
                            g.nbrs.get(), g.wts.get(), n, g.strength.get(), g.self_loop.get(), row_total.get(), ec_d,
                            flags);
         KERNEL_OK();
@@ -337,7 +337,7 @@ void graph_finish_unsorted(sc_graph &g, hipStream_t s) {
     DevBuf<unsigned int> flags(2);
     HIP_OK(hipMemsetAsync(flags.get(), 0, 2 * sizeof(unsigned int), s));
     if (n) {
-        hipLaunchKernelGGL(k_row_clean, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, g.ptrs.get(),
+        hipLaunchKernelGGL(k_row_clean, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, g.ptrs.get(),
                            g.nbrs.get(), g.wts.get(), n, flags.get(), flags.get() + 1);
         KERNEL_OK();
     }

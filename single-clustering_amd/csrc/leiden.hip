@@ -1538,9 +1538,8 @@ __global__ void __launch_bounds__(256) k_emit_fine(const uint32_t *members, cons
             vals[out] = wts[e];
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(n_valid, local);
+    local = block_sum(local);
+    if (threadIdx.x == 0 && local) atomicAdd(n_valid, local);
 }
 
 __global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
@@ -1692,7 +1691,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     DevBuf<unsigned long long> nvalid(1);
     HIP_OK(hipMemsetAsync(nvalid.get(), 0, sizeof(unsigned long long), s));
     if (nnz) {
-        hipLaunchKernelGGL((k_emit_fine<16>), dim3(grid_for(n, 16, 8192)), dim3(256), 0, s, members.get(), sg.get(), n,
+        hipLaunchKernelGGL((k_emit_fine<16>), dim3(grid_for(n, 16, 2048)), dim3(256), 0, s, members.get(), sg.get(), n,
                            moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, hbits, sentinel, keys.get(),
                            vals.get(), nvalid.get());
         KERNEL_OK();

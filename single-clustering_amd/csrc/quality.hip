@@ -24,7 +24,8 @@ __global__ void k_max_label(const uint64_t *labels, uint64_t n, unsigned long lo
         uint64_t l = labels[v];
         if (l > local) local = l;
     }
-    atomicMax(max_label, local);
+    local = block_max(local);
+    if (threadIdx.x == 0) atomicMax(max_label, local);
 }
 
 // G lanes per vertex stride its row; the lane partial sums combine in a fixed shuffle tree.
@@ -34,6 +35,14 @@ __global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, ui
                                                       const double *node_w, int s_in, int sK, int sW,
                                                       long long *in_fx, long long *K_fx, long long *W_fx) {
     constexpr int NG = 256 / G;
+    constexpr uint32_t TAB = 1024;  // per-workgroup label table (few labels recur within a block)
+    __shared__ unsigned long long t_key[TAB];
+    __shared__ long long t_in[TAB], t_K[TAB], t_W[TAB];
+    for (uint32_t i = threadIdx.x; i < TAB; i += blockDim.x) {
+        t_key[i] = ~0ull;
+        t_in[i] = t_K[i] = t_W[i] = 0;
+    }
+    __syncthreads();
     const int lane = threadIdx.x % G;
     for (uint64_t v = blockIdx.x * (uint64_t)NG + threadIdx.x / G; v < n; v += (uint64_t)gridDim.x * NG) {
         const uint64_t c = labels[v];
@@ -45,10 +54,34 @@ __global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, ui
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
         if (lane == 0) {
-            if (acc != 0.0) atomicAdd((unsigned long long *)&in_fx[c], (unsigned long long)to_fx(acc, s_in));
-            atomicAdd((unsigned long long *)&K_fx[c], (unsigned long long)to_fx(strength[v], sK));
-            atomicAdd((unsigned long long *)&W_fx[c], (unsigned long long)to_fx(node_w[v], sW));
+            const long long fi = acc != 0.0 ? to_fx(acc, s_in) : 0, fk = to_fx(strength[v], sK),
+                            fw = to_fx(node_w[v], sW);
+            // integer sums are order independent: pre-add in LDS, one global atomic per label
+            uint32_t h = hash32((uint32_t)c ^ (uint32_t)(c >> 32)) & (TAB - 1);
+            bool done = false;
+            for (uint32_t probe = 0; probe < 32 && !done; probe++, h = (h + 1) & (TAB - 1)) {
+                const unsigned long long prev = atomicCAS(&t_key[h], ~0ull, (unsigned long long)c);
+                if (prev == ~0ull || prev == c) {
+                    atomicAdd((unsigned long long *)&t_in[h], (unsigned long long)fi);
+                    atomicAdd((unsigned long long *)&t_K[h], (unsigned long long)fk);
+                    atomicAdd((unsigned long long *)&t_W[h], (unsigned long long)fw);
+                    done = true;
+                }
+            }
+            if (!done) {
+                atomicAdd((unsigned long long *)&in_fx[c], (unsigned long long)fi);
+                atomicAdd((unsigned long long *)&K_fx[c], (unsigned long long)fk);
+                atomicAdd((unsigned long long *)&W_fx[c], (unsigned long long)fw);
+            }
         }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < TAB; i += blockDim.x) {
+        const unsigned long long c = t_key[i];
+        if (c == ~0ull) continue;
+        if (t_in[i]) atomicAdd((unsigned long long *)&in_fx[c], (unsigned long long)t_in[i]);
+        atomicAdd((unsigned long long *)&K_fx[c], (unsigned long long)t_K[i]);
+        atomicAdd((unsigned long long *)&W_fx[c], (unsigned long long)t_W[i]);
     }
 }
 
@@ -92,7 +125,7 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
     const int sW = fixed_scale_for(g.total_node_weight + 1e-300);
     DevBuf<long long> acc(3 * nl);
     HIP_OK(hipMemsetAsync(acc.get(), 0, 3 * nl * sizeof(long long), s));
-    hipLaunchKernelGGL((k_vertex_terms<8>), dim3(grid_for(n, 32, 16384)), dim3(256), 0, s, d_labels, n, g.ptrs.get(),
+    hipLaunchKernelGGL((k_vertex_terms<8>), dim3(grid_for(n, 32, 1024)), dim3(256), 0, s, d_labels, n, g.ptrs.get(),
                        g.nbrs.get(), g.wts.get(), g.strength.get(), g.node_w.get(), s_in, sK, sW, acc.get(),
                        acc.get() + nl, acc.get() + 2 * nl);
     KERNEL_OK();
