@@ -44,7 +44,7 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
 constexpr int NUM_BUCKETS = 7;
-constexpr int MOVE_SUBSWEEPS = 4;
+constexpr int MOVE_SUBSWEEPS = 2;
 constexpr int MAX_MOVE_ITERS = 100;
 constexpr int MAX_REFINE_ROUNDS = 64;
 
