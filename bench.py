@@ -45,6 +45,22 @@ def cpu_info():
     return model, os.cpu_count()
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated
+    against a 1 GiB copy), from the committed profile summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload.replace('knn_leiden_100k_d50_k15', 'knn_leiden_100k')}_hbm_traffic.md")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("TRAFFIC "):
+                    _, name, val = line.split()
+                    out[name] = float(val)
+    except OSError:
+        return None, None
+    return out, os.path.relpath(path, ROOT)
+
+
 def make_workload(name):
     import workloads as W
 
@@ -137,7 +153,7 @@ def main():
     move_bytes = float(np.mean([s["move_kernel_bytes"] for s in lstats]))
     launches = float(np.mean([s["move_kernel_launches"] for s in lstats]))
     leiden_roof = {
-        "kernel": "k_decide (local-move decide, all degree buckets)",
+        "kernel": "k_sweep<local moving> (persistent; decide/apply/activate over all degree buckets)",
         "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
         "achieved": (move_bytes / (move_ms / 1e3) / 1e9) if move_ms > 0 else None,
         "kernel_ms_per_step": move_ms, "launches_per_step": launches,
@@ -153,6 +169,15 @@ def main():
                     "peak": BF16_PEAK_TFLOPS, "achieved": flops / (tile_ms / 1e3) / 1e12,
                     "kernel_ms_per_step": tile_ms, "traffic": None}
         knn_roof["frac"] = knn_roof["achieved"] / BF16_PEAK_TFLOPS
+    traffic, traffic_src = measured_traffic(wl["workload"])
+    if traffic:
+        if knn_roof and "k_knn_tile" in traffic and args.precision == "bf16":
+            knn_roof["traffic"] = traffic["k_knn_tile"]
+            knn_roof["traffic_source"] = traffic_src
+        if "k_sweep_move_per_step" in traffic:
+            leiden_roof["traffic_per_step"] = traffic["k_sweep_move_per_step"]
+            leiden_roof["traffic"] = traffic["k_sweep_move_per_step"] / max(launches, 1.0)
+            leiden_roof["traffic_source"] = traffic_src
     dominant = leiden_roof
     if knn_roof and knn_roof["kernel_ms_per_step"] > leiden_roof["kernel_ms_per_step"]:
         dominant = knn_roof
