@@ -72,6 +72,12 @@ def make_workload(name):
         x, truth = W.gaussian_mixture(4_000_000, 30, 200, 9.0, sd=1.0, alpha=1.0, seed=42)
         return {"kind": "knn", "x": x, "k": 7, "truth": truth, "objective": ("rb", 1.0),
                 "workload": "knn_leiden_4M_d30_k7", "n": 4_000_000, "d": 30}
+    if name == "batch":
+        sizes = W.batch_graph_sizes(64, 2000, 200000, seed=42)
+        parts = [W.lfr_like(int(n), 1000 + i) for i, n in enumerate(sizes)]
+        return {"kind": "batch", "parts": parts, "resolutions": np.linspace(0.25, 2.0, 8),
+                "objective": ("rb", 1.0), "workload": "batch_64_lfr_graphs_x_8_resolutions",
+                "n": int(sizes.sum())}
     if name == "small":
         ptrs, nbrs, wts, truth = W.planted_partition(seed=42)
         return {"kind": "graph", "csr": (ptrs, nbrs, wts), "truth": truth, "objective": ("rb", 1.0),
@@ -79,12 +85,74 @@ def make_workload(name):
     raise SystemExit(f"unknown config {name}")
 
 
+def bench_batch(args, wl, scgpu):
+    """Batched Leiden: 64 graphs x 8 Rb resolutions in one sc_leiden_batch call per step."""
+    graphs = [scgpu.CSRNetwork.from_csr_parts(*p[:3]) for p in wl["parts"]]
+    res = wl["resolutions"]
+    cfg = scgpu.LeidenConfig(seed=42)
+    m_total = float(sum(g.edge_count() for g in graphs)) * len(res)
+    for i in range(args.warmup):
+        scgpu.leiden_batch(graphs, res, cfg)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    times = []
+    jobs = None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        jobs = scgpu.leiden_batch(graphs, res, cfg)
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms for {len(jobs)} jobs")
+    t = float(np.mean(times))
+    # the same jobs as a loop of standalone calls (untimed for the metric; shows what batching buys)
+    t0 = time.perf_counter()
+    same = 0
+    for gi, g in enumerate(graphs):
+        for ri, r in enumerate(res):
+            c = scgpu.leiden(g, scgpu.LeidenConfig(seed=42, objective=scgpu.ObjectiveKind.Rb(float(r))))
+            same += int(np.array_equal(c.labels(), jobs[gi * len(res) + ri].labels()))
+    t_loop = time.perf_counter() - t0
+    log(f"[bench] loop of {len(jobs)} standalone calls: {t_loop * 1e3:.1f} ms, identical labels {same}/{len(jobs)}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        t_start, done_edges, n_jobs = time.perf_counter(), 0.0, 0
+        for gi, p in enumerate(wl["parts"]):
+            og = O.Graph.from_csr_parts(*p[:3])
+            for r in res:
+                O.leiden(og, seed=42, objective="rb", resolution=float(r))
+                done_edges += og.edge_count()
+                n_jobs += 1
+                if time.perf_counter() - t_start > args.cpu_baseline_seconds:
+                    break
+            if time.perf_counter() - t_start > args.cpu_baseline_seconds:
+                break
+        t_cpu = time.perf_counter() - t_start
+        model, ncpu = cpu_info()
+        cpu = {"value": done_edges / t_cpu, "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": f"the C oracle's leiden() on the first {n_jobs} jobs in batch order ({done_edges:.0f} edges, "
+                         f"{t_cpu:.1f} s), host {model} ({ncpu} logical CPUs visible)"}
+    st = scgpu.last_stats()
+    out = {
+        "metric": METRIC, "value": m_total / t, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (delta-quality), int64 fixed-point totals, f32 weights",
+        "data": "synthetic",
+        "config": {"workload": wl["workload"], "graphs": len(graphs), "resolutions": [float(r) for r in res],
+                   "jobs": len(jobs), "nodes_per_resolution": wl["n"], "edges_all_jobs": m_total,
+                   "leiden": "Rb, seed 42, defaults", "parallelism": "single GPU, one union graph"},
+        "loop_of_standalone_calls_ms": t_loop * 1e3, "batch_speedup_vs_loop": t_loop / t,
+        "jobs_identical_to_standalone": same, "leiden_stats": st, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="knn_leiden", choices=["knn_leiden", "small", "large"])
+    ap.add_argument("--config", default="knn_leiden", choices=["knn_leiden", "small", "large", "batch"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -111,6 +179,8 @@ def main():
 
     wl = make_workload(args.config)
     log(f"[bench] workload {wl['workload']} ready")
+    if wl["kind"] == "batch":
+        return bench_batch(args, wl, scgpu)
     obj_name, res = wl["objective"]
     cfg = scgpu.LeidenConfig(objective=scgpu.ObjectiveKind(obj_name, res), seed=42)
 

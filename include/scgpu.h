@@ -174,6 +174,17 @@ sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph
 sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *labels_out,
                     uint64_t *n_clusters_out, sc_error *err);
 
+/* Batched leiden: every graph at every resolution, as one run. Equivalent to the loop
+ *   for g in graphs, for r in resolutions: leiden(g, cfg.with_resolution(r))
+ * (leiden/mod.rs:168-204 once per job; LeidenConfig::with_resolution, mod.rs:108-112) with each
+ * job's labels equal to that standalone call's. Job j = g * n_resolutions + r; labels_out holds
+ * the jobs' labels one after another (job j's n_g entries, consecutive from 0), n_clusters_out
+ * one count per job. The jobs run as one disjoint-union graph, so every local-moving and
+ * refinement phase of a level is one persistent launch over all jobs. */
+sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, const double *resolutions,
+                          uint64_t n_resolutions, const sc_leiden_config *cfg, uint64_t *labels_out,
+                          uint64_t *n_clusters_out, sc_error *err);
+
 /* Counters of the last sc_leiden on the calling thread. */
 sc_status sc_leiden_last_stats(sc_leiden_stats *out);
 

@@ -160,6 +160,11 @@ double device_max_f64(const double *in, uint64_t n, hipStream_t s);
 // association order) in one launch pair and one synchronisation: out[i] = sum (or max, when
 // is_max[i]) of in[i][0..n). Null inputs give 0.
 void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_t n, double out[4], hipStream_t s);
+// Per-segment versions of the same fixed-shape reduction: out[j] = sum (or max) of
+// in[ranges[j] .. ranges[j+1]) with exactly the association order device_sum_f64 /
+// device_max_f64 use for an array of that length (one launch pair for all segments).
+void device_segment_reduce_f64(const double *in, const uint64_t *d_ranges, uint32_t nseg, bool is_max, double *d_out,
+                               hipStream_t s);
 
 // Workgroup reductions (blockDim.x a multiple of 64, at most 1024); every thread must call them.
 // The result is valid in thread 0.
@@ -225,6 +230,7 @@ struct sc_graph {
     scgpu::DevBuf<double> node_w;      // n
     scgpu::DevBuf<double> strength;    // n (self-loops twice)
     scgpu::DevBuf<double> self_loop;   // n
+    scgpu::DevBuf<double> row_total;   // n: the row's share (u >= v) of total_weight
     double total_weight = 0.0;
     uint64_t edge_count = 0;
     bool any_self_loops = false;

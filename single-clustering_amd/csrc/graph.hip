@@ -247,7 +247,8 @@ void graph_finish_sorted(sc_graph &g, hipStream_t s) {
     uint64_t n = g.n;
     g.strength.resize(n);
     g.self_loop.resize(n);
-    DevBuf<double> row_total(n);
+    g.row_total.resize(n);
+    DevBuf<double> &row_total = g.row_total;
     // [0..1] edge count, then flags: [2] any self-loop, [3] unused, [4] max degree
     DevBuf<unsigned int> cnt(6);
     HIP_OK(hipMemsetAsync(cnt.get(), 0, 6 * sizeof(unsigned int), s));

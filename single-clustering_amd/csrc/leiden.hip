@@ -94,15 +94,42 @@ struct DecideArgs {
     const uint64_t *goff;
     // counters
     unsigned long long *stats;  // [vertices, edges, candidates]
+    // batched runs (null otherwise): job of every vertex and the jobs' parameters
+    const uint32_t *vjob;
+    const struct JobParams *jobs;
 };
 
-__device__ __forceinline__ double delta_insert(const DecideArgs &a, double w_c, double loop, double k,
+// Parameters of one job of a batched run (one graph at one resolution, sharing a disjoint-union
+// graph with the other jobs). A job's vertices are contiguous at every level, starting at lo.
+struct JobParams {
+    double gamma, four_m, threshold, theta;
+    int sK, sW, sw;
+    uint32_t lo;      // first vertex of the job at the current level
+    uint32_t frozen;  // the job has stopped (its level loop or its passes ended): it never moves
+    uint32_t pad;
+};
+
+// Per-vertex view of the objective parameters: the graph's own, or its job's in a batched run.
+struct VParams {
+    double gamma, four_m, threshold, theta;
+    int sK, sW, sw;
+    uint32_t lo;
+    bool frozen;
+};
+
+__device__ __forceinline__ VParams vparams(const DecideArgs &a, uint32_t v) {
+    if (!a.vjob) return VParams{a.gamma, a.four_m, a.threshold, a.theta, a.sK, a.sW, a.sw, 0u, false};
+    const JobParams &p = a.jobs[a.vjob[v]];
+    return VParams{p.gamma, p.four_m, p.threshold, p.theta, p.sK, p.sW, p.sw, p.lo, p.frozen != 0u};
+}
+
+__device__ __forceinline__ double delta_insert(int objective, const VParams &p, double w_c, double loop, double k,
                                                double na, double Kc, double Wc) {
-    if (a.objective == 0) {
-        if (a.four_m == 0.0) return 0.0;
-        return (w_c + loop) - a.gamma * k * (2.0 * Kc + k) / a.four_m;
+    if (objective == 0) {
+        if (p.four_m == 0.0) return 0.0;
+        return (w_c + loop) - p.gamma * k * (2.0 * Kc + k) / p.four_m;
     }
-    return (w_c + loop) - a.gamma * na * (2.0 * Wc + na - 1.0) / 2.0;
+    return (w_c + loop) - p.gamma * na * (2.0 * Wc + na - 1.0) / 2.0;
 }
 
 __device__ __forceinline__ uint32_t subset_of(uint64_t seed, uint32_t v, uint32_t P) {
@@ -174,8 +201,9 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
     bool valid = idx < len;
     uint32_t pos = valid ? (sel ? sel[idx] : (uint32_t)idx) : 0u;
     uint32_t v = valid ? blist[pos] : 0u;
+    const VParams vp = vparams(a, v);
     if (valid && !sel) {
-        valid = a.active[v] != 0 && subset_of(sp.seed, v, a.P) == sp.sub;
+        valid = a.active[v] != 0 && !vp.frozen && subset_of(sp.seed, v - vp.lo, a.P) == sp.sub;
         if (REFINE) valid = valid && a.memb[v] == v && a.csize[v] == 1;
     }
     uint32_t *keys;
@@ -212,7 +240,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         if (u == v) continue;  // self-loops are handled by the partition (partition.rs:430)
         if (REFINE && a.constraint[u] != cv) continue;
         uint32_t c = a.memb[u];
-        long long wf = to_fx((double)a.wts[e], a.sw);
+        long long wf = to_fx((double)a.wts[e], vp.sw);
         table_insert(keys, vals, mask, c, wf);
     }
     __syncthreads();
@@ -227,7 +255,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         na = a.node_w[v];
         loop = a.self_loop[v];
         // refinement: the node leaves its singleton, which becomes empty (refine.rs:71-74)
-        if (REFINE) rbase = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
+        if (REFINE) rbase = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
         for (uint32_t s = lane; s < tsize; s += G) {
             uint32_t c = keys[s];
             if (c == EMPTY_KEY) continue;
@@ -237,14 +265,14 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
                 continue;
             }
             ds.c++;
-            double Wc = from_fx(a.Wfx[c], a.sW);
+            double Wc = from_fx(a.Wfx[c], vp.sW);
             if (a.has_max && !(Wc + na <= a.max_w)) continue;
-            double Kc = a.objective == 0 ? from_fx(a.Kfx[c], a.sK) : 0.0;
-            double gain = delta_insert(a, from_fx(wfx, a.sw), loop, k, na, Kc, Wc);
+            double Kc = a.objective == 0 ? from_fx(a.Kfx[c], vp.sK) : 0.0;
+            double gain = delta_insert(a.objective, vp, from_fx(wfx, vp.sw), loop, k, na, Kc, Wc);
             if (REFINE) {
                 double rel = gain - rbase;
                 if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
-                double score = a.theta > 0.0 ? rel / a.theta + gumbel(sp.seed, sp.round, v, c) : rel;
+                double score = vp.theta > 0.0 ? rel / vp.theta + gumbel(sp.seed, sp.round, v - vp.lo, c - vp.lo) : rel;
                 better(bs, bc, score, c);
             } else {
                 better(bs, bc, gain, c);
@@ -280,17 +308,17 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         ds.v++;
         ds.e += end - beg;
         if (!REFINE) {
-            long long kfx = a.objective == 0 ? to_fx(k, a.sK) : 0;
-            long long wfxv = to_fx(na, a.sW);
-            double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, a.sK) : 0.0;
-            double Wown = from_fx(a.Wfx[own] - wfxv, a.sW);
-            double base_q = delta_insert(a, from_fx(wown, a.sw), loop, k, na, Kown, Wown);
+            long long kfx = a.objective == 0 ? to_fx(k, vp.sK) : 0;
+            long long wfxv = to_fx(na, vp.sW);
+            double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, vp.sK) : 0.0;
+            double Wown = from_fx(a.Wfx[own] - wfxv, vp.sW);
+            double base_q = delta_insert(a.objective, vp, from_fx(wown, vp.sw), loop, k, na, Kown, Wown);
             // the empty community: this vertex's own id, when no one holds it
             if (a.csize[v] == 0 && (!a.has_max || na <= a.max_w)) {
-                double ge = delta_insert(a, 0.0, loop, k, na, 0.0, 0.0);
+                double ge = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
                 better(bs, bc, ge, v);
             }
-            bool accept = bc != NONE && bc != own && bs - base_q > a.threshold;
+            bool accept = bc != NONE && bc != own && bs - base_q > vp.threshold;
             if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
             if (accept) {
                 a.target[v] = bc;
@@ -720,7 +748,8 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
         if (i < n) {
             const uint32_t v = A.lists[i];
             b = bucket_of_pos(A, i);
-            take = a.active[v] != 0 && subset_of(seed, v, a.P) == sub;
+            const VParams vp = vparams(a, v);
+            take = a.active[v] != 0 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
             if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
         }
         uint64_t pending = __ballot(take);
@@ -791,7 +820,8 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
 }
 
 __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint64_t from, uint64_t to) {
-    const long long kf = to_fx(A.d.strength[v], A.d.sK), wf = to_fx(A.d.node_w[v], A.d.sW);
+    const VParams vp = vparams(A.d, (uint32_t)v);
+    const long long kf = to_fx(A.d.strength[v], vp.sK), wf = to_fx(A.d.node_w[v], vp.sW);
     atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
     atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)(-kf));
     atomicAdd((unsigned long long *)&A.Wfx[to], (unsigned long long)wf);
@@ -844,8 +874,9 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
     if (!grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap)) return;
     for (uint64_t v = gtid; v < n; v += gsize) {
         const uint64_t c = REFINE ? v : a.memb[v];
-        atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)to_fx(a.strength[v], a.sK));
-        atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)to_fx(a.node_w[v], a.sW));
+        const VParams vp = vparams(a, (uint32_t)v);
+        atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)to_fx(a.strength[v], vp.sK));
+        atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)to_fx(a.node_w[v], vp.sW));
         atomicAdd(&A.csize[c], 1u);
     }
     if (!grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap)) return;
@@ -1025,12 +1056,18 @@ struct LevelState {
     DevBuf<unsigned char> sweep_ctl;
 };
 
+struct BatchState;
+
 struct RunCtx {
     const sc_leiden_config *cfg;
     hipStream_t s;
     KernelTimer timer;
     sc_leiden_stats st{};
     double move_bytes = 0.0;
+    // batched run (null otherwise): the jobs and, for the current level, every vertex's job
+    BatchState *batch = nullptr;
+    const uint32_t *vjob = nullptr;
+    const JobParams *jobs = nullptr;
 };
 
 template <int BS, int G, int T, bool REFINE, bool GTAB>
@@ -1264,6 +1301,8 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     hipStream_t s = ctx.s;
     HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
     DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
+    a.vjob = ctx.vjob;
+    a.jobs = ctx.jobs;
     a.memb = memb;
     a.P = MOVE_SUBSWEEPS;
     if (b.start[6] < b.start[7]) {
@@ -1271,7 +1310,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         a.gvals = b.gvals.get();
         a.goff = b.goff.get();
     }
-    if (persistent_sweeps()) {
+    if (persistent_sweeps() || ctx.vjob) {
         unsigned int out[4];
         run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz, out);
         ctx.st.move_iterations += out[2];
@@ -1339,6 +1378,8 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     uint32_t n = (uint32_t)L.n;
     hipStream_t s = ctx.s;
     DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
+    a.vjob = ctx.vjob;
+    a.jobs = ctx.jobs;
     a.memb = rmemb;
     a.constraint = memb;
     a.P = 1;
@@ -1351,7 +1392,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
         a.gvals = b.gvals.get();
         a.goff = b.goff.get();
     }
-    if (persistent_sweeps()) {
+    if (persistent_sweeps() || ctx.vjob) {
         unsigned int out[4];
         run_sweep<true>(a, b, ls, ctx, level_seed, rmemb, L.nnz, out);
         ctx.st.refine_rounds += out[2];
@@ -1765,6 +1806,171 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     return out;
 }
 
+// ---- batched runs: many graphs x resolutions as one disjoint-union graph ----------------------
+// Every job (one graph at one resolution) keeps its vertices contiguous at every level; its
+// objective parameters, fixed-point scales and the per-level totals they come from are the
+// ones a run of that graph alone would use (same reductions, same association order), hashing
+// uses job-local vertex ids, and a job stops (is frozen) when its own level loop or its own
+// passes would stop. So each job's labels are the labels of the standalone run.
+struct BatchState {
+    uint32_t J = 0;
+    std::vector<double> gamma;       // per job
+    std::vector<double> tw0, tnw0;   // level-0 total weight and total node weight per job
+    std::vector<uint64_t> voff0;     // level-0 vertex offsets (J + 1)
+    std::vector<uint64_t> cnt;       // vertices per job at the current level
+    std::vector<uint8_t> pass_done;  // the job's passes have stopped (patience)
+    std::vector<uint8_t> level_done; // the job's level loop has stopped in this pass
+    DevBuf<uint32_t> vjob0;          // level-0 job of every vertex
+    DevBuf<uint64_t> d_voff0;
+    DevBuf<JobParams> params;
+    bool stopped(uint32_t j) const { return pass_done[j] || level_done[j]; }
+    bool all_stopped() const {
+        for (uint32_t j = 0; j < J; j++)
+            if (!stopped(j)) return false;
+        return true;
+    }
+    bool all_passes_done() const {
+        for (uint32_t j = 0; j < J; j++)
+            if (!pass_done[j]) return false;
+        return true;
+    }
+};
+
+// Undirected edges (u >= v, loops once) of each job's rows: one workgroup per job, a binary
+// search per sorted row.
+__global__ void k_job_edge_counts(const uint64_t *ptrs, const uint32_t *nbrs, const uint64_t *vranges,
+                                  unsigned long long *out) {
+    const uint32_t j = blockIdx.x;
+    unsigned long long c = 0;
+    for (uint64_t v = vranges[j] + threadIdx.x; v < vranges[j + 1]; v += blockDim.x) {
+        uint64_t lo = ptrs[v], hi = ptrs[v + 1];
+        const uint64_t end = hi;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (nbrs[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        c += end - lo;
+    }
+    c = block_sum(c);
+    if (threadIdx.x == 0) out[j] = c;
+}
+
+// Smallest and largest value of x over each job's range: one workgroup per job.
+__global__ void k_job_minmax(const uint32_t *x, const uint64_t *vranges, uint32_t *mn, uint32_t *mx) {
+    const uint32_t j = blockIdx.x;
+    uint32_t a = 0xFFFFFFFFu, b = 0;
+    for (uint64_t v = vranges[j] + threadIdx.x; v < vranges[j + 1]; v += blockDim.x) {
+        a = min(a, x[v]);
+        b = max(b, x[v]);
+    }
+    a = 0xFFFFFFFFu - block_max(0xFFFFFFFFu - a);
+    b = block_max(b);
+    if (threadIdx.x == 0) {
+        mn[j] = a;
+        mx[j] = b;
+    }
+}
+
+__global__ void k_scatter_job(const uint32_t *groups, const uint32_t *vjob, uint32_t n, uint32_t *cjob) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        cjob[groups[v]] = vjob[v];
+}
+
+__global__ void k_job_rebase(uint32_t *ids, const uint32_t *vjob, uint32_t n, const long long *shift) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        ids[v] = (uint32_t)((long long)ids[v] + shift[vjob[v]]);
+}
+
+static std::vector<uint64_t> batch_ranges(const BatchState &B);
+
+// Community ids compacted over the union are contiguous per job; moves them onto the job's
+// vertex range (job j's ids become lo_j + local id), so vertex ids and community ids share one
+// offset per job as they share one id space in a standalone run (the empty-community candidate
+// and the lower-id tie-break compare them).
+static void batch_rebase(const BatchState &B, uint32_t *ids, const uint32_t *vjob, uint32_t n, hipStream_t s) {
+    const uint32_t J = B.J;
+    const std::vector<uint64_t> r = batch_ranges(B);
+    DevBuf<uint64_t> d_r(J + 1);
+    to_device(d_r.get(), r.data(), J + 1, s);
+    DevBuf<uint32_t> mm(2 * (size_t)J);
+    hipLaunchKernelGGL(k_job_minmax, dim3(J), dim3(256), 0, s, ids, d_r.get(), mm.get(), mm.get() + J);
+    KERNEL_OK();
+    std::vector<uint32_t> h(2 * (size_t)J);
+    to_host(h.data(), mm.get(), 2 * (size_t)J, s);
+    HIP_OK(hipStreamSynchronize(s));
+    std::vector<long long> sh(J, 0);
+    for (uint32_t j = 0; j < J; j++) sh[j] = B.cnt[j] ? (long long)r[j] - (long long)h[j] : 0;
+    DevBuf<long long> d_sh(J);
+    to_device(d_sh.get(), sh.data(), J, s);
+    hipLaunchKernelGGL(k_job_rebase, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, vjob, n,
+                       (const long long *)d_sh.get());
+    KERNEL_OK();
+    HIP_OK(hipStreamSynchronize(s));  // sh is a host temporary
+}
+
+static std::vector<uint64_t> batch_ranges(const BatchState &B) {
+    std::vector<uint64_t> r(B.J + 1, 0);
+    for (uint32_t j = 0; j < B.J; j++) r[j + 1] = r[j] + B.cnt[j];
+    return r;
+}
+
+// Per-level parameters of every job, from the level graph's per-job totals (the same values
+// and reductions run_once derives from a graph of that job alone).
+static void batch_level_params(BatchState &B, const sc_graph &L, const sc_leiden_config &cfg, hipStream_t s) {
+    const uint32_t J = B.J;
+    const std::vector<uint64_t> r = batch_ranges(B);
+    DevBuf<uint64_t> d_r(J + 1);
+    to_device(d_r.get(), r.data(), J + 1, s);
+    DevBuf<double> red(3 * (size_t)J);
+    DevBuf<unsigned long long> ec(J);
+    device_segment_reduce_f64(L.row_total.get(), d_r.get(), J, false, red.get(), s);
+    device_segment_reduce_f64(L.node_w.get(), d_r.get(), J, false, red.get() + J, s);
+    device_segment_reduce_f64(L.strength.get(), d_r.get(), J, true, red.get() + 2 * J, s);
+    hipLaunchKernelGGL(k_job_edge_counts, dim3(J), dim3(256), 0, s, L.ptrs.get(), L.nbrs.get(), d_r.get(), ec.get());
+    KERNEL_OK();
+    std::vector<double> h(3 * (size_t)J);
+    std::vector<unsigned long long> hec(J);
+    to_host(h.data(), red.get(), 3 * (size_t)J, s);
+    to_host(hec.data(), ec.get(), J, s);
+    HIP_OK(hipStreamSynchronize(s));
+    std::vector<JobParams> P(J);
+    for (uint32_t j = 0; j < J; j++) {
+        const double tw = h[j], tnw = h[J + j], ms = B.cnt[j] ? h[2 * J + j] : 0.0;
+        JobParams &p = P[j];
+        p.gamma = B.gamma[j];
+        p.four_m = 4.0 * tw;
+        p.threshold = 1e-12 * std::max(std::fabs(tw), 1.0);
+        const double mean_w = hec[j] > 0 ? tw / (double)hec[j] : 1.0;  // refine.rs:44-50
+        p.theta = cfg.refine_randomness * mean_w;
+        p.sK = fixed_scale_for(2.0 * tw);
+        p.sW = fixed_scale_for(tnw);
+        p.sw = fixed_scale_for(ms);
+        p.lo = (uint32_t)r[j];
+        p.frozen = (B.stopped(j) || B.cnt[j] == 0) ? 1u : 0u;
+        p.pad = 0;
+    }
+    B.params.resize(J);
+    to_device(B.params.get(), P.data(), J, s);
+    HIP_OK(hipStreamSynchronize(s));  // P is a host temporary
+}
+
+// Groups per job after refinement (ids are ascending and contiguous per job).
+static std::vector<uint32_t> batch_group_counts(const BatchState &B, const uint32_t *groups, hipStream_t s) {
+    const uint32_t J = B.J;
+    const std::vector<uint64_t> r = batch_ranges(B);
+    DevBuf<uint64_t> d_r(J + 1);
+    to_device(d_r.get(), r.data(), J + 1, s);
+    DevBuf<uint32_t> mm(2 * (size_t)J);
+    hipLaunchKernelGGL(k_job_minmax, dim3(J), dim3(256), 0, s, groups, d_r.get(), mm.get(), mm.get() + J);
+    KERNEL_OK();
+    std::vector<uint32_t> h(2 * (size_t)J), out(J, 0);
+    to_host(h.data(), mm.get(), 2 * (size_t)J, s);
+    HIP_OK(hipStreamSynchronize(s));
+    for (uint32_t j = 0; j < J; j++) out[j] = B.cnt[j] ? h[J + j] - h[j] + 1 : 0;
+    return out;
+}
+
 // ---- driver (leiden/mod.rs:168-289) -----------------------------------------------------------
 // Host wall time per phase; each lap synchronizes the stream so device work is attributed to
 // the phase that queued it.
@@ -1807,6 +2013,17 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
     KERNEL_OK();
     LevelState ls;
     alloc_level(ls, n0);
+    BatchState *B = ctx.batch;
+    DevBuf<uint32_t> vjob_l, vjob_c;
+    if (B) {
+        vjob_l.resize(n0);
+        vjob_c.resize(n0);
+        HIP_OK(hipMemcpyAsync(vjob_l.get(), B->vjob0.get(), n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        for (uint32_t j = 0; j < B->J; j++) {
+            B->cnt[j] = B->voff0[j + 1] - B->voff0[j];
+            B->level_done[j] = 0;
+        }
+    }
     for (uint64_t lvl = 0; lvl < cfg.max_levels; lvl++) {
         const sc_graph &L = *level;
         uint32_t n = (uint32_t)L.n;
@@ -1816,6 +2033,11 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         int sw = fixed_scale_for(L.max_strength);
         uint64_t level_seed = mix64(pass_seed ^ (0x9E3779B97F4A7C15ull * (lvl + 1)));
         PhaseClock pc(ctx, s);
+        if (B) {
+            batch_level_params(*B, L, cfg, s);
+            ctx.vjob = vjob_l.get();
+            ctx.jobs = B->params.get();
+        }
         Buckets b;
         build_buckets(L, b, s);
         pc.lap(0);
@@ -1830,20 +2052,39 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
             ng = renumber(memb.get(), n, n, groups.get(), s);
         }
         pc.lap(3);
-        if (ng >= n) break;  // no shrink, nothing left to gain (mod.rs:263-265)
+        std::vector<uint32_t> jng;
+        if (B) {
+            // each job stops where its own level loop would (mod.rs:263-265)
+            jng = batch_group_counts(*B, groups.get(), s);
+            for (uint32_t j = 0; j < B->J; j++)
+                if (!B->stopped(j) && jng[j] >= B->cnt[j]) B->level_done[j] = 1;
+            if (B->all_stopped()) break;
+        } else if (ng >= n) {
+            break;  // no shrink, nothing left to gain (mod.rs:263-265)
+        }
         std::unique_ptr<sc_graph> coarse = aggregate_device(L, groups.get(), ng, s);
+        if (B) {
+            hipLaunchKernelGGL(k_scatter_job, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups.get(),
+                               vjob_l.get(), n, vjob_c.get());
+            KERNEL_OK();
+            std::swap(vjob_l, vjob_c);
+            for (uint32_t j = 0; j < B->J; j++) B->cnt[j] = jng[j];
+        }
         pc.lap(4);
         // several coarse nodes can share a working community (mod.rs:269-272); compact its ids
         hipLaunchKernelGGL(k_scatter_coarse_memb, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups.get(),
                            memb.get(), n, rmemb.get());
         KERNEL_OK();
         renumber(rmemb.get(), ng, n, memb.get(), s);
+        if (B) batch_rebase(*B, memb.get(), vjob_l.get(), ng, s);
         hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, groups.get(), node_at.get(), n0);
         KERNEL_OK();
         pc.lap(3);
         owned = std::move(coarse);
         level = owned.get();
     }
+    ctx.vjob = nullptr;
+    ctx.jobs = nullptr;
     uint32_t n = (uint32_t)level->n;
     uint32_t nc = renumber(memb.get(), n, n, rmemb.get(), s);
     // labels[v] = final community of v's node at the last level
@@ -1876,8 +2117,23 @@ static void validate_config(const sc_leiden_config &c) {
         bad("max_community_weight must be finite and positive, got " + std::to_string(c.max_community_weight));
 }
 
+void quality_jobs_device(const sc_graph &g, const uint64_t *d_labels, int mode, const uint32_t *vjob,
+                         const uint64_t *d_vranges, uint32_t J, const double *h_gamma, const double *h_tw,
+                         const double *h_tnw, double *h_q, hipStream_t s);
+
+static void batch_quality(const sc_graph &G0, const uint32_t *labels, const sc_leiden_config &cfg, BatchState &B,
+                          std::vector<double> &q, hipStream_t s) {
+    const uint32_t n = (uint32_t)G0.n;
+    DevBuf<uint64_t> l64(n);
+    hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, labels, l64.get(), n);
+    KERNEL_OK();
+    q.assign(B.J, 0.0);
+    quality_jobs_device(G0, l64.get(), cfg.objective == SC_RB ? 1 : 2, B.vjob0.get(), B.d_voff0.get(), B.J,
+                        B.gamma.data(), B.tw0.data(), B.tnw0.data(), q.data(), s);
+}
+
 void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_labels, uint64_t *n_clusters,
-                   hipStream_t s, sc_leiden_stats *stats_out) {
+                   hipStream_t s, sc_leiden_stats *stats_out, BatchState *batch = nullptr) {
     validate_config(cfg);
     RunCtx ctx;
     ctx.cfg = &cfg;
@@ -1894,7 +2150,45 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
         seed = ((uint64_t)rd() << 32) ^ rd();
     }
     uint64_t nc = 0;
-    if (n0 > 0) {
+    if (n0 > 0 && batch) {
+        // every job's passes, with its own improvement and patience (mod.rs:180-204)
+        ctx.batch = batch;
+        BatchState &B = *batch;
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, d_labels, n0);
+        KERNEL_OK();
+        std::vector<double> q_prev, q;
+        std::vector<uint64_t> stagnant(B.J, 0);
+        B.pass_done.assign(B.J, 0);
+        B.level_done.assign(B.J, 0);
+        B.cnt.assign(B.J, 0);
+        for (uint32_t j = 0; j < B.J; j++)
+            if (B.voff0[j + 1] == B.voff0[j]) B.pass_done[j] = 1;
+        PhaseClock pq(ctx, s);
+        batch_quality(G0, d_labels, cfg, B, q_prev, s);
+        pq.lap(5);
+        for (uint64_t pass = 0; pass < cfg.n_iterations && !B.all_passes_done(); pass++) {
+            ctx.st.passes++;
+            uint64_t pass_seed = mix64(seed + 0x9E3779B97F4A7C15ull * (pass + 1));
+            nc = run_once(G0, d_labels, pass_seed, ctx);
+            PhaseClock pq2(ctx, s);
+            batch_quality(G0, d_labels, cfg, B, q, s);
+            // the next pass starts from these labels: put each job's on its vertex range
+            for (uint32_t j = 0; j < B.J; j++) B.cnt[j] = B.voff0[j + 1] - B.voff0[j];
+            batch_rebase(B, d_labels, B.vjob0.get(), n0, s);
+            pq2.lap(5);
+            for (uint32_t j = 0; j < B.J; j++) {
+                if (B.pass_done[j]) continue;
+                const double improvement = q[j] - q_prev[j];
+                q_prev[j] = q[j];
+                if (improvement <= cfg.tolerance * std::max(std::fabs(B.tw0[j]), 1.0)) {
+                    if (++stagnant[j] >= cfg.patience) B.pass_done[j] = 1;
+                } else {
+                    stagnant[j] = 0;
+                }
+            }
+        }
+        ctx.batch = nullptr;
+    } else if (n0 > 0) {
         hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, d_labels, n0);
         KERNEL_OK();
         PhaseClock pq(ctx, s);
@@ -1932,11 +2226,147 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
     if (stats_out) *stats_out = ctx.st;
 }
 
+// One job's source graph and its place in the union.
+struct UnionSrc {
+    const uint64_t *ptrs;
+    const uint32_t *nbrs;
+    const float *wts;
+    const double *node_w, *strength, *self_loop, *row_total;
+    uint64_t n, nnz, voff, eoff;
+};
+
+// blockIdx.y = job: node arrays (row offsets shifted, every vertex tagged with its job).
+__global__ void k_union_nodes(const UnionSrc *src, uint64_t *ptrs, double *node_w, double *strength,
+                              double *self_loop, double *row_total, uint32_t *vjob) {
+    const UnionSrc u = src[blockIdx.y];
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < u.n; v += (uint64_t)gridDim.x * blockDim.x) {
+        ptrs[u.voff + v] = u.eoff + u.ptrs[v];
+        node_w[u.voff + v] = u.node_w[v];
+        strength[u.voff + v] = u.strength[v];
+        self_loop[u.voff + v] = u.self_loop[v];
+        row_total[u.voff + v] = u.row_total[v];
+        vjob[u.voff + v] = blockIdx.y;
+    }
+}
+
+// blockIdx.y = job: adjacency entries with neighbour ids shifted by the job's vertex offset.
+__global__ void k_union_edges(const UnionSrc *src, uint32_t *nbrs, float *wts) {
+    const UnionSrc u = src[blockIdx.y];
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < u.nnz; e += (uint64_t)gridDim.x * blockDim.x) {
+        nbrs[u.eoff + e] = (uint32_t)(u.voff + u.nbrs[e]);
+        wts[u.eoff + e] = u.wts[e];
+    }
+}
+
 }  // namespace scgpu
 
 using namespace scgpu;
 
 extern "C" {
+
+sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, const double *resolutions,
+                          uint64_t n_resolutions, const sc_leiden_config *cfg, uint64_t *labels_out,
+                          uint64_t *n_clusters_out, sc_error *err) {
+    try {
+        if (!graphs || !resolutions || !cfg || !n_clusters_out)
+            throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (n_graphs == 0 || n_resolutions == 0) throw ScError(SC_INVALID_ARGUMENT, "empty batch");
+        const uint64_t J64 = n_graphs * n_resolutions;
+        if (J64 > 65535) throw ScError(SC_INVALID_ARGUMENT, "at most 65535 jobs per batch");
+        const uint32_t J = (uint32_t)J64;
+        for (uint64_t r = 0; r < n_resolutions; r++) {
+            sc_leiden_config c = *cfg;
+            c.resolution = resolutions[r];
+            validate_config(c);
+        }
+        uint64_t N = 0, E = 0;
+        for (uint64_t g = 0; g < n_graphs; g++) {
+            if (!graphs[g]) throw ScError(SC_INVALID_ARGUMENT, "null graph");
+            N += graphs[g]->n * n_resolutions;
+            E += graphs[g]->nnz * n_resolutions;
+        }
+        if (N > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes in the batch");
+        if (N && !labels_out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        hipStream_t s = thread_stream();
+        // the union graph: job j = graph j / R at resolution j % R, vertices and entries in job order
+        sc_graph U;
+        U.n = N;
+        U.nnz = E;
+        U.ptrs.resize(N + 1);
+        U.nbrs.resize(E);
+        U.wts.resize(E);
+        U.node_w.resize(N);
+        U.strength.resize(N);
+        U.self_loop.resize(N);
+        U.row_total.resize(N);
+        BatchState B;
+        B.J = J;
+        B.gamma.resize(J);
+        B.tw0.resize(J);
+        B.tnw0.resize(J);
+        B.voff0.assign(J + 1, 0);
+        B.vjob0.resize(N);
+        std::vector<UnionSrc> src(J);
+        uint64_t voff = 0, eoff = 0;
+        uint32_t maxdeg = 0;
+        for (uint32_t j = 0; j < J; j++) {
+            const sc_graph &g = *graphs[j / n_resolutions];
+            src[j] = UnionSrc{g.ptrs.get(), g.nbrs.get(), g.wts.get(), g.node_w.get(), g.strength.get(),
+                              g.self_loop.get(), g.row_total.get(), g.n, g.nnz, voff, eoff};
+            B.gamma[j] = resolutions[j % n_resolutions];
+            B.tw0[j] = g.total_weight;
+            B.tnw0[j] = g.total_node_weight;
+            B.voff0[j] = voff;
+            U.total_weight += g.total_weight;
+            U.total_node_weight += g.total_node_weight;
+            U.edge_count += g.edge_count;
+            U.any_self_loops = U.any_self_loops || g.any_self_loops;
+            U.max_strength = std::max(U.max_strength, g.max_strength);
+            maxdeg = std::max(maxdeg, g.max_degree);
+            voff += g.n;
+            eoff += g.nnz;
+        }
+        B.voff0[J] = voff;
+        U.max_degree = maxdeg;
+        DevBuf<UnionSrc> d_src(J);
+        to_device(d_src.get(), src.data(), J, s);
+        B.d_voff0.resize(J + 1);
+        to_device(B.d_voff0.get(), B.voff0.data(), J + 1, s);
+        hipLaunchKernelGGL(k_union_nodes, dim3(64, J), dim3(256), 0, s, (const UnionSrc *)d_src.get(), U.ptrs.get(),
+                           U.node_w.get(), U.strength.get(), U.self_loop.get(), U.row_total.get(), B.vjob0.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_union_edges, dim3(256, J), dim3(256), 0, s, (const UnionSrc *)d_src.get(), U.nbrs.get(),
+                           U.wts.get());
+        KERNEL_OK();
+        to_device(U.ptrs.get() + N, &E, 1, s);
+        HIP_OK(hipStreamSynchronize(s));  // host tables above are temporaries
+        sc_leiden_config c = *cfg;
+        c.resolution = resolutions[0];
+        DevBuf<uint32_t> labels(N);
+        uint64_t nc = 0;
+        sc_leiden_stats st{};
+        leiden_device(U, c, labels.get(), &nc, s, &st, &B);
+        // per job: labels are consecutive over the union and contiguous per job
+        std::vector<uint32_t> h(N);
+        to_host(h.data(), labels.get(), N, s);
+        HIP_OK(hipStreamSynchronize(s));
+        for (uint32_t j = 0; j < J; j++) {
+            const uint64_t lo = B.voff0[j], hi = B.voff0[j + 1];
+            uint32_t mn = 0xFFFFFFFFu, mx = 0;
+            for (uint64_t v = lo; v < hi; v++) {
+                mn = std::min(mn, h[v]);
+                mx = std::max(mx, h[v]);
+            }
+            for (uint64_t v = lo; v < hi; v++) labels_out[v] = (uint64_t)(h[v] - mn);
+            n_clusters_out[j] = hi > lo ? (uint64_t)(mx - mn) + 1 : 0;
+        }
+        t_stats = st;
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
 
 sc_leiden_config sc_leiden_config_default(void) {
     sc_leiden_config c;

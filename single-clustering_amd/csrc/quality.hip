@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace scgpu {
 
@@ -33,7 +34,8 @@ template <int G>
 __global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, uint64_t n, const uint64_t *ptrs,
                                                       const uint32_t *nbrs, const float *wts, const double *strength,
                                                       const double *node_w, int s_in, int sK, int sW,
-                                                      long long *in_fx, long long *K_fx, long long *W_fx) {
+                                                      long long *in_fx, long long *K_fx, long long *W_fx,
+                                                      const uint32_t *vjob = nullptr, const int *jscale = nullptr) {
     constexpr int NG = 256 / G;
     constexpr uint32_t TAB = 1024;  // per-workgroup label table (few labels recur within a block)
     __shared__ unsigned long long t_key[TAB];
@@ -54,8 +56,15 @@ __global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, ui
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
         if (lane == 0) {
-            const long long fi = acc != 0.0 ? to_fx(acc, s_in) : 0, fk = to_fx(strength[v], sK),
-                            fw = to_fx(node_w[v], sW);
+            int si = s_in, sk = sK, sw = sW;
+            if (vjob) {  // batched run: the scales of this vertex's job
+                const uint32_t j = vjob[v];
+                si = jscale[3 * j];
+                sk = jscale[3 * j + 1];
+                sw = jscale[3 * j + 2];
+            }
+            const long long fi = acc != 0.0 ? to_fx(acc, si) : 0, fk = to_fx(strength[v], sk),
+                            fw = to_fx(node_w[v], sw);
             // integer sums are order independent: pre-add in LDS, one global atomic per label
             uint32_t h = hash32((uint32_t)c ^ (uint32_t)(c >> 32)) & (TAB - 1);
             bool done = false;
@@ -88,9 +97,18 @@ __global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, ui
 // mode 0: modularity term 2*in/2m - gamma*(K/2m)^2; mode 1: Rb quality term; mode 2: Cpm.
 __global__ void k_community_terms(const long long *in_fx, const long long *K_fx, const long long *W_fx,
                                   uint64_t nl, int s_in, int sK, int sW, int mode, double gamma, double m,
-                                  double *term) {
+                                  double *term, const uint32_t *ljob = nullptr, const int *jscale = nullptr,
+                                  const double *jgamma = nullptr, const double *jm = nullptr) {
     for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nl;
          c += (uint64_t)gridDim.x * blockDim.x) {
+        if (ljob) {  // batched run: the label's job supplies scales, resolution and m
+            const uint32_t j = ljob[c];
+            s_in = jscale[3 * j];
+            sK = jscale[3 * j + 1];
+            sW = jscale[3 * j + 2];
+            gamma = jgamma[j];
+            m = jm[j];
+        }
         double internal = from_fx(in_fx[c], s_in), st = from_fx(K_fx[c], sK), nw = from_fx(W_fx[c], sW);
         double t;
         if (mode == 0) {
@@ -134,6 +152,83 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
                        acc.get() + 2 * nl, nl, s_in, sK, sW, mode, gamma, g.total_weight, term.get());
     KERNEL_OK();
     return device_sum_f64(term.get(), nl, s);
+}
+
+// ---- batched runs: one quality per job -----------------------------------------------------
+__global__ void k_label_jobs(const uint64_t *labels, const uint32_t *vjob, uint64_t n, uint32_t *ljob) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        ljob[labels[v]] = vjob[v];
+}
+
+// Smallest label of each job (one workgroup per job over its vertex range).
+__global__ void k_job_min_label(const uint64_t *labels, const uint64_t *vranges, uint64_t *lranges) {
+    const uint32_t j = blockIdx.x;
+    unsigned long long mn = ~0ull;
+    for (uint64_t v = vranges[j] + threadIdx.x; v < vranges[j + 1]; v += blockDim.x)
+        mn = min(mn, (unsigned long long)labels[v]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mn = min(mn, __shfl_xor(mn, off));
+    __shared__ unsigned long long sh[16];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / 64; w++) mn = min(mn, sh[w]);
+        lranges[j] = mn;
+    }
+}
+
+// Quality of each job of a batched run: labels are consecutive over the union and contiguous
+// per job (ascending), so job j's communities are labels lranges[j] .. lranges[j+1). Per job
+// the fixed-point scales, resolution and m are the job's own, and the sum over its communities
+// uses the association order quality_device uses for a graph of that job alone.
+void quality_jobs_device(const sc_graph &g, const uint64_t *d_labels, int mode, const uint32_t *vjob,
+                         const uint64_t *d_vranges, uint32_t J, const double *h_gamma, const double *h_tw,
+                         const double *h_tnw, double *h_q, hipStream_t s) {
+    const uint64_t n = g.n;
+    for (uint32_t j = 0; j < J; j++) h_q[j] = 0.0;
+    if (n == 0 || J == 0) return;
+    DevBuf<unsigned long long> mx(1);
+    HIP_OK(hipMemsetAsync(mx.get(), 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_label, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d_labels, n, mx.get());
+    KERNEL_OK();
+    DevBuf<uint64_t> lr(J + 1);
+    hipLaunchKernelGGL(k_job_min_label, dim3(J), dim3(256), 0, s, d_labels, d_vranges, lr.get());
+    KERNEL_OK();
+    unsigned long long max_label = 0;
+    to_host(&max_label, mx.get(), 1, s);
+    HIP_OK(hipStreamSynchronize(s));
+    const uint64_t nl = max_label + 1;
+    to_device(lr.get() + J, &nl, 1, s);
+    std::vector<int> js(3 * (size_t)J);
+    std::vector<double> jm(J);
+    for (uint32_t j = 0; j < J; j++) {
+        js[3 * j] = fixed_scale_for(h_tw[j] + 1e-300);
+        js[3 * j + 1] = fixed_scale_for(2.0 * h_tw[j] + 1e-300);
+        js[3 * j + 2] = fixed_scale_for(h_tnw[j] + 1e-300);
+        jm[j] = h_tw[j];
+    }
+    DevBuf<int> d_js(3 * (size_t)J);
+    DevBuf<double> d_g(J), d_m(J), d_q(J);
+    to_device(d_js.get(), js.data(), 3 * (size_t)J, s);
+    to_device(d_g.get(), h_gamma, J, s);
+    to_device(d_m.get(), jm.data(), J, s);
+    DevBuf<uint32_t> ljob(nl);
+    hipLaunchKernelGGL(k_label_jobs, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d_labels, vjob, n, ljob.get());
+    KERNEL_OK();
+    DevBuf<long long> acc(3 * nl);
+    HIP_OK(hipMemsetAsync(acc.get(), 0, 3 * nl * sizeof(long long), s));
+    hipLaunchKernelGGL((k_vertex_terms<8>), dim3(grid_for(n, 32, 1024)), dim3(256), 0, s, d_labels, n, g.ptrs.get(),
+                       g.nbrs.get(), g.wts.get(), g.strength.get(), g.node_w.get(), 0, 0, 0, acc.get(),
+                       acc.get() + nl, acc.get() + 2 * nl, vjob, (const int *)d_js.get());
+    KERNEL_OK();
+    DevBuf<double> term(nl);
+    hipLaunchKernelGGL(k_community_terms, dim3(grid_for(nl, 256, 4096)), dim3(256), 0, s, acc.get(), acc.get() + nl,
+                       acc.get() + 2 * nl, nl, 0, 0, 0, mode, 0.0, 0.0, term.get(), (const uint32_t *)ljob.get(),
+                       (const int *)d_js.get(), (const double *)d_g.get(), (const double *)d_m.get());
+    KERNEL_OK();
+    device_segment_reduce_f64(term.get(), lr.get(), J, false, d_q.get(), s);
+    to_host(h_q, d_q.get(), J, s);
+    HIP_OK(hipStreamSynchronize(s));
 }
 
 // ---- connected components restricted to same-label edges --------------------------------

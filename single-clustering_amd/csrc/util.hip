@@ -245,6 +245,57 @@ void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_
     HIP_OK(hipStreamSynchronize(s));
 }
 
+// Segment j is in[ranges[j] .. ranges[j+1]); blockIdx.y = segment. Stage 1 writes
+// part[j * RED_BLOCKS + block], stage 2 reduces those into out[j], each with the strided
+// partials and tree of k_reduce_stage for an array of the segment's length.
+template <bool MAX>
+__global__ void __launch_bounds__(RED_THREADS) k_segment_reduce(const double *in, const uint64_t *ranges,
+                                                                 double *part, double *out, int final) {
+    __shared__ double sh[RED_THREADS];
+    const uint32_t j = blockIdx.y;
+    const uint64_t lo = ranges[j], len = ranges[j + 1] - ranges[j];
+    const double *src = final ? part + (size_t)j * RED_BLOCKS : in + lo;
+    const uint64_t n = final ? (len ? (uint64_t)RED_BLOCKS : 0) : len;
+    double acc = MAX ? -INFINITY : 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RED_THREADS) {
+        double x = src[i];
+        acc = MAX ? fmax(acc, x) : acc + x;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double o = sh[threadIdx.x + w];
+            sh[threadIdx.x] = MAX ? fmax(sh[threadIdx.x], o) : sh[threadIdx.x] + o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (final) out[j] = len ? sh[0] : (MAX ? -INFINITY : 0.0);
+        else part[(size_t)j * RED_BLOCKS + blockIdx.x] = sh[0];
+    }
+}
+
+void device_segment_reduce_f64(const double *in, const uint64_t *d_ranges, uint32_t nseg, bool is_max, double *d_out,
+                               hipStream_t s) {
+    if (nseg == 0) return;
+    DevBuf<double> part((size_t)nseg * RED_BLOCKS);
+    if (is_max) {
+        hipLaunchKernelGGL(k_segment_reduce<true>, dim3(RED_BLOCKS, nseg), dim3(RED_THREADS), 0, s, in, d_ranges,
+                           part.get(), d_out, 0);
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_segment_reduce<true>, dim3(1, nseg), dim3(RED_THREADS), 0, s, in, d_ranges, part.get(),
+                           d_out, 1);
+    } else {
+        hipLaunchKernelGGL(k_segment_reduce<false>, dim3(RED_BLOCKS, nseg), dim3(RED_THREADS), 0, s, in, d_ranges,
+                           part.get(), d_out, 0);
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_segment_reduce<false>, dim3(1, nseg), dim3(RED_THREADS), 0, s, in, d_ranges, part.get(),
+                           d_out, 1);
+    }
+    KERNEL_OK();
+}
+
 double device_sum_f64(const double *in, uint64_t n, hipStream_t s) { return reduce_f64<false>(in, n, s); }
 double device_max_f64(const double *in, uint64_t n, hipStream_t s) { return reduce_f64<true>(in, n, s); }
 

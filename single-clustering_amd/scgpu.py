@@ -112,6 +112,8 @@ SIGNATURES = {
     "sc_graph_validate_symmetry": ([_GP, C.POINTER(ScError)], C.c_int),
     "sc_graph_aggregate": ([_GP, u64p, C.POINTER(_GP), C.POINTER(ScError)], C.c_int),
     "sc_leiden": ([_GP, C.POINTER(ScLeidenConfig), u64p, u64p, C.POINTER(ScError)], C.c_int),
+    "sc_leiden_batch": ([C.POINTER(_GP), C.c_uint64, C.POINTER(C.c_double), C.c_uint64, C.POINTER(ScLeidenConfig),
+                         u64p, u64p, C.POINTER(ScError)], C.c_int),
     "sc_leiden_last_stats": ([C.POINTER(ScLeidenStats)], C.c_int),
     "sc_modularity": ([_GP, u64p, C.c_double], C.c_double),
     "sc_quality": ([_GP, u64p, C.c_int32, C.c_double, f64p, C.POINTER(ScError)], C.c_int),
@@ -421,6 +423,27 @@ def leiden(graph, config=None):
     err = ScError()
     _check(_lib.sc_leiden(graph.handle, C.byref(c), _p(labels, u64p), C.byref(nc), C.byref(err)), err)
     return Clustering(labels[:n], nc.value)
+
+
+def leiden_batch(graphs, resolutions, config=None):
+    """Every graph at every resolution in one batched run; returns the Clustering of job
+    g * len(resolutions) + r, each equal to leiden(graphs[g], config.with_resolution(r))."""
+    config = config or LeidenConfig()
+    graphs = list(graphs)
+    res = np.ascontiguousarray(resolutions, dtype=np.float64)
+    c = config.to_c()
+    sizes = [g.node_count() for g in graphs for _ in range(len(res))]
+    labels = np.zeros(max(sum(sizes), 1), dtype=np.uint64)
+    nc = np.zeros(max(len(sizes), 1), dtype=np.uint64)
+    handles = (_GP * len(graphs))(*[g.handle for g in graphs])
+    err = ScError()
+    _check(_lib.sc_leiden_batch(handles, len(graphs), res.ctypes.data_as(C.POINTER(C.c_double)), len(res),
+                                C.byref(c), _p(labels, u64p), _p(nc, u64p), C.byref(err)), err)
+    out, off = [], 0
+    for j, n in enumerate(sizes):
+        out.append(Clustering(labels[off:off + n].copy(), int(nc[j])))
+        off += n
+    return out
 
 
 def last_stats():

@@ -133,3 +133,74 @@ def edges_csr_from_list(n, edges):
     lo = np.minimum(a, b)
     hi = np.maximum(a, b)
     return edges_to_csr(n, lo, hi, w)
+
+
+def _power_law_ints(rng, size, exponent, lo, hi):
+    """Integers in [lo, hi] with P(x) proportional to x^-exponent (inverse-CDF sampling)."""
+    u = rng.random(size)
+    a = 1.0 - exponent
+    x = (lo ** a + u * (hi ** a - lo ** a)) ** (1.0 / a)
+    return np.clip(np.floor(x).astype(np.int64), lo, hi)
+
+
+def lfr_like(n, seed, mean_degree=15.0, tau_degree=2.5, tau_size=1.5, mu=0.3):
+    """Planted partition with power-law degrees (exponent tau_degree, mean ~mean_degree) and
+    power-law community sizes (exponent tau_size); a fraction mu of every vertex's stubs leaves
+    its community; stubs are paired configuration-model style (self-loops and repeats dropped);
+    weights ~ U(0.5, 1.5) f32. Returns (ptrs u64, nbrs u32, wts f32, truth)."""
+    rng = np.random.default_rng(seed)
+    dmax = int(min(n - 1, max(50, 4 * np.sqrt(n))))
+    dmin = 1
+    for cand in range(1, 40):  # smallest minimum degree whose expected mean reaches the target
+        d = np.arange(cand, dmax + 1, dtype=np.float64)
+        p = d ** -tau_degree
+        if (d * p).sum() / p.sum() >= mean_degree:
+            dmin = cand
+            break
+    deg = _power_law_ints(rng, n, tau_degree, dmin, dmax)
+    cmin, cmax = 20, max(100, n // 10)
+    sizes = []
+    while sum(sizes) < n:
+        sizes.extend(_power_law_ints(rng, 64, tau_size, cmin, cmax).tolist())
+    sizes = np.array(sizes, dtype=np.int64)
+    sizes = sizes[np.cumsum(sizes) - sizes < n]
+    sizes[-1] = n - (sizes.sum() - sizes[-1])
+    truth = rng.permutation(np.repeat(np.arange(len(sizes)), sizes))
+    k_in = np.minimum(np.rint((1.0 - mu) * deg).astype(np.int64), np.maximum(sizes[truth] - 1, 0))
+    k_out = deg - k_in
+    # intra-community stubs: shuffled within their community, paired consecutively
+    stubs = np.repeat(np.arange(n), k_in)
+    comm = truth[stubs]
+    order = np.lexsort((rng.random(len(stubs)), comm))
+    stubs, comm = stubs[order], comm[order]
+    counts = np.bincount(comm, minlength=len(sizes))
+    ends = np.cumsum(counts)
+    drop = ends[counts % 2 == 1] - 1
+    keep = np.ones(len(stubs), dtype=bool)
+    keep[drop] = False
+    stubs = stubs[keep]
+    a_in, b_in = stubs[0::2], stubs[1::2]
+    out = rng.permutation(np.repeat(np.arange(n), k_out))
+    if len(out) % 2:
+        out = out[:-1]
+    a = np.concatenate([a_in, out[0::2]])
+    b = np.concatenate([b_in, out[1::2]])
+    m = a != b
+    lo, hi = np.minimum(a[m], b[m]), np.maximum(a[m], b[m])
+    key = np.unique(lo.astype(np.int64) * n + hi)
+    lo, hi = key // n, key % n
+    w = rng.uniform(0.5, 1.5, len(lo)).astype(np.float32)
+    rows = np.concatenate([lo, hi])
+    cols = np.concatenate([hi, lo])
+    ww = np.concatenate([w, w])
+    o = np.lexsort((cols, rows))
+    rows, cols, ww = rows[o], cols[o], ww[o]
+    ptrs = np.zeros(n + 1, dtype=np.uint64)
+    ptrs[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    return ptrs, cols.astype(np.uint32), ww, truth
+
+
+def batch_graph_sizes(count=64, lo=2000, hi=200000, seed=42):
+    """Graph sizes log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
