@@ -185,6 +185,44 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
                           uint64_t n_resolutions, const sc_leiden_config *cfg, uint64_t *labels_out,
                           uint64_t *n_clusters_out, sc_error *err);
 
+/* ---- learned active-vertex filter (no counterpart in the reference) --------------------- */
+/* An 8 -> 64 -> 64 -> 1 perceptron (ReLU) predicts whether an active vertex will move when
+ * local moving decides it; parameters: sc_mlp_param_count() floats in the order
+ * W1[8][64], b1[64], W2[64][64], b2[64], w3[64], b3. Features: degree, community size, strength
+ * relative to the community mean, the community's share of 2m, moved earlier at this level,
+ * iteration, level, self-loop share. */
+uint32_t sc_mlp_param_count(void);
+/* He-uniform weights, zero biases, from a seed. */
+void sc_mlp_init(uint64_t seed, float *w);
+/* Mean loss (weighted BCE on the logit, positives weighted by pos_weight) and its gradient
+ * over n samples X[n][8], y[n] in {0, 1}; the HIP backward pass. */
+sc_status sc_mlp_grad(const float *w, const float *X, const uint8_t *y, uint64_t n, float pos_weight,
+                      float *grad_out, double *loss_out, sc_error *err);
+/* Minibatch Adam with HIP forward/backward kernels; loss_hist (nullable) gets each epoch's loss. */
+sc_status sc_mlp_train(const float *X, const uint8_t *y, uint64_t n, uint32_t epochs, uint32_t batch, float lr,
+                       float pos_weight, uint64_t seed, float *w_inout, double *loss_hist, sc_error *err);
+/* Logits of n samples (bf16 != 0: BF16-rounded weights, as the filter evaluates them). */
+sc_status sc_mlp_predict(const float *w, const float *X, uint64_t n, int32_t bf16, float *out, sc_error *err);
+
+/* Runs leiden() and records up to cap local-moving decisions as training samples
+ * (features x_out[cap][8], moved y_out[cap]); n_out = samples written. */
+sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64_t cap, float *x_out,
+                          uint8_t *y_out, uint64_t *n_out, sc_error *err);
+
+typedef struct {
+    uint64_t skipped;        /* filter decisions "stays" (vertex-steps not decided) */
+    uint64_t judged;         /* vertices the first plain (judge) iteration decided */
+    uint64_t judge_moves;    /* of them, vertices that moved: the filter's errors */
+    double skip_error_rate;  /* judge_moves / judged */
+} sc_filter_stats;
+
+/* leiden() whose local moving skips active vertices the model gives a move probability below
+ * threshold, until nothing moves; then plain iterations run to convergence (the judge), so the
+ * result is a converged plain local-moving state at every level. */
+sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, const float *weights,
+                             double threshold, uint64_t *labels_out, uint64_t *n_clusters_out,
+                             sc_filter_stats *stats, sc_error *err);
+
 /* Counters of the last sc_leiden on the calling thread. */
 sc_status sc_leiden_last_stats(sc_leiden_stats *out);
 

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libscgpu.so")
-SOURCES = ["util.hip", "graph.hip", "quality.hip", "leiden.hip", "knn.hip"]
+SOURCES = ["util.hip", "graph.hip", "quality.hip", "leiden.hip", "knn.hip", "mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -25,6 +25,7 @@
 //    ascending) via a stable radix sort by (group, neighbour group), so coarse weights are
 //    bit-identical to the CPU path.
 #include "common.hpp"
+#include "mlp.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -97,6 +98,13 @@ struct DecideArgs {
     // batched runs (null otherwise): job of every vertex and the jobs' parameters
     const uint32_t *vjob;
     const struct JobParams *jobs;
+    // learned filter support: the level index, and (trace runs) where local-moving decisions
+    // are recorded as (features, moved) samples
+    uint32_t level;
+    float *trace_x;
+    uint8_t *trace_y;
+    unsigned long long *trace_n;
+    uint64_t trace_cap;
 };
 
 // Parameters of one job of a batched run (one graph at one resolution, sharing a disjoint-union
@@ -130,6 +138,28 @@ __device__ __forceinline__ double delta_insert(int objective, const VParams &p, 
         return (w_c + loop) - p.gamma * k * (2.0 * Kc + k) / p.four_m;
     }
     return (w_c + loop) - p.gamma * na * (2.0 * Wc + na - 1.0) / 2.0;
+}
+
+// Features of an active vertex for the learned filter, from the state every decision reads:
+// degree, community size, the vertex's strength relative to its community's mean, the
+// community's share of 2m, whether it moved earlier at this level, iteration, level, and the
+// self-loop share of its strength. All O(1) per vertex.
+__device__ __forceinline__ void vertex_features(const DecideArgs &a, const VParams &vp, const uint32_t *mstep,
+                                                uint32_t v, uint32_t it, float (&x)[MLP_NF]) {
+    const uint32_t own = a.memb[v];
+    const double deg = (double)(a.ptrs[v + 1] - a.ptrs[v]);
+    const double cs = (double)a.csize[own];
+    const double K = from_fx(a.Kfx[own], vp.sK);
+    const double k = a.strength[v];
+    const double two_m = vp.four_m > 0.0 ? 0.5 * vp.four_m : 1.0;
+    x[0] = (float)(log2(1.0 + deg) / 16.0);
+    x[1] = (float)(log2(1.0 + cs) / 16.0);
+    x[2] = (float)(fmin(k * cs / fmax(K, 1e-300), 8.0) / 8.0);
+    x[3] = (float)(log2(fmax(K / two_m, 1e-12)) / 40.0 + 1.0);
+    x[4] = (mstep && mstep[v] != NONE) ? 1.0f : 0.0f;
+    x[5] = (float)min(it, 15u) / 15.0f;
+    x[6] = (float)min(a.level, 15u) / 15.0f;
+    x[7] = k > 0.0 ? (float)(a.self_loop[v] / k) : 0.0f;
 }
 
 __device__ __forceinline__ uint32_t subset_of(uint64_t seed, uint32_t v, uint32_t P) {
@@ -320,6 +350,16 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
             }
             bool accept = bc != NONE && bc != own && bs - base_q > vp.threshold;
             if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
+            if (a.trace_x) {
+                const unsigned long long slot = atomicAdd(a.trace_n, 1ull);
+                if (slot < a.trace_cap) {
+                    float x[MLP_NF];
+                    vertex_features(a, vp, sp.mstep, v, sp.round, x);
+#pragma unroll
+                    for (int f = 0; f < MLP_NF; f++) a.trace_x[slot * MLP_NF + f] = x[f];
+                    a.trace_y[slot] = accept ? 1 : 0;
+                }
+            }
             if (accept) {
                 a.target[v] = bc;
                 if (sp.mstep) sp.mstep[v] = sp.step;
@@ -637,6 +677,11 @@ struct SweepArgs {
     uint64_t level_seed;
     long long *trace;       // optional: real-time stamps after each barrier (workgroup 0)
     uint32_t trace_cap;
+    // learned filter (local moving): FP32 weights (null = plain), skip when logit < filter_thr;
+    // fstats: [0] skip events, [1] vertices the first plain iteration decided, [2] of them moved
+    const float *filter_w;
+    float filter_thr;
+    unsigned long long *fstats;
 };
 
 __device__ __forceinline__ unsigned int ld_acquire_ctr(const unsigned int *p) {
@@ -736,11 +781,13 @@ __device__ __forceinline__ int bucket_of_pos(const SweepArgs &A, uint32_t i) {
 // Positions (within their bucket) of the vertices that decide in this step, wave-aggregated
 // appends into work[bstart[b] + ...].
 template <bool REFINE>
-__device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t step) {
+__device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t step,
+                                              uint32_t it, const float *fw) {
     const DecideArgs &a = A.d;
     const uint32_t n = a.n;
     const int lane = threadIdx.x & 63;
     unsigned int *cnt = A.wcnt + (uint64_t)step * NUM_BUCKETS;
+    unsigned long long skipped = 0;
     for (uint32_t i0 = blockIdx.x * SWEEP_BS; i0 < n; i0 += gridDim.x * SWEEP_BS) {
         const uint32_t i = i0 + threadIdx.x;
         bool take = false;
@@ -751,6 +798,15 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
             const VParams vp = vparams(a, v);
             take = a.active[v] != 0 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
             if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
+            if (!REFINE && fw && take) {
+                // learned filter: a vertex predicted to stay is not decided now (it stays active)
+                float x[MLP_NF];
+                vertex_features(a, vp, A.mstep, v, it, x);
+                if (mlp_forward(fw, x) < A.filter_thr) {
+                    take = false;
+                    skipped++;
+                }
+            }
         }
         uint64_t pending = __ballot(take);
         while (pending) {
@@ -768,6 +824,10 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
             }
             pending &= ~m;
         }
+    }
+    if (!REFINE && fw) {
+        skipped = block_sum(skipped);
+        if (threadIdx.x == 0 && skipped) atomicAdd(&A.fstats[0], skipped);
     }
 }
 
@@ -886,6 +946,10 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
     constexpr uint32_t NGA = SWEEP_BS / SWEEP_ACT_G;
     uint32_t step = 0, it = 0;
     bool ok = true;
+    // with a learned filter, local moving first runs filtered to convergence, then plain
+    // iterations judge it (the first plain iteration's moves are the filter's errors)
+    bool filter_on = !REFINE && A.filter_w != nullptr;
+    uint32_t judge_it = 0xFFFFFFFFu;
     for (; it < A.max_iters && ok; it++) {
         const uint64_t seed = REFINE ? a.seed : mix64(A.level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
         const uint32_t P = REFINE ? 1u : a.P;
@@ -899,7 +963,14 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
                     if (REFINE) a.targeted_cnt[t] = 0;
                 }
             }
-            sweep_compact<REFINE>(A, seed, sub, step);
+            const float *fw = nullptr;
+            if (filter_on) {  // BF16-rounded weights in LDS (the decide tables reuse it afterwards)
+                float *sw = reinterpret_cast<float *>(s_vals);
+                for (int p = threadIdx.x; p < MLP_PARAMS; p += SWEEP_BS) sw[p] = bf16_round(A.filter_w[p]);
+                __syncthreads();
+                fw = sw;
+            }
+            sweep_compact<REFINE>(A, seed, sub, step, it, fw);
             if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
 
             StepParams sp{seed, sub, it, step, A.mcnt + step, REFINE ? nullptr : A.mstep};
@@ -984,7 +1055,19 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
         } else {
             for (uint32_t s2 = step - P; s2 < step; s2++) moved += ld_acquire_ctr(A.mcnt + s2);
         }
+        if (!REFINE && it == judge_it && gtid == 0) {
+            unsigned long long dec = 0;
+            for (uint32_t s2 = step - P; s2 < step; s2++)
+                for (int b = 0; b < NUM_BUCKETS; b++) dec += ld_acquire_ctr(A.wcnt + (uint64_t)s2 * NUM_BUCKETS + b);
+            atomicAdd(&A.fstats[1], dec);
+            atomicAdd(&A.fstats[2], (unsigned long long)moved);
+        }
         if (moved == 0) {
+            if (filter_on) {
+                filter_on = false;
+                judge_it = it + 1;
+                continue;
+            }
             it++;
             break;
         }
@@ -1068,6 +1151,20 @@ struct RunCtx {
     BatchState *batch = nullptr;
     const uint32_t *vjob = nullptr;
     const JobParams *jobs = nullptr;
+    // learned filter / trace collection (device pointers; null when off) and the level index
+    const struct FilterOpts *fopt = nullptr;
+    uint32_t level = 0;
+};
+
+// Learned-filter inference or trace collection for local moving (device pointers).
+struct FilterOpts {
+    const float *w = nullptr;             // FP32 weights; null: no filtering
+    float logit_thr = 0.0f;               // skip when the logit is below this
+    unsigned long long *fstats = nullptr; // [skip events, judged, judge moves]
+    float *trace_x = nullptr;             // trace runs: samples (features, moved)
+    uint8_t *trace_y = nullptr;
+    unsigned long long *trace_n = nullptr;
+    uint64_t trace_cap = 0;
 };
 
 template <int BS, int G, int T, bool REFINE, bool GTAB>
@@ -1225,6 +1322,11 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     A.claim = ctx.cfg->has_max_community_weight ? ls.claim.get() : nullptr;
     A.level_seed = level_seed;
     A.memb = memb_w;
+    if (!REFINE && ctx.fopt && ctx.fopt->w) {
+        A.filter_w = ctx.fopt->w;
+        A.filter_thr = ctx.fopt->logit_thr;
+        A.fstats = ctx.fopt->fstats;
+    }
     A.Kfx = ls.Kfx.get();
     A.Wfx = ls.Wfx.get();
     A.csize = ls.csize.get();
@@ -1303,6 +1405,13 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
     a.vjob = ctx.vjob;
     a.jobs = ctx.jobs;
+    a.level = ctx.level;
+    if (ctx.fopt && ctx.fopt->trace_x) {
+        a.trace_x = ctx.fopt->trace_x;
+        a.trace_y = ctx.fopt->trace_y;
+        a.trace_n = ctx.fopt->trace_n;
+        a.trace_cap = ctx.fopt->trace_cap;
+    }
     a.memb = memb;
     a.P = MOVE_SUBSWEEPS;
     if (b.start[6] < b.start[7]) {
@@ -1310,7 +1419,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         a.gvals = b.gvals.get();
         a.goff = b.goff.get();
     }
-    if (persistent_sweeps() || ctx.vjob) {
+    if (persistent_sweeps() || ctx.vjob || ctx.fopt) {
         unsigned int out[4];
         run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz, out);
         ctx.st.move_iterations += out[2];
@@ -2033,6 +2142,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         int sw = fixed_scale_for(L.max_strength);
         uint64_t level_seed = mix64(pass_seed ^ (0x9E3779B97F4A7C15ull * (lvl + 1)));
         PhaseClock pc(ctx, s);
+        ctx.level = (uint32_t)lvl;
         if (B) {
             batch_level_params(*B, L, cfg, s);
             ctx.vjob = vjob_l.get();
@@ -2133,11 +2243,13 @@ static void batch_quality(const sc_graph &G0, const uint32_t *labels, const sc_l
 }
 
 void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_labels, uint64_t *n_clusters,
-                   hipStream_t s, sc_leiden_stats *stats_out, BatchState *batch = nullptr) {
+                   hipStream_t s, sc_leiden_stats *stats_out, BatchState *batch = nullptr,
+                   const FilterOpts *fopt = nullptr) {
     validate_config(cfg);
     RunCtx ctx;
     ctx.cfg = &cfg;
     ctx.s = s;
+    ctx.fopt = fopt;
     ctx.timer.on = profiling_enabled();
     hipEvent_t t0, t1;
     HIP_OK(hipEventCreate(&t0));
@@ -2263,6 +2375,87 @@ __global__ void k_union_edges(const UnionSrc *src, uint32_t *nbrs, float *wts) {
 using namespace scgpu;
 
 extern "C" {
+
+static void labels_to_host(const uint32_t *d, uint64_t n, uint64_t *out, hipStream_t s) {
+    DevBuf<uint64_t> l64(n);
+    hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d, l64.get(), n);
+    KERNEL_OK();
+    to_host(out, l64.get(), n, s);
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64_t cap, float *x_out,
+                          uint8_t *y_out, uint64_t *n_out, sc_error *err) {
+    try {
+        if (!g || !cfg || !n_out || (cap && (!x_out || !y_out))) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
+        hipStream_t s = thread_stream();
+        const uint32_t n = (uint32_t)g->n;
+        DevBuf<float> tx(std::max<uint64_t>(cap, 1) * MLP_NF);
+        DevBuf<uint8_t> ty(std::max<uint64_t>(cap, 1));
+        DevBuf<unsigned long long> tn(1);
+        HIP_OK(hipMemsetAsync(tn.get(), 0, sizeof(unsigned long long), s));
+        FilterOpts fo;
+        fo.trace_x = tx.get();
+        fo.trace_y = ty.get();
+        fo.trace_n = tn.get();
+        fo.trace_cap = cap;
+        DevBuf<uint32_t> labels(n);
+        uint64_t nc = 0;
+        sc_leiden_stats st{};
+        leiden_device(*g, *cfg, labels.get(), &nc, s, &st, nullptr, &fo);
+        unsigned long long total = 0;
+        to_host(&total, tn.get(), 1, s);
+        HIP_OK(hipStreamSynchronize(s));
+        *n_out = std::min<uint64_t>(total, cap);
+        to_host(x_out, tx.get(), *n_out * MLP_NF, s);
+        to_host(y_out, ty.get(), *n_out, s);
+        HIP_OK(hipStreamSynchronize(s));
+        t_stats = st;
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
+
+sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, const float *weights, double threshold,
+                             uint64_t *labels_out, uint64_t *n_clusters_out, sc_filter_stats *fst, sc_error *err) {
+    try {
+        if (!g || !cfg || !weights || !n_clusters_out || (!labels_out && g->n))
+            throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (!(threshold > 0.0 && threshold < 1.0)) throw ScError(SC_INVALID_ARGUMENT, "threshold must be in (0, 1)");
+        if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
+        hipStream_t s = thread_stream();
+        const uint32_t n = (uint32_t)g->n;
+        DevBuf<float> w(MLP_PARAMS);
+        DevBuf<unsigned long long> fs(3);
+        to_device(w.get(), weights, MLP_PARAMS, s);
+        HIP_OK(hipMemsetAsync(fs.get(), 0, 3 * sizeof(unsigned long long), s));
+        FilterOpts fo;
+        fo.w = w.get();
+        fo.logit_thr = (float)std::log(threshold / (1.0 - threshold));
+        fo.fstats = fs.get();
+        DevBuf<uint32_t> labels(n);
+        sc_leiden_stats st{};
+        leiden_device(*g, *cfg, labels.get(), n_clusters_out, s, &st, nullptr, &fo);
+        if (n) labels_to_host(labels.get(), n, labels_out, s);
+        unsigned long long h[3] = {0, 0, 0};
+        to_host(h, fs.get(), 3, s);
+        HIP_OK(hipStreamSynchronize(s));
+        if (fst) {
+            fst->skipped = h[0];
+            fst->judged = h[1];
+            fst->judge_moves = h[2];
+            fst->skip_error_rate = h[1] ? (double)h[2] / (double)h[1] : 0.0;
+        }
+        t_stats = st;
+        return SC_OK;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+}
 
 sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, const double *resolutions,
                           uint64_t n_resolutions, const sc_leiden_config *cfg, uint64_t *labels_out,

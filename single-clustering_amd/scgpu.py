@@ -88,6 +88,11 @@ class ScKnnStats(C.Structure):
     ]
 
 
+class ScFilterStats(C.Structure):
+    _fields_ = [("skipped", C.c_uint64), ("judged", C.c_uint64), ("judge_moves", C.c_uint64),
+                ("skip_error_rate", C.c_double)]
+
+
 # Every symbol include/scgpu.h declares, with its ctypes signature.
 _GP = C.c_void_p
 SIGNATURES = {
@@ -112,6 +117,18 @@ SIGNATURES = {
     "sc_graph_validate_symmetry": ([_GP, C.POINTER(ScError)], C.c_int),
     "sc_graph_aggregate": ([_GP, u64p, C.POINTER(_GP), C.POINTER(ScError)], C.c_int),
     "sc_leiden": ([_GP, C.POINTER(ScLeidenConfig), u64p, u64p, C.POINTER(ScError)], C.c_int),
+    "sc_mlp_param_count": ([], C.c_uint32),
+    "sc_mlp_init": ([C.c_uint64, C.POINTER(C.c_float)], None),
+    "sc_mlp_grad": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_uint64, C.c_float,
+                     C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(ScError)], C.c_int),
+    "sc_mlp_train": ([C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
+                      C.c_float, C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(ScError)], C.c_int),
+    "sc_mlp_predict": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint64, C.c_int32, C.POINTER(C.c_float),
+                        C.POINTER(ScError)], C.c_int),
+    "sc_leiden_trace": ([_GP, C.POINTER(ScLeidenConfig), C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                         u64p, C.POINTER(ScError)], C.c_int),
+    "sc_leiden_filtered": ([_GP, C.POINTER(ScLeidenConfig), C.POINTER(C.c_float), C.c_double, u64p, u64p,
+                            C.POINTER(ScFilterStats), C.POINTER(ScError)], C.c_int),
     "sc_leiden_batch": ([C.POINTER(_GP), C.c_uint64, C.POINTER(C.c_double), C.c_uint64, C.POINTER(ScLeidenConfig),
                          u64p, u64p, C.POINTER(ScError)], C.c_int),
     "sc_leiden_last_stats": ([C.POINTER(ScLeidenStats)], C.c_int),
@@ -444,6 +461,95 @@ def leiden_batch(graphs, resolutions, config=None):
         out.append(Clustering(labels[off:off + n].copy(), int(nc[j])))
         off += n
     return out
+
+
+_f32p = C.POINTER(C.c_float)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_f32p)
+
+
+def mlp_param_count():
+    return int(_lib.sc_mlp_param_count())
+
+
+def mlp_init(seed=0):
+    """He-uniform initial parameters of the learned filter (8 -> 64 -> 64 -> 1)."""
+    w = np.zeros(mlp_param_count(), dtype=np.float32)
+    _lib.sc_mlp_init(int(seed), _fp(w))
+    return w
+
+
+def _mlp_data(X, y):
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+    y = np.ascontiguousarray(y, dtype=np.uint8).reshape(-1)
+    assert len(X) == len(y)
+    return X, y
+
+
+def mlp_grad(w, X, y, pos_weight=1.0):
+    """(mean loss, gradient) of the filter's loss on (X, y), from the HIP backward pass."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    X, y = _mlp_data(X, y)
+    g = np.zeros_like(w)
+    loss = C.c_double(0.0)
+    err = ScError()
+    _check(_lib.sc_mlp_grad(_fp(w), _fp(X), y.ctypes.data_as(C.POINTER(C.c_uint8)), len(y), float(pos_weight),
+                            _fp(g), C.byref(loss), C.byref(err)), err)
+    return loss.value, g
+
+
+def mlp_train(X, y, epochs=10, batch=4096, lr=2e-3, pos_weight=1.0, seed=0, w=None):
+    """Minibatch Adam on the GPU; returns (weights, per-epoch mean loss)."""
+    w = mlp_init(seed) if w is None else np.array(w, dtype=np.float32)
+    X, y = _mlp_data(X, y)
+    hist = np.zeros(epochs, dtype=np.float64)
+    err = ScError()
+    _check(_lib.sc_mlp_train(_fp(X), y.ctypes.data_as(C.POINTER(C.c_uint8)), len(y), int(epochs), int(batch),
+                             float(lr), float(pos_weight), int(seed), _fp(w),
+                             hist.ctypes.data_as(C.POINTER(C.c_double)), C.byref(err)), err)
+    return w, hist
+
+
+def mlp_predict(w, X, bf16=True):
+    """Logits of the filter (BF16-rounded weights by default, as local moving evaluates them)."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros(len(X), dtype=np.float32)
+    err = ScError()
+    _check(_lib.sc_mlp_predict(_fp(w), _fp(X), len(X), 1 if bf16 else 0, _fp(out), C.byref(err)), err)
+    return out
+
+
+def leiden_trace(graph, config=None, cap=1 << 22):
+    """Runs leiden() and returns up to cap local-moving decisions: (features[n][8], moved[n])."""
+    config = config or LeidenConfig()
+    c = config.to_c()
+    X = np.zeros((max(cap, 1), 8), dtype=np.float32)
+    y = np.zeros(max(cap, 1), dtype=np.uint8)
+    n = C.c_uint64(0)
+    err = ScError()
+    _check(_lib.sc_leiden_trace(graph.handle, C.byref(c), int(cap), _fp(X), y.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                C.byref(n), C.byref(err)), err)
+    return X[:n.value].copy(), y[:n.value].copy()
+
+
+def leiden_filtered(graph, weights, threshold=0.02, config=None):
+    """leiden() with the learned active-vertex filter in local moving; returns (Clustering,
+    stats) where stats holds the skip count, the judge sweep's size and moves, and the
+    skip-error rate."""
+    config = config or LeidenConfig()
+    c = config.to_c()
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    n = graph.node_count()
+    labels = np.zeros(max(n, 1), dtype=np.uint64)
+    nc = C.c_uint64(0)
+    st = ScFilterStats()
+    err = ScError()
+    _check(_lib.sc_leiden_filtered(graph.handle, C.byref(c), _fp(w), float(threshold), _p(labels, u64p), C.byref(nc),
+                                   C.byref(st), C.byref(err)), err)
+    return Clustering(labels[:n], nc.value), {k: getattr(st, k) for k, _ in ScFilterStats._fields_}
 
 
 def last_stats():

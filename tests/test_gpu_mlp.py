@@ -1,0 +1,81 @@
+"""The learned active-vertex filter: HIP forward/backward against the float64 oracle,
+training, trace collection, and filtered Leiden (no counterpart in the reference)."""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+import scgpu
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, 8)).astype(np.float32)
+    y = (rng.random(n) < 0.3).astype(np.uint8)
+    return X, y
+
+
+def test_hip_gradient_matches_oracle():
+    w = scgpu.mlp_init(5)
+    X, y = _data(3000, 1)
+    for pw in (1.0, 4.0):
+        loss, g = scgpu.mlp_grad(w, X, y, pw)
+        g_ref = M.grad(w.astype(np.float64), X, y, pw)
+        assert abs(loss - M.loss(w, X, y, pw)) <= 1e-5 * max(1.0, abs(loss))
+        assert np.max(np.abs(g - g_ref)) <= 2e-5 * max(1.0, np.max(np.abs(g_ref))), np.max(np.abs(g - g_ref))
+
+
+def test_predict_matches_oracle_fp32_and_bf16():
+    w = scgpu.mlp_init(9)
+    X, _ = _data(5000, 2)
+    z32 = scgpu.mlp_predict(w, X, bf16=False)
+    z_ref, _ = M.forward(w, X)
+    assert np.max(np.abs(z32 - z_ref)) <= 1e-4 * max(1.0, np.max(np.abs(z_ref)))
+    # BF16-rounded weights (round to nearest even), FP32 accumulation
+    u = w.view(np.uint32).astype(np.uint64)
+    wb = (((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)).view(np.float32)
+    zb = scgpu.mlp_predict(w, X, bf16=True)
+    zb_ref, _ = M.forward(wb, X)
+    assert np.max(np.abs(zb - zb_ref)) <= 1e-4 * max(1.0, np.max(np.abs(zb_ref)))
+
+
+def test_training_learns_a_separable_rule():
+    rng = np.random.default_rng(4)
+    X = rng.random((20000, 8)).astype(np.float32)
+    y = (X[:, 0] + 0.5 * X[:, 3] > 0.8).astype(np.uint8)
+    w0 = scgpu.mlp_init(1)
+    l0, _ = scgpu.mlp_grad(w0, X, y)
+    w, hist = scgpu.mlp_train(X, y, epochs=15, batch=1024, lr=3e-3, seed=1, w=w0)
+    assert hist[-1] < 0.5 * l0 and hist[-1] < hist[0]
+    acc = np.mean((scgpu.mlp_predict(w, X) > 0) == (y == 1))
+    assert acc > 0.95, acc
+    w2, hist2 = scgpu.mlp_train(X, y, epochs=15, batch=1024, lr=3e-3, seed=1, w=w0)
+    assert np.array_equal(w, w2) and np.array_equal(hist, hist2)  # deterministic
+
+
+def _knn(n, seed):
+    x, _ = W.gaussian_mixture(n, 16, 10, 9.0, seed=seed)
+    return scgpu.knn_graph(x, 10)
+
+
+def test_trace_and_filtered_leiden():
+    g_train, g = _knn(6000, 11), _knn(8000, 12)
+    X, y = scgpu.leiden_trace(g_train, scgpu.LeidenConfig(seed=3))
+    assert len(X) > 1000 and X.shape[1] == 8 and 0 < y.mean() < 1
+    assert np.all(np.isfinite(X))
+    w, _ = scgpu.mlp_train(X, y, epochs=20, batch=2048, lr=3e-3, seed=2)
+    plain = scgpu.leiden(g)
+    # a filter that never skips reproduces plain Leiden exactly
+    never = w.copy()
+    never[-1] = 1e4
+    c, st = scgpu.leiden_filtered(g, never, 0.5)
+    assert np.array_equal(c.labels(), plain.labels()) and st["skipped"] == 0
+    c, st = scgpu.leiden_filtered(g, w, 0.05)
+    q_plain = scgpu.modularity(g, plain.labels())
+    q = scgpu.modularity(g, c.labels())
+    print(f"filter: skipped {st['skipped']}, judged {st['judged']}, errors {st['judge_moves']} "
+          f"(rate {st['skip_error_rate']:.4f}); modularity {q:.6f} vs plain {q_plain:.6f}")
+    assert q >= q_plain - 0.01 * abs(q_plain)
+    assert 0.0 <= st["skip_error_rate"] <= 1.0

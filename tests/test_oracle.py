@@ -326,3 +326,27 @@ def test_gaussian_union_symmetrization():
     assert ptrs[-1] == 2 * len(pairs)
     g = O.Graph.from_csr_parts(ptrs, nbrs, wts)
     assert g.validate_symmetry()
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The filter's float64 oracle backward pass equals central differences of its loss on
+    random parameters (checked on 80 parameters spread over all six tensors)."""
+    import mlp_oracle as M
+
+    rng = np.random.default_rng(3)
+    sizes = M.SIZES
+    w = rng.normal(0, 0.4, sum(sizes))
+    X = rng.random((40, 8))
+    y = (rng.random(40) < 0.4).astype(np.uint8)
+    for pw in (1.0, 3.0):
+        g = M.grad(w, X, y, pw)
+        offs = np.cumsum([0] + sizes)
+        picks = np.concatenate([rng.choice(np.arange(offs[i], offs[i + 1]), min(sizes[i], 14), replace=False)
+                                for i in range(len(sizes))])
+        h = 1e-6
+        for p in picks:
+            wp, wm = w.copy(), w.copy()
+            wp[p] += h
+            wm[p] -= h
+            fd = (M.loss(wp, X, y, pw) - M.loss(wm, X, y, pw)) / (2 * h)
+            assert abs(fd - g[p]) <= 1e-6 + 1e-5 * abs(fd), (p, fd, g[p])
