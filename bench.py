@@ -151,11 +151,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=8,
+                    help="untimed steps (the online autotuner measures its candidates during these)")
     ap.add_argument("--config", default="knn_leiden", choices=["knn_leiden", "small", "large", "batch"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-filter comparison")
+    ap.add_argument("--filter-threshold", type=float, default=0.05)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -252,6 +255,44 @@ def main():
     if knn_roof and knn_roof["kernel_ms_per_step"] > leiden_roof["kernel_ms_per_step"]:
         dominant = knn_roof
 
+    # learned active-vertex filter vs plain Leiden on the same graph (trained on traces from
+    # graphs of the same distribution built with other seeds)
+    learned = None
+    if not args.no_learned and wl["kind"] == "knn":
+        import workloads as W
+
+        t0 = time.perf_counter()
+        Xs, ys = [], []
+        n_tr = min(wl["n"], 100_000)
+        for sd in (101, 102):
+            if args.config == "large":
+                xt, _ = W.gaussian_mixture(n_tr, 30, 200, 9.0, sd=1.0, alpha=1.0, seed=sd)
+            else:
+                xt, _ = W.gaussian_mixture(n_tr, wl["d"], 30, 16.0, sd=1.0, alpha=2.0, seed=sd)
+            gt = scgpu.knn_graph(xt, wl["k"])
+            X, y = scgpu.leiden_trace(gt, cfg)
+            Xs.append(X)
+            ys.append(y)
+        X, y = np.concatenate(Xs), np.concatenate(ys)
+        w, hist = scgpu.mlp_train(X, y, epochs=8, batch=8192, lr=3e-3, seed=7)
+        t_train = time.perf_counter() - t0
+        tl_f, runs = [], []
+        for i in range(max(args.steps, 1) + 1):
+            t1 = time.perf_counter()
+            cf, fst = scgpu.leiden_filtered(g, w, args.filter_threshold, cfg)
+            if i:  # first run is a warm-up
+                tl_f.append(time.perf_counter() - t1)
+            runs.append(fst)
+        q_f = scgpu.modularity(g, cf.labels(), 1.0)
+        learned = {"edges_per_s": m / float(np.mean(tl_f)), "leiden_ms": float(np.mean(tl_f)) * 1e3,
+                   "ratio_vs_plain": tl / float(np.mean(tl_f)), "modularity": q_f,
+                   "modularity_rel_diff": (q_f - q) / abs(q) if q else None, "threshold": args.filter_threshold,
+                   "skipped": runs[-1]["skipped"], "judged": runs[-1]["judged"],
+                   "judge_moves": runs[-1]["judge_moves"], "skip_error_rate": runs[-1]["skip_error_rate"],
+                   "train_samples": int(len(y)), "train_seconds": t_train, "final_train_loss": float(hist[-1])}
+        log(f"[bench] learned filter: {learned['leiden_ms']:.2f} ms (plain {tl * 1e3:.2f}), "
+            f"Q {q_f:.10f} vs {q:.10f}, skip-error rate {learned['skip_error_rate']:.4f}")
+
     # CPU baseline: the oracle's single-threaded Leiden on the same graph, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
@@ -301,8 +342,10 @@ def main():
         "knn_stats": kstats[-1],
         "roofline": dominant,
         "roofline_other": knn_roof if dominant is leiden_roof else leiden_roof,
+        "learned_filter": learned,
         "cpu_baseline": cpu,
     }
+    log("[bench] autotuner:\n" + scgpu.autotune_report())
     print(json.dumps(out), flush=True)
 
 

@@ -11,6 +11,7 @@
 #ifndef SCGPU_H
 #define SCGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -121,6 +122,13 @@ void sc_set_profiling(int32_t enable);
  * between sub-sweeps; 0 = one launch per kernel of each sub-sweep. Both give identical labels.
  * The environment variable SCGPU_PERSISTENT=0 selects 0 at start-up. */
 sc_status sc_set_sweep_mode(int32_t mode);
+/* No counterpart in the reference. Online autotuner of launch parameters that never change
+ * results (workgroups per persistent sweep phase, kNN column splits): the first calls for a
+ * shape try each candidate once, later calls use the fastest. The table is process-wide and
+ * mutex-guarded; SCGPU_AUTOTUNE=0 disables it. The report has one line per tuned shape (best
+ * time per candidate in ms, then the chosen index); returns its length. */
+size_t sc_autotune_report(char *buf, size_t len);
+void sc_autotune_reset(void);
 
 /* LeidenConfig::default (leiden/mod.rs:92-106). */
 sc_leiden_config sc_leiden_config_default(void);

@@ -20,6 +20,10 @@
 //  4. gaussian: sigma_i from the k-th (or median) distance, w_ij in f64 stored as f32,
 //     symmetrized, sorted into CSR and ingested without leaving HBM.
 #include "common.hpp"
+#include "tune.hpp"
+
+#include <string>
+#include <vector>
 
 #include <algorithm>
 #include <cmath>
@@ -610,6 +614,16 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
             splits = sp;
         }
     }
+    // the online tuner measures the alternatives when the grid is small enough for the last
+    // wave of workgroups to matter (the result is exact for every split count)
+    std::vector<uint32_t> cand{splits};
+    if (row_blocks < 4 * slots)
+        for (uint32_t sp = splits > 2 ? splits - 2 : 1; sp <= splits + 2 && sp <= 8 && sp * K2 <= (uint32_t)KNN_MAXCAND; sp++)
+            if (sp != splits) cand.push_back(sp);
+    const std::string tkey = "knn_tile/rb2^" + std::to_string(bits_for(row_blocks)) + "/kc" + std::to_string(nkc) +
+                             "/km" + std::to_string(K2) + (bf16 ? "/bf16" : "/fp32");
+    const TuneChoice tc = tune_enabled() && cand.size() > 1 ? tune_pick(tkey, (int)cand.size(), 1) : TuneChoice{0, false};
+    splits = cand[tc.index];
     uint32_t cps = (npad + splits - 1) / splits;
     cps = (cps + KNN_TN - 1) / KNN_TN * KNN_TN;
     splits = (npad + cps - 1) / cps;
@@ -687,6 +701,8 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
     (void)hipEventDestroy(e3);
+    // the whole search is timed: a split count that starves the certificate pays in fallbacks
+    if (tune_enabled() && cand.size() > 1) tune_record(tkey, tc.index, t_all);
     if (st) {
         st->n = n;
         st->candidates_rescored = nr;

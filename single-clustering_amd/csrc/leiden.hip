@@ -26,6 +26,7 @@
 //    bit-identical to the CPU path.
 #include "common.hpp"
 #include "mlp.hpp"
+#include "tune.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -1270,8 +1271,9 @@ static bool persistent_sweeps() {
 
 // Workgroups of a sweep launch: all must be resident at once (cooperative launch); small
 // levels use fewer, which makes each grid barrier cheaper.
+// Workgroups of a sweep launch that can be resident at once.
 template <bool REFINE>
-static uint32_t sweep_grid(uint32_t n, uint64_t nnz) {
+static uint32_t sweep_resident() {
     static thread_local int resident[64] = {0};
     int dev = 0;
     HIP_OK(hipGetDevice(&dev));
@@ -1282,11 +1284,17 @@ static uint32_t sweep_grid(uint32_t n, uint64_t nnz) {
         HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         r = std::max(1, per) * std::max(1, cus);
     }
+    return (uint32_t)r;
+}
+
+template <bool REFINE>
+static uint32_t sweep_grid(uint32_t n, uint64_t nnz) {
+    const uint32_t r = sweep_resident<REFINE>();
     static const long long forced = [] {
         const char *e = std::getenv("SCGPU_SWEEP_BLOCKS");
         return e ? std::atoll(e) : 0ll;
     }();
-    const uint64_t by_work = std::max<uint64_t>((n + 63) / 64, (nnz + 2047) / 2048);
+    const uint64_t by_work = std::max<uint64_t>(((uint64_t)n + 63) / 64, nnz / 2048 + 1);
     const uint32_t want = forced > 0 ? (uint32_t)forced : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8, by_work), 1u << 20);
     return std::min<uint32_t>((uint32_t)r, want);
 }
@@ -1332,7 +1340,20 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     A.csize = ls.csize.get();
     HIP_OK(hipMemsetAsync(ls.sweep_ctl.get(), 0, sizeof(BarrierCtl), s));
     HIP_OK(hipMemsetAsync(A.ctl, 0, 4 * sizeof(unsigned int), s));
-    const uint32_t grid = sweep_grid<REFINE>(n, nnz);
+    // workgroup count: the heuristic first, then a few alternatives the online tuner measures
+    // (any count gives identical labels, so exploring never changes a result)
+    const uint32_t heur = sweep_grid<REFINE>(n, nnz);
+    std::vector<uint32_t> grids{heur};
+    {
+        const uint32_t res = sweep_resident<REFINE>();
+        const uint32_t lo = std::max<uint32_t>(8u, heur / 4);
+        for (uint32_t c : {heur / 2, heur * 2, res})
+            if (c >= lo && c <= res && std::find(grids.begin(), grids.end(), c) == grids.end()) grids.push_back(c);
+    }
+    const std::string tkey = std::string(REFINE ? "sweep/refine" : "sweep/move") + "/n2^" +
+                             std::to_string(bits_for(n)) + "/e2^" + std::to_string(bits_for(nnz));
+    const TuneChoice tc = tune_enabled() ? tune_pick(tkey, (int)grids.size()) : TuneChoice{0, false};
+    const uint32_t grid = grids[tc.index];
     constexpr uint32_t TRACE_CAP = 2048;
     DevBuf<long long> trace;
     if (sweep_trace_on()) {
@@ -1343,14 +1364,27 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         KERNEL_OK();
     }
     void *args[] = {&A};
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr, t0 = nullptr, t1 = nullptr;
+    if (tune_enabled()) {
+        HIP_OK(hipEventCreate(&t0));
+        HIP_OK(hipEventCreate(&t1));
+        HIP_OK(hipEventRecord(t0, s));
+    }
     if (ctx.timer.on && !REFINE) ctx.timer.begin(s, ev);
     HIP_OK(hipLaunchCooperativeKernel((const void *)k_sweep<REFINE>, dim3(grid), dim3(SWEEP_BS), args, 0, s));
     if (ctx.timer.on && !REFINE) ctx.timer.end(s, ev);
+    if (t1) HIP_OK(hipEventRecord(t1, s));
     unsigned int ctl[4], berr = 0;
     to_host(ctl, A.ctl, 4, s);
     to_host(&berr, &A.bc->err, 1, s);
     HIP_OK(hipStreamSynchronize(s));
+    if (t1) {
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        tune_record(tkey, tc.index, ms);
+    }
     if (ctx.timer.on) ctx.timer.harvest();
     if (berr) throw ScError(SC_DEVICE_ERROR, "Leiden sweep: a grid barrier timed out");
     for (int i = 0; i < 4; i++) out[i] = ctl[i];

@@ -1,6 +1,9 @@
 // util.hip -- allocator, streams, error plumbing and device-wide primitives.
 #include "common.hpp"
 
+#include <algorithm>
+#include <cstring>
+
 #include <hipcub/hipcub.hpp>
 
 #include <atomic>
@@ -311,5 +314,96 @@ void sc_shutdown(void) {
     (void)hipDeviceSynchronize();
     scgpu::DevicePool::get().trim();
 }
+
+}
+
+// ---- online autotuner ---------------------------------------------------------------------
+#include "tune.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+
+namespace scgpu {
+namespace {
+struct TuneEntry {
+    std::vector<double> best;  // best time seen per candidate (ms), < 0 = not measured
+    std::vector<int> count;    // measurements per candidate
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneEntry> g_tune;
+}  // namespace
+
+bool tune_enabled() {
+    static const bool on = [] {
+        const char *e = std::getenv("SCGPU_AUTOTUNE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+TuneChoice tune_pick(const std::string &key, int n_candidates, int reps) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneEntry &e = g_tune[key];
+    if ((int)e.best.size() != n_candidates) {
+        e.best.assign(n_candidates, -1.0);
+        e.count.assign(n_candidates, 0);
+    }
+    int least = 0;
+    for (int i = 1; i < n_candidates; i++)
+        if (e.count[i] < e.count[least]) least = i;
+    if (e.count[least] < reps) return TuneChoice{least, true};
+    int b = 0;
+    for (int i = 1; i < n_candidates; i++)
+        if (e.best[i] < e.best[b]) b = i;
+    return TuneChoice{b, false};
+}
+
+void tune_record(const std::string &key, int index, double ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneEntry &e = g_tune[key];
+    if (index < 0 || index >= (int)e.best.size()) return;
+    e.count[index]++;
+    if (e.best[index] < 0.0 || ms < e.best[index]) e.best[index] = ms;
+}
+
+std::string tune_report() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out;
+    for (const auto &kv : g_tune) {
+        out += kv.first + ":";
+        int b = -1;
+        for (int i = 0; i < (int)kv.second.best.size(); i++) {
+            char buf[48];
+            std::snprintf(buf, sizeof(buf), " %.3f", kv.second.best[i]);
+            out += buf;
+            if (kv.second.best[i] >= 0.0 && (b < 0 || kv.second.best[i] < kv.second.best[b])) b = i;
+        }
+        out += " -> " + std::to_string(b) + "\n";
+    }
+    return out;
+}
+
+void tune_reset() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+}  // namespace scgpu
+
+extern "C" {
+
+size_t sc_autotune_report(char *buf, size_t len) {
+    const std::string r = scgpu::tune_report();
+    if (buf && len) {
+        const size_t c = std::min(len - 1, r.size());
+        std::memcpy(buf, r.data(), c);
+        buf[c] = 0;
+    }
+    return r.size();
+}
+
+void sc_autotune_reset(void) { scgpu::tune_reset(); }
 
 }

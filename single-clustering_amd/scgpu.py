@@ -117,6 +117,8 @@ SIGNATURES = {
     "sc_graph_validate_symmetry": ([_GP, C.POINTER(ScError)], C.c_int),
     "sc_graph_aggregate": ([_GP, u64p, C.POINTER(_GP), C.POINTER(ScError)], C.c_int),
     "sc_leiden": ([_GP, C.POINTER(ScLeidenConfig), u64p, u64p, C.POINTER(ScError)], C.c_int),
+    "sc_autotune_report": ([C.c_char_p, C.c_size_t], C.c_size_t),
+    "sc_autotune_reset": ([], None),
     "sc_mlp_param_count": ([], C.c_uint32),
     "sc_mlp_init": ([C.c_uint64, C.POINTER(C.c_float)], None),
     "sc_mlp_grad": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_uint64, C.c_float,
@@ -468,6 +470,18 @@ _f32p = C.POINTER(C.c_float)
 
 def _fp(a):
     return a.ctypes.data_as(_f32p)
+
+
+def autotune_report():
+    """The online autotuner's table: one line per tuned shape."""
+    n = _lib.sc_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    _lib.sc_autotune_report(buf, n + 1)
+    return buf.value.decode()
+
+
+def autotune_reset():
+    _lib.sc_autotune_reset()
 
 
 def mlp_param_count():

@@ -1,0 +1,28 @@
+"""The online autotuner only tunes launch parameters whose every candidate gives identical
+results: repeated calls (which explore all candidates) must return identical outputs."""
+import numpy as np
+import pytest
+
+import scgpu
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotuner_explores_without_changing_results():
+    scgpu.autotune_reset()
+    x, _ = W.gaussian_mixture(20_000, 24, 12, 9.0, seed=8)
+    first_idx, first_d2 = scgpu.knn(x, 10)
+    g = scgpu.knn_graph(x, 10)
+    first = scgpu.leiden(g)
+    for _ in range(8):
+        idx, d2 = scgpu.knn(x, 10)
+        assert np.array_equal(idx, first_idx) and np.array_equal(d2, first_d2)
+        assert np.array_equal(scgpu.leiden(g).labels(), first.labels())
+    rep = scgpu.autotune_report()
+    print(rep)
+    assert "sweep/move" in rep and "sweep/refine" in rep and "knn_tile" in rep
+    # every candidate of the kNN key has been measured by now
+    line = [l for l in rep.splitlines() if l.startswith("knn_tile")][0]
+    times = [float(t) for t in line.split(":")[1].split("->")[0].split()]
+    assert all(t >= 0 for t in times)
