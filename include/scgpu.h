@@ -209,7 +209,8 @@ sc_status sc_mlp_grad(const float *w, const float *X, const uint8_t *y, uint64_t
 /* Minibatch Adam with HIP forward/backward kernels; loss_hist (nullable) gets each epoch's loss. */
 sc_status sc_mlp_train(const float *X, const uint8_t *y, uint64_t n, uint32_t epochs, uint32_t batch, float lr,
                        float pos_weight, uint64_t seed, float *w_inout, double *loss_hist, sc_error *err);
-/* Logits of n samples (bf16 != 0: BF16-rounded weights, as the filter evaluates them). */
+/* Logits of n samples. bf16 == 0: FP32 weights (the training arithmetic); bf16 != 0: the
+ * filter's inference, BF16-rounded weights on the wave-level MFMA path. */
 sc_status sc_mlp_predict(const float *w, const float *X, uint64_t n, int32_t bf16, float *out, sc_error *err);
 
 /* Runs leiden() and records up to cap local-moving decisions as training samples
@@ -225,10 +226,13 @@ typedef struct {
 } sc_filter_stats;
 
 /* leiden() whose local moving skips active vertices the model gives a move probability below
- * threshold, until nothing moves; then plain iterations run to convergence (the judge), so the
- * result is a converged plain local-moving state at every level. */
+ * threshold: a skipped vertex leaves the active set as if it had decided to stay (a neighbour's
+ * move re-activates it), until nothing moves. judge != 0: the vertices whose last evaluation was
+ * a skip are then decided by plain iterations run to convergence (the judge), so every level
+ * ends in a converged plain local-moving state and stats counts the filter's errors (moves in
+ * the first judge iteration). judge == 0: no judge runs and only stats->skipped is counted. */
 sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, const float *weights,
-                             double threshold, uint64_t *labels_out, uint64_t *n_clusters_out,
+                             double threshold, int32_t judge, uint64_t *labels_out, uint64_t *n_clusters_out,
                              sc_filter_stats *stats, sc_error *err);
 
 /* Counters of the last sc_leiden on the calling thread. */

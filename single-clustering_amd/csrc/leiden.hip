@@ -104,6 +104,7 @@ struct DecideArgs {
     uint32_t level;
     float *trace_x;
     uint8_t *trace_y;
+    uint64_t *trace_key;  // (level << 56 | iteration << 32 | vertex): the samples' canonical order
     unsigned long long *trace_n;
     uint64_t trace_cap;
 };
@@ -359,6 +360,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
 #pragma unroll
                     for (int f = 0; f < MLP_NF; f++) a.trace_x[slot * MLP_NF + f] = x[f];
                     a.trace_y[slot] = accept ? 1 : 0;
+                    a.trace_key[slot] = ((uint64_t)a.level << 56) | ((uint64_t)min(sp.round, 0xFFFFFFu) << 32) | v;
                 }
             }
             if (accept) {
@@ -678,10 +680,14 @@ struct SweepArgs {
     uint64_t level_seed;
     long long *trace;       // optional: real-time stamps after each barrier (workgroup 0)
     uint32_t trace_cap;
-    // learned filter (local moving): FP32 weights (null = plain), skip when logit < filter_thr;
-    // fstats: [0] skip events, [1] vertices the first plain iteration decided, [2] of them moved
-    const float *filter_w;
+    // learned filter (local moving): packed BF16 weights (null = plain), skip when the logit is
+    // below filter_thr. A skipped vertex leaves the active set like one that decided to stay;
+    // filter_judge: it keeps a mark, and when the filtered iterations converge the marked
+    // vertices are decided by plain iterations run to convergence (the judge).
+    // fstats: [0] skip events, [1] vertices the judge iteration decided, [2] of them moved
+    const MlpPacked *filter_pk;
     float filter_thr;
+    int filter_judge;
     unsigned long long *fstats;
 };
 
@@ -781,9 +787,9 @@ __device__ __forceinline__ int bucket_of_pos(const SweepArgs &A, uint32_t i) {
 
 // Positions (within their bucket) of the vertices that decide in this step, wave-aggregated
 // appends into work[bstart[b] + ...].
-template <bool REFINE>
+template <bool REFINE, bool FILTER>
 __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t step,
-                                              uint32_t it, const float *fw) {
+                                              uint32_t it, bool filter_on) {
     const DecideArgs &a = A.d;
     const uint32_t n = a.n;
     const int lane = threadIdx.x & 63;
@@ -793,20 +799,26 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
         const uint32_t i = i0 + threadIdx.x;
         bool take = false;
         int b = 0;
+        uint32_t v = 0;
+        float x[MLP_NF] = {};
         if (i < n) {
-            const uint32_t v = A.lists[i];
+            v = A.lists[i];
             b = bucket_of_pos(A, i);
             const VParams vp = vparams(a, v);
-            take = a.active[v] != 0 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
+            take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
             if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
-            if (!REFINE && fw && take) {
-                // learned filter: a vertex predicted to stay is not decided now (it stays active)
-                float x[MLP_NF];
-                vertex_features(a, vp, A.mstep, v, it, x);
-                if (mlp_forward(fw, x) < A.filter_thr) {
-                    take = false;
-                    skipped++;
-                }
+            if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
+        }
+        // learned filter: a vertex predicted to stay is not decided now. The loop bounds are
+        // workgroup-uniform, so every lane of the wave reaches the wave-collective inference.
+        if (FILTER && filter_on && __any(take)) {
+            const float z = mlp_forward_wave(A.filter_pk, x);
+            if (take && z < A.filter_thr) {
+                // it leaves the active set like a vertex that decided to stay (a neighbour's
+                // move re-activates it); with a judge it keeps a mark (2) for the judge
+                take = false;
+                skipped++;
+                a.active[v] = A.filter_judge ? 2 : 0;
             }
         }
         uint64_t pending = __ballot(take);
@@ -826,7 +838,7 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
             pending &= ~m;
         }
     }
-    if (!REFINE && fw) {
+    if (FILTER && filter_on) {
         skipped = block_sum(skipped);
         if (threadIdx.x == 0 && skipped) atomicAdd(&A.fstats[0], skipped);
     }
@@ -893,8 +905,8 @@ __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint
 
 __global__ void k_stamp_clock(long long *out) { *out = wall_clock64(); }
 
-template <bool REFINE>
-__global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
+template <bool REFINE, bool FILTER>
+__global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3))) k_sweep(SweepArgs A) {
     __shared__ uint32_t s_keys[SWEEP_TAB];
     __shared__ long long s_vals[SWEEP_TAB];
     __shared__ double r_s[SWEEP_BS / 64];
@@ -947,9 +959,9 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
     constexpr uint32_t NGA = SWEEP_BS / SWEEP_ACT_G;
     uint32_t step = 0, it = 0;
     bool ok = true;
-    // with a learned filter, local moving first runs filtered to convergence, then plain
-    // iterations judge it (the first plain iteration's moves are the filter's errors)
-    bool filter_on = !REFINE && A.filter_w != nullptr;
+    // with a learned filter, local moving runs filtered to convergence; with a judge, plain
+    // iterations follow (the first plain iteration's moves are the filter's errors)
+    bool filter_on = FILTER;
     uint32_t judge_it = 0xFFFFFFFFu;
     for (; it < A.max_iters && ok; it++) {
         const uint64_t seed = REFINE ? a.seed : mix64(A.level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
@@ -964,14 +976,7 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
                     if (REFINE) a.targeted_cnt[t] = 0;
                 }
             }
-            const float *fw = nullptr;
-            if (filter_on) {  // BF16-rounded weights in LDS (the decide tables reuse it afterwards)
-                float *sw = reinterpret_cast<float *>(s_vals);
-                for (int p = threadIdx.x; p < MLP_PARAMS; p += SWEEP_BS) sw[p] = bf16_round(A.filter_w[p]);
-                __syncthreads();
-                fw = sw;
-            }
-            sweep_compact<REFINE>(A, seed, sub, step, it, fw);
+            sweep_compact<REFINE, FILTER>(A, seed, sub, step, it, filter_on);
             if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
 
             StepParams sp{seed, sub, it, step, A.mcnt + step, REFINE ? nullptr : A.mstep};
@@ -1064,9 +1069,14 @@ __global__ void __launch_bounds__(SWEEP_BS) k_sweep(SweepArgs A) {
             atomicAdd(&A.fstats[2], (unsigned long long)moved);
         }
         if (moved == 0) {
-            if (filter_on) {
+            if (filter_on && A.filter_judge) {
+                // the judge: vertices whose last evaluation was a skip are decided by plain
+                // iterations, which then run to convergence
                 filter_on = false;
                 judge_it = it + 1;
+                for (uint64_t v = gtid; v < n; v += gsize)
+                    if (a.active[v] == 2) a.active[v] = 1;
+                if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
                 continue;
             }
             it++;
@@ -1159,11 +1169,13 @@ struct RunCtx {
 
 // Learned-filter inference or trace collection for local moving (device pointers).
 struct FilterOpts {
-    const float *w = nullptr;             // FP32 weights; null: no filtering
+    const MlpPacked *pk = nullptr;        // packed BF16 weights; null: no filtering
+    bool judge = true;                    // decide the skipped vertices with plain iterations at the end
     float logit_thr = 0.0f;               // skip when the logit is below this
     unsigned long long *fstats = nullptr; // [skip events, judged, judge moves]
-    float *trace_x = nullptr;             // trace runs: samples (features, moved)
+    float *trace_x = nullptr;             // trace runs: samples (features, moved, order key)
     uint8_t *trace_y = nullptr;
+    uint64_t *trace_key = nullptr;
     unsigned long long *trace_n = nullptr;
     uint64_t trace_cap = 0;
 };
@@ -1272,7 +1284,7 @@ static bool persistent_sweeps() {
 // Workgroups of a sweep launch: all must be resident at once (cooperative launch); small
 // levels use fewer, which makes each grid barrier cheaper.
 // Workgroups of a sweep launch that can be resident at once.
-template <bool REFINE>
+template <bool REFINE, bool FILTER = false>
 static uint32_t sweep_resident() {
     static thread_local int resident[64] = {0};
     int dev = 0;
@@ -1280,16 +1292,16 @@ static uint32_t sweep_resident() {
     int &r = resident[dev & 63];
     if (!r) {
         int per = 0, cus = 0;
-        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep<REFINE>, SWEEP_BS, 0));
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep<REFINE, FILTER>, SWEEP_BS, 0));
         HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         r = std::max(1, per) * std::max(1, cus);
     }
     return (uint32_t)r;
 }
 
-template <bool REFINE>
+template <bool REFINE, bool FILTER = false>
 static uint32_t sweep_grid(uint32_t n, uint64_t nnz) {
-    const uint32_t r = sweep_resident<REFINE>();
+    const uint32_t r = sweep_resident<REFINE, FILTER>();
     static const long long forced = [] {
         const char *e = std::getenv("SCGPU_SWEEP_BLOCKS");
         return e ? std::atoll(e) : 0ll;
@@ -1330,9 +1342,11 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     A.claim = ctx.cfg->has_max_community_weight ? ls.claim.get() : nullptr;
     A.level_seed = level_seed;
     A.memb = memb_w;
-    if (!REFINE && ctx.fopt && ctx.fopt->w) {
-        A.filter_w = ctx.fopt->w;
+    const bool filtered = !REFINE && ctx.fopt && ctx.fopt->pk;
+    if (filtered) {
+        A.filter_pk = ctx.fopt->pk;
         A.filter_thr = ctx.fopt->logit_thr;
+        A.filter_judge = ctx.fopt->judge ? 1 : 0;
         A.fstats = ctx.fopt->fstats;
     }
     A.Kfx = ls.Kfx.get();
@@ -1342,15 +1356,15 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     HIP_OK(hipMemsetAsync(A.ctl, 0, 4 * sizeof(unsigned int), s));
     // workgroup count: the heuristic first, then a few alternatives the online tuner measures
     // (any count gives identical labels, so exploring never changes a result)
-    const uint32_t heur = sweep_grid<REFINE>(n, nnz);
+    const uint32_t heur = filtered ? sweep_grid<false, true>(n, nnz) : sweep_grid<REFINE>(n, nnz);
     std::vector<uint32_t> grids{heur};
     {
-        const uint32_t res = sweep_resident<REFINE>();
+        const uint32_t res = filtered ? sweep_resident<false, true>() : sweep_resident<REFINE>();
         const uint32_t lo = std::max<uint32_t>(8u, heur / 4);
         for (uint32_t c : {heur / 2, heur * 2, res})
             if (c >= lo && c <= res && std::find(grids.begin(), grids.end(), c) == grids.end()) grids.push_back(c);
     }
-    const std::string tkey = std::string(REFINE ? "sweep/refine" : "sweep/move") + "/n2^" +
+    const std::string tkey = std::string(REFINE ? "sweep/refine" : (filtered ? "sweep/move-filtered" : "sweep/move")) + "/n2^" +
                              std::to_string(bits_for(n)) + "/e2^" + std::to_string(bits_for(nnz));
     const TuneChoice tc = tune_enabled() ? tune_pick(tkey, (int)grids.size()) : TuneChoice{0, false};
     const uint32_t grid = grids[tc.index];
@@ -1371,7 +1385,8 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         HIP_OK(hipEventRecord(t0, s));
     }
     if (ctx.timer.on && !REFINE) ctx.timer.begin(s, ev);
-    HIP_OK(hipLaunchCooperativeKernel((const void *)k_sweep<REFINE>, dim3(grid), dim3(SWEEP_BS), args, 0, s));
+    const void *kern = filtered ? (const void *)k_sweep<false, true> : (const void *)k_sweep<REFINE, false>;
+    HIP_OK(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(SWEEP_BS), args, 0, s));
     if (ctx.timer.on && !REFINE) ctx.timer.end(s, ev);
     if (t1) HIP_OK(hipEventRecord(t1, s));
     unsigned int ctl[4], berr = 0;
@@ -1443,6 +1458,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     if (ctx.fopt && ctx.fopt->trace_x) {
         a.trace_x = ctx.fopt->trace_x;
         a.trace_y = ctx.fopt->trace_y;
+        a.trace_key = ctx.fopt->trace_key;
         a.trace_n = ctx.fopt->trace_n;
         a.trace_cap = ctx.fopt->trace_cap;
     }
@@ -2427,11 +2443,13 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
         const uint32_t n = (uint32_t)g->n;
         DevBuf<float> tx(std::max<uint64_t>(cap, 1) * MLP_NF);
         DevBuf<uint8_t> ty(std::max<uint64_t>(cap, 1));
+        DevBuf<uint64_t> tk(std::max<uint64_t>(cap, 1));
         DevBuf<unsigned long long> tn(1);
         HIP_OK(hipMemsetAsync(tn.get(), 0, sizeof(unsigned long long), s));
         FilterOpts fo;
         fo.trace_x = tx.get();
         fo.trace_y = ty.get();
+        fo.trace_key = tk.get();
         fo.trace_n = tn.get();
         fo.trace_cap = cap;
         DevBuf<uint32_t> labels(n);
@@ -2441,10 +2459,29 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
         unsigned long long total = 0;
         to_host(&total, tn.get(), 1, s);
         HIP_OK(hipStreamSynchronize(s));
-        *n_out = std::min<uint64_t>(total, cap);
-        to_host(x_out, tx.get(), *n_out * MLP_NF, s);
-        to_host(y_out, ty.get(), *n_out, s);
+        const uint64_t m = std::min<uint64_t>(total, cap);
+        *n_out = m;
+        // samples arrive in atomic-slot order; sorting by (level, iteration, vertex) and then by
+        // content (levels repeat across passes) makes the trace, and a model trained on it,
+        // reproducible (when the cap is not reached)
+        std::vector<float> hx(m * MLP_NF);
+        std::vector<uint8_t> hy(m);
+        std::vector<uint64_t> hk(m);
+        to_host(hx.data(), tx.get(), m * MLP_NF, s);
+        to_host(hy.data(), ty.get(), m, s);
+        to_host(hk.data(), tk.get(), m, s);
         HIP_OK(hipStreamSynchronize(s));
+        std::vector<uint64_t> order(m);
+        for (uint64_t i = 0; i < m; i++) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) {
+            if (hk[p] != hk[q]) return hk[p] < hk[q];
+            const int c = std::memcmp(hx.data() + p * MLP_NF, hx.data() + q * MLP_NF, MLP_NF * sizeof(float));
+            return c != 0 ? c < 0 : hy[p] < hy[q];
+        });
+        for (uint64_t i = 0; i < m; i++) {
+            std::memcpy(x_out + i * MLP_NF, hx.data() + order[i] * MLP_NF, MLP_NF * sizeof(float));
+            y_out[i] = hy[order[i]];
+        }
         t_stats = st;
         return SC_OK;
     } catch (const ScError &e) {
@@ -2454,7 +2491,8 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
 }
 
 sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, const float *weights, double threshold,
-                             uint64_t *labels_out, uint64_t *n_clusters_out, sc_filter_stats *fst, sc_error *err) {
+                             int32_t judge, uint64_t *labels_out, uint64_t *n_clusters_out, sc_filter_stats *fst,
+                             sc_error *err) {
     try {
         if (!g || !cfg || !weights || !n_clusters_out || (!labels_out && g->n))
             throw ScError(SC_INVALID_ARGUMENT, "null argument");
@@ -2463,11 +2501,14 @@ sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, con
         hipStream_t s = thread_stream();
         const uint32_t n = (uint32_t)g->n;
         DevBuf<float> w(MLP_PARAMS);
+        DevBuf<MlpPacked> pk(1);
         DevBuf<unsigned long long> fs(3);
         to_device(w.get(), weights, MLP_PARAMS, s);
+        mlp_pack_device(w.get(), pk.get(), s);
         HIP_OK(hipMemsetAsync(fs.get(), 0, 3 * sizeof(unsigned long long), s));
         FilterOpts fo;
-        fo.w = w.get();
+        fo.pk = pk.get();
+        fo.judge = judge != 0;
         fo.logit_thr = (float)std::log(threshold / (1.0 - threshold));
         fo.fstats = fs.get();
         DevBuf<uint32_t> labels(n);

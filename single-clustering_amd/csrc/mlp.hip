@@ -87,9 +87,10 @@ __global__ void k_adam(float *w, const float *g, float *m, float *v, float lr, f
     w[p] -= lr * (m[p] / c1) / (sqrtf(v[p] / c2) + eps);
 }
 
-__global__ void k_mlp_predict(const float *w, const float *X, uint64_t n, int bf16, float *out) {
+// FP32 weights, one sample per thread (the training arithmetic)
+__global__ void k_mlp_predict(const float *w, const float *X, uint64_t n, float *out) {
     __shared__ float sw[MLP_PARAMS];
-    for (int p = threadIdx.x; p < MLP_PARAMS; p += blockDim.x) sw[p] = bf16 ? bf16_round(w[p]) : w[p];
+    for (int p = threadIdx.x; p < MLP_PARAMS; p += blockDim.x) sw[p] = w[p];
     __syncthreads();
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
         float x[MLP_NF];
@@ -97,6 +98,59 @@ __global__ void k_mlp_predict(const float *w, const float *X, uint64_t n, int bf
         for (int f = 0; f < MLP_NF; f++) x[f] = X[s * MLP_NF + f];
         out[s] = mlp_forward(sw, x);
     }
+}
+
+// BF16 inference: the wave MFMA path the filtered sweep runs (loop bounds are block-uniform,
+// so every lane of a wave reaches mlp_forward_wave together)
+__global__ void k_mlp_predict_bf16(const MlpPacked *P, const float *X, uint64_t n, float *out) {
+    for (uint64_t s0 = blockIdx.x * (uint64_t)blockDim.x; s0 < n; s0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = s0 + threadIdx.x;
+        float x[MLP_NF];
+#pragma unroll
+        for (int f = 0; f < MLP_NF; f++) x[f] = s < n ? X[s * MLP_NF + f] : 0.0f;
+        const float z = mlp_forward_wave(P, x);
+        if (s < n) out[s] = z;
+    }
+}
+
+// One thread per packed element (see MlpPacked)
+__global__ void k_mlp_pack(const float *w, MlpPacked *P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 4 * 64) {  // a1[mt][lane]
+        const int mt = t >> 6, lane = t & 63, g = lane >> 4, j = 16 * mt + (lane & 15);
+        mlp_u32x4 v = {0u, 0u, 0u, 0u};
+        if (g < 2)
+            for (int q = 0; q < 4; q++)
+                v[q] = bf16_bits(w[(2 * q) * MLP_H + j]) | (bf16_bits(w[(2 * q + 1) * MLP_H + j]) << 16);
+        else if (g == 2)
+            v[0] = bf16_bits(w[MLP_OFF_B1 + j]);
+        P->a1[mt][lane] = v;
+    } else if (t < 4 * 64 + 8 * 64) {  // a2[ot][s][lane]
+        const int u = t - 4 * 64, ot = u >> 7, s = (u >> 6) & 1, lane = u & 63, g = lane >> 4;
+        const int o = 16 * ot + (lane & 15);
+        mlp_u32x4 v;
+        for (int q = 0; q < 4; q++) {
+            uint32_t half[2];
+            for (int e = 0; e < 2; e++) {
+                const int p = 2 * q + e;
+                const int j = 16 * (2 * s + (p >> 2)) + 4 * g + (p & 3);
+                half[e] = bf16_bits(w[MLP_OFF_W2 + j * MLP_H + o]);
+            }
+            v[q] = half[0] | (half[1] << 16);
+        }
+        P->a2[ot][s][lane] = v;
+    } else if (t < 4 * 64 + 8 * 64 + MLP_H) {
+        const int i = t - 12 * 64;
+        P->b1[i] = bf16_round(w[MLP_OFF_B1 + i]);
+        P->b2[i] = bf16_round(w[MLP_OFF_B2 + i]);
+        P->w3[i] = bf16_round(w[MLP_OFF_W3 + i]);
+        if (i == 0) P->b3 = bf16_round(w[MLP_OFF_B3]);
+    }
+}
+
+void mlp_pack_device(const float *w, MlpPacked *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3((12 * 64 + MLP_H + 255) / 256), dim3(256), 0, s, w, out);
+    KERNEL_OK();
 }
 
 // loss and gradient of one batch (all n samples) at weights w
@@ -213,8 +267,18 @@ sc_status sc_mlp_predict(const float *w, const float *X, uint64_t n, int32_t bf1
         DevBuf<float> dw(MLP_PARAMS), dX(n * MLP_NF), dout(n);
         to_device(dw.get(), w, MLP_PARAMS, s);
         to_device(dX.get(), X, n * MLP_NF, s);
+        if (bf16) {
+            DevBuf<MlpPacked> pk(1);
+            mlp_pack_device(dw.get(), pk.get(), s);
+            hipLaunchKernelGGL(k_mlp_predict_bf16, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s,
+                               (const MlpPacked *)pk.get(), (const float *)dX.get(), n, dout.get());
+            KERNEL_OK();
+            to_host(out, dout.get(), n, s);
+            HIP_OK(hipStreamSynchronize(s));
+            return SC_OK;
+        }
         hipLaunchKernelGGL(k_mlp_predict, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const float *)dw.get(),
-                           (const float *)dX.get(), n, bf16, dout.get());
+                           (const float *)dX.get(), n, dout.get());
         KERNEL_OK();
         to_host(out, dout.get(), n, s);
         HIP_OK(hipStreamSynchronize(s));

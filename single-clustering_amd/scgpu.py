@@ -129,7 +129,7 @@ SIGNATURES = {
                         C.POINTER(ScError)], C.c_int),
     "sc_leiden_trace": ([_GP, C.POINTER(ScLeidenConfig), C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_uint8),
                          u64p, C.POINTER(ScError)], C.c_int),
-    "sc_leiden_filtered": ([_GP, C.POINTER(ScLeidenConfig), C.POINTER(C.c_float), C.c_double, u64p, u64p,
+    "sc_leiden_filtered": ([_GP, C.POINTER(ScLeidenConfig), C.POINTER(C.c_float), C.c_double, C.c_int32, u64p, u64p,
                             C.POINTER(ScFilterStats), C.POINTER(ScError)], C.c_int),
     "sc_leiden_batch": ([C.POINTER(_GP), C.c_uint64, C.POINTER(C.c_double), C.c_uint64, C.POINTER(ScLeidenConfig),
                          u64p, u64p, C.POINTER(ScError)], C.c_int),
@@ -549,10 +549,12 @@ def leiden_trace(graph, config=None, cap=1 << 22):
     return X[:n.value].copy(), y[:n.value].copy()
 
 
-def leiden_filtered(graph, weights, threshold=0.02, config=None):
+def leiden_filtered(graph, weights, threshold=0.02, config=None, judge=True):
     """leiden() with the learned active-vertex filter in local moving; returns (Clustering,
     stats) where stats holds the skip count, the judge sweep's size and moves, and the
-    skip-error rate."""
+    skip-error rate. A skipped vertex leaves the active set until a neighbour moves; with
+    judge=True the vertices last skipped are decided by plain iterations at the end of each
+    local-moving phase, judge=False leaves them (only the skip count is then reported)."""
     config = config or LeidenConfig()
     c = config.to_c()
     w = np.ascontiguousarray(weights, dtype=np.float32)
@@ -561,7 +563,8 @@ def leiden_filtered(graph, weights, threshold=0.02, config=None):
     nc = C.c_uint64(0)
     st = ScFilterStats()
     err = ScError()
-    _check(_lib.sc_leiden_filtered(graph.handle, C.byref(c), _fp(w), float(threshold), _p(labels, u64p), C.byref(nc),
+    _check(_lib.sc_leiden_filtered(graph.handle, C.byref(c), _fp(w), float(threshold), 1 if judge else 0,
+                                   _p(labels, u64p), C.byref(nc),
                                    C.byref(st), C.byref(err)), err)
     return Clustering(labels[:n], nc.value), {k: getattr(st, k) for k, _ in ScFilterStats._fields_}
 

@@ -70,8 +70,9 @@ def test_trace_and_filtered_leiden():
     # a filter that never skips reproduces plain Leiden exactly
     never = w.copy()
     never[-1] = 1e4
-    c, st = scgpu.leiden_filtered(g, never, 0.5)
-    assert np.array_equal(c.labels(), plain.labels()) and st["skipped"] == 0
+    for judge in (True, False):
+        c, st = scgpu.leiden_filtered(g, never, 0.5, judge=judge)
+        assert np.array_equal(c.labels(), plain.labels()) and st["skipped"] == 0
     c, st = scgpu.leiden_filtered(g, w, 0.05)
     q_plain = scgpu.modularity(g, plain.labels())
     q = scgpu.modularity(g, c.labels())
@@ -79,3 +80,43 @@ def test_trace_and_filtered_leiden():
           f"(rate {st['skip_error_rate']:.4f}); modularity {q:.6f} vs plain {q_plain:.6f}")
     assert q >= q_plain - 0.01 * abs(q_plain)
     assert 0.0 <= st["skip_error_rate"] <= 1.0
+    # without the judge nothing re-decides the skipped vertices: the result rests on the model
+    # alone (a sanity bound here), and is still deterministic
+    cf, fst = scgpu.leiden_filtered(g, w, 0.05, judge=False)
+    qf = scgpu.modularity(g, cf.labels())
+    print(f"filter (no judge): skipped {fst['skipped']}, modularity {qf:.6f}")
+    assert fst["judged"] == 0 and fst["skipped"] > 0
+    assert qf >= q_plain - 0.1 * abs(q_plain)
+    assert np.array_equal(scgpu.leiden_filtered(g, w, 0.05, judge=False)[0].labels(), cf.labels())
+    assert np.array_equal(scgpu.leiden_filtered(g, w, 0.05)[0].labels(), c.labels())
+
+
+def test_trace_is_reproducible():
+    g = _knn(5000, 13)
+    X1, y1 = scgpu.leiden_trace(g, scgpu.LeidenConfig(seed=5))
+    X2, y2 = scgpu.leiden_trace(g, scgpu.LeidenConfig(seed=5))
+    assert np.array_equal(X1, X2) and np.array_equal(y1, y2)
+
+
+def test_bf16_inference_wave_layout():
+    """The MFMA inference's packed layout, by impulses: with one nonzero path through the network
+    (feature f -> hidden i -> hidden j -> output, or bias b1[i] -> j with bias b2[j]) the logit has
+    a closed form, which fails if a fragment's row, column or k order is transposed. Weights are
+    exact in BF16."""
+    W1, B1, W2, B2, W3, B3 = 0, 512, 576, 4672, 4736, 4800
+    rng = np.random.default_rng(8)
+    X = rng.random((300, 8)).astype(np.float32)
+    x64 = X.astype(np.float64)
+    for case in range(12):
+        f, i, j = int(rng.integers(8)), int(rng.integers(64)), int(rng.integers(64))
+        w = np.zeros(4801, dtype=np.float32)
+        w[W3 + j], w[B3] = 0.5, 0.25
+        if case % 2 == 0:
+            w[W1 + f * 64 + i], w[W2 + i * 64 + j] = 1.5, 0.75
+            want = 0.25 + 0.5 * (1.5 * 0.75) * x64[:, f]
+        else:
+            w[B1 + i], w[W2 + i * 64 + j], w[B2 + j] = 0.5, 1.25, -0.25
+            w[W1 + f * 64 + i] = -1.0   # h1[i] = relu(0.5 - x_f)
+            want = 0.25 + 0.5 * np.maximum(1.25 * np.maximum(0.5 - x64[:, f], 0) - 0.25, 0)
+        z = scgpu.mlp_predict(w, X, bf16=True)
+        assert np.max(np.abs(z - want)) <= 1e-5, (case, f, i, j, np.max(np.abs(z - want)))
