@@ -158,6 +158,7 @@ def main():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-filter comparison")
+    ap.add_argument("--no-sweep-modes", action="store_true", help="skip timing the multi-launch and graph modes")
     ap.add_argument("--filter-threshold", type=float, default=None,
                     help="skip probability threshold of the learned filter (default: calibrated on a "
                          "training graph: the fastest of a few whose modularity stays within 1e-6)")
@@ -259,6 +260,23 @@ def main():
 
     # learned active-vertex filter vs plain Leiden on the same graph (trained on traces from
     # graphs of the same distribution built with other seeds)
+    # the three launch modes of local moving / refinement on the same graph (identical labels)
+    sweep_modes = None
+    if not args.no_sweep_modes:
+        sweep_modes = {}
+        for mode in ("persistent", "graph", "multi"):
+            scgpu.set_sweep_mode(mode)
+            ts = []
+            for i in range(3):
+                t1 = time.perf_counter()
+                cm = scgpu.leiden(g, cfg)
+                if i:
+                    ts.append(time.perf_counter() - t1)
+            sweep_modes[mode] = {"leiden_ms": 1e3 * float(np.mean(ts)),
+                                 "labels_equal": bool(np.array_equal(cm.labels(), c.labels()))}
+        scgpu.set_sweep_mode("persistent")
+        log("[bench] sweep modes: " + ", ".join(f"{k} {v['leiden_ms']:.2f} ms" for k, v in sweep_modes.items()))
+
     learned = None
     if not args.no_learned and wl["kind"] == "knn":
         import workloads as W
@@ -372,6 +390,7 @@ def main():
         "knn_stats": kstats[-1],
         "roofline": dominant,
         "roofline_other": knn_roof if dominant is leiden_roof else leiden_roof,
+        "sweep_modes": sweep_modes,
         "learned_filter": learned,
         "cpu_baseline": cpu,
     }

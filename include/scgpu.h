@@ -119,8 +119,11 @@ void sc_shutdown(void);
 void sc_set_profiling(int32_t enable);
 /* No counterpart in the reference. Selects how local moving and refinement are launched:
  * 1 (default) = one cooperative persistent launch per phase of a level, with grid barriers
- * between sub-sweeps; 0 = one launch per kernel of each sub-sweep. Both give identical labels.
- * The environment variable SCGPU_PERSISTENT=0 selects 0 at start-up. */
+ * between sub-sweeps; 0 = one launch per kernel of each sub-sweep; 2 = mode 0's launches of one
+ * iteration captured into a hipGraph once per level and phase, and the graph replayed per
+ * iteration (the iteration index lives in device memory). All give identical labels. The
+ * environment variable SCGPU_SWEEP_MODE=0/1/2 (or SCGPU_PERSISTENT=0) selects it at start-up.
+ * Returns SC_INVALID_ARGUMENT for other values. */
 sc_status sc_set_sweep_mode(int32_t mode);
 /* No counterpart in the reference. Online autotuner of launch parameters that never change
  * results (workgroups per persistent sweep phase, kNN column splits): the first calls for a

@@ -90,6 +90,11 @@ struct DecideArgs {
     uint32_t sub;
     uint32_t round;
     double theta;
+    // graph-replayed iterations: the iteration index is read from device memory (null: seed
+    // and round above are used); local moving derives its seed from iter_seed_base the way the
+    // persistent sweep does
+    const uint32_t *iter_dev;
+    uint64_t iter_seed_base;
     // global hash tables for the highest-degree bucket
     uint32_t *gkeys;
     long long *gvals;
@@ -412,7 +417,13 @@ __global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
     __shared__ long long r_w[BS / 64];
     __shared__ unsigned long long s_stats[3];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
-    StepParams sp{a.seed, a.sub, a.round, 0u, a.n_movers, nullptr};
+    uint64_t seed = a.seed;
+    uint32_t round = a.round;
+    if (a.iter_dev) {
+        round = *a.iter_dev;
+        if (!REFINE) seed = mix64(a.iter_seed_base ^ (0x5851F42D4C957F2Dull * (uint64_t)(round + 1)));
+    }
+    StepParams sp{seed, a.sub, round, 0u, a.n_movers, nullptr};
     DecideStats ds;
     for (uint64_t base = (uint64_t)blockIdx.x * NG; base < a.list_len; base += (uint64_t)gridDim.x * NG)
         decide_unit<BS, G, T, REFINE, GTAB>(a, sp, a.list, nullptr, a.list_len, base, s_keys, s_vals, r_s, r_c,
@@ -439,6 +450,7 @@ struct ApplyArgs {
     uint32_t *moved;
     unsigned int *n_moved;
     const uint32_t *claim;  // with a community-weight cap: lowest mover id per target, else null
+    const uint32_t *iter_dev;  // graph-replayed rounds: the round index in device memory, else null
 };
 
 // Local moving: every decided mover moves (targets were checked against the snapshot).
@@ -471,6 +483,7 @@ __global__ void k_apply_move(ApplyArgs a) {
 // took the high half of a 64-bit index from a stale register.
 __global__ void k_apply_refine(ApplyArgs a) {
     unsigned int cnt = *a.n_movers;
+    const uint32_t round = a.iter_dev ? *a.iter_dev : a.round;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint64_t v = a.movers[i];
         const uint64_t t = a.target[v];
@@ -478,7 +491,7 @@ __global__ void k_apply_refine(ApplyArgs a) {
         const uint32_t tr = a.tround[t];
         const uint64_t tt = a.target[t];
         const uint32_t cl = a.claim ? a.claim[t] : (uint32_t)v;
-        const bool go = ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == a.round) & (tt == v))) & (cl == (uint32_t)v);
+        const bool go = ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == round) & (tt == v))) & (cl == (uint32_t)v);
         if (!go) {
             a.active[v] = 1;  // blocked: look again next round
             continue;
@@ -549,6 +562,48 @@ __global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t x) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = x;
 }
+
+// Fills and counters of the multi-launch iterations are kernels (not memset nodes), so that an
+// iteration can be captured into a graph and replayed.
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t x) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = x;
+}
+
+__global__ void k_iter_advance(uint32_t *it) { *it += 1u; }
+
+static void fill_u32(uint32_t *p, uint64_t n, uint32_t x, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, x);
+    KERNEL_OK();
+}
+
+// One iteration's launch sequence captured once (per level) and replayed per iteration.
+struct IterGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    IterGraph() = default;
+    IterGraph(const IterGraph &) = delete;
+    IterGraph &operator=(const IterGraph &) = delete;
+    template <typename F>
+    void capture(hipStream_t s, F &&record) {
+        HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            record();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_OK(hipStreamEndCapture(s, &graph));
+        HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    void launch(hipStream_t s) { HIP_OK(hipGraphLaunch(exec, s)); }
+    ~IterGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
 
 // ---- degree buckets ------------------------------------------------------------------------
 __device__ __forceinline__ int bucket_of(uint64_t d) {
@@ -905,6 +960,15 @@ __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint
 
 __global__ void k_stamp_clock(long long *out) { *out = wall_clock64(); }
 
+__global__ void k_sweep_collect(const unsigned int *ctl, const unsigned int *err, unsigned long long *run,
+                                unsigned long long *run_err) {
+    if (threadIdx.x == 0) {
+        run[0] += ctl[2];
+        run[1] += ctl[3];
+        if (*err) *run_err = 1ull;
+    }
+}
+
 template <bool REFINE, bool FILTER>
 __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3))) k_sweep(SweepArgs A) {
     __shared__ uint32_t s_keys[SWEEP_TAB];
@@ -1152,12 +1216,44 @@ struct LevelState {
 
 struct BatchState;
 
+// Counters a run accumulates on the device and reads back once, at its end (no readback per
+// phase): per phase (0 local moving, 1 refinement) the decide statistics [vertices, edges,
+// candidates] and the sweep results [iterations, moves]; then a sweep barrier-timeout flag.
+constexpr int RUN_STAT = 0, RUN_SWEEP = 6, RUN_ERR = 10, RUN_COUNTERS = 11;
+
+struct EventPool {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    ~EventPool() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// A tuner measurement whose events are read when the run ends.
+struct TuneSample {
+    std::string key;
+    int index;
+    hipEvent_t t0, t1;
+};
+
 struct RunCtx {
     const sc_leiden_config *cfg;
     hipStream_t s;
     KernelTimer timer;
     sc_leiden_stats st{};
     double move_bytes = 0.0;
+    DevBuf<unsigned long long> counters;                 // RUN_COUNTERS device counters
+    EventPool events;                                    // phase marks and tuner samples
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> phase_marks;
+    std::vector<TuneSample> tune_samples;
     // batched run (null otherwise): the jobs and, for the current level, every vertex's job
     BatchState *batch = nullptr;
     const uint32_t *vjob = nullptr;
@@ -1268,18 +1364,24 @@ static void alloc_level(LevelState &ls, uint32_t n) {
 }
 
 // Sweep mode: 1 = one cooperative launch per phase (default), 0 = one launch per sub-sweep
-// kernel. SCGPU_PERSISTENT=0 in the environment selects 0 at start-up.
+// kernel, 2 = mode 0's launches of one iteration captured into a graph once per level and
+// replayed per iteration. SCGPU_SWEEP_MODE=0/1/2 in the environment selects the mode at
+// start-up (SCGPU_PERSISTENT=0 selects 0).
 static std::atomic<int> g_sweep_mode{-1};
 
-static bool persistent_sweeps() {
+static int sweep_mode() {
     int m = g_sweep_mode.load(std::memory_order_relaxed);
     if (m < 0) {
-        const char *e = std::getenv("SCGPU_PERSISTENT");
-        m = (e && e[0] == '0') ? 0 : 1;
+        const char *e = std::getenv("SCGPU_SWEEP_MODE");
+        const char *p = std::getenv("SCGPU_PERSISTENT");
+        m = (e && e[0] >= '0' && e[0] <= '2' && !e[1]) ? e[0] - '0' : ((p && p[0] == '0') ? 0 : 1);
         g_sweep_mode.store(m, std::memory_order_relaxed);
     }
-    return m == 1;
+    return m;
 }
+
+static bool persistent_sweeps() { return sweep_mode() == 1; }
+static bool graph_sweeps() { return sweep_mode() == 2; }
 
 // Workgroups of a sweep launch: all must be resident at once (cooperative launch); small
 // levels use fewer, which makes each grid barrier cheaper.
@@ -1322,7 +1424,7 @@ static bool sweep_trace_on() {
 // One cooperative launch for the whole phase; returns {steps, iterations, moves}.
 template <bool REFINE>
 static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, RunCtx &ctx, uint64_t level_seed,
-                      uint32_t *memb_w, uint64_t nnz, unsigned int out[4]) {
+                      uint32_t *memb_w, uint64_t nnz) {
     hipStream_t s = ctx.s;
     const uint32_t n = d.n;
     SweepArgs A{};
@@ -1378,32 +1480,32 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         KERNEL_OK();
     }
     void *args[] = {&A};
-    hipEvent_t ev = nullptr, t0 = nullptr, t1 = nullptr;
-    if (tune_enabled()) {
-        HIP_OK(hipEventCreate(&t0));
-        HIP_OK(hipEventCreate(&t1));
-        HIP_OK(hipEventRecord(t0, s));
+    hipEvent_t ev = nullptr;
+    // a tuner measurement is taken only while the shape is explored; its events are read when
+    // the run ends, so nothing here waits for the device
+    const bool sample = tune_enabled() && tc.exploring;
+    TuneSample ts{};
+    if (sample) {
+        ts = TuneSample{tkey, tc.index, ctx.events.next(), ctx.events.next()};
+        HIP_OK(hipEventRecord(ts.t0, s));
     }
     if (ctx.timer.on && !REFINE) ctx.timer.begin(s, ev);
     const void *kern = filtered ? (const void *)k_sweep<false, true> : (const void *)k_sweep<REFINE, false>;
     HIP_OK(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(SWEEP_BS), args, 0, s));
     if (ctx.timer.on && !REFINE) ctx.timer.end(s, ev);
-    if (t1) HIP_OK(hipEventRecord(t1, s));
-    unsigned int ctl[4], berr = 0;
-    to_host(ctl, A.ctl, 4, s);
-    to_host(&berr, &A.bc->err, 1, s);
-    HIP_OK(hipStreamSynchronize(s));
-    if (t1) {
-        float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, t0, t1));
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
-        tune_record(tkey, tc.index, ms);
+    if (sample) {
+        HIP_OK(hipEventRecord(ts.t1, s));
+        ctx.tune_samples.push_back(ts);
     }
-    if (ctx.timer.on) ctx.timer.harvest();
-    if (berr) throw ScError(SC_DEVICE_ERROR, "Leiden sweep: a grid barrier timed out");
-    for (int i = 0; i < 4; i++) out[i] = ctl[i];
+    // iterations, moves and the barrier-timeout flag accumulate on the device
+    hipLaunchKernelGGL(k_sweep_collect, dim3(1), dim3(64), 0, s, (const unsigned int *)A.ctl,
+                       (const unsigned int *)&A.bc->err, ctx.counters.get() + RUN_SWEEP + (REFINE ? 2 : 0),
+                       ctx.counters.get() + RUN_ERR);
+    KERNEL_OK();
     if (sweep_trace_on()) {
+        unsigned int ctl[4];
+        to_host(ctl, A.ctl, 4, s);
+        HIP_OK(hipStreamSynchronize(s));
         // barriers: 2 prologue, then per step 3 (4 with a cap); print the phase times in us
         const uint32_t per = A.claim ? 4u : 3u;
         const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per);
@@ -1431,18 +1533,53 @@ static void init_totals(const sc_graph &L, const uint32_t *memb, LevelState &ls,
     KERNEL_OK();
 }
 
+__global__ void k_stats_collect(unsigned long long *stat, unsigned long long *run) {
+    if (threadIdx.x < 3) {
+        run[threadIdx.x] += stat[threadIdx.x];
+        stat[threadIdx.x] = 0;
+    }
+}
+
+// The phase's decide statistics join the run's device counters (read when the run ends).
 static void harvest_stats(LevelState &ls, RunCtx &ctx, bool move_phase) {
-    unsigned long long h[3];
-    to_host(h, ls.stat.get(), 3, ctx.s);
+    hipLaunchKernelGGL(k_stats_collect, dim3(1), dim3(64), 0, ctx.s, ls.stat.get(),
+                       ctx.counters.get() + RUN_STAT + (move_phase ? 0 : 3));
+    KERNEL_OK();
+}
+
+// Run-end readback of the device counters, phase marks, tuner samples and kernel timers
+// (after the stream has been synchronized).
+static void finish_run_stats(RunCtx &ctx) {
+    unsigned long long h[RUN_COUNTERS];
+    to_host(h, ctx.counters.get(), RUN_COUNTERS, ctx.s);
     HIP_OK(hipStreamSynchronize(ctx.s));
-    HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 3 * sizeof(unsigned long long), ctx.s));
-    ctx.st.vertex_visits += h[0];
-    ctx.st.edge_visits += h[1];
-    // minimum unique bytes: neighbour id + weight + membership per entry, 32 B per vertex, one
-    // 8-byte community total per candidate scored
-    double bytes = 12.0 * (double)h[1] + 32.0 * (double)h[0] + 8.0 * (double)h[2];
-    ctx.st.algorithmic_bytes += bytes;
-    if (move_phase) ctx.move_bytes += bytes;
+    for (int ph = 0; ph < 2; ph++) {
+        const unsigned long long *c = h + RUN_STAT + 3 * ph;
+        ctx.st.vertex_visits += c[0];
+        ctx.st.edge_visits += c[1];
+        // minimum unique bytes: neighbour id + weight + membership per entry, 32 B per vertex,
+        // one 8-byte community total per candidate scored
+        const double bytes = 12.0 * (double)c[1] + 32.0 * (double)c[0] + 8.0 * (double)c[2];
+        ctx.st.algorithmic_bytes += bytes;
+        if (ph == 0) ctx.move_bytes += bytes;
+    }
+    ctx.st.move_iterations += h[RUN_SWEEP + 0];
+    ctx.st.moves += h[RUN_SWEEP + 1];
+    ctx.st.refine_rounds += h[RUN_SWEEP + 2];
+    for (auto &pm : ctx.phase_marks) {
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, pm.second.first, pm.second.second));
+        ctx.st.phase_ms[pm.first] += ms;
+    }
+    ctx.phase_marks.clear();
+    for (auto &t : ctx.tune_samples) {
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, t.t0, t.t1));
+        tune_record(t.key, t.index, ms);
+    }
+    ctx.tune_samples.clear();
+    if (ctx.timer.on) ctx.timer.harvest();
+    if (h[RUN_ERR]) throw ScError(SC_DEVICE_ERROR, "Leiden sweep: a grid barrier timed out");
 }
 
 // Local moving on one level (local_move.rs:39-122), synchronous sub-sweeps.
@@ -1470,10 +1607,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         a.goff = b.goff.get();
     }
     if (persistent_sweeps() || ctx.vjob || ctx.fopt) {
-        unsigned int out[4];
-        run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz, out);
-        ctx.st.move_iterations += out[2];
-        ctx.st.moves += out[3];
+        run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz);
         harvest_stats(ls, ctx, true);
         return;
     }
@@ -1496,16 +1630,26 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     const bool capped = ctx.cfg->has_max_community_weight != 0;
     ap.claim = capped ? ls.claim.get() : nullptr;
     DevBuf<unsigned int> iter_moves(MOVE_SUBSWEEPS);
-    for (int it = 0; it < MAX_MOVE_ITERS; it++) {
-        a.seed = mix64(level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
+    const bool use_graph = graph_sweeps();
+    DevBuf<uint32_t> iter_dev;
+    if (use_graph) {
+        iter_dev.resize(1);
+        fill_u32(iter_dev.get(), 1, 0u, s);
+        a.iter_dev = iter_dev.get();
+        a.iter_seed_base = level_seed;
+    }
+    // one iteration: both sub-sweeps' decide / claim / apply / activate (sync-free, so it can
+    // be captured); with a graph the iteration index and seed come from iter_dev
+    auto iteration = [&](int it) {
+        if (!use_graph) a.seed = mix64(level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
         for (uint32_t sub = 0; sub < (uint32_t)MOVE_SUBSWEEPS; sub++) {
             a.sub = sub;
             a.n_movers = iter_moves.get() + sub;
             ap.n_movers = iter_moves.get() + sub;
-            HIP_OK(hipMemsetAsync(iter_moves.get() + sub, 0, sizeof(unsigned int), s));
-            decide_all<false>(a, b, s, ctx.timer.on ? &ctx.timer : nullptr);
+            fill_u32(iter_moves.get() + sub, 1, 0u, s);
+            decide_all<false>(a, b, s, (ctx.timer.on && !use_graph) ? &ctx.timer : nullptr);
             if (capped) {
-                HIP_OK(hipMemsetAsync(ls.claim.get(), 0xFF, n * sizeof(uint32_t), s));
+                fill_u32(ls.claim.get(), n, 0xFFFFFFFFu, s);
                 hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), iter_moves.get() + sub,
                                    ls.target.get(), ls.claim.get());
                 KERNEL_OK();
@@ -1518,6 +1662,18 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
                                (const uint32_t *)(capped ? ls.claim.get() : nullptr));
             KERNEL_OK();
         }
+        if (use_graph) {
+            hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(1), 0, s, iter_dev.get());
+            KERNEL_OK();
+        }
+    };
+    IterGraph ig;
+    if (use_graph) ig.capture(s, [&] { iteration(0); });
+    for (int it = 0; it < MAX_MOVE_ITERS; it++) {
+        if (use_graph)
+            ig.launch(s);
+        else
+            iteration(it);
         unsigned int mv[MOVE_SUBSWEEPS];
         to_host(mv, iter_moves.get(), MOVE_SUBSWEEPS, s);
         HIP_OK(hipStreamSynchronize(s));
@@ -1552,9 +1708,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
         a.goff = b.goff.get();
     }
     if (persistent_sweeps() || ctx.vjob) {
-        unsigned int out[4];
-        run_sweep<true>(a, b, ls, ctx, level_seed, rmemb, L.nnz, out);
-        ctx.st.refine_rounds += out[2];
+        run_sweep<true>(a, b, ls, ctx, level_seed, rmemb, L.nnz);
         harvest_stats(ls, ctx, false);
         return;
     }
@@ -1583,14 +1737,24 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     ap.n_moved = ls.counters.get() + 1;
     const bool capped = ctx.cfg->has_max_community_weight != 0;
     ap.claim = capped ? ls.claim.get() : nullptr;
-    for (uint32_t r = 0; r < (uint32_t)MAX_REFINE_ROUNDS; r++) {
-        a.round = r;
-        ap.round = r;
-        HIP_OK(hipMemsetAsync(ls.counters.get(), 0, 2 * sizeof(unsigned int), s));
-        HIP_OK(hipMemsetAsync(ls.targeted.get(), 0, n * sizeof(uint32_t), s));
+    const bool use_graph = graph_sweeps();
+    DevBuf<uint32_t> iter_dev;
+    if (use_graph) {
+        iter_dev.resize(1);
+        fill_u32(iter_dev.get(), 1, 0u, s);
+        a.iter_dev = iter_dev.get();
+        ap.iter_dev = iter_dev.get();
+    }
+    auto round_launches = [&](uint32_t r) {
+        if (!use_graph) {
+            a.round = r;
+            ap.round = r;
+        }
+        fill_u32(ls.counters.get(), 2, 0u, s);
+        fill_u32(ls.targeted.get(), n, 0u, s);
         decide_all<true>(a, b, s, nullptr);
         if (capped) {
-            HIP_OK(hipMemsetAsync(ls.claim.get(), 0xFF, n * sizeof(uint32_t), s));
+            fill_u32(ls.claim.get(), n, 0xFFFFFFFFu, s);
             hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), ls.counters.get(),
                                ls.target.get(), ls.claim.get());
             KERNEL_OK();
@@ -1601,6 +1765,18 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
                            ls.counters.get() + 1, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)rmemb, memb,
                            ls.active.get());
         KERNEL_OK();
+        if (use_graph) {
+            hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(1), 0, s, iter_dev.get());
+            KERNEL_OK();
+        }
+    };
+    IterGraph ig;
+    if (use_graph) ig.capture(s, [&] { round_launches(0); });
+    for (uint32_t r = 0; r < (uint32_t)MAX_REFINE_ROUNDS; r++) {
+        if (use_graph)
+            ig.launch(s);
+        else
+            round_launches(r);
         unsigned int c[2];
         to_host(c, ls.counters.get(), 2, s);
         HIP_OK(hipStreamSynchronize(s));
@@ -1629,10 +1805,10 @@ static uint32_t renumber(const uint32_t *ids, uint32_t n, uint32_t id_space, uin
     hipLaunchKernelGGL(k_used_flags, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, n, flags.get());
     KERNEL_OK();
     exclusive_scan_u32(flags.get(), newid.get(), id_space + 1, s);
-    uint32_t count = 0;
-    to_host(&count, newid.get() + id_space, 1, s);
     hipLaunchKernelGGL(k_remap, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, n, newid.get(), out);
     KERNEL_OK();
+    uint32_t count = 0;
+    to_host(&count, newid.get() + id_space, 1, s);
     HIP_OK(hipStreamSynchronize(s));
     return count;
 }
@@ -2131,17 +2307,17 @@ static std::vector<uint32_t> batch_group_counts(const BatchState &B, const uint3
 }
 
 // ---- driver (leiden/mod.rs:168-289) -----------------------------------------------------------
-// Host wall time per phase; each lap synchronizes the stream so device work is attributed to
-// the phase that queued it.
+// Stream time per phase: events mark the phase boundaries in stream order (no host
+// synchronization); the intervals are read when the run ends.
 struct PhaseClock {
     RunCtx &ctx;
     hipStream_t s;
-    std::chrono::steady_clock::time_point t;
-    PhaseClock(RunCtx &c, hipStream_t st) : ctx(c), s(st), t(std::chrono::steady_clock::now()) {}
+    hipEvent_t t;
+    PhaseClock(RunCtx &c, hipStream_t st) : ctx(c), s(st), t(c.events.next()) { HIP_OK(hipEventRecord(t, s)); }
     void lap(int phase) {
-        HIP_OK(hipStreamSynchronize(s));
-        auto now = std::chrono::steady_clock::now();
-        ctx.st.phase_ms[phase] += std::chrono::duration<double, std::milli>(now - t).count();
+        hipEvent_t now = ctx.events.next();
+        HIP_OK(hipEventRecord(now, s));
+        ctx.phase_marks.push_back({phase, {t, now}});
         t = now;
     }
 };
@@ -2301,6 +2477,8 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
     ctx.s = s;
     ctx.fopt = fopt;
     ctx.timer.on = profiling_enabled();
+    ctx.counters.resize(RUN_COUNTERS);
+    HIP_OK(hipMemsetAsync(ctx.counters.get(), 0, RUN_COUNTERS * sizeof(unsigned long long), s));
     hipEvent_t t0, t1;
     HIP_OK(hipEventCreate(&t0));
     HIP_OK(hipEventCreate(&t1));
@@ -2380,6 +2558,7 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
     HIP_OK(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    finish_run_stats(ctx);
     ctx.st.total_ms = ms;
     ctx.st.move_kernel_ms = ctx.timer.total_ms;
     ctx.st.move_kernel_launches = ctx.timer.launches;
@@ -2680,7 +2859,7 @@ sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *la
 }
 
 sc_status sc_set_sweep_mode(int32_t mode) {
-    if (mode != 0 && mode != 1) return SC_INVALID_ARGUMENT;
+    if (mode < 0 || mode > 2) return SC_INVALID_ARGUMENT;
     g_sweep_mode.store(mode, std::memory_order_relaxed);
     return SC_OK;
 }

@@ -160,7 +160,8 @@ __global__ void k_label_jobs(const uint64_t *labels, const uint32_t *vjob, uint6
         ljob[labels[v]] = vjob[v];
 }
 
-// Smallest label of each job (one workgroup per job over its vertex range).
+// Smallest label of each job (one workgroup per job over its vertex range; ~0 for a job
+// without vertices).
 __global__ void k_job_min_label(const uint64_t *labels, const uint64_t *vranges, uint64_t *lranges) {
     const uint32_t j = blockIdx.x;
     unsigned long long mn = ~0ull;
@@ -195,10 +196,17 @@ void quality_jobs_device(const sc_graph &g, const uint64_t *d_labels, int mode, 
     hipLaunchKernelGGL(k_job_min_label, dim3(J), dim3(256), 0, s, d_labels, d_vranges, lr.get());
     KERNEL_OK();
     unsigned long long max_label = 0;
+    std::vector<uint64_t> hlr(J + 1);
     to_host(&max_label, mx.get(), 1, s);
+    to_host(hlr.data(), lr.get(), J, s);
     HIP_OK(hipStreamSynchronize(s));
     const uint64_t nl = max_label + 1;
-    to_device(lr.get() + J, &nl, 1, s);
+    // a job without vertices has no smallest label (the kernel leaves ~0): its label range is
+    // the empty range at the next job's start, so the ranges stay ascending
+    hlr[J] = nl;
+    for (uint32_t j = J; j-- > 0;)
+        if (hlr[j] > hlr[j + 1]) hlr[j] = hlr[j + 1];
+    to_device(lr.get(), hlr.data(), J + 1, s);
     std::vector<int> js(3 * (size_t)J);
     std::vector<double> jm(J);
     for (uint32_t j = 0; j < J; j++) {

@@ -222,11 +222,16 @@ def set_profiling(on):
     _lib.sc_set_profiling(1 if on else 0)
 
 
-def set_sweep_mode(persistent):
-    """True: one cooperative launch per local-moving / refinement phase; False: one launch per
-    sub-sweep kernel. Labels are identical either way."""
-    if _lib.sc_set_sweep_mode(1 if persistent else 0) != 0:
-        raise ValueError("invalid sweep mode")
+SWEEP_MODES = {"multi": 0, "persistent": 1, "graph": 2}
+
+
+def set_sweep_mode(mode):
+    """"persistent" (or True): one cooperative launch per local-moving / refinement phase;
+    "multi" (or False): one launch per sub-sweep kernel; "graph": the multi-launch iteration
+    captured into a hipGraph once per level and replayed per iteration. Labels are identical."""
+    code = SWEEP_MODES.get(mode) if isinstance(mode, str) else (1 if mode else 0)
+    if code is None or _lib.sc_set_sweep_mode(code) != 0:
+        raise ValueError(f"invalid sweep mode {mode!r}")
 
 
 @dataclass(frozen=True)

@@ -159,21 +159,28 @@ def test_planted_partition_small_config_vs_oracle():
 
 
 def test_persistent_sweeps_match_multi_launch():
-    """The cooperative one-launch-per-phase sweeps and the one-launch-per-kernel sweeps run the
-    same synchronous algorithm, so labels must be identical. 40k points give ~600 workgroups,
-    many per XCD, which exercises the grid barrier's cross-XCD visibility."""
+    """The cooperative one-launch-per-phase sweeps, the one-launch-per-kernel sweeps and the
+    graph-replayed iterations run the same synchronous algorithm, so labels must be identical.
+    40k points give ~600 workgroups, many per XCD, which exercises the grid barrier's cross-XCD
+    visibility; each graph is replayed once per iteration of its level."""
     x, _ = W.gaussian_mixture(40_000, 24, 20, 9.0, sd=1.0, alpha=2.0, seed=3)
     g = scgpu.knn_graph(x, 10)
     cfgs = [scgpu.LeidenConfig(seed=42), scgpu.LeidenConfig(seed=7, objective=scgpu.ObjectiveKind.Cpm(0.05)),
             scgpu.LeidenConfig(seed=11, max_community_weight=400.0)]
     try:
         for cfg in cfgs:
-            scgpu.set_sweep_mode(True)
+            scgpu.set_sweep_mode("persistent")
             p = [scgpu.leiden(g, cfg).labels() for _ in range(3)]
-            scgpu.set_sweep_mode(False)
+            scgpu.set_sweep_mode("multi")
             m = scgpu.leiden(g, cfg).labels()
+            scgpu.set_sweep_mode("graph")
+            gr = [scgpu.leiden(g, cfg).labels() for _ in range(2)]
             for r in p[1:]:
                 assert np.array_equal(p[0], r), "persistent sweeps are not deterministic"
             assert np.array_equal(p[0], m), f"persistent and multi-launch labels differ for {cfg}"
+            for r in gr:
+                assert np.array_equal(m, r), f"graph-replayed and multi-launch labels differ for {cfg}"
     finally:
-        scgpu.set_sweep_mode(True)
+        scgpu.set_sweep_mode("persistent")
+    with pytest.raises(ValueError):
+        scgpu.set_sweep_mode("bogus")
