@@ -222,18 +222,21 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
                           uint8_t *y_out, uint64_t *n_out, sc_error *err);
 
 typedef struct {
-    uint64_t skipped;        /* filter decisions "stays" (vertex-steps not decided) */
-    uint64_t judged;         /* vertices the first plain (judge) iteration decided */
-    uint64_t judge_moves;    /* of them, vertices that moved: the filter's errors */
-    double skip_error_rate;  /* judge_moves / judged */
+    uint64_t skipped;           /* filter decisions "stays" (vertex-steps not decided) */
+    uint64_t judged;            /* vertices the first plain (judge) iteration decided */
+    uint64_t judge_moves;       /* of them, vertices that moved */
+    uint64_t skipped_at_judge;  /* vertices whose last evaluation before the judge was a skip */
+    uint64_t skipped_moved;     /* of them, vertices the judge iteration moved: the filter's errors */
+    double skip_error_rate;     /* skipped_moved / skipped_at_judge */
 } sc_filter_stats;
 
 /* leiden() whose local moving skips active vertices the model gives a move probability below
  * threshold: a skipped vertex leaves the active set as if it had decided to stay (a neighbour's
- * move re-activates it), until nothing moves. judge != 0: the vertices whose last evaluation was
- * a skip are then decided by plain iterations run to convergence (the judge), so every level
- * ends in a converged plain local-moving state and stats counts the filter's errors (moves in
- * the first judge iteration). judge == 0: no judge runs and only stats->skipped is counted. */
+ * move re-activates it), until nothing moves. Then the judge: judge == 1 activates every vertex
+ * (a full plain sweep), judge == 2 only the vertices whose last evaluation was a skip, and plain
+ * iterations run to convergence, so every level ends in a converged plain local-moving state;
+ * stats counts the skipped vertices the judge moved. judge == 0: no judge runs and only
+ * stats->skipped is counted. */
 sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, const float *weights,
                              double threshold, int32_t judge, uint64_t *labels_out, uint64_t *n_clusters_out,
                              sc_filter_stats *stats, sc_error *err);

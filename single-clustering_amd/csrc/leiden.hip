@@ -112,6 +112,12 @@ struct DecideArgs {
     uint64_t *trace_key;  // (level << 56 | iteration << 32 | vertex): the samples' canonical order
     unsigned long long *trace_n;
     uint64_t trace_cap;
+    // per-vertex filter features kept by local moving when a filter or a trace is on (else
+    // null): the last decision's gain margin and own-community weight fraction (-1 before the
+    // vertex's first decision at this level), and neighbour moves since that decision
+    float *f_margin;
+    float *f_own;
+    uint32_t *f_nmv;
 };
 
 // Parameters of one job of a batched run (one graph at one resolution, sharing a disjoint-union
@@ -147,26 +153,25 @@ __device__ __forceinline__ double delta_insert(int objective, const VParams &p, 
     return (w_c + loop) - p.gamma * na * (2.0 * Wc + na - 1.0) / 2.0;
 }
 
-// Features of an active vertex for the learned filter, from the state every decision reads:
-// degree, community size, the vertex's strength relative to its community's mean, the
-// community's share of 2m, whether it moved earlier at this level, iteration, level, and the
-// self-loop share of its strength. All O(1) per vertex.
+// Features of an active vertex for the learned filter, from the state every decision reads
+// and the vertex's last decision at this level: degree, own-community weight fraction and
+// best-gain margin of the last decision, neighbour moves since then, whether it has decided at
+// this level yet, iteration, level and own community size (each scaled to about [0, 1]).
 __device__ __forceinline__ void vertex_features(const DecideArgs &a, const VParams &vp, const uint32_t *mstep,
                                                 uint32_t v, uint32_t it, float (&x)[MLP_NF]) {
+    (void)vp;
+    (void)mstep;
     const uint32_t own = a.memb[v];
     const double deg = (double)(a.ptrs[v + 1] - a.ptrs[v]);
-    const double cs = (double)a.csize[own];
-    const double K = from_fx(a.Kfx[own], vp.sK);
-    const double k = a.strength[v];
-    const double two_m = vp.four_m > 0.0 ? 0.5 * vp.four_m : 1.0;
+    const float fo = a.f_own[v];
     x[0] = (float)(log2(1.0 + deg) / 16.0);
-    x[1] = (float)(log2(1.0 + cs) / 16.0);
-    x[2] = (float)(fmin(k * cs / fmax(K, 1e-300), 8.0) / 8.0);
-    x[3] = (float)(log2(fmax(K / two_m, 1e-12)) / 40.0 + 1.0);
-    x[4] = (mstep && mstep[v] != NONE) ? 1.0f : 0.0f;
+    x[1] = fo >= 0.0f ? fo : 0.0f;
+    x[2] = a.f_margin[v];
+    x[3] = (float)(log2(1.0 + (double)a.f_nmv[v]) / 8.0);
+    x[4] = fo >= 0.0f ? 1.0f : 0.0f;
     x[5] = (float)min(it, 15u) / 15.0f;
     x[6] = (float)min(a.level, 15u) / 15.0f;
-    x[7] = k > 0.0 ? (float)(a.self_loop[v] / k) : 0.0f;
+    x[7] = (float)(log2(1.0 + (double)a.csize[own]) / 16.0);
 }
 
 __device__ __forceinline__ uint32_t subset_of(uint64_t seed, uint32_t v, uint32_t P) {
@@ -367,6 +372,14 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
                     a.trace_y[slot] = accept ? 1 : 0;
                     a.trace_key[slot] = ((uint64_t)a.level << 56) | ((uint64_t)min(sp.round, 0xFFFFFFu) << 32) | v;
                 }
+            }
+            if (a.f_margin) {
+                // this decision becomes the vertex's "last decision" for its next features
+                const double mg = bc != NONE ? (bs - base_q) / fmax(k, 1e-300) : -1e300;
+                const float m = (float)fmin(fmax(mg, -1e30), 1e30);
+                a.f_margin[v] = m / (1.0f + fabsf(m));
+                a.f_own[v] = k > 0.0 ? (float)fmin(fmax(from_fx(wown, vp.sw) / k, 0.0), 1.0) : 0.0f;
+                a.f_nmv[v] = 0u;
             }
             if (accept) {
                 a.target[v] = bc;
@@ -736,14 +749,15 @@ struct SweepArgs {
     long long *trace;       // optional: real-time stamps after each barrier (workgroup 0)
     uint32_t trace_cap;
     // learned filter (local moving): packed BF16 weights (null = plain), skip when the logit is
-    // below filter_thr. A skipped vertex leaves the active set like one that decided to stay;
-    // filter_judge: it keeps a mark, and when the filtered iterations converge the marked
-    // vertices are decided by plain iterations run to convergence (the judge).
+    // below filter_thr. A skipped vertex leaves the active set like one that decided to stay,
+    // with a mark; when the filtered iterations converge, the judge activates every vertex
+    // (filter_judge 1) or the marked ones (2) and plain iterations run to convergence.
     // fstats: [0] skip events, [1] vertices the judge iteration decided, [2] of them moved
     const MlpPacked *filter_pk;
     float filter_thr;
-    int filter_judge;
-    unsigned long long *fstats;
+    int filter_judge;             // 0 none, 1 full plain sweep, 2 the marked vertices only
+    uint8_t *f_mark;              // 1 while a vertex's last evaluation was a skip
+    unsigned long long *fstats;   // also [3] marked vertices at the judge, [4] of them moved
 };
 
 __device__ __forceinline__ unsigned int ld_acquire_ctr(const unsigned int *p) {
@@ -870,10 +884,13 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
             const float z = mlp_forward_wave(A.filter_pk, x);
             if (take && z < A.filter_thr) {
                 // it leaves the active set like a vertex that decided to stay (a neighbour's
-                // move re-activates it); with a judge it keeps a mark (2) for the judge
+                // move re-activates it), marked as skipped
                 take = false;
                 skipped++;
-                a.active[v] = A.filter_judge ? 2 : 0;
+                a.active[v] = A.filter_judge == 2 ? 2 : 0;
+                A.f_mark[v] = 1;
+            } else if (take) {
+                A.f_mark[v] = 0;
             }
         }
         uint64_t pending = __ballot(take);
@@ -1000,6 +1017,12 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             a.targeted_cnt[v] = 0;
         } else {
             A.mstep[v] = NONE;
+            if (a.f_margin) {
+                a.f_margin[v] = 0.0f;
+                a.f_own[v] = -1.0f;
+                a.f_nmv[v] = 0u;
+            }
+            if (FILTER) A.f_mark[v] = 0;
         }
         if (capped) A.claim[v] = NONE;
     }
@@ -1076,6 +1099,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                         const uint64_t from = REFINE ? v : a.memb[v];
                         A.memb[v] = (uint32_t)t;
                         move_totals(A, v, from, t);
+                        if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                     }
                 }
                 if (REFINE) {
@@ -1097,6 +1121,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                         } else {
                             const uint32_t cu = A.mstep[u] == step ? a.target[u] : a.memb[u];
                             act = cu != (uint32_t)t;
+                            if (a.f_nmv) atomicAdd(&a.f_nmv[u], 1u);
                         }
                         if (act) a.active[u] = 1;
                     }
@@ -1111,7 +1136,9 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                     if (A.claim[t] != (uint32_t)v) continue;
                     for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
                         const uint32_t u = a.nbrs[e];
-                        if (u != v && a.memb[u] != t) a.active[u] = 1;
+                        if (u == v) continue;
+                        if (a.f_nmv) atomicAdd(&a.f_nmv[u], 1u);
+                        if (a.memb[u] != t) a.active[u] = 1;
                     }
                 }
                 if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
@@ -1134,12 +1161,17 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         }
         if (moved == 0) {
             if (filter_on && A.filter_judge) {
-                // the judge: vertices whose last evaluation was a skip are decided by plain
-                // iterations, which then run to convergence
+                // the judge: plain iterations from every vertex (a full sweep) or from the
+                // vertices whose last evaluation was a skip, run to convergence
                 filter_on = false;
                 judge_it = it + 1;
-                for (uint64_t v = gtid; v < n; v += gsize)
-                    if (a.active[v] == 2) a.active[v] = 1;
+                unsigned long long marked = 0;
+                for (uint64_t v = gtid; v < n; v += gsize) {
+                    marked += A.f_mark[v];
+                    if (A.filter_judge == 1 || a.active[v] == 2) a.active[v] = 1;
+                }
+                marked = block_sum(marked);
+                if (threadIdx.x == 0 && marked) atomicAdd(&A.fstats[3], marked);
                 if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
                 continue;
             }
@@ -1208,6 +1240,10 @@ struct LevelState {
     DevBuf<uint32_t> target, tround, movers, moved, targeted, claim;
     DevBuf<unsigned int> counters;           // [0] movers, [1] moved
     DevBuf<unsigned long long> stat;         // vertices, edges, candidates
+    // learned filter / trace runs only: decision features and skip marks (see DecideArgs)
+    DevBuf<float> f_margin, f_own;
+    DevBuf<uint32_t> f_nmv;
+    DevBuf<uint8_t> f_mark;
     // persistent sweeps: compacted step lists, mover stamps, step counters + barrier + control
     DevBuf<uint32_t> work, mstep;
     DevBuf<unsigned int> sweep;
@@ -1266,7 +1302,7 @@ struct RunCtx {
 // Learned-filter inference or trace collection for local moving (device pointers).
 struct FilterOpts {
     const MlpPacked *pk = nullptr;        // packed BF16 weights; null: no filtering
-    bool judge = true;                    // decide the skipped vertices with plain iterations at the end
+    int judge = 1;                        // 0 none, 1 full plain sweep, 2 skipped vertices only
     float logit_thr = 0.0f;               // skip when the logit is below this
     unsigned long long *fstats = nullptr; // [skip events, judged, judge moves]
     float *trace_x = nullptr;             // trace runs: samples (features, moved, order key)
@@ -1448,7 +1484,8 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     if (filtered) {
         A.filter_pk = ctx.fopt->pk;
         A.filter_thr = ctx.fopt->logit_thr;
-        A.filter_judge = ctx.fopt->judge ? 1 : 0;
+        A.filter_judge = ctx.fopt->judge;
+        A.f_mark = ls.f_mark.get();
         A.fstats = ctx.fopt->fstats;
     }
     A.Kfx = ls.Kfx.get();
@@ -1598,6 +1635,18 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         a.trace_key = ctx.fopt->trace_key;
         a.trace_n = ctx.fopt->trace_n;
         a.trace_cap = ctx.fopt->trace_cap;
+    }
+    if (ctx.fopt) {
+        // filter features (kept by the sweep; the arrays hold the largest level's size)
+        if (ls.f_margin.size() < n) {
+            ls.f_margin.resize(n);
+            ls.f_own.resize(n);
+            ls.f_nmv.resize(n);
+            ls.f_mark.resize(n);
+        }
+        a.f_margin = ls.f_margin.get();
+        a.f_own = ls.f_own.get();
+        a.f_nmv = ls.f_nmv.get();
     }
     a.memb = memb;
     a.P = MOVE_SUBSWEEPS;
@@ -2676,32 +2725,35 @@ sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, con
         if (!g || !cfg || !weights || !n_clusters_out || (!labels_out && g->n))
             throw ScError(SC_INVALID_ARGUMENT, "null argument");
         if (!(threshold > 0.0 && threshold < 1.0)) throw ScError(SC_INVALID_ARGUMENT, "threshold must be in (0, 1)");
+        if (judge < 0 || judge > 2) throw ScError(SC_INVALID_ARGUMENT, "judge must be 0, 1 or 2");
         if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
         hipStream_t s = thread_stream();
         const uint32_t n = (uint32_t)g->n;
         DevBuf<float> w(MLP_PARAMS);
         DevBuf<MlpPacked> pk(1);
-        DevBuf<unsigned long long> fs(3);
+        DevBuf<unsigned long long> fs(5);
         to_device(w.get(), weights, MLP_PARAMS, s);
         mlp_pack_device(w.get(), pk.get(), s);
-        HIP_OK(hipMemsetAsync(fs.get(), 0, 3 * sizeof(unsigned long long), s));
+        HIP_OK(hipMemsetAsync(fs.get(), 0, 5 * sizeof(unsigned long long), s));
         FilterOpts fo;
         fo.pk = pk.get();
-        fo.judge = judge != 0;
+        fo.judge = judge;
         fo.logit_thr = (float)std::log(threshold / (1.0 - threshold));
         fo.fstats = fs.get();
         DevBuf<uint32_t> labels(n);
         sc_leiden_stats st{};
         leiden_device(*g, *cfg, labels.get(), n_clusters_out, s, &st, nullptr, &fo);
         if (n) labels_to_host(labels.get(), n, labels_out, s);
-        unsigned long long h[3] = {0, 0, 0};
-        to_host(h, fs.get(), 3, s);
+        unsigned long long h[5] = {0, 0, 0, 0, 0};
+        to_host(h, fs.get(), 5, s);
         HIP_OK(hipStreamSynchronize(s));
         if (fst) {
             fst->skipped = h[0];
             fst->judged = h[1];
             fst->judge_moves = h[2];
-            fst->skip_error_rate = h[1] ? (double)h[2] / (double)h[1] : 0.0;
+            fst->skipped_at_judge = h[3];
+            fst->skipped_moved = h[4];
+            fst->skip_error_rate = h[3] ? (double)h[4] / (double)h[3] : 0.0;
         }
         t_stats = st;
         return SC_OK;

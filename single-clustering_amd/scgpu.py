@@ -90,7 +90,7 @@ class ScKnnStats(C.Structure):
 
 class ScFilterStats(C.Structure):
     _fields_ = [("skipped", C.c_uint64), ("judged", C.c_uint64), ("judge_moves", C.c_uint64),
-                ("skip_error_rate", C.c_double)]
+                ("skipped_at_judge", C.c_uint64), ("skipped_moved", C.c_uint64), ("skip_error_rate", C.c_double)]
 
 
 # Every symbol include/scgpu.h declares, with its ctypes signature.
@@ -554,12 +554,16 @@ def leiden_trace(graph, config=None, cap=1 << 22):
     return X[:n.value].copy(), y[:n.value].copy()
 
 
-def leiden_filtered(graph, weights, threshold=0.02, config=None, judge=True):
+JUDGES = {False: 0, True: 1, "none": 0, "full": 1, "skipped": 2}
+
+
+def leiden_filtered(graph, weights, threshold=0.02, config=None, judge="full"):
     """leiden() with the learned active-vertex filter in local moving; returns (Clustering,
-    stats) where stats holds the skip count, the judge sweep's size and moves, and the
-    skip-error rate. A skipped vertex leaves the active set until a neighbour moves; with
-    judge=True the vertices last skipped are decided by plain iterations at the end of each
-    local-moving phase, judge=False leaves them (only the skip count is then reported)."""
+    stats) where stats holds the skip count, the judge iteration's size and moves, the skipped
+    vertices it moved and the skip-error rate. A skipped vertex leaves the active set until a
+    neighbour moves. At each level's convergence the judge runs plain iterations from every
+    vertex ("full" or True), from the vertices last skipped ("skipped"), or not at all ("none"
+    or False: only the skip count is then reported)."""
     config = config or LeidenConfig()
     c = config.to_c()
     w = np.ascontiguousarray(weights, dtype=np.float32)
@@ -568,7 +572,9 @@ def leiden_filtered(graph, weights, threshold=0.02, config=None, judge=True):
     nc = C.c_uint64(0)
     st = ScFilterStats()
     err = ScError()
-    _check(_lib.sc_leiden_filtered(graph.handle, C.byref(c), _fp(w), float(threshold), 1 if judge else 0,
+    if judge not in JUDGES:
+        raise ValueError(f"invalid judge {judge!r}")
+    _check(_lib.sc_leiden_filtered(graph.handle, C.byref(c), _fp(w), float(threshold), JUDGES[judge],
                                    _p(labels, u64p), C.byref(nc),
                                    C.byref(st), C.byref(err)), err)
     return Clustering(labels[:n], nc.value), {k: getattr(st, k) for k, _ in ScFilterStats._fields_}

@@ -67,28 +67,36 @@ def test_trace_and_filtered_leiden():
     assert np.all(np.isfinite(X))
     w, _ = scgpu.mlp_train(X, y, epochs=20, batch=2048, lr=3e-3, seed=2)
     plain = scgpu.leiden(g)
-    # a filter that never skips reproduces plain Leiden exactly
+    q_plain = scgpu.modularity(g, plain.labels())
+    # a filter that never skips reproduces plain Leiden exactly, unless the judge is a full
+    # sweep (which also re-decides vertices plain local moving left inactive)
     never = w.copy()
     never[-1] = 1e4
-    for judge in (True, False):
+    for judge in ("none", "skipped"):
         c, st = scgpu.leiden_filtered(g, never, 0.5, judge=judge)
         assert np.array_equal(c.labels(), plain.labels()) and st["skipped"] == 0
-    c, st = scgpu.leiden_filtered(g, w, 0.05)
-    q_plain = scgpu.modularity(g, plain.labels())
-    q = scgpu.modularity(g, c.labels())
-    print(f"filter: skipped {st['skipped']}, judged {st['judged']}, errors {st['judge_moves']} "
-          f"(rate {st['skip_error_rate']:.4f}); modularity {q:.6f} vs plain {q_plain:.6f}")
-    assert q >= q_plain - 0.01 * abs(q_plain)
-    assert 0.0 <= st["skip_error_rate"] <= 1.0
+    c, st = scgpu.leiden_filtered(g, never, 0.5, judge="full")
+    assert st["skipped"] == 0 and st["skipped_at_judge"] == 0 and st["judged"] > 0
+    assert scgpu.modularity(g, c.labels()) >= q_plain - 1e-3 * abs(q_plain)
+    for judge in ("full", "skipped"):
+        c, st = scgpu.leiden_filtered(g, w, 0.05, judge=judge)
+        q = scgpu.modularity(g, c.labels())
+        print(f"filter ({judge} judge): skipped {st['skipped']}, judged {st['judged']}, "
+              f"skipped at judge {st['skipped_at_judge']}, moved {st['skipped_moved']} "
+              f"(rate {st['skip_error_rate']:.4f}); modularity {q:.6f} vs plain {q_plain:.6f}")
+        assert q >= q_plain - 0.01 * abs(q_plain)
+        assert 0.0 <= st["skip_error_rate"] <= 1.0 and st["skipped_moved"] <= st["skipped_at_judge"]
+        assert np.array_equal(scgpu.leiden_filtered(g, w, 0.05, judge=judge)[0].labels(), c.labels())
     # without the judge nothing re-decides the skipped vertices: the result rests on the model
     # alone (a sanity bound here), and is still deterministic
-    cf, fst = scgpu.leiden_filtered(g, w, 0.05, judge=False)
+    cf, fst = scgpu.leiden_filtered(g, w, 0.05, judge="none")
     qf = scgpu.modularity(g, cf.labels())
     print(f"filter (no judge): skipped {fst['skipped']}, modularity {qf:.6f}")
     assert fst["judged"] == 0 and fst["skipped"] > 0
     assert qf >= q_plain - 0.1 * abs(q_plain)
-    assert np.array_equal(scgpu.leiden_filtered(g, w, 0.05, judge=False)[0].labels(), cf.labels())
-    assert np.array_equal(scgpu.leiden_filtered(g, w, 0.05)[0].labels(), c.labels())
+    assert np.array_equal(scgpu.leiden_filtered(g, w, 0.05, judge="none")[0].labels(), cf.labels())
+    with pytest.raises(ValueError):
+        scgpu.leiden_filtered(g, w, 0.05, judge="sometimes")
 
 
 def test_trace_is_reproducible():
