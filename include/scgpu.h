@@ -199,9 +199,10 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
 /* ---- learned active-vertex filter (no counterpart in the reference) --------------------- */
 /* An 8 -> 64 -> 64 -> 1 perceptron (ReLU) predicts whether an active vertex will move when
  * local moving decides it; parameters: sc_mlp_param_count() floats in the order
- * W1[8][64], b1[64], W2[64][64], b2[64], w3[64], b3. Features: degree, community size, strength
- * relative to the community mean, the community's share of 2m, moved earlier at this level,
- * iteration, level, self-loop share. */
+ * W1[8][64], b1[64], W2[64][64], b2[64], w3[64], b3. Features (each about [0, 1]): degree, the
+ * own-community weight fraction and best-gain margin of the vertex's last decision on this
+ * level, neighbour moves since that decision, whether it has decided on this level yet,
+ * iteration, level, own community size. */
 uint32_t sc_mlp_param_count(void);
 /* He-uniform weights, zero biases, from a seed. */
 void sc_mlp_init(uint64_t seed, float *w);
