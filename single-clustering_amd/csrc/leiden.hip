@@ -214,14 +214,32 @@ __device__ __forceinline__ void group_reduce_wave(double &bs, uint32_t &bc, long
     }
 }
 
+// Ordering point between the phases of a decide unit (clear table / insert / scan). A group of
+// G <= 64 lanes lies inside one wave and owns its LDS table, and a wave's LDS operations are
+// performed in order, so only the compiler must not move them across this point: the waves of a
+// workgroup then run their units independently. Wider groups and global tables synchronize the
+// workgroup.
+template <int G, bool GTAB>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (GTAB || G > 64) {
+        __syncthreads();
+        if (GTAB) __threadfence_block();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // Per-step values of a decide pass (the multi-launch path takes them from DecideArgs).
 struct StepParams {
     uint64_t seed;
     uint32_t sub;
     uint32_t round;
     uint32_t step;
-    unsigned int *n_movers;
-    uint32_t *mstep;  // step stamp of accepted movers (persistent local moving), else null
+    unsigned int *n_movers;  // mover count: global, or the workgroup's LDS counter with mv_list
+    uint32_t *mstep;         // step stamp of accepted movers (persistent local moving), else null
+    uint32_t *mv_list;       // persistent sweeps: the workgroup's own mover list (else a.movers)
 };
 
 struct DecideStats {
@@ -265,8 +283,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         keys[s] = EMPTY_KEY;
         vals[s] = 0;
     }
-    __syncthreads();
-    if (GTAB) __threadfence_block();
+    group_sync<G, GTAB>();
 
     uint64_t beg = 0, end = 0;
     uint32_t own = 0, cv = 0;
@@ -277,16 +294,32 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         if (REFINE) cv = a.constraint[v];
     }
     const uint32_t mask = tsize - 1;
-    for (uint64_t e = beg + lane; e < end; e += G) {
-        uint32_t u = a.nbrs[e];
-        if (u == v) continue;  // self-loops are handled by the partition (partition.rs:430)
-        if (REFINE && a.constraint[u] != cv) continue;
-        uint32_t c = a.memb[u];
-        long long wf = to_fx((double)a.wts[e], vp.sw);
-        table_insert(keys, vals, mask, c, wf);
+    // EB entries per lane in flight: their neighbour ids, then their memberships, are gathered
+    // with independent loads before the table inserts (the inserts' order does not matter:
+    // fixed-point sums are exact and candidates are ranked by gain, then id)
+    constexpr int EB = 4;
+    for (uint64_t e0 = beg + lane; e0 < end; e0 += (uint64_t)EB * G) {
+        uint32_t u[EB];
+        float w[EB];
+#pragma unroll
+        for (int j = 0; j < EB; j++) {
+            const uint64_t e = e0 + (uint64_t)j * G;
+            const bool in = e < end;
+            u[j] = in ? a.nbrs[e] : v;  // v: no entry (self-loops are the partition's, partition.rs:430)
+            w[j] = in ? a.wts[e] : 0.0f;
+        }
+        uint32_t c[EB];
+#pragma unroll
+        for (int j = 0; j < EB; j++) {
+            bool ok = u[j] != v;
+            c[j] = ok ? a.memb[u[j]] : EMPTY_KEY;
+            if (REFINE && ok && a.constraint[u[j]] != cv) c[j] = EMPTY_KEY;
+        }
+#pragma unroll
+        for (int j = 0; j < EB; j++)
+            if (c[j] != EMPTY_KEY) table_insert(keys, vals, mask, c[j], to_fx((double)w[j], vp.sw));
     }
-    __syncthreads();
-    if (GTAB) __threadfence_block();
+    group_sync<G, GTAB>();
 
     double bs = -INFINITY;
     uint32_t bc = NONE;
@@ -401,10 +434,11 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         if ((int)(threadIdx.x & 63) == leader) slot0 = atomicAdd(sp.n_movers, (unsigned int)__popcll(pm));
         slot0 = __shfl(slot0, leader);
         if (push)
-            a.movers[slot0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] = v;
+            (sp.mv_list ? sp.mv_list : a.movers)[slot0 + __builtin_amdgcn_mbcnt_hi(
+                                                             (uint32_t)(pm >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] = v;
     }
-    __syncthreads();
+    group_sync<G, GTAB>();
 }
 
 __device__ __forceinline__ void flush_decide_stats(const DecideStats &ds, unsigned long long *s_stats,
@@ -436,7 +470,7 @@ __global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
         round = *a.iter_dev;
         if (!REFINE) seed = mix64(a.iter_seed_base ^ (0x5851F42D4C957F2Dull * (uint64_t)(round + 1)));
     }
-    StepParams sp{seed, a.sub, round, 0u, a.n_movers, nullptr};
+    StepParams sp{seed, a.sub, round, 0u, a.n_movers, nullptr, nullptr};
     DecideStats ds;
     for (uint64_t base = (uint64_t)blockIdx.x * NG; base < a.list_len; base += (uint64_t)gridDim.x * NG)
         decide_unit<BS, G, T, REFINE, GTAB>(a, sp, a.list, nullptr, a.list_len, base, s_keys, s_vals, r_s, r_c,
@@ -736,6 +770,7 @@ struct SweepArgs {
     unsigned int *mcnt;     // [max_steps] deciders that chose to move
     unsigned int *ccnt;     // [max_steps] vertices that moved (refinement)
     uint32_t *mstep;        // step stamp of accepted movers (local moving)
+    uint32_t mv_stride;     // each workgroup's mover list: d.movers + blockIdx.x * mv_stride
     uint32_t *claim;        // lowest mover id per capped target, or null without a weight cap
     BarrierCtl *bc;         // grid barrier words (zeroed before the launch)
     unsigned int *ctl;      // [0] steps run, [2] iterations, [3] moves
@@ -856,64 +891,73 @@ __device__ __forceinline__ int bucket_of_pos(const SweepArgs &A, uint32_t i) {
 
 // Positions (within their bucket) of the vertices that decide in this step, wave-aggregated
 // appends into work[bstart[b] + ...].
+// Workgroup b's share of bucket k's list positions, [lo, hi): every bucket is split evenly over
+// the workgroups, so each compacts and decides its own share (no global counters, no barrier
+// between compaction and decide).
+__device__ __forceinline__ void bucket_share(const SweepArgs &A, int k, uint32_t &lo, uint32_t &hi) {
+    const uint64_t L = (uint64_t)(A.bstart[k + 1] - A.bstart[k]);
+    lo = A.bstart[k] + (uint32_t)(L * blockIdx.x / gridDim.x);
+    hi = A.bstart[k] + (uint32_t)(L * (blockIdx.x + 1) / gridDim.x);
+}
+
+// Positions (relative to their bucket) of this workgroup's share that decide in this step,
+// stored at the start of the share's own range of work[]; s_wc[k] gets their count.
 template <bool REFINE, bool FILTER>
-__device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t step,
-                                              uint32_t it, bool filter_on) {
+__device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t it,
+                                              bool filter_on, unsigned int *s_wc) {
     const DecideArgs &a = A.d;
-    const uint32_t n = a.n;
     const int lane = threadIdx.x & 63;
-    unsigned int *cnt = A.wcnt + (uint64_t)step * NUM_BUCKETS;
     unsigned long long skipped = 0;
-    for (uint32_t i0 = blockIdx.x * SWEEP_BS; i0 < n; i0 += gridDim.x * SWEEP_BS) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool take = false;
-        int b = 0;
-        uint32_t v = 0;
-        float x[MLP_NF] = {};
-        if (i < n) {
-            v = A.lists[i];
-            b = bucket_of_pos(A, i);
-            const VParams vp = vparams(a, v);
-            take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
-            if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
-            if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
-        }
-        // learned filter: a vertex predicted to stay is not decided now. The loop bounds are
-        // workgroup-uniform, so every lane of the wave reaches the wave-collective inference.
-        if (FILTER && filter_on && __any(take)) {
-            const float z = mlp_forward_wave(A.filter_pk, x);
-            if (take && z < A.filter_thr) {
-                // it leaves the active set like a vertex that decided to stay (a neighbour's
-                // move re-activates it), marked as skipped
-                take = false;
-                skipped++;
-                a.active[v] = A.filter_judge == 2 ? 2 : 0;
-                A.f_mark[v] = 1;
-            } else if (take) {
-                A.f_mark[v] = 0;
+    for (int k = 0; k < NUM_BUCKETS; k++) {
+        uint32_t lo, hi;
+        bucket_share(A, k, lo, hi);
+        for (uint32_t i0 = lo; i0 < hi; i0 += SWEEP_BS) {  // workgroup-uniform bounds
+            const uint32_t i = i0 + threadIdx.x;
+            bool take = false;
+            uint32_t v = 0;
+            float x[MLP_NF] = {};
+            if (i < hi) {
+                v = A.lists[i];
+                const VParams vp = vparams(a, v);
+                take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
+                if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
+                if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
             }
-        }
-        uint64_t pending = __ballot(take);
-        while (pending) {
-            const int leader = __ffsll((unsigned long long)pending) - 1;
-            const int bl = __shfl(b, leader);
-            const bool mine = take && b == bl;
-            const uint64_t m = __ballot(mine);
-            unsigned int basei = 0;
-            if (lane == leader) basei = atomicAdd(&cnt[bl], (unsigned int)__popcll(m));
-            basei = __shfl(basei, leader);
-            if (mine) {
-                const uint32_t below =
-                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                A.work[A.bstart[bl] + basei + below] = i - A.bstart[bl];
+            // learned filter: a vertex predicted to stay is not decided now; every lane of the
+            // wave reaches the wave-collective inference
+            if (FILTER && filter_on && __any(take)) {
+                const float z = mlp_forward_wave(A.filter_pk, x);
+                if (take && z < A.filter_thr) {
+                    // it leaves the active set like a vertex that decided to stay (a
+                    // neighbour's move re-activates it), marked as skipped
+                    take = false;
+                    skipped++;
+                    a.active[v] = A.filter_judge == 2 ? 2 : 0;
+                    A.f_mark[v] = 1;
+                } else if (take) {
+                    A.f_mark[v] = 0;
+                }
             }
-            pending &= ~m;
+            const uint64_t m = __ballot(take);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                unsigned int basei = 0;
+                if (lane == leader) basei = atomicAdd(&s_wc[k], (unsigned int)__popcll(m));
+                basei = __shfl(basei, leader);
+                if (take) {
+                    const uint32_t below =
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    A.work[lo + basei + below] = i - A.bstart[k];
+                }
+            }
         }
     }
     if (FILTER && filter_on) {
         skipped = block_sum(skipped);
         if (threadIdx.x == 0 && skipped) atomicAdd(&A.fstats[0], skipped);
     }
+    __threadfence_block();  // the work[] entries are read by the workgroup's other waves
+    __syncthreads();
 }
 
 template <int B>
@@ -927,41 +971,37 @@ template <> struct BucketShape<4> { static constexpr int G = 256, T = 2048; };
 template <> struct BucketShape<5> { static constexpr int G = 256, T = 4096; };
 template <> struct BucketShape<6> { static constexpr int G = 256, T = 1; };
 
-template <int B, bool REFINE>
-__device__ __forceinline__ void sweep_decide_unit(const SweepArgs &A, const StepParams &sp, uint32_t cntb,
-                                                  uint64_t u, uint32_t *s_keys, long long *s_vals, double *r_s,
-                                                  uint32_t *r_c, long long *r_w, DecideStats &ds) {
-    constexpr int G = BucketShape<B>::G, T = BucketShape<B>::T, NG = SWEEP_BS / G;
-    static_assert(B == 6 || NG * T <= SWEEP_TAB, "bucket table does not fit");
-    decide_unit<SWEEP_BS, G, T, REFINE, B == 6>(A.d, sp, A.lists + A.bstart[B], A.work + A.bstart[B], cntb,
-                                                 u * NG, s_keys, s_vals, r_s, r_c, r_w, ds);
-}
-
 template <bool REFINE>
-__device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParams &sp, uint32_t *s_keys,
-                                             long long *s_vals, double *r_s, uint32_t *r_c, long long *r_w,
-                                             DecideStats &ds) {
-    constexpr int NGs[NUM_BUCKETS] = {SWEEP_BS / BucketShape<0>::G, SWEEP_BS / BucketShape<1>::G,
-                                      SWEEP_BS / BucketShape<2>::G, SWEEP_BS / BucketShape<3>::G,
-                                      SWEEP_BS / BucketShape<4>::G, SWEEP_BS / BucketShape<5>::G,
-                                      SWEEP_BS / BucketShape<6>::G};
-    uint32_t cnt[NUM_BUCKETS];
-    uint64_t ustart[NUM_BUCKETS + 1];
-    ustart[0] = 0;
-#pragma unroll
-    for (int b = 0; b < NUM_BUCKETS; b++) {
-        cnt[b] = ld_acquire_ctr(A.wcnt + (uint64_t)sp.step * NUM_BUCKETS + b);
-        ustart[b + 1] = ustart[b] + (cnt[b] + NGs[b] - 1) / NGs[b];
+__device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParams &sp, const unsigned int *s_wc,
+                                             uint32_t *s_keys, long long *s_vals, double *r_s, uint32_t *r_c,
+                                             long long *r_w, DecideStats &ds) {
+    // bucket by bucket over the workgroup's own compacted share; waves run the units of a bucket
+    // independently (group_sync), and a bucket with another table layout starts after every
+    // wave has left the previous one
+    bool first = true;
+#define SC_DECIDE_BUCKET(BK)                                                                          \
+    {                                                                                                 \
+        constexpr int G = BucketShape<BK>::G, T = BucketShape<BK>::T, NG = SWEEP_BS / G;               \
+        static_assert(BK == 6 || NG * T <= SWEEP_TAB, "bucket table does not fit");                   \
+        const uint32_t len = s_wc[BK];                                                                \
+        if (len) {                                                                                    \
+            if (!first) __syncthreads();                                                              \
+            first = false;                                                                            \
+            uint32_t lo, hi;                                                                          \
+            bucket_share(A, BK, lo, hi);                                                              \
+            for (uint64_t base = 0; base < len; base += NG)                                           \
+                decide_unit<SWEEP_BS, G, T, REFINE, BK == 6>(A.d, sp, A.lists + A.bstart[BK], A.work + lo, \
+                                                             len, base, s_keys, s_vals, r_s, r_c, r_w, ds); \
+        }                                                                                             \
     }
-    for (uint64_t u = blockIdx.x; u < ustart[NUM_BUCKETS]; u += gridDim.x) {
-        if (u < ustart[1]) sweep_decide_unit<0, REFINE>(A, sp, cnt[0], u, s_keys, s_vals, r_s, r_c, r_w, ds);
-        else if (u < ustart[2]) sweep_decide_unit<1, REFINE>(A, sp, cnt[1], u - ustart[1], s_keys, s_vals, r_s, r_c, r_w, ds);
-        else if (u < ustart[3]) sweep_decide_unit<2, REFINE>(A, sp, cnt[2], u - ustart[2], s_keys, s_vals, r_s, r_c, r_w, ds);
-        else if (u < ustart[4]) sweep_decide_unit<3, REFINE>(A, sp, cnt[3], u - ustart[3], s_keys, s_vals, r_s, r_c, r_w, ds);
-        else if (u < ustart[5]) sweep_decide_unit<4, REFINE>(A, sp, cnt[4], u - ustart[4], s_keys, s_vals, r_s, r_c, r_w, ds);
-        else if (u < ustart[6]) sweep_decide_unit<5, REFINE>(A, sp, cnt[5], u - ustart[5], s_keys, s_vals, r_s, r_c, r_w, ds);
-        else sweep_decide_unit<6, REFINE>(A, sp, cnt[6], u - ustart[6], s_keys, s_vals, r_s, r_c, r_w, ds);
-    }
+    SC_DECIDE_BUCKET(0)
+    SC_DECIDE_BUCKET(1)
+    SC_DECIDE_BUCKET(2)
+    SC_DECIDE_BUCKET(3)
+    SC_DECIDE_BUCKET(4)
+    SC_DECIDE_BUCKET(5)
+    SC_DECIDE_BUCKET(6)
+#undef SC_DECIDE_BUCKET
 }
 
 __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint64_t from, uint64_t to) {
@@ -995,6 +1035,8 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
     __shared__ long long r_w[SWEEP_BS / 64];
     __shared__ unsigned long long s_stats[3];
     __shared__ int s_flag;
+    __shared__ unsigned int s_wc[NUM_BUCKETS];  // this step's compacted vertices per bucket
+    __shared__ unsigned int s_mc;               // this step's movers of the workgroup
     const DecideArgs &a = A.d;
     const uint32_t n = a.n;
     const uint64_t gtid = (uint64_t)blockIdx.x * SWEEP_BS + threadIdx.x;
@@ -1044,43 +1086,66 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
     const int lane = threadIdx.x % SWEEP_ACT_G;
     const uint32_t gslot = threadIdx.x / SWEEP_ACT_G;
     constexpr uint32_t NGA = SWEEP_BS / SWEEP_ACT_G;
+    uint32_t *const mv_list = a.movers + (uint64_t)blockIdx.x * A.mv_stride;  // this workgroup's movers
+    if (threadIdx.x == 0) s_mc = 0;
     uint32_t step = 0, it = 0;
     bool ok = true;
     // with a learned filter, local moving runs filtered to convergence; with a judge, plain
     // iterations follow (the first plain iteration's moves are the filter's errors)
     bool filter_on = FILTER;
     uint32_t judge_it = 0xFFFFFFFFu;
+    unsigned long long total_moves = 0, judge_dec = 0, judge_mv = 0;  // thread 0's tallies
     for (; it < A.max_iters && ok; it++) {
         const uint64_t seed = REFINE ? a.seed : mix64(A.level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
         const uint32_t P = REFINE ? 1u : a.P;
         for (uint32_t sub = 0; sub < P && ok; sub++, step++) {
-            // compact; clear the claims and join counts the previous step left behind
+            // clear the claims and join counts this workgroup's movers of the previous step
+            // left behind; every workgroup finishes that before anyone decides again
             if (step > 0 && (capped || REFINE)) {
-                const unsigned int pm = ld_acquire_ctr(A.mcnt + step - 1);
-                for (uint64_t i = gtid; i < pm; i += gsize) {
-                    const uint32_t t = a.target[a.movers[i]];
+                const unsigned int pm = s_mc;
+                for (uint32_t i = threadIdx.x; i < pm; i += SWEEP_BS) {
+                    const uint32_t t = a.target[mv_list[i]];
                     if (capped) A.claim[t] = NONE;
                     if (REFINE) a.targeted_cnt[t] = 0;
                 }
+                if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
             }
-            sweep_compact<REFINE, FILTER>(A, seed, sub, step, it, filter_on);
+            __syncthreads();
+            if (threadIdx.x < NUM_BUCKETS) s_wc[threadIdx.x] = 0;
+            if (threadIdx.x == 0) s_mc = 0;
+            __syncthreads();
+            // compact and decide the workgroup's own share of every bucket
+            sweep_compact<REFINE, FILTER>(A, seed, sub, it, filter_on, s_wc);
+            StepParams sp{seed, sub, it, step, &s_mc, REFINE ? nullptr : A.mstep, mv_list};
+            sweep_decide<REFINE>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
+            __syncthreads();
+            const unsigned int nm = s_mc;
+            if (threadIdx.x == 0) {
+                if (!REFINE) {
+                    if (nm) A.mcnt[step] = 1u;  // someone chose to move: not converged
+                    total_moves += nm;
+                }
+                if (!REFINE && it == judge_it) {
+                    unsigned long long dec = 0;
+                    for (int b = 0; b < NUM_BUCKETS; b++) dec += s_wc[b];
+                    judge_dec += dec;
+                    judge_mv += nm;
+                }
+            }
             if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
 
-            StepParams sp{seed, sub, it, step, A.mcnt + step, REFINE ? nullptr : A.mstep};
-            sweep_decide<REFINE>(A, sp, s_keys, s_vals, r_s, r_c, r_w, ds);
-            if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
-
-            const unsigned int nm = ld_acquire_ctr(A.mcnt + step);
             if (capped) {
-                for (uint64_t i = gtid; i < nm; i += gsize) {
-                    const uint64_t v = a.movers[i];
-                    atomicMin(&A.claim[a.target[v]], (uint32_t)v);
+                for (uint32_t i = threadIdx.x; i < nm; i += SWEEP_BS) {
+                    const uint32_t v = mv_list[i];
+                    atomicMin(&A.claim[a.target[v]], v);
                 }
                 if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
             }
-            // apply; re-activation is fused with it where it cannot race with the moves
-            for (uint64_t i = (uint64_t)blockIdx.x * NGA + gslot; i < nm; i += (uint64_t)gridDim.x * NGA) {
-                const uint64_t v = a.movers[i];
+            // apply this workgroup's movers; re-activation is fused with it where it cannot race
+            // with the moves
+            unsigned int applied = 0;
+            for (uint32_t i = gslot; i < nm; i += NGA) {
+                const uint64_t v = mv_list[i];
                 const uint64_t t = a.target[v];
                 bool go;
                 if (REFINE) {
@@ -1099,13 +1164,9 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                         const uint64_t from = REFINE ? v : a.memb[v];
                         A.memb[v] = (uint32_t)t;
                         move_totals(A, v, from, t);
+                        applied++;
                         if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                     }
-                }
-                if (REFINE) {
-                    const uint64_t mm = __ballot(go && lane == 0);
-                    if (mm && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)mm) - 1)
-                        atomicAdd(A.ccnt + step, (unsigned int)__popcll(mm));
                 }
                 if (go && (REFINE || !capped)) {
                     // refinement: neighbours in the same working community (refine.rs); local
@@ -1127,11 +1188,18 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                     }
                 }
             }
+            if (REFINE) {
+                const unsigned long long moved_here = block_sum((unsigned long long)applied);
+                if (threadIdx.x == 0) {
+                    if (moved_here) A.ccnt[step] = 1u;  // someone moved: not converged
+                    total_moves += moved_here;
+                }
+            }
             if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
             if (!REFINE && capped) {
                 // local moving with a cap: activate after all moves landed
-                for (uint64_t i = (uint64_t)blockIdx.x * NGA + gslot; i < nm; i += (uint64_t)gridDim.x * NGA) {
-                    const uint64_t v = a.movers[i];
+                for (uint32_t i = gslot; i < nm; i += NGA) {
+                    const uint64_t v = mv_list[i];
                     const uint32_t t = a.target[v];
                     if (A.claim[t] != (uint32_t)v) continue;
                     for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
@@ -1145,19 +1213,17 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             }
         }
         if (!ok) break;
-        // convergence, read identically by every workgroup after the barrier
+        // convergence, read identically by every workgroup after the barrier: the step flags
+        // say whether anyone chose to move (local moving) or moved (refinement)
         unsigned int moved = 0;
         if (REFINE) {
             moved = ld_acquire_ctr(A.ccnt + step - 1);
         } else {
-            for (uint32_t s2 = step - P; s2 < step; s2++) moved += ld_acquire_ctr(A.mcnt + s2);
+            for (uint32_t s2 = step - P; s2 < step; s2++) moved |= ld_acquire_ctr(A.mcnt + s2);
         }
-        if (!REFINE && it == judge_it && gtid == 0) {
-            unsigned long long dec = 0;
-            for (uint32_t s2 = step - P; s2 < step; s2++)
-                for (int b = 0; b < NUM_BUCKETS; b++) dec += ld_acquire_ctr(A.wcnt + (uint64_t)s2 * NUM_BUCKETS + b);
-            atomicAdd(&A.fstats[1], dec);
-            atomicAdd(&A.fstats[2], (unsigned long long)moved);
+        if (!REFINE && it == judge_it && threadIdx.x == 0) {
+            atomicAdd(&A.fstats[1], judge_dec);
+            atomicAdd(&A.fstats[2], judge_mv);
         }
         if (moved == 0) {
             if (filter_on && A.filter_judge) {
@@ -1180,12 +1246,10 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         }
     }
     flush_decide_stats(ds, s_stats, a.stats);
+    if (threadIdx.x == 0 && total_moves) atomicAdd(&A.ctl[3], (unsigned int)total_moves);
     if (gtid == 0) {
-        unsigned int total = 0;
-        for (uint32_t s2 = 0; s2 < step; s2++) total += ld_acquire_ctr((REFINE ? A.ccnt : A.mcnt) + s2);
         A.ctl[0] = step;
         A.ctl[2] = it;
-        A.ctl[3] = total;
     }
 }
 
@@ -1507,6 +1571,12 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
                              std::to_string(bits_for(n)) + "/e2^" + std::to_string(bits_for(nnz));
     const TuneChoice tc = tune_enabled() ? tune_pick(tkey, (int)grids.size()) : TuneChoice{0, false};
     const uint32_t grid = grids[tc.index];
+    // each workgroup keeps its movers in its own slice of the mover array: its shares of the
+    // buckets hold at most ceil(n / grid) + NUM_BUCKETS vertices
+    const uint64_t stride = ((uint64_t)n + grid - 1) / grid + NUM_BUCKETS;
+    if (ls.movers.size() < stride * grid) ls.movers.resize(stride * grid);
+    A.d.movers = ls.movers.get();
+    A.mv_stride = (uint32_t)stride;
     constexpr uint32_t TRACE_CAP = 2048;
     DevBuf<long long> trace;
     if (sweep_trace_on()) {
@@ -1543,9 +1613,12 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         unsigned int ctl[4];
         to_host(ctl, A.ctl, 4, s);
         HIP_OK(hipStreamSynchronize(s));
-        // barriers: 2 prologue, then per step 3 (4 with a cap); print the phase times in us
-        const uint32_t per = A.claim ? 4u : 3u;
-        const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per);
+        // barriers: 2 prologue, then per step: decide, apply; a cleanup first (refinement, cap);
+        // with a cap also claims and, in local moving, activation. Phase times in us.
+        const bool cp = A.claim != nullptr;
+        const uint32_t per = 2u + (cp ? 1u : 0u) + ((!REFINE && cp) ? 1u : 0u);
+        const uint32_t cleanups = (REFINE || cp) && ctl[0] ? ctl[0] - 1u : 0u;
+        const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per + cleanups);
         std::vector<long long> h(nb + 1);
         to_host(h.data(), trace.get(), nb + 1, s);
         HIP_OK(hipStreamSynchronize(s));
