@@ -153,6 +153,10 @@ void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream
 void radix_sort_pairs_u64_f32(uint64_t *keys, float *vals, uint64_t n, int end_bit, hipStream_t s);
 void radix_sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s);
 void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s);
+// Sorts the first n pairs; the buffers may come back exchanged for the sort's second buffers
+// (of size n), so callers must re-read .get() afterwards.
+void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<float> &vals, uint64_t n, int end_bit, hipStream_t s);
+void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals, uint64_t n, int end_bit, hipStream_t s);
 // Deterministic sum of n doubles (fixed two-level tree).
 double device_sum_f64(const double *in, uint64_t n, hipStream_t s);
 double device_max_f64(const double *in, uint64_t n, hipStream_t s);

@@ -356,7 +356,7 @@ void graph_finish_unsorted(sc_graph &g, hipStream_t s) {
     hipLaunchKernelGGL(k_expand_keys, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, g.ptrs.get(),
                        g.nbrs.get(), n, keys.get());
     KERNEL_OK();
-    radix_sort_pairs_u64_f32(keys.get(), vals.get(), nnz, 32 + bits_for(n), s);
+    radix_sort_pairs(keys, vals, nnz, 32 + bits_for(n), s);
     DevBuf<uint32_t> head(nnz), run_idx(nnz);
     hipLaunchKernelGGL(k_run_heads, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), nnz, head.get());
     KERNEL_OK();

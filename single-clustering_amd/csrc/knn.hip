@@ -821,7 +821,7 @@ std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const dou
     hipLaunchKernelGGL(k_gauss_emit, dim3(grid_for((uint64_t)n * k, 256, 8192)), dim3(256), 0, s, d_idx, d_d2,
                        sigma.get(), n, k, mode, keys.get(), vals.get());
     KERNEL_OK();
-    radix_sort_pairs_u64_f32(keys.get(), vals.get(), m, 64, s);
+    radix_sort_pairs(keys, vals, m, 64, s);
     DevBuf<uint32_t> head(m), pos(m);
     hipLaunchKernelGGL(k_dedup_heads, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, keys.get(), m, head.get());
     KERNEL_OK();

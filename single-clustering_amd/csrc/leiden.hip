@@ -1017,13 +1017,18 @@ __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint
 
 __global__ void k_stamp_clock(long long *out) { *out = wall_clock64(); }
 
-__global__ void k_sweep_collect(const unsigned int *ctl, const unsigned int *err, unsigned long long *run,
-                                unsigned long long *run_err) {
+// After a sweep: its iterations, moves and barrier-timeout flag join the run's counters, and the
+// barrier words and result words are zeroed for the next sweep (no memset between sweeps).
+__global__ void k_sweep_collect(unsigned int *ctl, unsigned int *bc_words, uint32_t n_bc_words, uint32_t err_word,
+                                unsigned long long *run, unsigned long long *run_err) {
     if (threadIdx.x == 0) {
         run[0] += ctl[2];
         run[1] += ctl[3];
-        if (*err) *run_err = 1ull;
+        if (bc_words[err_word]) *run_err = 1ull;
     }
+    __syncthreads();
+    if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < n_bc_words; i += blockDim.x) bc_words[i] = 0;
 }
 
 template <bool REFINE, bool FILTER>
@@ -1444,7 +1449,7 @@ static DecideArgs base_args(const sc_graph &L, LevelState &ls, const sc_leiden_c
     return a;
 }
 
-static void alloc_level(LevelState &ls, uint32_t n) {
+static void alloc_level(LevelState &ls, uint32_t n, hipStream_t s) {
     ls.Kfx.resize(n);
     ls.Wfx.resize(n);
     ls.csize.resize(n);
@@ -1457,10 +1462,14 @@ static void alloc_level(LevelState &ls, uint32_t n) {
     ls.claim.resize(n);
     ls.counters.resize(4);
     ls.stat.resize(4);
+    HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
     ls.work.resize(n);
     ls.mstep.resize(n);
     ls.sweep.resize((size_t)MAX_MOVE_ITERS * MOVE_SUBSWEEPS * (NUM_BUCKETS + 2) + 8);
     ls.sweep_ctl.resize(sizeof(BarrierCtl));
+    // sweep counters and barrier words start at zero; each sweep leaves them zero again
+    HIP_OK(hipMemsetAsync(ls.sweep.get(), 0, ls.sweep.bytes(), s));
+    HIP_OK(hipMemsetAsync(ls.sweep_ctl.get(), 0, ls.sweep_ctl.bytes(), s));
 }
 
 // Sweep mode: 1 = one cooperative launch per phase (default), 0 = one launch per sub-sweep
@@ -1555,8 +1564,8 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     A.Kfx = ls.Kfx.get();
     A.Wfx = ls.Wfx.get();
     A.csize = ls.csize.get();
-    HIP_OK(hipMemsetAsync(ls.sweep_ctl.get(), 0, sizeof(BarrierCtl), s));
-    HIP_OK(hipMemsetAsync(A.ctl, 0, 4 * sizeof(unsigned int), s));
+    // the barrier and result words are zero here: alloc_level clears them, and each sweep's
+    // k_sweep_collect clears them for the next one
     // workgroup count: the heuristic first, then a few alternatives the online tuner measures
     // (any count gives identical labels, so exploring never changes a result)
     const uint32_t heur = filtered ? sweep_grid<false, true>(n, nnz) : sweep_grid<REFINE>(n, nnz);
@@ -1604,14 +1613,15 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         HIP_OK(hipEventRecord(ts.t1, s));
         ctx.tune_samples.push_back(ts);
     }
+    unsigned int ctl[4] = {0, 0, 0, 0};
+    if (sweep_trace_on()) to_host(ctl, A.ctl, 4, s);  // before the collect kernel clears them
     // iterations, moves and the barrier-timeout flag accumulate on the device
-    hipLaunchKernelGGL(k_sweep_collect, dim3(1), dim3(64), 0, s, (const unsigned int *)A.ctl,
-                       (const unsigned int *)&A.bc->err, ctx.counters.get() + RUN_SWEEP + (REFINE ? 2 : 0),
-                       ctx.counters.get() + RUN_ERR);
+    hipLaunchKernelGGL(k_sweep_collect, dim3(1), dim3(256), 0, s, A.ctl, reinterpret_cast<unsigned int *>(A.bc),
+                       (uint32_t)(sizeof(BarrierCtl) / sizeof(unsigned int)),
+                       (uint32_t)(offsetof(BarrierCtl, err) / sizeof(unsigned int)),
+                       ctx.counters.get() + RUN_SWEEP + (REFINE ? 2 : 0), ctx.counters.get() + RUN_ERR);
     KERNEL_OK();
     if (sweep_trace_on()) {
-        unsigned int ctl[4];
-        to_host(ctl, A.ctl, 4, s);
         HIP_OK(hipStreamSynchronize(s));
         // barriers: 2 prologue, then per step: decide, apply; a cleanup first (refinement, cap);
         // with a cap also claims and, in local moving, activation. Phase times in us.
@@ -1697,7 +1707,6 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
                        uint64_t level_seed, int sK, int sW, int sw) {
     uint32_t n = (uint32_t)L.n;
     hipStream_t s = ctx.s;
-    HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
     DecideArgs a = base_args(L, ls, *ctx.cfg, sK, sW, sw);
     a.vjob = ctx.vjob;
     a.jobs = ctx.jobs;
@@ -2165,7 +2174,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     DevBuf<uint32_t> sg(n), members(n);
     hipLaunchKernelGGL(k_pairs_for_sort, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups, n, sg.get(), members.get());
     KERNEL_OK();
-    radix_sort_pairs_u32_u32(sg.get(), members.get(), n, std::max(1, bits_for(ng)), s);
+    radix_sort_pairs(sg, members, n, std::max(1, bits_for(ng)), s);
     DevBuf<uint64_t> starts(ng + 1);
     hipLaunchKernelGGL(k_group_starts, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, sg.get(), n, ng, starts.get());
     KERNEL_OK();
@@ -2193,7 +2202,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
                            moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, hbits, sentinel, keys.get(),
                            vals.get(), nvalid.get());
         KERNEL_OK();
-        radix_sort_pairs_u64_f32(keys.get(), vals.get(), nnz, kbits, s);
+        radix_sort_pairs(keys, vals, nnz, kbits, s);
     }
     unsigned long long nv = 0;
     to_host(&nv, nvalid.get(), 1, s);
@@ -2245,7 +2254,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         hipLaunchKernelGGL(k_pair_emit, dim3(grid_for(np, 256, 4096)), dim3(256), 0, s, pa.get(), pb.get(), pw.get(),
                            keep.get(), off.get(), np, cbits, ckeys.get(), cvals.get());
         KERNEL_OK();
-        radix_sort_pairs_u64_f32(ckeys.get(), cvals.get(), total_entries, std::min(64, 2 * cbits), s);
+        radix_sort_pairs(ckeys, cvals, total_entries, std::min(64, 2 * cbits), s);
     }
     out->nnz = total_entries;
     out->nbrs.resize(total_entries);
@@ -2469,7 +2478,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
     hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, node_at.get(), n0);
     KERNEL_OK();
     LevelState ls;
-    alloc_level(ls, n0);
+    alloc_level(ls, n0, s);
     BatchState *B = ctx.batch;
     DevBuf<uint32_t> vjob_l, vjob_c;
     if (B) {

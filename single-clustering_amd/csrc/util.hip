@@ -146,6 +146,30 @@ static void sort_pairs(K *keys, V *vals, uint64_t n, int end_bit, hipStream_t s)
 void radix_sort_pairs_u64_f32(uint64_t *keys, float *vals, uint64_t n, int end_bit, hipStream_t s) {
     sort_pairs(keys, vals, n, end_bit, s);
 }
+
+// Sorts the first n pairs; the buffers may come back swapped with the sort's second buffers
+// (their sizes become n), which saves the copy back when the last pass wrote the other buffer.
+template <typename K, typename V>
+static void sort_pairs_bufs(DevBuf<K> &keys, DevBuf<V> &vals, uint64_t n, int end_bit, hipStream_t s) {
+    if (n <= 1) return;
+    DevBuf<K> k2(n);
+    DevBuf<V> v2(n);
+    hipcub::DoubleBuffer<K> kb(keys.get(), k2.get());
+    hipcub::DoubleBuffer<V> vb(vals.get(), v2.get());
+    size_t tmp = 0;
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)n, 0, end_bit, s));
+    DevBuf<unsigned char> t(tmp);
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, kb, vb, (int)n, 0, end_bit, s));
+    if (kb.Current() != keys.get()) std::swap(keys, k2);
+    if (vb.Current() != vals.get()) std::swap(vals, v2);
+}
+
+void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<float> &vals, uint64_t n, int end_bit, hipStream_t s) {
+    sort_pairs_bufs(keys, vals, n, end_bit, s);
+}
+void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals, uint64_t n, int end_bit, hipStream_t s) {
+    sort_pairs_bufs(keys, vals, n, end_bit, s);
+}
 void radix_sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s) {
     sort_pairs(keys, vals, n, end_bit, s);
 }
