@@ -92,6 +92,11 @@ class DevBuf {
 // Stream owned by the calling thread (default priority), created lazily.
 hipStream_t thread_stream();
 
+// Waits until the stream's work is done. It polls first (a blocking wait can put the thread
+// to sleep, and waking it costs tens of microseconds per small readback) and blocks only when
+// the work takes longer than a couple of milliseconds.
+void sync_stream(hipStream_t s);
+
 // Profiling switch: when on, the Leiden driver records events around its decide kernels.
 bool profiling_enabled();
 void set_profiling(int on);
@@ -216,9 +221,17 @@ template <typename T>
 void to_device(T *dst, const T *src, size_t n, hipStream_t s) {
     if (n) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
+// Small device-to-host readbacks go through a per-thread pinned staging buffer (a copy into
+// pageable memory waits inside the copy call); the values reach dst when sync_stream returns.
+bool stage_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+void stage_drop();
+
+// dst is valid after the next sync_stream(s).
 template <typename T>
 void to_host(T *dst, const T *src, size_t n, hipStream_t s) {
-    if (n) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    if (!n) return;
+    if (stage_d2h(dst, src, n * sizeof(T), s)) return;
+    HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
 }
 
 }  // namespace scgpu

@@ -49,8 +49,8 @@ static uint64_t first_bad(void (*launch)(unsigned long long *, hipStream_t), hip
     unsigned long long init = NONE64, out = 0;
     HIP_OK(hipMemcpyAsync(d.get(), &init, sizeof(init), hipMemcpyHostToDevice, s));
     launch(d.get(), s);
-    HIP_OK(hipMemcpyAsync(&out, d.get(), sizeof(out), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    to_host(&out, d.get(), 1, s);
+    sync_stream(s);
     return out;
 }
 
@@ -67,7 +67,7 @@ static void validate_csr_device(const uint64_t *d_ptrs, uint64_t n, const uint32
     }
     uint64_t last = 0;
     to_host(&last, d_ptrs + n, 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     if (last != nnz)
         throw ScError(SC_INVALID_CSR, "malformed CSR input: node_ptrs ends at " + std::to_string(last) +
                                           " but there are " + std::to_string(nnz) + " entries");
@@ -79,8 +79,8 @@ static void validate_csr_device(const uint64_t *d_ptrs, uint64_t n, const uint32
         HIP_OK(hipMemcpyAsync(d.get(), &init, sizeof(init), hipMemcpyHostToDevice, s));
         if (n) hipLaunchKernelGGL(k_check_ptrs, dim3(grid_for(n, block, 4096)), dim3(block), 0, s, d_ptrs, n, d.get());
         KERNEL_OK();
-        HIP_OK(hipMemcpyAsync(&bad, d.get(), sizeof(bad), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        to_host(reinterpret_cast<unsigned long long *>(&bad), d.get(), 1, s);
+        sync_stream(s);
     }
     if (bad != NONE64)
         throw ScError(SC_INVALID_CSR, "malformed CSR input: node_ptrs is not non-decreasing at " +
@@ -91,15 +91,15 @@ static void validate_csr_device(const uint64_t *d_ptrs, uint64_t n, const uint32
         HIP_OK(hipMemcpyAsync(d.get(), &init, sizeof(init), hipMemcpyHostToDevice, s));
         if (nnz) hipLaunchKernelGGL(k_check_entries, dim3(grid_for(nnz, block, 8192)), dim3(block), 0, s, d_nbrs, d_wts, nnz, n, d.get());
         KERNEL_OK();
-        HIP_OK(hipMemcpyAsync(&bad, d.get(), sizeof(bad), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        to_host(reinterpret_cast<unsigned long long *>(&bad), d.get(), 1, s);
+        sync_stream(s);
     }
     if (bad != NONE64) {
         uint32_t u = 0;
         float w = 0.0f;
         to_host(&u, d_nbrs + bad, 1, s);
         to_host(&w, d_wts + bad, 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         // the reference reports edge (0, u) for these, not the real row (csr_network.rs:228-238)
         if ((uint64_t)u >= n) {
             ScError e(SC_NODE_INDEX_OUT_OF_RANGE, "node index " + std::to_string(u) +
@@ -129,12 +129,12 @@ static void validate_csr_device(const uint64_t *d_ptrs, uint64_t n, const uint32
         HIP_OK(hipMemcpyAsync(d.get(), &init, sizeof(init), hipMemcpyHostToDevice, s));
         if (n) hipLaunchKernelGGL(k_check_node_w, dim3(grid_for(n, block, 4096)), dim3(block), 0, s, d_nw, n, d.get());
         KERNEL_OK();
-        HIP_OK(hipMemcpyAsync(&bad, d.get(), sizeof(bad), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        to_host(reinterpret_cast<unsigned long long *>(&bad), d.get(), 1, s);
+        sync_stream(s);
         if (bad != NONE64) {
             double w = 0.0;
             to_host(&w, d_nw + bad, 1, s);
-            HIP_OK(hipStreamSynchronize(s));
+            sync_stream(s);
             char buf[160];
             std::snprintf(buf, sizeof(buf), "node %llu has invalid weight %g; must be finite and non-negative",
                           (unsigned long long)bad, w);
@@ -344,7 +344,7 @@ void graph_finish_unsorted(sc_graph &g, hipStream_t s) {
     }
     unsigned int fl[2];
     to_host(fl, flags.get(), 2, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     if (fl[0] == 0) {
         graph_finish_sorted(g, s);
         return;
@@ -364,7 +364,7 @@ void graph_finish_unsorted(sc_graph &g, hipStream_t s) {
     uint32_t last_idx = 0, last_head = 0;
     to_host(&last_idx, run_idx.get() + nnz - 1, 1, s);
     to_host(&last_head, head.get() + nnz - 1, 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     uint64_t runs = (uint64_t)last_idx + last_head;
     DevBuf<uint32_t> keep(runs), out_idx(runs);
     DevBuf<double> run_w(runs);
@@ -376,7 +376,7 @@ void graph_finish_unsorted(sc_graph &g, hipStream_t s) {
     uint32_t lk = 0, lo = 0;
     to_host(&lk, keep.get() + runs - 1, 1, s);
     to_host(&lo, out_idx.get() + runs - 1, 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     uint64_t out_nnz = (uint64_t)lo + lk;
     DevBuf<uint32_t> nbrs(out_nnz);
     DevBuf<float> wts(out_nnz);
@@ -472,7 +472,7 @@ static sc_status build_from_csr(const uint64_t *ptrs, uint64_t n, const uint32_t
         validate_csr_device(g->ptrs.get(), n, g->nbrs.get(), g->wts.get(), nnz,
                             node_w ? g->node_w.get() : nullptr, s);
         graph_finish_unsorted(*g, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         *out = g;
         return SC_OK;
     } catch (const ScError &e) {
@@ -573,7 +573,7 @@ sc_status sc_graph_from_edges(uint64_t n, const uint64_t *from, const uint64_t *
             hipStream_t s = thread_stream();
             upload_graph(*g, ptrs.data(), n, nbrs.data(), wts.data(), slots, node_weights, false, s);
             graph_finish_unsorted(*g, s);
-            HIP_OK(hipStreamSynchronize(s));
+            sync_stream(s);
         } catch (...) {
             delete g;
             throw;
@@ -609,7 +609,7 @@ sc_status sc_graph_to_csr_parts(const sc_graph *g, uint64_t *node_ptrs, uint32_t
         if (weights) to_host(weights, g->wts.get(), g->nnz, s);
         if (node_weights) to_host(node_weights, g->node_w.get(), g->n, s);
         if (strengths) to_host(strengths, g->strength.get(), g->n, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return SC_OK;
     } catch (const ScError &e) {
         fill_error(err, e);
@@ -628,8 +628,8 @@ sc_status sc_graph_validate_symmetry(const sc_graph *g, sc_error *err) {
             hipLaunchKernelGGL(k_symmetry, dim3(grid_for(g->n, 256, 8192)), dim3(256), 0, s, g->ptrs.get(),
                                g->nbrs.get(), g->wts.get(), g->n, d.get());
         KERNEL_OK();
-        HIP_OK(hipMemcpyAsync(&bad, d.get(), sizeof(bad), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        to_host(reinterpret_cast<unsigned long long *>(&bad), d.get(), 1, s);
+        sync_stream(s);
         if (bad == NONE64) return SC_OK;
         // locate the row of entry `bad` on the host (rare path)
         std::vector<uint64_t> ptrs(g->n + 1);
@@ -638,7 +638,7 @@ sc_status sc_graph_validate_symmetry(const sc_graph *g, sc_error *err) {
         float w = 0.0f;
         to_host(&u, g->nbrs.get() + bad, 1, s);
         to_host(&w, g->wts.get() + bad, 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         uint64_t v = (uint64_t)(std::upper_bound(ptrs.begin(), ptrs.end(), bad) - ptrs.begin()) - 1;
         ScError e(SC_INVALID_CSR, "malformed CSR input: edge (" + std::to_string(v) + ", " + std::to_string(u) +
                                       ") has no matching reverse edge of weight " + std::to_string(w));

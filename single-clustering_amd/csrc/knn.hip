@@ -664,7 +664,7 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     HIP_OK(hipEventRecord(e2, s));
     unsigned int ymax_bits = 0;
     to_host(&ymax_bits, ymax.get(), 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     float ymaxf;
     std::memcpy(&ymaxf, &ymax_bits, 4);
     double Y = (double)ymaxf;
@@ -687,13 +687,13 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     unsigned long long nr = 0;
     to_host(&nf, nfb.get(), 1, s);
     to_host(&nr, nres.get(), 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     if (nf) {
         hipLaunchKernelGGL(k_knn_fallback, dim3(nf), dim3(FB_THREADS), 0, s, d_X, n, d, k, fb.get(), d_idx, d_d2);
         KERNEL_OK();
     }
     HIP_OK(hipEventRecord(e3, s));
-    HIP_OK(hipEventSynchronize(e3));
+    sync_stream(s);
     float t_tile = 0.0f, t_all = 0.0f;
     HIP_OK(hipEventElapsedTime(&t_tile, e1, e2));
     HIP_OK(hipEventElapsedTime(&t_all, e0, e3));
@@ -829,7 +829,7 @@ std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const dou
     uint32_t lp = 0, lh = 0;
     to_host(&lp, pos.get() + m - 1, 1, s);
     to_host(&lh, head.get() + m - 1, 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     uint64_t nnz = (uint64_t)lp + lh;
     auto g = std::make_unique<sc_graph>();
     g->n = n;
@@ -847,7 +847,7 @@ std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const dou
     std::vector<double> ones(n, 1.0);
     to_device(g->node_w.get(), ones.data(), n, s);
     graph_finish_sorted(*g, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     return g;
 }
 
@@ -873,7 +873,7 @@ sc_status sc_knn(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc
         knn_device(X.get(), (uint32_t)n, d, k, knn_precision(opts), idx.get(), d2.get(), stats, s);
         if (idx_out) to_host(idx_out, idx.get(), (size_t)n * k, s);
         if (d2_out) to_host(d2_out, d2.get(), (size_t)n * k, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return SC_OK;
     } catch (const ScError &e) {
         fill_error(err, e);
@@ -945,7 +945,7 @@ sc_status sc_knn_gaussian_csr(const uint32_t *idx, const double *d2, uint64_t n,
         to_host(node_ptrs, g->ptrs.get(), n + 1, s);
         to_host(neighbors, g->nbrs.get(), g->nnz, s);
         to_host(weights, g->wts.get(), g->nnz, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         *nnz_out = g->nnz;
         return SC_OK;
     } catch (const ScError &e) {

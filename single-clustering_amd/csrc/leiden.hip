@@ -723,7 +723,7 @@ static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
     KERNEL_OK();
     uint32_t h[NUM_BUCKETS];
     to_host(h, cnt.get(), NUM_BUCKETS, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     b.start[0] = 0;
     for (int i = 0; i < NUM_BUCKETS; i++) b.start[i + 1] = b.start[i] + h[i];
     DevBuf<uint32_t> starts(NUM_BUCKETS + 1);
@@ -742,7 +742,7 @@ static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
         exclusive_scan_u64(sizes.get(), b.goff.get(), nb6 + 1, s);
         uint64_t total = 0;
         to_host(&total, b.goff.get() + nb6, 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         b.gkeys.resize(total);
         b.gvals.resize(total);
     }
@@ -1622,7 +1622,7 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
                        ctx.counters.get() + RUN_SWEEP + (REFINE ? 2 : 0), ctx.counters.get() + RUN_ERR);
     KERNEL_OK();
     if (sweep_trace_on()) {
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         // barriers: 2 prologue, then per step: decide, apply; a cleanup first (refinement, cap);
         // with a cap also claims and, in local moving, activation. Phase times in us.
         const bool cp = A.claim != nullptr;
@@ -1631,7 +1631,7 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per + cleanups);
         std::vector<long long> h(nb + 1);
         to_host(h.data(), trace.get(), nb + 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         std::string line = std::string(REFINE ? "[sweep refine]" : "[sweep move]") + " n=" + std::to_string(n) +
                            " grid=" + std::to_string(grid) + " steps=" + std::to_string(ctl[0]) + " us:";
         for (uint32_t i = 1; i <= nb; i++) {
@@ -1672,7 +1672,7 @@ static void harvest_stats(LevelState &ls, RunCtx &ctx, bool move_phase) {
 static void finish_run_stats(RunCtx &ctx) {
     unsigned long long h[RUN_COUNTERS];
     to_host(h, ctx.counters.get(), RUN_COUNTERS, ctx.s);
-    HIP_OK(hipStreamSynchronize(ctx.s));
+    sync_stream(ctx.s);
     for (int ph = 0; ph < 2; ph++) {
         const unsigned long long *c = h + RUN_STAT + 3 * ph;
         ctx.st.vertex_visits += c[0];
@@ -1807,7 +1807,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
             iteration(it);
         unsigned int mv[MOVE_SUBSWEEPS];
         to_host(mv, iter_moves.get(), MOVE_SUBSWEEPS, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         if (ctx.timer.on) ctx.timer.harvest();
         uint64_t moves = 0;
         for (int i = 0; i < MOVE_SUBSWEEPS; i++) moves += mv[i];
@@ -1910,7 +1910,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
             round_launches(r);
         unsigned int c[2];
         to_host(c, ls.counters.get(), 2, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         ctx.st.refine_rounds++;
         if (c[1] == 0) break;
     }
@@ -1940,7 +1940,7 @@ static uint32_t renumber(const uint32_t *ids, uint32_t n, uint32_t id_space, uin
     KERNEL_OK();
     uint32_t count = 0;
     to_host(&count, newid.get() + id_space, 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     return count;
 }
 
@@ -2206,7 +2206,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     }
     unsigned long long nv = 0;
     to_host(&nv, nvalid.get(), 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     uint64_t npairs_max = nv;
     DevBuf<uint32_t> head(nv + 1), run(nv + 1);
     uint32_t np = 0;
@@ -2217,7 +2217,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         uint32_t lr = 0, lh = 0;
         to_host(&lr, run.get() + nv - 1, 1, s);
         to_host(&lh, head.get() + nv - 1, 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         np = lr + lh;
     }
     (void)npairs_max;
@@ -2244,7 +2244,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         uint32_t lo = 0, lk = 0;
         to_host(&lo, off.get() + np - 1, 1, s);
         to_host(&lk, keep.get() + np - 1, 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         total_entries = (uint64_t)lo + lk;
     }
     const int cbits = std::max(1, bits_for(ng));
@@ -2364,7 +2364,7 @@ static void batch_rebase(const BatchState &B, uint32_t *ids, const uint32_t *vjo
     KERNEL_OK();
     std::vector<uint32_t> h(2 * (size_t)J);
     to_host(h.data(), mm.get(), 2 * (size_t)J, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     std::vector<long long> sh(J, 0);
     for (uint32_t j = 0; j < J; j++) sh[j] = B.cnt[j] ? (long long)r[j] - (long long)h[j] : 0;
     DevBuf<long long> d_sh(J);
@@ -2372,7 +2372,7 @@ static void batch_rebase(const BatchState &B, uint32_t *ids, const uint32_t *vjo
     hipLaunchKernelGGL(k_job_rebase, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, vjob, n,
                        (const long long *)d_sh.get());
     KERNEL_OK();
-    HIP_OK(hipStreamSynchronize(s));  // sh is a host temporary
+    sync_stream(s);  // sh is a host temporary
 }
 
 static std::vector<uint64_t> batch_ranges(const BatchState &B) {
@@ -2399,7 +2399,7 @@ static void batch_level_params(BatchState &B, const sc_graph &L, const sc_leiden
     std::vector<unsigned long long> hec(J);
     to_host(h.data(), red.get(), 3 * (size_t)J, s);
     to_host(hec.data(), ec.get(), J, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     std::vector<JobParams> P(J);
     for (uint32_t j = 0; j < J; j++) {
         const double tw = h[j], tnw = h[J + j], ms = B.cnt[j] ? h[2 * J + j] : 0.0;
@@ -2418,7 +2418,7 @@ static void batch_level_params(BatchState &B, const sc_graph &L, const sc_leiden
     }
     B.params.resize(J);
     to_device(B.params.get(), P.data(), J, s);
-    HIP_OK(hipStreamSynchronize(s));  // P is a host temporary
+    sync_stream(s);  // P is a host temporary
 }
 
 // Groups per job after refinement (ids are ascending and contiguous per job).
@@ -2432,7 +2432,7 @@ static std::vector<uint32_t> batch_group_counts(const BatchState &B, const uint3
     KERNEL_OK();
     std::vector<uint32_t> h(2 * (size_t)J), out(J, 0);
     to_host(h.data(), mm.get(), 2 * (size_t)J, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     for (uint32_t j = 0; j < J; j++) out[j] = B.cnt[j] ? h[J + j] - h[j] + 1 : 0;
     return out;
 }
@@ -2558,7 +2558,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
     HIP_OK(hipMemcpyAsync(labels, node_at.get(), n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, rmemb.get(), labels, n0);
     KERNEL_OK();
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     return nc;
 }
 
@@ -2684,7 +2684,7 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
         }
     }
     HIP_OK(hipEventRecord(t1, s));
-    HIP_OK(hipEventSynchronize(t1));
+    sync_stream(s);
     float ms = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
@@ -2741,7 +2741,7 @@ static void labels_to_host(const uint32_t *d, uint64_t n, uint64_t *out, hipStre
     hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d, l64.get(), n);
     KERNEL_OK();
     to_host(out, l64.get(), n, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
 }
 
 sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64_t cap, float *x_out,
@@ -2768,7 +2768,7 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
         leiden_device(*g, *cfg, labels.get(), &nc, s, &st, nullptr, &fo);
         unsigned long long total = 0;
         to_host(&total, tn.get(), 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         const uint64_t m = std::min<uint64_t>(total, cap);
         *n_out = m;
         // samples arrive in atomic-slot order; sorting by (level, iteration, vertex) and then by
@@ -2780,7 +2780,7 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
         to_host(hx.data(), tx.get(), m * MLP_NF, s);
         to_host(hy.data(), ty.get(), m, s);
         to_host(hk.data(), tk.get(), m, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         std::vector<uint64_t> order(m);
         for (uint64_t i = 0; i < m; i++) order[i] = i;
         std::sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) {
@@ -2828,7 +2828,7 @@ sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, con
         if (n) labels_to_host(labels.get(), n, labels_out, s);
         unsigned long long h[5] = {0, 0, 0, 0, 0};
         to_host(h, fs.get(), 5, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         if (fst) {
             fst->skipped = h[0];
             fst->judged = h[1];
@@ -2920,7 +2920,7 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
                            U.wts.get());
         KERNEL_OK();
         to_device(U.ptrs.get() + N, &E, 1, s);
-        HIP_OK(hipStreamSynchronize(s));  // host tables above are temporaries
+        sync_stream(s);  // host tables above are temporaries
         sc_leiden_config c = *cfg;
         c.resolution = resolutions[0];
         DevBuf<uint32_t> labels(N);
@@ -2930,7 +2930,7 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
         // per job: labels are consecutive over the union and contiguous per job
         std::vector<uint32_t> h(N);
         to_host(h.data(), labels.get(), N, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         for (uint32_t j = 0; j < J; j++) {
             const uint64_t lo = B.voff0[j], hi = B.voff0[j + 1];
             uint32_t mn = 0xFFFFFFFFu, mx = 0;
@@ -2983,7 +2983,7 @@ sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *la
             KERNEL_OK();
             to_host(labels_out, l64.get(), n, s);
         }
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         t_stats = st;
         return SC_OK;
     } catch (const ScError &e) {
@@ -3027,7 +3027,7 @@ sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph
             c->ptrs.resize(1);
             HIP_OK(hipMemsetAsync(c->ptrs.get(), 0, sizeof(uint64_t), s));
         }
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         *out = c.release();
         return SC_OK;
     } catch (const ScError &e) {

@@ -167,7 +167,7 @@ static double batch_grad(const float *d_w, const float *d_X, const uint8_t *d_y,
     KERNEL_OK();
     std::vector<float> h(B);
     to_host(h.data(), L.get(), B, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     double tot = 0.0;
     for (uint32_t b = 0; b < B; b++) tot += h[b];
     return tot / (double)B;
@@ -205,7 +205,7 @@ sc_status sc_mlp_grad(const float *w, const float *X, const uint8_t *y, uint64_t
         to_device(dy.get(), y, n, s);
         *loss_out = batch_grad(dw.get(), dX.get(), dy.get(), nullptr, (uint32_t)n, pos_weight, dg.get(), s);
         to_host(grad_out, dg.get(), MLP_PARAMS, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return SC_OK;
     } catch (const ScError &e) {
         fill_error(err, e);
@@ -250,7 +250,7 @@ sc_status sc_mlp_train(const float *X, const uint8_t *y, uint64_t n, uint32_t ep
             if (loss_hist) loss_hist[e] = lsum / (double)nb;
         }
         to_host(w_inout, dw.get(), MLP_PARAMS, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return SC_OK;
     } catch (const ScError &e) {
         fill_error(err, e);
@@ -274,14 +274,14 @@ sc_status sc_mlp_predict(const float *w, const float *X, uint64_t n, int32_t bf1
                                (const MlpPacked *)pk.get(), (const float *)dX.get(), n, dout.get());
             KERNEL_OK();
             to_host(out, dout.get(), n, s);
-            HIP_OK(hipStreamSynchronize(s));
+            sync_stream(s);
             return SC_OK;
         }
         hipLaunchKernelGGL(k_mlp_predict, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const float *)dw.get(),
                            (const float *)dX.get(), n, dout.get());
         KERNEL_OK();
         to_host(out, dout.get(), n, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return SC_OK;
     } catch (const ScError &e) {
         fill_error(err, e);

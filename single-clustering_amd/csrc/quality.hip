@@ -135,7 +135,7 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
     KERNEL_OK();
     unsigned long long max_label = 0;
     to_host(&max_label, mx.get(), 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     if (max_label >= 0xFFFFFFFFull) throw ScError(SC_INVALID_ARGUMENT, "labels must be below 2^32 - 1");
     const uint64_t nl = max_label + 1;
     const int s_in = fixed_scale_for(g.total_weight + 1e-300);
@@ -199,7 +199,7 @@ void quality_jobs_device(const sc_graph &g, const uint64_t *d_labels, int mode, 
     std::vector<uint64_t> hlr(J + 1);
     to_host(&max_label, mx.get(), 1, s);
     to_host(hlr.data(), lr.get(), J, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     const uint64_t nl = max_label + 1;
     // a job without vertices has no smallest label (the kernel leaves ~0): its label range is
     // the empty range at the next job's start, so the ranges stay ascending
@@ -236,7 +236,7 @@ void quality_jobs_device(const sc_graph &g, const uint64_t *d_labels, int mode, 
     KERNEL_OK();
     device_segment_reduce_f64(term.get(), lr.get(), J, false, d_q.get(), s);
     to_host(h_q, d_q.get(), J, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
 }
 
 // ---- connected components restricted to same-label edges --------------------------------
@@ -319,7 +319,7 @@ uint64_t disconnected_count_device(const sc_graph &g, const uint32_t *d_labels32
         KERNEL_OK();
         unsigned int ch = 0;
         to_host(&ch, changed.get(), 1, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         if (!ch) break;
     }
     DevBuf<uint32_t> ncomp(n_labels);
@@ -334,7 +334,7 @@ uint64_t disconnected_count_device(const sc_graph &g, const uint32_t *d_labels32
     KERNEL_OK();
     unsigned long long out = 0;
     to_host(&out, bad.get(), 1, s);
-    HIP_OK(hipStreamSynchronize(s));
+    sync_stream(s);
     return out;
 }
 
@@ -351,9 +351,10 @@ double sc_modularity(const sc_graph *g, const uint64_t *labels, double resolutio
         DevBuf<uint64_t> d(g->n);
         to_device(d.get(), labels, g->n, s);
         double q = quality_device(*g, d.get(), 0, resolution, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return q;
     } catch (const ScError &) {
+        stage_drop();
         return NAN;
     }
 }
@@ -367,7 +368,7 @@ sc_status sc_quality(const sc_graph *g, const uint64_t *labels, int32_t objectiv
         DevBuf<uint64_t> d(g->n);
         to_device(d.get(), labels, g->n, s);
         *out = quality_device(*g, d.get(), objective == SC_RB ? 1 : 2, resolution, s);
-        HIP_OK(hipStreamSynchronize(s));
+        sync_stream(s);
         return SC_OK;
     } catch (const ScError &e) {
         fill_error(err, e);

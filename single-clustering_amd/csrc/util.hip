@@ -2,6 +2,9 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <vector>
+#include <tuple>
+#include <chrono>
 #include <cstring>
 
 #include <hipcub/hipcub.hpp>
@@ -20,6 +23,7 @@ void throw_hip(hipError_t e, const char *what, const char *file, int line) {
 }
 
 void fill_error(sc_error *err, const ScError &e) {
+    stage_drop();  // the failed call's pending readbacks point into frames that are gone
     if (!err) return;
     *err = e.payload;
     err->code = (int32_t)e.code;
@@ -85,6 +89,70 @@ void DevicePool::trim() {
     for (auto &kv : free_) (void)hipFree(kv.second);
     free_.clear();
     cached_ = 0;
+}
+
+namespace {
+struct HostStage {
+    char *buf = nullptr;
+    bool failed = false;
+    size_t used = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::tuple<void *, size_t, size_t>> pending;  // (dst, offset, bytes)
+};
+thread_local HostStage t_stage;
+constexpr size_t STAGE_BYTES = 1 << 16;
+constexpr size_t STAGE_MAX_COPY = 4096;
+
+void stage_flush(hipStream_t s) {
+    HostStage &st = t_stage;
+    if (st.pending.empty() || st.stream != s) return;
+    for (auto &p : st.pending) std::memcpy(std::get<0>(p), st.buf + std::get<1>(p), std::get<2>(p));
+    st.pending.clear();
+    st.used = 0;
+    st.stream = nullptr;
+}
+}  // namespace
+
+bool stage_d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    HostStage &st = t_stage;
+    if (bytes > STAGE_MAX_COPY || st.failed) return false;
+    if (!st.buf) {
+        if (hipHostMalloc((void **)&st.buf, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            st.buf = nullptr;
+            st.failed = true;
+            return false;
+        }
+    }
+    if (!st.pending.empty() && st.stream != s) return false;
+    const size_t off = (st.used + 15) & ~size_t(15);
+    if (off + bytes > STAGE_BYTES) return false;
+    HIP_OK(hipMemcpyAsync(st.buf + off, src, bytes, hipMemcpyDeviceToHost, s));
+    st.pending.emplace_back(dst, off, bytes);
+    st.used = off + bytes;
+    st.stream = s;
+    return true;
+}
+
+// Readbacks of a call that failed before its sync are dropped (their destinations are gone).
+void stage_drop() {
+    t_stage.pending.clear();
+    t_stage.used = 0;
+    t_stage.stream = nullptr;
+}
+
+void sync_stream(hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) HIP_OK(e);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000)) {
+            HIP_OK(hipStreamSynchronize(s));
+            break;
+        }
+    }
+    stage_flush(s);
 }
 
 hipStream_t thread_stream() {
@@ -206,7 +274,10 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_stage(const double *in, 
 
 template <bool MAX>
 static double reduce_f64(const double *in, uint64_t n, hipStream_t s) {
-    if (n == 0) return MAX ? -INFINITY : 0.0;
+    if (n == 0) {
+        sync_stream(s);  // like the general path: the stream is synchronized on return
+        return MAX ? -INFINITY : 0.0;
+    }
     DevBuf<double> part(RED_BLOCKS + 1);
     hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, in, n,
                        part.get());
@@ -215,8 +286,8 @@ static double reduce_f64(const double *in, uint64_t n, hipStream_t s) {
                        (uint64_t)RED_BLOCKS, part.get() + RED_BLOCKS);
     KERNEL_OK();
     double r = 0.0;
-    HIP_OK(hipMemcpyAsync(&r, part.get() + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    to_host(&r, part.get() + RED_BLOCKS, 1, s);
+    sync_stream(s);
     return r;
 }
 
@@ -256,6 +327,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce4_stage(Reduce4Args a, ui
 void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_t n, double out[4], hipStream_t s) {
     if (n == 0) {
         for (int i = 0; i < 4; i++) out[i] = is_max[i] ? -INFINITY : 0.0;
+        sync_stream(s);  // callers rely on this call having synchronized (and flushed readbacks)
         return;
     }
     Reduce4Args a;
@@ -268,8 +340,8 @@ void device_reduce4_f64(const double *const in[4], const bool is_max[4], uint64_
     KERNEL_OK();
     hipLaunchKernelGGL(k_reduce4_stage, dim3(1, 4), dim3(RED_THREADS), 0, s, a, n, part.get(), 1);
     KERNEL_OK();
-    HIP_OK(hipMemcpyAsync(out, part.get() + (size_t)4 * RED_BLOCKS, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    to_host(out, part.get() + (size_t)4 * RED_BLOCKS, 4, s);
+    sync_stream(s);
 }
 
 // Segment j is in[ranges[j] .. ranges[j+1]); blockIdx.y = segment. Stage 1 writes
