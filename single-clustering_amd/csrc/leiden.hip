@@ -45,7 +45,7 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
-constexpr int NUM_BUCKETS = 7;
+constexpr int NUM_BUCKETS = 8;
 constexpr int MOVE_SUBSWEEPS = 2;
 constexpr int MAX_MOVE_ITERS = 100;
 constexpr int MAX_REFINE_ROUNDS = 64;
@@ -653,14 +653,17 @@ struct IterGraph {
 };
 
 // ---- degree buckets ------------------------------------------------------------------------
+constexpr int GTAB_BUCKET = 7;  // rows too long for an LDS table: global-memory tables
+
 __device__ __forceinline__ int bucket_of(uint64_t d) {
     if (d <= 8) return 0;
     if (d <= 16) return 1;
     if (d <= 32) return 2;
     if (d <= 128) return 3;
-    if (d <= 1024) return 4;
-    if (d <= 2048) return 5;
-    return 6;
+    if (d <= 512) return 4;
+    if (d <= 1024) return 5;
+    if (d <= 2048) return 6;
+    return GTAB_BUCKET;
 }
 
 __global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, uint32_t *counts) {
@@ -732,12 +735,12 @@ static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
     hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, starts.get(),
                        cnt.get() + NUM_BUCKETS, b.lists.get());
     KERNEL_OK();
-    uint32_t nb6 = h[6];
+    uint32_t nb6 = h[GTAB_BUCKET];
     if (nb6) {
         b.goff.resize(nb6 + 1);
         DevBuf<uint64_t> sizes(nb6 + 1);
         hipLaunchKernelGGL(k_gtab_sizes, dim3(grid_for(nb6 + 1, 256, 1024)), dim3(256), 0, s,
-                           b.lists.get() + b.start[6], nb6, L.ptrs.get(), sizes.get());
+                           b.lists.get() + b.start[GTAB_BUCKET], nb6, L.ptrs.get(), sizes.get());
         KERNEL_OK();
         exclusive_scan_u64(sizes.get(), b.goff.get(), nb6 + 1, s);
         uint64_t total = 0;
@@ -967,9 +970,10 @@ template <> struct BucketShape<0> { static constexpr int G = 4, T = 16; };
 template <> struct BucketShape<1> { static constexpr int G = 4, T = 32; };
 template <> struct BucketShape<2> { static constexpr int G = 8, T = 64; };
 template <> struct BucketShape<3> { static constexpr int G = 16, T = 256; };
-template <> struct BucketShape<4> { static constexpr int G = 256, T = 2048; };
-template <> struct BucketShape<5> { static constexpr int G = 256, T = 4096; };
-template <> struct BucketShape<6> { static constexpr int G = 256, T = 1; };
+template <> struct BucketShape<4> { static constexpr int G = 64, T = 1024; };  // a wave per vertex
+template <> struct BucketShape<5> { static constexpr int G = 256, T = 2048; };
+template <> struct BucketShape<6> { static constexpr int G = 256, T = 4096; };
+template <> struct BucketShape<7> { static constexpr int G = 256, T = 1; };
 
 template <bool REFINE>
 __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParams &sp, const unsigned int *s_wc,
@@ -982,7 +986,7 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
 #define SC_DECIDE_BUCKET(BK)                                                                          \
     {                                                                                                 \
         constexpr int G = BucketShape<BK>::G, T = BucketShape<BK>::T, NG = SWEEP_BS / G;               \
-        static_assert(BK == 6 || NG * T <= SWEEP_TAB, "bucket table does not fit");                   \
+        static_assert(BK == GTAB_BUCKET || NG * T <= SWEEP_TAB, "bucket table does not fit");         \
         const uint32_t len = s_wc[BK];                                                                \
         if (len) {                                                                                    \
             if (!first) __syncthreads();                                                              \
@@ -990,7 +994,8 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
             uint32_t lo, hi;                                                                          \
             bucket_share(A, BK, lo, hi);                                                              \
             for (uint64_t base = 0; base < len; base += NG)                                           \
-                decide_unit<SWEEP_BS, G, T, REFINE, BK == 6>(A.d, sp, A.lists + A.bstart[BK], A.work + lo, \
+                decide_unit<SWEEP_BS, G, T, REFINE, BK == GTAB_BUCKET>(A.d, sp, A.lists + A.bstart[BK],  \
+                                                                       A.work + lo,                   \
                                                              len, base, s_keys, s_vals, r_s, r_c, r_w, ds); \
         }                                                                                             \
     }
@@ -1001,6 +1006,7 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
     SC_DECIDE_BUCKET(4)
     SC_DECIDE_BUCKET(5)
     SC_DECIDE_BUCKET(6)
+    SC_DECIDE_BUCKET(7)
 #undef SC_DECIDE_BUCKET
 }
 
@@ -1403,8 +1409,9 @@ static void decide_all(DecideArgs a, const Buckets &b, hipStream_t s, KernelTime
             case 1: launch_decide<256, 16, 32, REFINE, false>(a, s); break;
             case 2: launch_decide<256, 32, 64, REFINE, false>(a, s); break;
             case 3: launch_decide<256, 64, 256, REFINE, false>(a, s); break;
-            case 4: launch_decide<64, 64, 2048, REFINE, false>(a, s); break;
-            case 5: launch_decide<256, 256, 4096, REFINE, false>(a, s); break;
+            case 4: launch_decide<256, 64, 1024, REFINE, false>(a, s); break;
+            case 5: launch_decide<64, 64, 2048, REFINE, false>(a, s); break;
+            case 6: launch_decide<256, 256, 4096, REFINE, false>(a, s); break;
             default:
                 a.gkeys = b.gkeys.get();
                 a.gvals = b.gvals.get();
@@ -1633,7 +1640,10 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         to_host(h.data(), trace.get(), nb + 1, s);
         sync_stream(s);
         std::string line = std::string(REFINE ? "[sweep refine]" : "[sweep move]") + " n=" + std::to_string(n) +
-                           " grid=" + std::to_string(grid) + " steps=" + std::to_string(ctl[0]) + " us:";
+                           " grid=" + std::to_string(grid) + " steps=" + std::to_string(ctl[0]) + " buckets=";
+        for (int k = 0; k < NUM_BUCKETS; k++)
+            line += std::to_string(b.start[k + 1] - b.start[k]) + (k + 1 < NUM_BUCKETS ? "/" : "");
+        line += " us:";
         for (uint32_t i = 1; i <= nb; i++) {
             char buf[32];
             std::snprintf(buf, sizeof(buf), " %.1f", (double)(h[i] - h[i - 1]) / 100.0);
@@ -1732,7 +1742,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     }
     a.memb = memb;
     a.P = MOVE_SUBSWEEPS;
-    if (b.start[6] < b.start[7]) {
+    if (b.start[GTAB_BUCKET] < b.start[GTAB_BUCKET + 1]) {
         a.gkeys = b.gkeys.get();
         a.gvals = b.gvals.get();
         a.goff = b.goff.get();
@@ -1833,7 +1843,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     double mean_w = L.edge_count > 0 ? L.total_weight / (double)L.edge_count : 1.0;
     a.theta = ctx.cfg->refine_randomness * mean_w;  // refine.rs:44-50
     a.seed = mix64(level_seed ^ 0xA0761D6478BD642Full);
-    if (b.start[6] < b.start[7]) {
+    if (b.start[GTAB_BUCKET] < b.start[GTAB_BUCKET + 1]) {
         a.gkeys = b.gkeys.get();
         a.gvals = b.gvals.get();
         a.goff = b.goff.get();
@@ -2507,6 +2517,9 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         }
         Buckets b;
         build_buckets(L, b, s);
+        if (sweep_trace_on())
+            std::fprintf(stderr, "[level %llu] n=%u nnz=%llu max_degree=%llu\n", (unsigned long long)lvl, n,
+                         (unsigned long long)L.nnz, (unsigned long long)L.max_degree);
         pc.lap(0);
         local_move(L, memb.get(), ls, b, ctx, level_seed, sK, sW, sw);
         pc.lap(1);
