@@ -73,8 +73,9 @@ def make_workload(name):
         return {"kind": "knn", "x": x, "k": 7, "truth": truth, "objective": ("rb", 1.0),
                 "workload": "knn_leiden_4M_d30_k7", "n": 4_000_000, "d": 30}
     if name == "filter":
-        # the learned filter's configuration: 8M x 30, k = 7 (~57M edges before symmetrization)
-        x, truth = W.gaussian_mixture(8_000_000, 30, 200, 9.0, sd=1.0, alpha=1.0, seed=42)
+        # the learned filter's configuration: 8M x 30 from a 400-component mixture (otherwise as
+        # the 4M configuration), k = 7 (~57M edges before symmetrization)
+        x, truth = W.gaussian_mixture(8_000_000, 30, 400, 9.0, sd=1.0, alpha=1.0, seed=42)
         return {"kind": "knn", "x": x, "k": 7, "truth": truth, "objective": ("rb", 1.0),
                 "workload": "knn_leiden_8M_d30_k7", "n": 8_000_000, "d": 30}
     if name == "batch":
@@ -162,6 +163,8 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--knn-cpu-rows", type=int, default=8000,
+                    help="query rows timed for the CPU kNN baseline (evenly spaced sample)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-filter comparison")
     ap.add_argument("--no-sweep-modes", action="store_true", help="skip timing the multi-launch and graph modes")
     ap.add_argument("--filter-threshold", type=float, default=None,
@@ -367,6 +370,40 @@ def main():
                          f"median {t_cpu:.3f} s, host {model} ({ncpu} logical CPUs visible)",
                "modularity": q_cpu, "ari_vs_gpu": O.ari(c.labels(), ol), "n_clusters": int(onc)}
 
+    # CPU baseline of the kNN half: the kd-tree restatement of the reference's exact path
+    # (kiddo nearest_n(k + 1) on min(64, threads) workers) on a bounded sample of query rows of
+    # the same points, scaled to points/s; the GPU's index sets are compared on the same rows
+    cpu_knn = None
+    if not args.no_cpu_baseline and wl["kind"] == "knn":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        try:
+            threads = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            threads = min(16, os.cpu_count() or 1)
+        threads = min(64, threads)
+        n_rows = min(wl["n"], args.knn_cpu_rows)
+        rows = np.linspace(0, wl["n"] - 1, n_rows).astype(np.uint64)
+        t0 = time.perf_counter()
+        ki, kd, t_build = O.knn_kdtree(wl["x"], wl["k"], rows=rows, threads=threads, return_build_seconds=True)
+        t_q = time.perf_counter() - t0 - t_build
+        gi, gd = scgpu.knn(wl["x"], wl["k"], precision=args.precision)
+        same = int(np.sum(np.all(gi[rows.astype(np.int64)] == ki, axis=1)))
+        d2_same = bool(np.array_equal(gd[rows.astype(np.int64)], kd))
+        model, ncpu = cpu_info()
+        # whole-job estimate: tree build once + every row's query at the sampled rate
+        t_est = t_build + t_q * wl["n"] / n_rows
+        cpu_knn = {"value": wl["n"] / t_est, "unit": "points/s", "cores": threads, "kind": "port",
+                   "sample": f"kd-tree kNN restatement (oracle/oracle.c or_knn_kdtree, exact FP64 distances) "
+                             f"on {threads} threads: tree over all {wl['n']} points built in {t_build:.3f} s, "
+                             f"{n_rows} evenly spaced query rows in {t_q:.2f} s, scaled to all rows; "
+                             f"host {model} ({ncpu} logical CPUs visible)",
+                   "gpu_rows_identical": same, "rows_compared": n_rows, "gpu_d2_identical": d2_same,
+                   "gpu_speedup": (wl["n"] / tk) / (wl["n"] / t_est)}
+        log(f"[bench] cpu kNN baseline: {cpu_knn['value']:.0f} points/s on {threads} threads; GPU rows identical "
+            f"{same}/{n_rows}, distances identical {d2_same}")
+
     out = {
         "metric": METRIC,
         "value": edges_per_s,
@@ -395,6 +432,7 @@ def main():
         "sweep_modes": sweep_modes,
         "learned_filter": learned,
         "cpu_baseline": cpu,
+        "cpu_baseline_knn": cpu_knn,
     }
     log("[bench] autotuner:\n" + scgpu.autotune_report())
     print(json.dumps(out), flush=True)

@@ -2,14 +2,18 @@
  * oracle.c -- CPU restatement of single-clustering v0.7.0's Leiden core and kNN Gaussian
  * builder, for testing the HIP implementation. TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
- * Single-threaded, plain C11. Each function cites the reference file:line it restates.
+ * Plain C11 (POSIX threads only in the kd-tree kNN, which restates the reference's threaded
+ * search); everything else is single-threaded. Each function cites the reference file:line it
+ * restates.
  */
+#define _POSIX_C_SOURCE 200809L
 #include "oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#include <time.h>
 
 #define UNASSIGNED 0xFFFFFFFFu
 
@@ -1178,6 +1182,187 @@ void or_knn_bruteforce(const float *data, uint64_t n, uint32_t d, uint32_t k,
         }
     }
     free(bd); free(bi);
+}
+
+/* ---- exact kNN by kd-tree (the reference's n <= 250 000 path) ------------------------------
+ * knn_arrayd_kiddo_gaussian (neighborhood/mod.rs:51-98) builds a kiddo 5.2.2 KdTree over the
+ * points and knn_search_large_batches (neighborhood/mod.rs:233-314) queries nearest_n(k + 1)
+ * for every row, self included, on min(64, threads) workers over contiguous row chunks. This
+ * restatement builds a balanced tree (split dimension cycling with depth, median split, leaves of
+ * at most 32 points, kiddo's default bucket size) and answers with the same exact result as
+ * or_knn_bruteforce: k nearest by FP64 squared distance, ties to the lower index, self excluded.
+ * A subtree is skipped only when its splitting plane is strictly farther than the current k-th
+ * distance: every term of the FP64 sum is non-negative and rounding is monotone, so a computed
+ * distance is never below its computed plane term and pruning never drops a point that ties. */
+#include <pthread.h>
+
+#define KD_LEAF 32u
+
+typedef struct {
+    const float *data;
+    uint32_t d;
+    uint32_t *perm;  /* point order: every node owns a contiguous range */
+    uint32_t *lo, *hi, *left, *right, *dim;
+    float *split;
+    uint32_t n_nodes;
+} kd_tree;
+
+static uint32_t kd_build_rec(kd_tree *t, uint32_t lo, uint32_t hi, uint32_t depth) {
+    uint32_t id = t->n_nodes++;
+    t->lo[id] = lo;
+    t->hi[id] = hi;
+    t->left[id] = t->right[id] = UNASSIGNED;
+    if (hi - lo <= KD_LEAF) return id;
+    uint32_t dm = depth % t->d, mid = lo + (hi - lo) / 2;
+    /* quickselect of the median on dimension dm (ties ordered by index, so the split is deterministic) */
+    uint32_t a = lo, b = hi - 1;
+    while (a < b) {
+        uint32_t pv = t->perm[a + (b - a) / 2];
+        float pvx = t->data[(uint64_t)pv * t->d + dm];
+        uint32_t i = a, j = b;
+        while (i <= j) {
+            for (;;) {
+                uint32_t q = t->perm[i];
+                float x = t->data[(uint64_t)q * t->d + dm];
+                if (x < pvx || (x == pvx && q < pv)) i++; else break;
+            }
+            for (;;) {
+                uint32_t q = t->perm[j];
+                float x = t->data[(uint64_t)q * t->d + dm];
+                if (x > pvx || (x == pvx && q > pv)) j--; else break;
+            }
+            if (i <= j) {
+                uint32_t tmp = t->perm[i]; t->perm[i] = t->perm[j]; t->perm[j] = tmp;
+                i++;
+                if (j == 0) break;
+                j--;
+            }
+        }
+        if (mid <= j) b = j;
+        else if (mid >= i) a = i;
+        else break;
+    }
+    t->dim[id] = dm;
+    t->split[id] = t->data[(uint64_t)t->perm[mid] * t->d + dm];
+    uint32_t l = kd_build_rec(t, lo, mid, depth + 1);
+    uint32_t r = kd_build_rec(t, mid, hi, depth + 1);
+    t->left[id] = l;
+    t->right[id] = r;
+    return id;
+}
+
+typedef struct {
+    double *bd;
+    uint32_t *bi;
+    uint32_t cnt, k;
+} kd_best;
+
+static void kd_offer(kd_best *b, double dd, uint32_t j) {
+    if (b->cnt == b->k && !(dd < b->bd[b->k - 1] || (dd == b->bd[b->k - 1] && j < b->bi[b->k - 1]))) return;
+    uint32_t pos = b->cnt < b->k ? b->cnt : b->k - 1;
+    while (pos > 0 && (dd < b->bd[pos - 1] || (dd == b->bd[pos - 1] && j < b->bi[pos - 1]))) {
+        b->bd[pos] = b->bd[pos - 1];
+        b->bi[pos] = b->bi[pos - 1];
+        pos--;
+    }
+    b->bd[pos] = dd;
+    b->bi[pos] = j;
+    if (b->cnt < b->k) b->cnt++;
+}
+
+static void kd_search(const kd_tree *t, uint32_t node, const float *x, uint32_t self, kd_best *b) {
+    if (t->left[node] == UNASSIGNED) {
+        for (uint32_t p = t->lo[node]; p < t->hi[node]; p++) {
+            uint32_t j = t->perm[p];
+            if (j == self) continue;
+            kd_offer(b, d2_f64(x, t->data + (uint64_t)j * t->d, t->d), j);
+        }
+        return;
+    }
+    uint32_t dm = t->dim[node];
+    double diff = (double)x[dm] - (double)t->split[node];
+    /* points equal to the split value can sit on either side: the near side is searched first
+     * and the far side is pruned only by a strictly larger plane distance */
+    uint32_t near = diff < 0.0 ? t->left[node] : t->right[node];
+    uint32_t far = diff < 0.0 ? t->right[node] : t->left[node];
+    kd_search(t, near, x, self, b);
+    double plane = diff * diff;
+    if (b->cnt < b->k || !(plane > b->bd[b->k - 1])) kd_search(t, far, x, self, b);
+}
+
+typedef struct {
+    const kd_tree *t;
+    const uint64_t *rows;
+    uint64_t q0, q1;
+    uint32_t k;
+    uint32_t *idx_out;
+    double *d2_out;
+} kd_job;
+
+static void *kd_worker(void *arg) {
+    kd_job *jb = (kd_job *)arg;
+    kd_best b;
+    b.k = jb->k;
+    b.bd = (double *)malloc(jb->k * sizeof(double));
+    b.bi = (uint32_t *)malloc(jb->k * sizeof(uint32_t));
+    for (uint64_t qi = jb->q0; qi < jb->q1; qi++) {
+        uint64_t i = jb->rows ? jb->rows[qi] : qi;
+        b.cnt = 0;
+        kd_search(jb->t, 0, jb->t->data + i * jb->t->d, (uint32_t)i, &b);
+        for (uint32_t s = 0; s < jb->k; s++) {
+            jb->idx_out[qi * jb->k + s] = s < b.cnt ? b.bi[s] : UNASSIGNED;
+            jb->d2_out[qi * jb->k + s] = s < b.cnt ? b.bd[s] : INFINITY;
+        }
+    }
+    free(b.bd);
+    free(b.bi);
+    return NULL;
+}
+
+int or_knn_kdtree(const float *data, uint64_t n, uint32_t d, uint32_t k, const uint64_t *rows,
+                  uint64_t n_rows, uint32_t threads, uint32_t *idx_out, double *d2_out,
+                  double *build_seconds) {
+    if (n == 0 || d == 0 || k == 0 || n > 0xFFFFFFF0ull) return 1;
+    kd_tree t;
+    memset(&t, 0, sizeof(t));
+    t.data = data;
+    t.d = d;
+    uint64_t cap = 2 * (n / KD_LEAF + 1) * 2 + 4;
+    t.perm = (uint32_t *)malloc(n * sizeof(uint32_t));
+    t.lo = (uint32_t *)malloc(cap * sizeof(uint32_t));
+    t.hi = (uint32_t *)malloc(cap * sizeof(uint32_t));
+    t.left = (uint32_t *)malloc(cap * sizeof(uint32_t));
+    t.right = (uint32_t *)malloc(cap * sizeof(uint32_t));
+    t.dim = (uint32_t *)calloc(cap, sizeof(uint32_t));
+    t.split = (float *)calloc(cap, sizeof(float));
+    for (uint64_t i = 0; i < n; i++) t.perm[i] = (uint32_t)i;
+    struct timespec ts0, ts1;
+    clock_gettime(CLOCK_MONOTONIC, &ts0);
+    kd_build_rec(&t, 0, (uint32_t)n, 0);
+    clock_gettime(CLOCK_MONOTONIC, &ts1);
+    if (build_seconds) *build_seconds = (double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec);
+    uint64_t nq = rows ? n_rows : n;
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64; /* min(64, threads), neighborhood/mod.rs:245 */
+    uint64_t chunk = (nq + threads - 1) / threads;
+    pthread_t *tid = (pthread_t *)malloc(threads * sizeof(pthread_t));
+    kd_job *jobs = (kd_job *)malloc(threads * sizeof(kd_job));
+    uint32_t started = 0;
+    for (uint32_t w = 0; w < threads; w++) {
+        uint64_t q0 = (uint64_t)w * chunk, q1 = q0 + chunk < nq ? q0 + chunk : nq;
+        if (q0 >= q1) break;
+        jobs[w] = (kd_job){&t, rows, q0, q1, k, idx_out, d2_out};
+        if (pthread_create(&tid[w], NULL, kd_worker, &jobs[w]) != 0) {
+            kd_worker(&jobs[w]);
+            tid[w] = 0;
+        }
+        started = w + 1;
+    }
+    for (uint32_t w = 0; w < started; w++)
+        if (tid[w]) pthread_join(tid[w], NULL);
+    free(tid); free(jobs);
+    free(t.perm); free(t.lo); free(t.hi); free(t.left); free(t.right); free(t.dim); free(t.split);
+    return 0;
 }
 
 typedef struct { uint32_t r, c; float w; } trip_t;

@@ -119,6 +119,13 @@ void or_knn_bruteforce(const float *data, uint64_t n, uint32_t d, uint32_t k,
                        const uint64_t *rows, uint64_t n_rows, uint32_t *idx_out,
                        double *d2_out);
 
+/* Exact kNN through a kd-tree on `threads` workers (the reference's kd-tree path,
+ * neighborhood/mod.rs:51-98, 233-314): same result as or_knn_bruteforce. build_seconds gets the
+ * tree construction time (nullable). Returns nonzero on invalid sizes. */
+int or_knn_kdtree(const float *data, uint64_t n, uint32_t d, uint32_t k, const uint64_t *rows,
+                  uint64_t n_rows, uint32_t threads, uint32_t *idx_out, double *d2_out,
+                  double *build_seconds);
+
 /* Gaussian connectivity over a kNN result (k neighbours per row, self excluded).
  * mode 0: sigma_i = distance to the k-th neighbour, union (max) symmetrization.
  * mode 1: reference-compatible: sigma_i = upper median of nonzero distances over the k+1

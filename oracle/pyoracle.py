@@ -97,6 +97,9 @@ def lib():
                                         C.c_uint64, u32p, f64p]
         L.or_knn_gaussian.argtypes = [u32p, f64p, C.c_uint64, C.c_uint32, C.c_int, u64p, u32p,
                                       f32p]
+        L.or_knn_kdtree.argtypes = [f32p, C.c_uint64, C.c_uint32, C.c_uint32, u64p, C.c_uint64,
+                                    C.c_uint32, u32p, f64p, f64p]
+        L.or_knn_kdtree.restype = C.c_int
         L.or_knn_gaussian.restype = C.c_uint64
         _lib = L
     return _lib
@@ -294,6 +297,26 @@ def knn_bruteforce(data, k, rows=None):
     d2 = np.zeros((nq, k), dtype=np.float64)
     lib().or_knn_bruteforce(_p(data, f32p), n, d, k, rp, nq, _p(idx, u32p), _p(d2, f64p))
     return idx, d2
+
+
+def knn_kdtree(data, k, rows=None, threads=1, return_build_seconds=False):
+    """Exact kNN through the kd-tree restatement of the reference's n <= 250 000 path
+    (neighborhood/mod.rs:51-98, 233-314), on `threads` workers; equals knn_bruteforce."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    n, d = data.shape
+    if rows is None:
+        nq, rp = n, None
+    else:
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        nq, rp = len(rows), _p(rows, u64p)
+    idx = np.zeros((nq, k), dtype=np.uint32)
+    d2 = np.zeros((nq, k), dtype=np.float64)
+    tb = C.c_double(0.0)
+    rc = lib().or_knn_kdtree(_p(data, f32p), n, d, k, rp, nq, int(threads), _p(idx, u32p), _p(d2, f64p),
+                             C.byref(tb))
+    if rc:
+        raise OracleError(rc)
+    return (idx, d2, tb.value) if return_build_seconds else (idx, d2)
 
 
 def knn_gaussian(idx, d2, mode=0):

@@ -305,6 +305,32 @@ def test_knn_bruteforce_matches_numpy():
     assert idx[10, 0] == 11 and d2[10, 0] == 0.0
 
 
+def test_knn_kdtree_equals_bruteforce():
+    # the kd-tree restatement (CPU baseline of the kNN half) returns exactly the brute-force
+    # result, ties included: duplicated points, duplicated coordinates on split dimensions,
+    # clustered data, several thread counts, a row subset
+    rng = np.random.default_rng(1)
+    centres = rng.normal(0.0, 4.0, size=(6, 12))
+    x = (centres[rng.integers(0, 6, 1500)] + rng.normal(size=(1500, 12))).astype(np.float32)
+    x[100] = x[101]
+    x[102] = x[101]
+    x[200:260, 0] = 0.5  # many equal values on the first split dimension
+    x[300:310] = np.round(x[300:310])
+    for d in (12, 3):
+        xd = np.ascontiguousarray(x[:, :d])
+        bi, bd = O.knn_bruteforce(xd, 9)
+        for threads in (1, 3):
+            ki, kd = O.knn_kdtree(xd, 9, threads=threads)
+            assert np.array_equal(ki, bi)
+            assert np.array_equal(kd, bd)
+    rows = np.array([5, 100, 101, 250, 1499], dtype=np.uint64)
+    ki, kd = O.knn_kdtree(x, 4, rows=rows, threads=2)
+    bi, bd = O.knn_bruteforce(x, 4, rows=rows)
+    assert np.array_equal(ki, bi) and np.array_equal(kd, bd)
+    # the tie at distance 0 keeps the lower index first
+    assert ki[1, 0] == 101 and ki[1, 1] == 102 and kd[1, 0] == 0.0
+
+
 def test_gaussian_weights_closed_form():
     idx = np.array([[1], [0]], dtype=np.uint32)
     d2 = np.array([[4.0], [4.0]])
