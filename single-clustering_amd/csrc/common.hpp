@@ -238,6 +238,24 @@ void to_host(T *dst, const T *src, size_t n, hipStream_t s) {
 
 // The device-resident graph behind sc_graph (CSRNetworkData, csr_network.rs:39-60), plus the
 // per-node self-loop weight, cached so kernels never binary-search a row for it.
+namespace scgpu {
+// Degree classes of the sweeps' decide step (the LDS table layout each class uses); the last
+// class keeps its tables in global memory.
+constexpr int NUM_BUCKETS = 8;
+constexpr int GTAB_BUCKET = 7;
+
+__host__ __device__ __forceinline__ int bucket_of(uint64_t d) {
+    if (d <= 8) return 0;
+    if (d <= 16) return 1;
+    if (d <= 32) return 2;
+    if (d <= 128) return 3;
+    if (d <= 512) return 4;
+    if (d <= 1024) return 5;
+    if (d <= 2048) return 6;
+    return GTAB_BUCKET;
+}
+}  // namespace scgpu
+
 struct sc_graph {
     uint64_t n = 0;
     uint64_t nnz = 0;
@@ -254,6 +272,9 @@ struct sc_graph {
     double total_node_weight = 0.0;
     double max_strength = 0.0;
     uint32_t max_degree = 0;
+    // vertices per degree class (bucket_of), counted when the graph was finished
+    bool has_bucket_counts = false;
+    uint32_t bucket_counts[scgpu::NUM_BUCKETS] = {0};
 };
 
 namespace scgpu {

@@ -176,9 +176,12 @@ template <int G>
 __global__ void __launch_bounds__(256) k_row_stats(const uint64_t *ptrs, const uint32_t *nbrs, const float *wts,
                                                    uint64_t n, double *strength, double *self_loop,
                                                    double *row_total, unsigned long long *edge_count,
-                                                   unsigned int *flags) {
+                                                   unsigned int *flags, unsigned int *bucket_counts) {
     constexpr int NG = 256 / G;
     const int lane = threadIdx.x % G;
+    __shared__ unsigned int s_bc[NUM_BUCKETS];
+    if (threadIdx.x < NUM_BUCKETS) s_bc[threadIdx.x] = 0;
+    __syncthreads();
     unsigned long long local_edges = 0;
     unsigned int max_deg = 0, loop_seen = 0;
     for (uint64_t v0 = (uint64_t)blockIdx.x * NG; v0 < n; v0 += (uint64_t)gridDim.x * NG) {
@@ -230,6 +233,7 @@ __global__ void __launch_bounds__(256) k_row_stats(const uint64_t *ptrs, const u
             self_loop[v] = loop;
             row_total[v] = tot;
             max_deg = max(max_deg, (unsigned int)min<uint64_t>(hi - lo, 0xFFFFFFFFull));
+            atomicAdd(&s_bc[bucket_of(hi - lo)], 1u);
         }
     }
     // one atomic per workgroup for each counter (same-address atomics serialize)
@@ -241,6 +245,8 @@ __global__ void __launch_bounds__(256) k_row_stats(const uint64_t *ptrs, const u
         if (loop_seen) atomicOr(&flags[0], 1u);
         if (max_deg) atomicMax(&flags[2], max_deg);
     }
+    __syncthreads();
+    if (threadIdx.x < NUM_BUCKETS && s_bc[threadIdx.x]) atomicAdd(&bucket_counts[threadIdx.x], s_bc[threadIdx.x]);
 }
 
 void graph_finish_sorted(sc_graph &g, hipStream_t s) {
@@ -249,19 +255,21 @@ void graph_finish_sorted(sc_graph &g, hipStream_t s) {
     g.self_loop.resize(n);
     g.row_total.resize(n);
     DevBuf<double> &row_total = g.row_total;
-    // [0..1] edge count, then flags: [2] any self-loop, [3] unused, [4] max degree
-    DevBuf<unsigned int> cnt(6);
-    HIP_OK(hipMemsetAsync(cnt.get(), 0, 6 * sizeof(unsigned int), s));
+    // [0..1] edge count, then flags: [2] any self-loop, [3] unused, [4] max degree, [5] unused,
+    // [6..] vertices per degree class
+    constexpr int NW = 6 + NUM_BUCKETS;
+    DevBuf<unsigned int> cnt(NW);
+    HIP_OK(hipMemsetAsync(cnt.get(), 0, NW * sizeof(unsigned int), s));
     unsigned long long *ec_d = reinterpret_cast<unsigned long long *>(cnt.get());
     unsigned int *flags = cnt.get() + 2;
     if (n) {
         hipLaunchKernelGGL((k_row_stats<16>), dim3(grid_for(n, 16, 1024)), dim3(256), 0, s, g.ptrs.get(),
                            g.nbrs.get(), g.wts.get(), n, g.strength.get(), g.self_loop.get(), row_total.get(), ec_d,
-                           flags);
+                           flags, cnt.get() + 6);
         KERNEL_OK();
     }
-    unsigned int h[6] = {0, 0, 0, 0, 0, 0};
-    to_host(h, cnt.get(), 6, s);
+    unsigned int h[NW] = {0};
+    to_host(h, cnt.get(), NW, s);
     const double *in[4] = {row_total.get(), g.strength.get(), g.strength.get(), g.node_w.get()};
     const bool is_max[4] = {false, false, true, false};
     double r[4];
@@ -271,6 +279,8 @@ void graph_finish_sorted(sc_graph &g, hipStream_t s) {
     g.edge_count = ec;
     g.any_self_loops = h[2] != 0;
     g.max_degree = h[4];
+    for (int i = 0; i < NUM_BUCKETS; i++) g.bucket_counts[i] = h[6 + i];
+    g.has_bucket_counts = n <= 0xFFFFFFFFull;
     g.total_weight = r[0];
     const double deg_sum = r[1];
     g.max_strength = n ? r[2] : 0.0;

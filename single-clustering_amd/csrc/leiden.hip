@@ -45,7 +45,6 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
-constexpr int NUM_BUCKETS = 8;
 constexpr int MOVE_SUBSWEEPS = 2;
 constexpr int MAX_MOVE_ITERS = 100;
 constexpr int MAX_REFINE_ROUNDS = 64;
@@ -652,20 +651,7 @@ struct IterGraph {
     }
 };
 
-// ---- degree buckets ------------------------------------------------------------------------
-constexpr int GTAB_BUCKET = 7;  // rows too long for an LDS table: global-memory tables
-
-__device__ __forceinline__ int bucket_of(uint64_t d) {
-    if (d <= 8) return 0;
-    if (d <= 16) return 1;
-    if (d <= 32) return 2;
-    if (d <= 128) return 3;
-    if (d <= 512) return 4;
-    if (d <= 1024) return 5;
-    if (d <= 2048) return 6;
-    return GTAB_BUCKET;
-}
-
+// ---- degree buckets (bucket_of: common.hpp) ---------------------------------------------
 __global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, uint32_t *counts) {
     __shared__ uint32_t sc[NUM_BUCKETS];
     if (threadIdx.x < NUM_BUCKETS) sc[threadIdx.x] = 0;
@@ -676,8 +662,12 @@ __global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, uint32_t *count
     if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
 }
 
+struct BucketStarts {
+    uint32_t s[NUM_BUCKETS + 1];
+};
+
 // Block-local counting, one global reservation per bucket per block, then placement.
-__global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, const uint32_t *starts, uint32_t *cursor,
+__global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, BucketStarts starts, uint32_t *cursor,
                               uint32_t *lists) {
     __shared__ uint32_t cnt[NUM_BUCKETS], base[NUM_BUCKETS];
     for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gridDim.x * blockDim.x) {
@@ -694,7 +684,7 @@ __global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, const uint32_t *
         if (threadIdx.x < NUM_BUCKETS && cnt[threadIdx.x])
             base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
         __syncthreads();
-        if (v < n) lists[starts[b] + base[b] + local] = v;
+        if (v < n) lists[starts.s[b] + base[b] + local] = v;
         __syncthreads();
     }
 }
@@ -722,17 +712,22 @@ static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
     uint32_t n = (uint32_t)L.n;
     DevBuf<uint32_t> cnt(2 * NUM_BUCKETS + 2);
     HIP_OK(hipMemsetAsync(cnt.get(), 0, (2 * NUM_BUCKETS + 2) * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_bucket_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, cnt.get());
-    KERNEL_OK();
     uint32_t h[NUM_BUCKETS];
-    to_host(h, cnt.get(), NUM_BUCKETS, s);
-    sync_stream(s);
+    if (L.has_bucket_counts) {
+        // counted while the graph was finished (read back with its other aggregates)
+        for (int i = 0; i < NUM_BUCKETS; i++) h[i] = L.bucket_counts[i];
+    } else {
+        hipLaunchKernelGGL(k_bucket_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, cnt.get());
+        KERNEL_OK();
+        to_host(h, cnt.get(), NUM_BUCKETS, s);
+        sync_stream(s);
+    }
     b.start[0] = 0;
     for (int i = 0; i < NUM_BUCKETS; i++) b.start[i + 1] = b.start[i] + h[i];
-    DevBuf<uint32_t> starts(NUM_BUCKETS + 1);
-    to_device(starts.get(), b.start, NUM_BUCKETS + 1, s);
+    BucketStarts starts;
+    for (int i = 0; i <= NUM_BUCKETS; i++) starts.s[i] = b.start[i];
     b.lists.resize(n);
-    hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, starts.get(),
+    hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, starts,
                        cnt.get() + NUM_BUCKETS, b.lists.get());
     KERNEL_OK();
     uint32_t nb6 = h[GTAB_BUCKET];
@@ -1085,12 +1080,60 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         else A.ccnt[i - (uint64_t)A.max_steps * (NUM_BUCKETS + 1)] = 0;
     }
     if (!grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap)) return;
-    for (uint64_t v = gtid; v < n; v += gsize) {
-        const uint64_t c = REFINE ? v : a.memb[v];
-        const VParams vp = vparams(a, (uint32_t)v);
-        atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)to_fx(a.strength[v], vp.sK));
-        atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)to_fx(a.node_w[v], vp.sW));
-        atomicAdd(&A.csize[c], 1u);
+    if (REFINE) {
+        // singletons: every community gets exactly one vertex, no two atomics share an address
+        for (uint64_t v = gtid; v < n; v += gsize) {
+            const VParams vp = vparams(a, (uint32_t)v);
+            atomicAdd((unsigned long long *)&A.Kfx[v], (unsigned long long)to_fx(a.strength[v], vp.sK));
+            atomicAdd((unsigned long long *)&A.Wfx[v], (unsigned long long)to_fx(a.node_w[v], vp.sW));
+            atomicAdd(&A.csize[v], 1u);
+        }
+    } else {
+        // a level that starts from earlier communities has few, large ones: the workgroup first
+        // sums its vertices per community in an LDS table (integer sums, so any order gives the
+        // same totals), then adds each community's sums with one global atomic per array; a
+        // vertex whose community finds no slot within a few probes is added directly
+        constexpr uint32_t PT = 1024;
+        static_assert(3 * PT <= SWEEP_TAB, "prologue table does not fit the sweep's LDS");
+        uint32_t *tk = s_keys;
+        long long *tK = s_vals, *tW = s_vals + PT, *tN = s_vals + 2 * PT;
+        for (uint32_t i = threadIdx.x; i < PT; i += SWEEP_BS) {
+            tk[i] = EMPTY_KEY;
+            tK[i] = 0;
+            tW[i] = 0;
+            tN[i] = 0;
+        }
+        __syncthreads();
+        for (uint64_t v = gtid; v < n; v += gsize) {
+            const uint32_t c = a.memb[v];
+            const VParams vp = vparams(a, (uint32_t)v);
+            const long long kf = to_fx(a.strength[v], vp.sK), wf = to_fx(a.node_w[v], vp.sW);
+            uint32_t h = hash32(c) & (PT - 1);
+            bool done = false;
+            for (int p = 0; p < 8 && !done; p++) {
+                const uint32_t prev = atomicCAS(&tk[h], EMPTY_KEY, c);
+                if (prev == EMPTY_KEY || prev == c) {
+                    atomicAdd((unsigned long long *)&tK[h], (unsigned long long)kf);
+                    atomicAdd((unsigned long long *)&tW[h], (unsigned long long)wf);
+                    atomicAdd((unsigned long long *)&tN[h], 1ull);
+                    done = true;
+                }
+                h = (h + 1) & (PT - 1);
+            }
+            if (!done) {
+                atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)kf);
+                atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)wf);
+                atomicAdd(&A.csize[c], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < PT; i += SWEEP_BS) {
+            const uint32_t c = tk[i];
+            if (c == EMPTY_KEY) continue;
+            atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)tK[i]);
+            atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)tW[i]);
+            atomicAdd(&A.csize[c], (uint32_t)tN[i]);
+        }
     }
     if (!grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap)) return;
 
@@ -1939,15 +1982,20 @@ __global__ void k_remap(const uint32_t *ids, uint32_t n, const uint32_t *newid, 
 }
 
 // Ascending compaction of the ids in use (Partition::renumber, partition.rs:265-296).
-// ids are < id_space. Writes compacted ids to out (may alias ids) and returns their count.
-static uint32_t renumber(const uint32_t *ids, uint32_t n, uint32_t id_space, uint32_t *out, hipStream_t s) {
+// ids are < id_space. Writes compacted ids to out (may alias ids) and returns their count, or
+// 0 without waiting for the device when the caller does not need it.
+static uint32_t renumber(const uint32_t *ids, uint32_t n, uint32_t id_space, uint32_t *out, hipStream_t s,
+                         bool need_count = true) {
     DevBuf<uint32_t> flags(id_space + 1), newid(id_space + 1);
-    HIP_OK(hipMemsetAsync(flags.get(), 0, (id_space + 1) * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((uint64_t)id_space + 1, 256, 4096)), dim3(256), 0, s, flags.get(),
+                       (uint64_t)id_space + 1, 0u);
+    KERNEL_OK();
     hipLaunchKernelGGL(k_used_flags, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, n, flags.get());
     KERNEL_OK();
     exclusive_scan_u32(flags.get(), newid.get(), id_space + 1, s);
     hipLaunchKernelGGL(k_remap, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ids, n, newid.get(), out);
     KERNEL_OK();
+    if (!need_count) return 0;
     uint32_t count = 0;
     to_host(&count, newid.get() + id_space, 1, s);
     sync_stream(s);
@@ -2034,11 +2082,9 @@ template <int G>
 __global__ void __launch_bounds__(256) k_emit_fine(const uint32_t *members, const uint32_t *sorted_groups, uint32_t n,
                                                    const uint64_t *moff, const uint64_t *ptrs, const uint32_t *nbrs,
                                                    const float *wts, const uint32_t *groups, int hbits,
-                                                   uint64_t sentinel, uint64_t *keys, float *vals,
-                                                   unsigned long long *n_valid) {
+                                                   uint64_t sentinel, uint64_t *keys, float *vals) {
     constexpr int NG = 256 / G;
     const int lane = threadIdx.x % G;
-    unsigned long long local = 0;
     for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
         uint32_t v = members[j];
         uint64_t g = sorted_groups[j];
@@ -2046,22 +2092,16 @@ __global__ void __launch_bounds__(256) k_emit_fine(const uint32_t *members, cons
         for (uint64_t e = lo + lane; e < hi; e += G) {
             uint64_t h = groups[nbrs[e]];
             uint64_t out = o + (e - lo);
-            if (h >= g) {
-                keys[out] = (g << hbits) | h;
-                local++;
-            } else {
-                keys[out] = sentinel;
-            }
+            keys[out] = h >= g ? (g << hbits) | h : sentinel;
             vals[out] = wts[e];
         }
     }
-    local = block_sum(local);
-    if (threadIdx.x == 0 && local) atomicAdd(n_valid, local);
 }
 
-__global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
+// Run heads of the sorted fine keys; the sentinel keys (sorted last) start no run.
+__global__ void k_key_heads(const uint64_t *keys, uint64_t n, uint64_t sentinel, uint32_t *head) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+        head[i] = (keys[i] != sentinel && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
 }
 
 // Coarse pair sums. The reference adds each pair's fine weights sequentially in f64 in the
@@ -2080,10 +2120,17 @@ __device__ __forceinline__ void emit_pair(uint64_t key, double acc, uint32_t r, 
     pw[r] = (g == h) ? (float)(0.5 * (acc + cloops[g])) : (float)acc;
 }
 
-__global__ void k_seg_starts(const uint32_t *head, const uint32_t *run, uint64_t n, uint32_t np, uint64_t *start) {
+// Segment (pair) starts; start[np] is the end of the valid keys: the first sentinel, or n.
+__global__ void k_seg_starts(const uint64_t *keys, uint64_t sentinel, const uint32_t *head, const uint32_t *run,
+                             uint64_t n, uint32_t np, uint64_t *start) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
-        if (i == n) start[np] = n;
-        else if (head[i]) start[run[i]] = i;
+        if (i == n) {
+            if (n == 0 || keys[n - 1] != sentinel) start[np] = n;
+        } else if (keys[i] == sentinel) {
+            if (i == 0 || keys[i - 1] != sentinel) start[np] = i;
+        } else if (head[i]) {
+            start[run[i]] = i;
+        }
     }
 }
 
@@ -2205,32 +2252,28 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     const int kbits = std::min(64, hbits + bits_for(ng) + 1);
     DevBuf<uint64_t> keys(nnz);
     DevBuf<float> vals(nnz);
-    DevBuf<unsigned long long> nvalid(1);
-    HIP_OK(hipMemsetAsync(nvalid.get(), 0, sizeof(unsigned long long), s));
     if (nnz) {
         hipLaunchKernelGGL((k_emit_fine<16>), dim3(grid_for(n, 16, 2048)), dim3(256), 0, s, members.get(), sg.get(), n,
                            moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, hbits, sentinel, keys.get(),
-                           vals.get(), nvalid.get());
+                           vals.get());
         KERNEL_OK();
         radix_sort_pairs(keys, vals, nnz, kbits, s);
     }
-    unsigned long long nv = 0;
-    to_host(&nv, nvalid.get(), 1, s);
-    sync_stream(s);
-    uint64_t npairs_max = nv;
-    DevBuf<uint32_t> head(nv + 1), run(nv + 1);
+    // runs of equal keys over all nnz entries (sentinels start none), so the number of valid
+    // keys is never needed on the host: one wait gives the pair count
+    DevBuf<uint32_t> head(nnz + 1), run(nnz + 1);
     uint32_t np = 0;
-    if (nv) {
-        hipLaunchKernelGGL(k_key_heads, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, keys.get(), (uint64_t)nv, head.get());
+    if (nnz) {
+        hipLaunchKernelGGL(k_key_heads, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), nnz, sentinel,
+                           head.get());
         KERNEL_OK();
-        exclusive_scan_u32(head.get(), run.get(), nv, s);
+        exclusive_scan_u32(head.get(), run.get(), nnz, s);
         uint32_t lr = 0, lh = 0;
-        to_host(&lr, run.get() + nv - 1, 1, s);
-        to_host(&lh, head.get() + nv - 1, 1, s);
+        to_host(&lr, run.get() + nnz - 1, 1, s);
+        to_host(&lh, head.get() + nnz - 1, 1, s);
         sync_stream(s);
         np = lr + lh;
     }
-    (void)npairs_max;
     DevBuf<uint32_t> pa(np + 1), pb(np + 1), keep(np + 1), off(np + 1);
     DevBuf<float> pw(np + 1);
     uint64_t total_entries = 0;
@@ -2239,8 +2282,8 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         DevBuf<uint32_t> long_list(np);
         DevBuf<unsigned int> n_long(1);
         HIP_OK(hipMemsetAsync(n_long.get(), 0, sizeof(unsigned int), s));
-        hipLaunchKernelGGL(k_seg_starts, dim3(grid_for(nv + 1, 256, 8192)), dim3(256), 0, s, head.get(), run.get(),
-                           (uint64_t)nv, np, seg.get());
+        hipLaunchKernelGGL(k_seg_starts, dim3(grid_for(nnz + 1, 256, 8192)), dim3(256), 0, s, keys.get(), sentinel,
+                           head.get(), run.get(), nnz, np, seg.get());
         KERNEL_OK();
         hipLaunchKernelGGL(k_pair_sums_short, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, keys.get(), vals.get(),
                            seg.get(), np, hbits, cloops.get(), pa.get(), pb.get(), pw.get(), keep.get(),
@@ -2555,7 +2598,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         hipLaunchKernelGGL(k_scatter_coarse_memb, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups.get(),
                            memb.get(), n, rmemb.get());
         KERNEL_OK();
-        renumber(rmemb.get(), ng, n, memb.get(), s);
+        renumber(rmemb.get(), ng, n, memb.get(), s, false);
         if (B) batch_rebase(*B, memb.get(), vjob_l.get(), ng, s);
         hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, groups.get(), node_at.get(), n0);
         KERNEL_OK();
@@ -2903,6 +2946,7 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
         std::vector<UnionSrc> src(J);
         uint64_t voff = 0, eoff = 0;
         uint32_t maxdeg = 0;
+        bool bucket_known = true;
         for (uint32_t j = 0; j < J; j++) {
             const sc_graph &g = *graphs[j / n_resolutions];
             src[j] = UnionSrc{g.ptrs.get(), g.nbrs.get(), g.wts.get(), g.node_w.get(), g.strength.get(),
@@ -2917,11 +2961,14 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
             U.any_self_loops = U.any_self_loops || g.any_self_loops;
             U.max_strength = std::max(U.max_strength, g.max_strength);
             maxdeg = std::max(maxdeg, g.max_degree);
+            bucket_known = bucket_known && g.has_bucket_counts;
+            for (int k = 0; k < NUM_BUCKETS; k++) U.bucket_counts[k] += g.bucket_counts[k];
             voff += g.n;
             eoff += g.nnz;
         }
         B.voff0[J] = voff;
         U.max_degree = maxdeg;
+        U.has_bucket_counts = bucket_known && N <= 0xFFFFFFFFull;
         DevBuf<UnionSrc> d_src(J);
         to_device(d_src.get(), src.data(), J, s);
         B.d_voff0.resize(J + 1);
