@@ -2223,7 +2223,9 @@ __global__ void k_ptrs_from_keys(const uint64_t *keys, uint64_t n, int cbits, ui
     }
 }
 
-std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *groups, uint32_t ng, hipStream_t s) {
+// 64-bit-key variant (any size): fine entries sorted by (g, h), coarse entries by (row, column).
+static std::unique_ptr<sc_graph> aggregate_device_wide(const sc_graph &L, const uint32_t *groups, uint32_t ng,
+                                                       hipStream_t s) {
     uint32_t n = (uint32_t)L.n;
     auto out = std::make_unique<sc_graph>();
     out->n = ng;
@@ -2321,6 +2323,280 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     hipLaunchKernelGGL(k_ptrs_from_keys, dim3(grid_for((uint64_t)ng + 1, 256, 4096)), dim3(256), 0, s, ckeys.get(),
                        total_entries, cbits, ng, out->ptrs.get());
     KERNEL_OK();
+    graph_finish_sorted(*out, s);
+    return out;
+}
+
+// ---- aggregation with 32-bit keys --------------------------------------------------------------
+// The reference walks the groups in order and, per group g, adds the weights of its members'
+// entries (members ascending, rows ascending) towards groups h >= g (csr_network.rs:548-564).
+// Here every fine entry t (member order) is keyed by its neighbour group h (h >= g), or by the
+// sentinel ng; a stable sort by h leaves each pair's entries contiguous and in t order, with
+// the pairs ordered by (h, g). So the pair list directly gives each row r its entries (g, r),
+// g <= r, ascending in g; a second stable sort of the pairs with g < h by g gives each row its
+// entries (r, h), h > r, ascending in h. Both sorts use 32-bit keys of bits_for(ng) bits.
+
+template <int G>
+__global__ void __launch_bounds__(256) k_emit_fine_h(const uint32_t *members, const uint32_t *sorted_groups, uint32_t n,
+                                                     const uint64_t *moff, const uint64_t *ptrs, const uint32_t *nbrs,
+                                                     const float *wts, const uint32_t *groups, uint32_t ng,
+                                                     uint32_t *keys, uint32_t *eidx, uint32_t *egrp, float *evals) {
+    constexpr int NG = 256 / G;
+    const int lane = threadIdx.x % G;
+    for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
+        const uint32_t v = members[j];
+        const uint32_t g = sorted_groups[j];
+        const uint64_t o = moff[j], lo = ptrs[v], hi = ptrs[v + 1];
+        for (uint64_t e = lo + lane; e < hi; e += G) {
+            const uint32_t h = groups[nbrs[e]];
+            const uint64_t t = o + (e - lo);
+            keys[t] = h >= g ? h : ng;
+            eidx[t] = (uint32_t)t;
+            egrp[t] = g;
+            evals[t] = wts[e];
+        }
+    }
+}
+
+// A pair starts where the neighbour group or the row group changes (sentinels start none).
+__global__ void k_pair_heads_h(const uint32_t *keys, const uint32_t *eidx, const uint32_t *egrp, uint64_t n,
+                               uint32_t sent, uint32_t *head) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = keys[i];
+        head[i] = (k != sent && (i == 0 || k != keys[i - 1] || egrp[eidx[i]] != egrp[eidx[i - 1]])) ? 1u : 0u;
+    }
+}
+
+// Pair starts; start[np] is the first sentinel position (or n).
+__global__ void k_seg_starts_h(const uint32_t *keys, uint32_t sent, const uint32_t *head, const uint32_t *run,
+                               uint64_t n, uint32_t np, uint64_t *start) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i == n) {
+            if (n == 0 || keys[n - 1] != sent) start[np] = n;
+        } else if (keys[i] == sent) {
+            if (i == 0 || keys[i - 1] != sent) start[np] = i;
+        } else if (head[i]) {
+            start[run[i]] = i;
+        }
+    }
+}
+
+__device__ __forceinline__ void emit_pair_h(uint32_t g, uint32_t h, double acc, uint32_t r, const double *cloops,
+                                            uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep) {
+    pa[r] = g;
+    pb[r] = h;
+    keep[r] = acc > 0.0 ? 1u : 0u;  // non-positive sums are not emitted (csr_network.rs:568-570)
+    pw[r] = (g == h) ? (float)(0.5 * (acc + cloops[g])) : (float)acc;
+}
+
+// Short pairs: one thread adds the entries in their t order.
+__global__ void k_pair_sums_short_h(const uint32_t *keys, const uint32_t *eidx, const uint32_t *egrp,
+                                    const float *evals, const uint64_t *start, uint32_t np, const double *cloops,
+                                    uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep, uint32_t *long_list,
+                                    unsigned int *n_long) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x) {
+        const uint64_t b = start[r], e = start[r + 1];
+        if (e - b > PAIR_SHORT) {
+            long_list[atomicAdd(n_long, 1u)] = r;
+            continue;
+        }
+        double acc = 0.0;
+        for (uint64_t j = b; j < e; j++) acc += (double)evals[eidx[j]];
+        emit_pair_h(egrp[eidx[b]], keys[b], acc, r, cloops, pa, pb, pw, keep);
+    }
+}
+
+// Long pairs: one wave each; the tree sum equals the sequential chain whenever the f64_quantum
+// bound holds, otherwise lane 0 adds sequentially.
+__global__ void __launch_bounds__(256) k_pair_sums_long_h(const uint32_t *keys, const uint32_t *eidx,
+                                                          const uint32_t *egrp, const float *evals,
+                                                          const uint64_t *start, const uint32_t *long_list,
+                                                          const unsigned int *n_long, const double *cloops,
+                                                          uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep) {
+    const unsigned int cnt = *n_long;
+    const int lane = threadIdx.x & 63;
+    for (unsigned int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < cnt; q += gridDim.x * 4) {
+        const uint32_t r = long_list[q];
+        const uint64_t b = start[r], e = start[r + 1];
+        double acc = 0.0, sabs = 0.0;
+        int qz = 4096;
+        for (uint64_t j = b + lane; j < e; j += 64) {
+            const double w = (double)evals[eidx[j]];
+            acc += w;
+            sabs += fabs(w);
+            qz = min(qz, f64_quantum(w));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            acc += __shfl_xor(acc, off, 64);
+            sabs += __shfl_xor(sabs, off, 64);
+            qz = min(qz, __shfl_xor(qz, off, 64));
+        }
+        const bool exact = sabs < ldexp(1.0, min(53 + qz, 1000));
+        if (lane == 0) {
+            if (!exact) {
+                acc = 0.0;
+                for (uint64_t j = b; j < e; j++) acc += (double)evals[eidx[j]];
+            }
+            emit_pair_h(egrp[eidx[b]], keys[b], acc, r, cloops, pa, pb, pw, keep);
+        }
+    }
+}
+
+// Key of the second sort: the row group of a kept pair with g < h (an upper entry of row g),
+// the sentinel otherwise.
+__global__ void k_upper_keys(const uint32_t *pa, const uint32_t *pb, const uint32_t *keep, uint32_t np, uint32_t sent,
+                             uint32_t *key2, uint32_t *val2) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x) {
+        key2[r] = (keep[r] && pa[r] != pb[r]) ? pa[r] : sent;
+        val2[r] = r;
+    }
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Row r: its part A (kept pairs (g, r), g <= r) is the pair list range with pb == r, its part B
+// (pairs (r, h), h > r) the range of the second sort with key2 == r.
+__global__ void k_row_parts(const uint32_t *pb, const uint32_t *kscan, const uint32_t *key2, uint32_t np,
+                            uint32_t ng, uint32_t *a0, uint32_t *b0, uint32_t *cnt_a, uint64_t *row_len) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= ng; r += gridDim.x * blockDim.x) {
+        if (r == ng) {
+            row_len[ng] = 0;
+            continue;
+        }
+        const uint32_t x0 = lower_bound_u32(pb, np, r), x1 = lower_bound_u32(pb, np, r + 1);
+        const uint32_t y0 = lower_bound_u32(key2, np, r), y1 = lower_bound_u32(key2, np, r + 1);
+        const uint32_t ca = kscan[x1] - kscan[x0];
+        a0[r] = x0;
+        b0[r] = y0;
+        cnt_a[r] = ca;
+        row_len[r] = (uint64_t)ca + (y1 - y0);
+    }
+}
+
+__global__ void k_fill_rows(const uint32_t *pa, const uint32_t *pb, const float *pw, const uint32_t *keep,
+                            const uint32_t *kscan, const uint32_t *key2, const uint32_t *val2, uint32_t np,
+                            uint32_t sent, const uint32_t *a0, const uint32_t *b0, const uint32_t *cnt_a,
+                            const uint64_t *ptrs, uint32_t *nbrs, float *wts) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+        if (keep[i]) {
+            // entry (g, r) of row r = pb[i], at its rank among the row's kept part-A pairs
+            const uint32_t r = pb[i];
+            const uint64_t o = ptrs[r] + (kscan[i] - kscan[a0[r]]);
+            nbrs[o] = pa[i];
+            wts[o] = pw[i];
+        }
+        const uint32_t g = key2[i];
+        if (g != sent) {
+            // entry (g, h) of row g, after the row's part A
+            const uint32_t q = val2[i];
+            const uint64_t o = ptrs[g] + cnt_a[g] + (i - b0[g]);
+            nbrs[o] = pb[q];
+            wts[o] = pw[q];
+        }
+    }
+}
+
+std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *groups, uint32_t ng, hipStream_t s) {
+    const uint32_t n = (uint32_t)L.n;
+    const uint64_t nnz = L.nnz;
+    if (nnz >= 0xFFFFFFFFull || ng >= 0x7FFFFFFFu) return aggregate_device_wide(L, groups, ng, s);
+    auto out = std::make_unique<sc_graph>();
+    out->n = ng;
+    // members grouped by group, ascending vertex within a group (stable sort)
+    DevBuf<uint32_t> sg(n), members(n);
+    hipLaunchKernelGGL(k_pairs_for_sort, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups, n, sg.get(), members.get());
+    KERNEL_OK();
+    radix_sort_pairs(sg, members, n, std::max(1, bits_for(ng)), s);
+    DevBuf<uint64_t> starts(ng + 1);
+    hipLaunchKernelGGL(k_group_starts, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, sg.get(), n, ng, starts.get());
+    KERNEL_OK();
+    out->node_w.resize(ng);
+    DevBuf<double> cloops(ng);
+    hipLaunchKernelGGL(k_group_sums, dim3(grid_for(ng, 16, 4096)), dim3(256), 0, s, starts.get(), members.get(), ng,
+                       L.node_w.get(), L.self_loop.get(), out->node_w.get(), cloops.get());
+    KERNEL_OK();
+    DevBuf<uint64_t> deg(n + 1), moff(n + 1);
+    hipLaunchKernelGGL(k_member_degrees, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, members.get(), n,
+                       L.ptrs.get(), deg.get());
+    KERNEL_OK();
+    exclusive_scan_u64(deg.get(), moff.get(), n + 1, s);
+    const uint32_t sent = ng;
+    const int kbits = std::max(1, bits_for(ng));
+    DevBuf<uint32_t> keys(nnz), eidx(nnz), egrp(nnz);
+    DevBuf<float> evals(nnz);
+    uint32_t np = 0;
+    DevBuf<uint32_t> head(nnz + 1), run(nnz + 1);
+    if (nnz) {
+        hipLaunchKernelGGL((k_emit_fine_h<16>), dim3(grid_for(n, 16, 2048)), dim3(256), 0, s, members.get(), sg.get(),
+                           n, moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, ng, keys.get(), eidx.get(),
+                           egrp.get(), evals.get());
+        KERNEL_OK();
+        radix_sort_pairs(keys, eidx, nnz, kbits, s);
+        hipLaunchKernelGGL(k_pair_heads_h, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), eidx.get(),
+                           egrp.get(), nnz, sent, head.get());
+        KERNEL_OK();
+        exclusive_scan_u32(head.get(), run.get(), nnz, s);
+        uint32_t lr = 0, lh = 0;
+        to_host(&lr, run.get() + nnz - 1, 1, s);
+        to_host(&lh, head.get() + nnz - 1, 1, s);
+        sync_stream(s);
+        np = lr + lh;
+    }
+    DevBuf<uint32_t> pa(np + 1), pb(np + 1), keep(np + 1), kscan(np + 1), key2(np + 1), val2(np + 1);
+    DevBuf<float> pw(np + 1);
+    DevBuf<uint32_t> a0(ng + 1), b0(ng + 1), cnt_a(ng + 1);
+    DevBuf<uint64_t> row_len(ng + 1);
+    out->ptrs.resize((uint64_t)ng + 1);
+    if (np) {
+        DevBuf<uint64_t> seg(np + 1);
+        DevBuf<uint32_t> long_list(np);
+        DevBuf<unsigned int> n_long(1);
+        HIP_OK(hipMemsetAsync(n_long.get(), 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_seg_starts_h, dim3(grid_for(nnz + 1, 256, 8192)), dim3(256), 0, s, keys.get(), sent,
+                           head.get(), run.get(), nnz, np, seg.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_pair_sums_short_h, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, keys.get(), eidx.get(),
+                           egrp.get(), evals.get(), seg.get(), np, cloops.get(), pa.get(), pb.get(), pw.get(),
+                           keep.get(), long_list.get(), n_long.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_pair_sums_long_h, dim3(1024), dim3(256), 0, s, keys.get(), eidx.get(), egrp.get(),
+                           evals.get(), seg.get(), long_list.get(), n_long.get(), cloops.get(), pa.get(), pb.get(),
+                           pw.get(), keep.get());
+        KERNEL_OK();
+        // kscan[np] = kept pairs in total
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, keep.get() + np, (uint64_t)1, 0u);
+        KERNEL_OK();
+        exclusive_scan_u32(keep.get(), kscan.get(), (uint64_t)np + 1, s);
+        hipLaunchKernelGGL(k_upper_keys, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, pa.get(), pb.get(),
+                           keep.get(), np, sent, key2.get(), val2.get());
+        KERNEL_OK();
+        radix_sort_pairs(key2, val2, np, kbits, s);
+    }
+    hipLaunchKernelGGL(k_row_parts, dim3(grid_for((uint64_t)ng + 1, 256, 8192)), dim3(256), 0, s, pb.get(),
+                       kscan.get(), key2.get(), np, ng, a0.get(), b0.get(), cnt_a.get(), row_len.get());
+    KERNEL_OK();
+    exclusive_scan_u64(row_len.get(), out->ptrs.get(), (uint64_t)ng + 1, s);
+    uint64_t total = 0;
+    to_host(&total, out->ptrs.get() + ng, 1, s);
+    sync_stream(s);
+    out->nnz = total;
+    out->nbrs.resize(total);
+    out->wts.resize(total);
+    if (np) {
+        hipLaunchKernelGGL(k_fill_rows, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, pa.get(), pb.get(), pw.get(),
+                           keep.get(), kscan.get(), key2.get(), val2.get(), np, sent, a0.get(), b0.get(), cnt_a.get(),
+                           out->ptrs.get(), out->nbrs.get(), out->wts.get());
+        KERNEL_OK();
+    }
     graph_finish_sorted(*out, s);
     return out;
 }

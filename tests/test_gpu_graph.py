@@ -119,6 +119,62 @@ def test_aggregate_bit_exact_vs_oracle():
             assert_same_graph(g.aggregate(groups), og.aggregate(groups))
 
 
+def _csr_from_directed(n, rows, cols, vals):
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    ptrs = np.zeros(n + 1, np.uint64)
+    np.add.at(ptrs, rows + 1, 1)
+    return np.cumsum(ptrs).astype(np.uint64), cols.astype(np.uint32), vals.astype(np.float32)
+
+
+def test_aggregate_long_pairs_and_wide_exponents():
+    # few large groups: pairs with thousands of fine entries (the wave-parallel sums), weights
+    # spread over eleven decades so that many pair sums fail the exactness bound and take the
+    # sequential chain; both must equal the reference's order bit for bit
+    rng = np.random.default_rng(21)
+    n = 3000
+    m = 40000
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    w = (10.0 ** rng.uniform(-8, 3, m)).astype(np.float32)
+    loops = rng.integers(0, n, 50)
+    rows = np.concatenate([a, b, loops])
+    cols = np.concatenate([b, a, loops])
+    vals = np.concatenate([w, w, np.full(50, 0.75, np.float32)])
+    ptrs, nbrs, wts = _csr_from_directed(n, rows, cols, vals)
+    g = scgpu.CSRNetwork.from_csr_parts(ptrs, nbrs, wts)
+    og = O.Graph.from_csr_parts(ptrs, nbrs, wts)
+    for k in (1, 3, 17, 400):
+        groups = rng.integers(0, k, size=n)
+        _, groups = np.unique(groups, return_inverse=True)
+        assert_same_graph(g.aggregate(groups), og.aggregate(groups))
+
+
+def test_aggregate_balanced_asymmetric_weights():
+    # the two directions of a pair carry different weights (a permutation, so the degree sum
+    # still balances and ingest accepts it): coarse pair weights come from the lower group's
+    # side only (csr_network.rs:548-581), which a symmetric shortcut would get wrong
+    rng = np.random.default_rng(8)
+    n = 500
+    a = rng.integers(0, n, 3000)
+    b = rng.integers(0, n, 3000)
+    keep = a < b
+    a, b = a[keep], b[keep]
+    pairs = np.unique(np.stack([a, b], 1), axis=0)
+    a, b = pairs[:, 0], pairs[:, 1]
+    w_up = rng.integers(1, 9, len(a)).astype(np.float32)
+    w_dn = rng.permutation(w_up)
+    ptrs, nbrs, wts = _csr_from_directed(n, np.concatenate([a, b]), np.concatenate([b, a]),
+                                         np.concatenate([w_up, w_dn]))
+    g = scgpu.CSRNetwork.from_csr_parts(ptrs, nbrs, wts)
+    og = O.Graph.from_csr_parts(ptrs, nbrs, wts)
+    assert not og.validate_symmetry()
+    for k in (2, 9, 60, 250):
+        groups = rng.integers(0, k, size=n)
+        _, groups = np.unique(groups, return_inverse=True)
+        assert_same_graph(g.aggregate(groups), og.aggregate(groups))
+
+
 def test_modularity_and_quality_on_fixture_memberships():
     graphs, cases = load_fixture()
     built = {k: (scgpu.CSRNetwork.from_edges(n, e), O.Graph.from_edges(n, e)) for k, (n, e) in graphs.items()}
