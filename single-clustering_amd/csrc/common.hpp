@@ -162,6 +162,7 @@ void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int en
 // (of size n), so callers must re-read .get() afterwards.
 void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<float> &vals, uint64_t n, int end_bit, hipStream_t s);
 void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals, uint64_t n, int end_bit, hipStream_t s);
+void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint64_t> &vals, uint64_t n, int end_bit, hipStream_t s);
 // Deterministic sum of n doubles (fixed two-level tree).
 double device_sum_f64(const double *in, uint64_t n, hipStream_t s);
 double device_max_f64(const double *in, uint64_t n, hipStream_t s);

@@ -2331,16 +2331,21 @@ static std::unique_ptr<sc_graph> aggregate_device_wide(const sc_graph &L, const 
 // The reference walks the groups in order and, per group g, adds the weights of its members'
 // entries (members ascending, rows ascending) towards groups h >= g (csr_network.rs:548-564).
 // Here every fine entry t (member order) is keyed by its neighbour group h (h >= g), or by the
-// sentinel ng; a stable sort by h leaves each pair's entries contiguous and in t order, with
-// the pairs ordered by (h, g). So the pair list directly gives each row r its entries (g, r),
-// g <= r, ascending in g; a second stable sort of the pairs with g < h by g gives each row its
-// entries (r, h), h > r, ascending in h. Both sorts use 32-bit keys of bits_for(ng) bits.
+// sentinel ng, and carries (g, w) packed in 64 bits; a stable sort by h leaves each pair's
+// entries contiguous and in t order, with the pairs ordered by (h, g). So the pair list directly
+// gives each row r its entries (g, r), g <= r, ascending in g; a second stable sort of the
+// pairs with g < h by g (carrying (h, w)) gives each row its entries (r, h), h > r, ascending in
+// h. Both sorts use 32-bit keys of bits_for(ng) bits.
+
+__device__ __forceinline__ uint64_t pack_gw(uint32_t g, float w) { return ((uint64_t)g << 32) | __float_as_uint(w); }
+__device__ __forceinline__ uint32_t gw_group(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ float gw_weight(uint64_t x) { return __uint_as_float((uint32_t)x); }
 
 template <int G>
 __global__ void __launch_bounds__(256) k_emit_fine_h(const uint32_t *members, const uint32_t *sorted_groups, uint32_t n,
                                                      const uint64_t *moff, const uint64_t *ptrs, const uint32_t *nbrs,
                                                      const float *wts, const uint32_t *groups, uint32_t ng,
-                                                     uint32_t *keys, uint32_t *eidx, uint32_t *egrp, float *evals) {
+                                                     uint32_t *keys, uint64_t *gw) {
     constexpr int NG = 256 / G;
     const int lane = threadIdx.x % G;
     for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
@@ -2351,19 +2356,16 @@ __global__ void __launch_bounds__(256) k_emit_fine_h(const uint32_t *members, co
             const uint32_t h = groups[nbrs[e]];
             const uint64_t t = o + (e - lo);
             keys[t] = h >= g ? h : ng;
-            eidx[t] = (uint32_t)t;
-            egrp[t] = g;
-            evals[t] = wts[e];
+            gw[t] = pack_gw(g, wts[e]);
         }
     }
 }
 
 // A pair starts where the neighbour group or the row group changes (sentinels start none).
-__global__ void k_pair_heads_h(const uint32_t *keys, const uint32_t *eidx, const uint32_t *egrp, uint64_t n,
-                               uint32_t sent, uint32_t *head) {
+__global__ void k_pair_heads_h(const uint32_t *keys, const uint64_t *gw, uint64_t n, uint32_t sent, uint32_t *head) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t k = keys[i];
-        head[i] = (k != sent && (i == 0 || k != keys[i - 1] || egrp[eidx[i]] != egrp[eidx[i - 1]])) ? 1u : 0u;
+        head[i] = (k != sent && (i == 0 || k != keys[i - 1] || gw_group(gw[i]) != gw_group(gw[i - 1]))) ? 1u : 0u;
     }
 }
 
@@ -2390,10 +2392,9 @@ __device__ __forceinline__ void emit_pair_h(uint32_t g, uint32_t h, double acc, 
 }
 
 // Short pairs: one thread adds the entries in their t order.
-__global__ void k_pair_sums_short_h(const uint32_t *keys, const uint32_t *eidx, const uint32_t *egrp,
-                                    const float *evals, const uint64_t *start, uint32_t np, const double *cloops,
-                                    uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep, uint32_t *long_list,
-                                    unsigned int *n_long) {
+__global__ void k_pair_sums_short_h(const uint32_t *keys, const uint64_t *gw, const uint64_t *start, uint32_t np,
+                                    const double *cloops, uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep,
+                                    uint32_t *long_list, unsigned int *n_long) {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x) {
         const uint64_t b = start[r], e = start[r + 1];
         if (e - b > PAIR_SHORT) {
@@ -2401,27 +2402,37 @@ __global__ void k_pair_sums_short_h(const uint32_t *keys, const uint32_t *eidx, 
             continue;
         }
         double acc = 0.0;
-        for (uint64_t j = b; j < e; j++) acc += (double)evals[eidx[j]];
-        emit_pair_h(egrp[eidx[b]], keys[b], acc, r, cloops, pa, pb, pw, keep);
+        for (uint64_t j = b; j < e; j++) acc += (double)gw_weight(gw[j]);
+        emit_pair_h(gw_group(gw[b]), keys[b], acc, r, cloops, pa, pb, pw, keep);
     }
 }
 
-// Long pairs: one wave each; the tree sum equals the sequential chain whenever the f64_quantum
-// bound holds, otherwise lane 0 adds sequentially.
-__global__ void __launch_bounds__(256) k_pair_sums_long_h(const uint32_t *keys, const uint32_t *eidx,
-                                                          const uint32_t *egrp, const float *evals,
+// Long pairs: one workgroup each; the tree sum equals the sequential chain whenever the
+// f64_quantum bound holds, otherwise thread 0 adds sequentially.
+__global__ void __launch_bounds__(256) k_pair_sums_long_h(const uint32_t *keys, const uint64_t *gw,
                                                           const uint64_t *start, const uint32_t *long_list,
                                                           const unsigned int *n_long, const double *cloops,
                                                           uint32_t *pa, uint32_t *pb, float *pw, uint32_t *keep) {
+    __shared__ double s_acc[4], s_abs[4];
+    __shared__ int s_q[4];
     const unsigned int cnt = *n_long;
-    const int lane = threadIdx.x & 63;
-    for (unsigned int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < cnt; q += gridDim.x * 4) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (unsigned int q = blockIdx.x; q < cnt; q += gridDim.x) {
         const uint32_t r = long_list[q];
         const uint64_t b = start[r], e = start[r + 1];
         double acc = 0.0, sabs = 0.0;
         int qz = 4096;
-        for (uint64_t j = b + lane; j < e; j += 64) {
-            const double w = (double)evals[eidx[j]];
+        // four independent loads in flight per thread
+        uint64_t j = b + threadIdx.x;
+        for (; j + 768 < e; j += 1024) {
+            const double w0 = (double)gw_weight(gw[j]), w1 = (double)gw_weight(gw[j + 256]);
+            const double w2 = (double)gw_weight(gw[j + 512]), w3 = (double)gw_weight(gw[j + 768]);
+            acc += (w0 + w1) + (w2 + w3);
+            sabs += (fabs(w0) + fabs(w1)) + (fabs(w2) + fabs(w3));
+            qz = min(qz, min(min(f64_quantum(w0), f64_quantum(w1)), min(f64_quantum(w2), f64_quantum(w3))));
+        }
+        for (; j < e; j += 256) {
+            const double w = (double)gw_weight(gw[j]);
             acc += w;
             sabs += fabs(w);
             qz = min(qz, f64_quantum(w));
@@ -2432,24 +2443,33 @@ __global__ void __launch_bounds__(256) k_pair_sums_long_h(const uint32_t *keys, 
             sabs += __shfl_xor(sabs, off, 64);
             qz = min(qz, __shfl_xor(qz, off, 64));
         }
-        const bool exact = sabs < ldexp(1.0, min(53 + qz, 1000));
         if (lane == 0) {
-            if (!exact) {
-                acc = 0.0;
-                for (uint64_t j = b; j < e; j++) acc += (double)evals[eidx[j]];
-            }
-            emit_pair_h(egrp[eidx[b]], keys[b], acc, r, cloops, pa, pb, pw, keep);
+            s_acc[wave] = acc;
+            s_abs[wave] = sabs;
+            s_q[wave] = qz;
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            acc = (s_acc[0] + s_acc[1]) + (s_acc[2] + s_acc[3]);
+            sabs = (s_abs[0] + s_abs[1]) + (s_abs[2] + s_abs[3]);
+            qz = min(min(s_q[0], s_q[1]), min(s_q[2], s_q[3]));
+            if (!(sabs < ldexp(1.0, min(53 + qz, 1000)))) {
+                acc = 0.0;
+                for (uint64_t i = b; i < e; i++) acc += (double)gw_weight(gw[i]);
+            }
+            emit_pair_h(gw_group(gw[b]), keys[b], acc, r, cloops, pa, pb, pw, keep);
+        }
+        __syncthreads();
     }
 }
 
 // Key of the second sort: the row group of a kept pair with g < h (an upper entry of row g),
-// the sentinel otherwise.
-__global__ void k_upper_keys(const uint32_t *pa, const uint32_t *pb, const uint32_t *keep, uint32_t np, uint32_t sent,
-                             uint32_t *key2, uint32_t *val2) {
+// the sentinel otherwise; the payload carries (h, w).
+__global__ void k_upper_keys(const uint32_t *pa, const uint32_t *pb, const float *pw, const uint32_t *keep,
+                             uint32_t np, uint32_t sent, uint32_t *key2, uint64_t *hw) {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x) {
         key2[r] = (keep[r] && pa[r] != pb[r]) ? pa[r] : sent;
-        val2[r] = r;
+        hw[r] = pack_gw(pb[r], pw[r]);
     }
 }
 
@@ -2483,7 +2503,7 @@ __global__ void k_row_parts(const uint32_t *pb, const uint32_t *kscan, const uin
 }
 
 __global__ void k_fill_rows(const uint32_t *pa, const uint32_t *pb, const float *pw, const uint32_t *keep,
-                            const uint32_t *kscan, const uint32_t *key2, const uint32_t *val2, uint32_t np,
+                            const uint32_t *kscan, const uint32_t *key2, const uint64_t *hw, uint32_t np,
                             uint32_t sent, const uint32_t *a0, const uint32_t *b0, const uint32_t *cnt_a,
                             const uint64_t *ptrs, uint32_t *nbrs, float *wts) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
@@ -2497,10 +2517,10 @@ __global__ void k_fill_rows(const uint32_t *pa, const uint32_t *pb, const float 
         const uint32_t g = key2[i];
         if (g != sent) {
             // entry (g, h) of row g, after the row's part A
-            const uint32_t q = val2[i];
+            const uint64_t x = hw[i];
             const uint64_t o = ptrs[g] + cnt_a[g] + (i - b0[g]);
-            nbrs[o] = pb[q];
-            wts[o] = pw[q];
+            nbrs[o] = gw_group(x);
+            wts[o] = gw_weight(x);
         }
     }
 }
@@ -2531,18 +2551,17 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     exclusive_scan_u64(deg.get(), moff.get(), n + 1, s);
     const uint32_t sent = ng;
     const int kbits = std::max(1, bits_for(ng));
-    DevBuf<uint32_t> keys(nnz), eidx(nnz), egrp(nnz);
-    DevBuf<float> evals(nnz);
+    DevBuf<uint32_t> keys(nnz);
+    DevBuf<uint64_t> gw(nnz);
     uint32_t np = 0;
     DevBuf<uint32_t> head(nnz + 1), run(nnz + 1);
     if (nnz) {
         hipLaunchKernelGGL((k_emit_fine_h<16>), dim3(grid_for(n, 16, 2048)), dim3(256), 0, s, members.get(), sg.get(),
-                           n, moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, ng, keys.get(), eidx.get(),
-                           egrp.get(), evals.get());
+                           n, moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, ng, keys.get(), gw.get());
         KERNEL_OK();
-        radix_sort_pairs(keys, eidx, nnz, kbits, s);
-        hipLaunchKernelGGL(k_pair_heads_h, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), eidx.get(),
-                           egrp.get(), nnz, sent, head.get());
+        radix_sort_pairs(keys, gw, nnz, kbits, s);
+        hipLaunchKernelGGL(k_pair_heads_h, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), gw.get(), nnz,
+                           sent, head.get());
         KERNEL_OK();
         exclusive_scan_u32(head.get(), run.get(), nnz, s);
         uint32_t lr = 0, lh = 0;
@@ -2551,7 +2570,8 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         sync_stream(s);
         np = lr + lh;
     }
-    DevBuf<uint32_t> pa(np + 1), pb(np + 1), keep(np + 1), kscan(np + 1), key2(np + 1), val2(np + 1);
+    DevBuf<uint32_t> pa(np + 1), pb(np + 1), keep(np + 1), kscan(np + 1), key2(np + 1);
+    DevBuf<uint64_t> hw(np + 1);
     DevBuf<float> pw(np + 1);
     DevBuf<uint32_t> a0(ng + 1), b0(ng + 1), cnt_a(ng + 1);
     DevBuf<uint64_t> row_len(ng + 1);
@@ -2564,22 +2584,21 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         hipLaunchKernelGGL(k_seg_starts_h, dim3(grid_for(nnz + 1, 256, 8192)), dim3(256), 0, s, keys.get(), sent,
                            head.get(), run.get(), nnz, np, seg.get());
         KERNEL_OK();
-        hipLaunchKernelGGL(k_pair_sums_short_h, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, keys.get(), eidx.get(),
-                           egrp.get(), evals.get(), seg.get(), np, cloops.get(), pa.get(), pb.get(), pw.get(),
-                           keep.get(), long_list.get(), n_long.get());
+        hipLaunchKernelGGL(k_pair_sums_short_h, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, keys.get(), gw.get(),
+                           seg.get(), np, cloops.get(), pa.get(), pb.get(), pw.get(), keep.get(), long_list.get(),
+                           n_long.get());
         KERNEL_OK();
-        hipLaunchKernelGGL(k_pair_sums_long_h, dim3(1024), dim3(256), 0, s, keys.get(), eidx.get(), egrp.get(),
-                           evals.get(), seg.get(), long_list.get(), n_long.get(), cloops.get(), pa.get(), pb.get(),
-                           pw.get(), keep.get());
+        hipLaunchKernelGGL(k_pair_sums_long_h, dim3(2048), dim3(256), 0, s, keys.get(), gw.get(), seg.get(),
+                           long_list.get(), n_long.get(), cloops.get(), pa.get(), pb.get(), pw.get(), keep.get());
         KERNEL_OK();
         // kscan[np] = kept pairs in total
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, keep.get() + np, (uint64_t)1, 0u);
         KERNEL_OK();
         exclusive_scan_u32(keep.get(), kscan.get(), (uint64_t)np + 1, s);
-        hipLaunchKernelGGL(k_upper_keys, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, pa.get(), pb.get(),
-                           keep.get(), np, sent, key2.get(), val2.get());
+        hipLaunchKernelGGL(k_upper_keys, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, pa.get(), pb.get(), pw.get(),
+                           keep.get(), np, sent, key2.get(), hw.get());
         KERNEL_OK();
-        radix_sort_pairs(key2, val2, np, kbits, s);
+        radix_sort_pairs(key2, hw, np, kbits, s);
     }
     hipLaunchKernelGGL(k_row_parts, dim3(grid_for((uint64_t)ng + 1, 256, 8192)), dim3(256), 0, s, pb.get(),
                        kscan.get(), key2.get(), np, ng, a0.get(), b0.get(), cnt_a.get(), row_len.get());
@@ -2593,7 +2612,7 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     out->wts.resize(total);
     if (np) {
         hipLaunchKernelGGL(k_fill_rows, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, pa.get(), pb.get(), pw.get(),
-                           keep.get(), kscan.get(), key2.get(), val2.get(), np, sent, a0.get(), b0.get(), cnt_a.get(),
+                           keep.get(), kscan.get(), key2.get(), hw.get(), np, sent, a0.get(), b0.get(), cnt_a.get(),
                            out->ptrs.get(), out->nbrs.get(), out->wts.get());
         KERNEL_OK();
     }

@@ -238,6 +238,9 @@ void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<float> &vals, uint64_t n, i
 void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals, uint64_t n, int end_bit, hipStream_t s) {
     sort_pairs_bufs(keys, vals, n, end_bit, s);
 }
+void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint64_t> &vals, uint64_t n, int end_bit, hipStream_t s) {
+    sort_pairs_bufs(keys, vals, n, end_bit, s);
+}
 void radix_sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t n, int end_bit, hipStream_t s) {
     sort_pairs(keys, vals, n, end_bit, s);
 }
