@@ -1175,6 +1175,13 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             __syncthreads();
             const unsigned int nm = s_mc;
             if (threadIdx.x == 0) {
+                if (A.trace && step < A.max_steps) {
+                    // traced runs: deciders and movers of every step
+                    unsigned int dec = 0;
+                    for (int b = 0; b < NUM_BUCKETS; b++) dec += s_wc[b];
+                    if (dec) atomicAdd(&A.wcnt[(uint64_t)step * NUM_BUCKETS + 0], dec);
+                    if (nm) atomicAdd(&A.wcnt[(uint64_t)step * NUM_BUCKETS + 1], nm);
+                }
                 if (!REFINE) {
                     if (nm) A.mcnt[step] = 1u;  // someone chose to move: not converged
                     total_moves += nm;
@@ -1681,6 +1688,9 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per + cleanups);
         std::vector<long long> h(nb + 1);
         to_host(h.data(), trace.get(), nb + 1, s);
+        const uint32_t ns = std::min<uint32_t>(ctl[0], A.max_steps);
+        std::vector<unsigned int> wc((size_t)ns * NUM_BUCKETS + 1);
+        if (ns) to_host(wc.data(), A.wcnt, (size_t)ns * NUM_BUCKETS, s);
         sync_stream(s);
         std::string line = std::string(REFINE ? "[sweep refine]" : "[sweep move]") + " n=" + std::to_string(n) +
                            " grid=" + std::to_string(grid) + " steps=" + std::to_string(ctl[0]) + " buckets=";
@@ -1692,6 +1702,9 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
             std::snprintf(buf, sizeof(buf), " %.1f", (double)(h[i] - h[i - 1]) / 100.0);
             line += buf;
         }
+        line += " deciders/movers:";
+        for (uint32_t i = 0; i < ns; i++)
+            line += " " + std::to_string(wc[(size_t)i * NUM_BUCKETS]) + "/" + std::to_string(wc[(size_t)i * NUM_BUCKETS + 1]);
         std::fprintf(stderr, "%s\n", line.c_str());
     }
 }
