@@ -163,8 +163,9 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--knn-cpu-rows", type=int, default=8000,
-                    help="query rows timed for the CPU kNN baseline (evenly spaced sample)")
+    ap.add_argument("--knn-cpu-rows", type=int, default=0,
+                    help="query rows timed for the CPU kNN baseline (evenly spaced sample; default 8000, "
+                         "or 1000 above 200k points)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-filter comparison")
     ap.add_argument("--no-sweep-modes", action="store_true", help="skip timing the multi-launch and graph modes")
     ap.add_argument("--filter-threshold", type=float, default=None,
@@ -383,7 +384,8 @@ def main():
         except AttributeError:
             threads = min(16, os.cpu_count() or 1)
         threads = min(64, threads)
-        n_rows = min(wl["n"], args.knn_cpu_rows)
+        # about 10 s of host work: the per-query cost grows with n, so large workloads sample fewer rows
+        n_rows = min(wl["n"], args.knn_cpu_rows or (8000 if wl["n"] <= 200_000 else 1000))
         rows = np.linspace(0, wl["n"] - 1, n_rows).astype(np.uint64)
         t0 = time.perf_counter()
         ki, kd, t_build = O.knn_kdtree(wl["x"], wl["k"], rows=rows, threads=threads, return_build_seconds=True)
