@@ -107,7 +107,9 @@ typedef struct sc_knn_stats {
     uint64_t rows_fallback;        /* rows whose certified candidate set overflowed */
     double tile_kernel_ms;
     double total_ms;
-    double flops;                  /* 2 * n^2 * d (unpadded) */
+    double flops;                  /* 2 * pairs_computed * d (d unpadded) */
+    double pairs_computed;         /* point pairs the distance tiles computed, padding included:
+                                      the brute force's npad^2, or fewer with block pruning */
 } sc_knn_stats;
 
 /* ---- library ------------------------------------------------------------------------- */

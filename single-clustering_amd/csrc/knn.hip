@@ -22,6 +22,7 @@
 #include "common.hpp"
 #include "tune.hpp"
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -54,13 +55,16 @@ __device__ __forceinline__ float bf16_bits_to_f32(unsigned short b) {
 }
 
 // One thread per element of the padded npad x Kp layout: split-BF16 copies and the f32 copy
-// (coalesced; padding rows and columns are zero).
-__global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, uint64_t total, unsigned short *Xh,
-                           unsigned short *Xl, float *Xf) {
+// (coalesced; padding rows and columns are zero). With perm, row j of the layout is point
+// perm[j], or padding where perm[j] == KNONE (the grouped layout of the pruned search, whose
+// rows number npad with perm); without it rows j < n are the points.
+__global__ void k_knn_prep(const float *X, const uint32_t *perm, uint32_t n, uint32_t d, uint32_t Kp, uint64_t total,
+                           unsigned short *Xh, unsigned short *Xl, float *Xf) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t j = (uint32_t)(e / Kp), t = (uint32_t)(e % Kp);
-        const float v = (j < n && t < d) ? X[(uint64_t)j * d + t] : 0.0f;
+        const uint32_t src = perm ? perm[j] : (j < n ? j : KNONE);
+        const float v = (src != KNONE && t < d) ? X[(uint64_t)src * d + t] : 0.0f;
         const unsigned short h = bf16_bits_rn(v);
         Xh[e] = h;
         Xl[e] = bf16_bits_rn(v - bf16_bits_to_f32(h));
@@ -69,14 +73,16 @@ __global__ void k_knn_prep(const float *X, uint32_t n, uint32_t d, uint32_t Kp, 
 }
 
 // One thread per padded point: f32 norm (+inf for padding points, so that they never pass a
-// candidate threshold), exact FP64 norm, and the largest norm (one atomic per wave).
-__global__ void k_knn_norms(const float *X, uint32_t n, uint32_t d, uint32_t npad, float *ny, double *nx64,
-                            unsigned int *ymax_bits) {
+// candidate threshold), exact FP64 norm, and the largest norm (one atomic per wave). With perm,
+// ny follows the layout's order (ny[j] of point perm[j]) and nx64 the points' own order.
+__global__ void k_knn_norms(const float *X, const uint32_t *perm, uint32_t n, uint32_t d, uint32_t npad, float *ny,
+                            double *nx64, unsigned int *ymax_bits) {
     for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < npad; j0 += gridDim.x * blockDim.x) {
         const uint32_t j = j0 + threadIdx.x;
         float m = 0.0f;
-        if (j < n) {
-            const float *x = X + (uint64_t)j * d;
+        const uint32_t src = j < npad ? (perm ? perm[j] : (j < n ? j : KNONE)) : KNONE;
+        if (src != KNONE) {
+            const float *x = X + (uint64_t)src * d;
             float s = 0.0f;
             double s64 = 0.0;
             for (uint32_t t = 0; t < d; t++) {
@@ -86,11 +92,11 @@ __global__ void k_knn_norms(const float *X, uint32_t n, uint32_t d, uint32_t npa
                 s64 = __dadd_rn(s64, __dmul_rn(dv, dv));
             }
             ny[j] = s;
-            nx64[j] = s64;
+            nx64[src] = s64;
             m = sqrtf(s) * 1.0001f;
         } else if (j < npad) {
             ny[j] = INFINITY;
-            nx64[j] = 0.0;
+            if (!perm) nx64[j] = 0.0;
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
@@ -229,10 +235,14 @@ __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float ny
     return need;
 }
 
+// Column stages: the split's contiguous range of cols_per_split points, or, for the pruned
+// search (blist non-null), every gridDim.y-th entry, from the split's index on, of this row
+// block's list of column blocks of TN points: blist[boff[row block] .. boff[row block + 1]).
 template <bool BF16, int NKC, int KM>
 __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, const float *ny, uint32_t n, uint32_t npad,
-                                                  uint32_t cols_per_split, uint32_t *cand_idx, float *cand_s) {
+                                                  uint32_t cols_per_split, const uint32_t *blist, const uint32_t *boff,
+                                                  uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
     constexpr int BCAP = KnnTileShape<NKC>::BCAP;
@@ -258,8 +268,20 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     const uint32_t row0 = blockIdx.x * KNN_BM;
     const uint32_t wrow0 = row0 + wave * 64;
     const uint32_t split = blockIdx.y;
-    const uint32_t cbeg = split * cols_per_split;
-    const uint32_t cend = min(npad, cbeg + cols_per_split);
+    uint32_t cbeg = 0, ns = 0, jb = 0;
+    const uint32_t jstep = blist ? gridDim.y : 1u;
+    if (blist) {
+        // splits take every gridDim.y-th block of the list: each split sees a sample of the near
+        // blocks, so no split's list is filled by the nearest points alone (which would leave
+        // the certificate no margin)
+        const uint32_t L0 = boff[blockIdx.x], L = boff[blockIdx.x + 1] - L0;
+        jb = L0 + split;
+        ns = L > split ? (L - split + gridDim.y - 1) / gridDim.y : 0u;
+    } else {
+        cbeg = split * cols_per_split;
+        const uint32_t cend = min(npad, cbeg + cols_per_split);
+        ns = cend > cbeg ? (cend - cbeg) / TN : 0;
+    }
 
     sThr[tid] = INFINITY;
 
@@ -335,8 +357,9 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     // block-local row of this lane's value i of row tile rt, minus 16 rt + i
     const uint32_t rl_base = wave * 64 + 4 * quarter;
     const uint64_t qmask = 0xFFFFull << (16 * quarter);
-    if (cbeg < cend) KNN_PREFETCH(cbeg);
-    for (uint32_t c0 = cbeg; c0 < cend; c0 += TN) {
+    if (ns) KNN_PREFETCH(blist ? blist[jb] * TN : cbeg);
+    for (uint32_t st = 0; st < ns; st++) {
+        const uint32_t c0 = blist ? blist[jb + st * jstep] * TN : cbeg + st * TN;
         __syncthreads();  // previous stage fully consumed
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -344,7 +367,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
             else *reinterpret_cast<u32x4 *>(sBf + dst_off[c]) = pre[c];
         }
         if (tid < TN) sNy[tid] = ny_pre;
-        if (c0 + TN < cend) KNN_PREFETCH(c0 + TN);  // lands while this stage computes
+        if (st + 1 < ns) KNN_PREFETCH(blist ? blist[jb + (st + 1) * jstep] * TN : c0 + TN);  // lands while this stage computes
         __syncthreads();
 
         // two accumulator sets: the products of sub-tile t + 1 are issued before the threshold
@@ -563,24 +586,420 @@ __global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uin
     }
 }
 
+// ---- pruned search: spatial grouping and block bounds -----------------------------------------
+// The points are reordered so that nearby points are consecutive (k-means groups), every block
+// of consecutive points gets a bounding sphere, and a row block R skips the column blocks B that
+// provably hold none of its rows' k nearest points: R holds at least k + 1 points, so every row
+// x of R has k other points within 2 rR (rR: R's radius), while every y in B is at least
+// |cR - cB| - rR - rB away; B is skipped when |cR - cB| - rR - rB > 2 rR. A skipped point is
+// therefore strictly farther than the k-th nearest and cannot tie with it: the result is the
+// brute-force result. Only the amount of skipped work depends on the grouping.
+constexpr int KMC = 64;  // centroids per LDS chunk of the assignment
+
+__global__ void k_fill_u32_knn(uint32_t *p, uint32_t n, uint32_t x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = x;
+}
+
+__global__ void k_knn_iota(uint32_t *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
+// Initial centroids: C points spread evenly over the input order; f32 squared norms.
+__global__ void k_kmeans_init(const float *X, uint32_t n, uint32_t d, uint32_t C, float *cen) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < (uint64_t)C * d;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = (uint32_t)(e / d), t = (uint32_t)(e % d);
+        const uint64_t src = (uint64_t)c * n / C;
+        cen[e] = X[src * d + t];
+    }
+}
+
+__global__ void k_kmeans_norms(const float *cen, uint32_t d, uint32_t C, float *cnorm) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (uint32_t t = 0; t < d; t++) s = fmaf(cen[(uint64_t)c * d + t], cen[(uint64_t)c * d + t], s);
+        cnorm[c] = s;
+    }
+}
+
+// Nearest centroid of every point by |c|^2 - 2 x.c (f32; ties to the lower centroid). The
+// centroids pass through LDS in chunks of KMC, transposed so a thread reads one value per FMA.
+__global__ void __launch_bounds__(256) k_kmeans_assign(const float *X, uint32_t n, uint32_t d, const float *cen,
+                                                       const float *cnorm, uint32_t C, uint32_t *assign) {
+    __shared__ float sc[KMC * 128];
+    __shared__ float sn[KMC];
+    for (uint32_t i0 = blockIdx.x * 256u; i0 < n; i0 += gridDim.x * 256u) {  // workgroup-uniform
+        const uint32_t i = i0 + threadIdx.x;
+        const bool ok = i < n;
+        float best = INFINITY;
+        uint32_t bid = 0;
+        for (uint32_t c0 = 0; c0 < C; c0 += KMC) {
+            const uint32_t cc = min((uint32_t)KMC, C - c0);
+            __syncthreads();
+            for (uint32_t e = threadIdx.x; e < cc * d; e += 256) {
+                const uint32_t c = e / d, t = e % d;
+                sc[t * KMC + c] = cen[(uint64_t)(c0 + c) * d + t];
+            }
+            if (threadIdx.x < cc) sn[threadIdx.x] = cnorm[c0 + threadIdx.x];
+            __syncthreads();
+            if (ok) {
+                float acc[KMC];
+#pragma unroll
+                for (int c = 0; c < KMC; c++) acc[c] = 0.0f;
+                const float *x = X + (uint64_t)i * d;
+                for (uint32_t t = 0; t < d; t++) {
+                    const float xv = x[t];
+#pragma unroll
+                    for (int c = 0; c < KMC; c++) acc[c] = fmaf(xv, sc[t * KMC + c], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < KMC; c++) {
+                    const float dist = sn[c] - 2.0f * acc[c];
+                    if ((uint32_t)c < cc && dist < best) {
+                        best = dist;
+                        bid = c0 + (uint32_t)c;
+                    }
+                }
+            }
+        }
+        if (ok) assign[i] = bid;
+    }
+}
+
+// start[g] = first position of the sorted keys holding a key >= g (g = 0..G).
+__global__ void k_key_starts_u32(const uint32_t *keys, uint32_t n, uint32_t G, uint32_t *start) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (keys[mid] < g) lo = mid + 1;
+            else hi = mid;
+        }
+        start[g] = lo;
+    }
+}
+
+__device__ __forceinline__ double block_sum_f64(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__device__ __forceinline__ double block_max_f64(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Centroid of every group: the mean of its members (positions start[g] .. start[g + 1] of perm);
+// an empty group keeps its centroid. One workgroup per group: thread (slice, dimension) adds the
+// dimension over every slices-th member, then the slices' sums are added in LDS (the order only
+// moves the grouping, never a result).
+__global__ void __launch_bounds__(256) k_kmeans_update(const float *X, uint32_t d, const uint32_t *perm,
+                                                       const uint32_t *start, uint32_t C, float *cen) {
+    __shared__ float part[256];
+    const uint32_t slices = 256 / d;  // d <= 128
+    const uint32_t sl = threadIdx.x / d, t = threadIdx.x % d;
+    for (uint32_t g = blockIdx.x; g < C; g += gridDim.x) {
+        const uint32_t b = start[g], e = start[g + 1];
+        if (b == e) continue;  // workgroup-uniform
+        float acc = 0.0f, acc2 = 0.0f;
+        if (sl < slices) {
+            uint32_t j = b + sl;
+            for (; j + slices < e; j += 2 * slices) {  // two loads in flight
+                acc += X[(uint64_t)perm[j] * d + t];
+                acc2 += X[(uint64_t)perm[j + slices] * d + t];
+            }
+            if (j < e) acc += X[(uint64_t)perm[j] * d + t];
+        }
+        acc += acc2;
+        __syncthreads();
+        part[threadIdx.x] = acc;
+        __syncthreads();
+        if (threadIdx.x < d) {
+            float sum = 0.0f;
+            for (uint32_t q = 0; q < slices; q++) sum += part[q * d + threadIdx.x];
+            cen[(uint64_t)g * d + threadIdx.x] = sum / (float)(e - b);
+        }
+    }
+}
+
+// Bounding sphere of every block of bsz (<= 256) consecutive layout positions (n of them,
+// perm[j] == KNONE for padding): centre (mean of the block's points, f64) and radius (the
+// largest distance to it, rounded up so that it bounds every point's exact distance); cnt =
+// points in the block.
+__global__ void __launch_bounds__(256) k_block_spheres(const float *X, const uint32_t *perm, uint32_t n, uint32_t d,
+                                                       uint32_t bsz, uint32_t nblocks, double *cen, double *rad,
+                                                       uint32_t *cnt) {
+    __shared__ double sc[128];
+    __shared__ double red[4];
+    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint64_t j = (uint64_t)b * bsz + threadIdx.x;
+        const uint32_t src = (threadIdx.x < bsz && j < n) ? perm[j] : KNONE;  // n: layout rows here
+        const bool real = src != KNONE;
+        const double c = block_sum_f64(real ? 1.0 : 0.0, red);
+        for (uint32_t t = 0; t < d; t++) {
+            const double v = block_sum_f64(real ? (double)X[(uint64_t)src * d + t] : 0.0, red);
+            if (threadIdx.x == 0) sc[t] = c > 0.0 ? v / c : 0.0;
+        }
+        __syncthreads();
+        double r2 = 0.0;
+        if (real)
+            for (uint32_t t = 0; t < d; t++) {
+                const double df = (double)X[(uint64_t)src * d + t] - sc[t];
+                r2 += df * df;
+            }
+        r2 = block_max_f64(r2, red);
+        if (threadIdx.x == 0) {
+            rad[b] = c > 0.0 ? sqrt(r2) * (1.0 + 1e-12) + 1e-300 : 0.0;
+            cnt[b] = (uint32_t)c;
+        }
+        for (uint32_t t = threadIdx.x; t < d; t += 256) cen[(uint64_t)b * d + t] = sc[t];
+        __syncthreads();
+    }
+}
+
+// Column blocks each row block needs, in the order of a sweep that starts at the row block's
+// own columns and wraps around: B is skipped when |cR - cB| - rR - rB > uR, uR bounding every
+// row's k-th nearest distance (k_block_kth; infinite when the block holds too few points).
+// Count pass (blist null): counts[R]; fill pass: blist[boff[R] + rank] = B, in sweep order.
+__global__ void __launch_bounds__(256) k_need_lists(const double *cR, const double *rR, const double *uR,
+                                                    const double *cB, const double *rB, const uint32_t *nB,
+                                                    uint32_t d, uint32_t nrb, uint32_t ncb, uint32_t bstep,
+                                                    uint32_t *counts, const uint32_t *boff, uint32_t *blist) {
+    __shared__ double sc[128];
+    __shared__ uint32_t s_w[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < d; t += 256) sc[t] = cR[(uint64_t)R * d + t];
+        __syncthreads();
+        const double rr = rR[R], ur = uR[R];
+        const bool all = !(ur < INFINITY);
+        const uint32_t B0 = (uint32_t)(((uint64_t)R * bstep) % ncb);
+        uint32_t total = 0;
+        if (ur >= 0.0) {  // negative: the block holds no points
+            for (uint32_t t0 = 0; t0 < ncb; t0 += 256) {  // workgroup-uniform
+                const uint32_t t = t0 + threadIdx.x;
+                bool need = false;
+                uint32_t B = 0;
+                if (t < ncb) {
+                    B = B0 + t;
+                    if (B >= ncb) B -= ncb;
+                    if (nB[B] > 0) {
+                        if (all) {
+                            need = true;
+                        } else {
+                            double s2 = 0.0;
+                            for (uint32_t u = 0; u < d; u++) {
+                                const double df = sc[u] - cB[(uint64_t)B * d + u];
+                                s2 += df * df;
+                            }
+                            const double dist = sqrt(s2) * (1.0 - 1e-12);
+                            need = !(dist > (rr + rB[B] + ur) * (1.0 + 1e-12));
+                        }
+                    }
+                }
+                const uint64_t m = __ballot(need);
+                if (lane == 0) s_w[wave] = (uint32_t)__popcll(m);
+                __syncthreads();
+                uint32_t before = 0;
+                for (int w = 0; w < wave; w++) before += s_w[w];
+                const uint32_t wtot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+                if (blist && need)
+                    blist[boff[R] + total + before +
+                          __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = B;
+                total += wtot;
+                __syncthreads();
+            }
+        }
+        if (!blist && threadIdx.x == 0) counts[R] = total;
+    }
+}
+
+// Candidate lists of layout row j (per_row entries; rows layout rows) -> the point's own row,
+// columns mapped through perm, so the rescore works on the points in their own order and index
+// ties are broken on the points' own indices. Padding rows are dropped.
+__global__ void k_knn_remap(const uint32_t *perm, uint32_t rows, uint32_t per_row, const uint32_t *cidx,
+                            const float *cs, uint32_t *oidx, float *os) {
+    const uint64_t total = (uint64_t)rows * per_row;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(e / per_row), q = (uint32_t)(e % per_row);
+        const uint32_t src = perm[j];
+        if (src == KNONE) continue;  // padding row
+        const uint32_t c = cidx[e];
+        const uint64_t o = (uint64_t)src * per_row + q;
+        oidx[o] = c < rows ? perm[c] : KNONE;
+        os[o] = cs[e];
+    }
+}
+
+// uR of every row block: an upper bound, over its points, of the distance to the point's k-th
+// nearest other point within a window of two row blocks (the block and the one before it, or
+// after it for the first block), so every point of the block has k points within uR; +inf when
+// the window holds k other points or fewer, -1 when the block holds none. Distances are summed in f32 from
+// the stored f32 coordinates: every term is non-negative, so the sum's relative error is below
+// (d + 2) 2^-24 and uR inflates the largest k-th squared distance by 1e-4 before the square
+// root. One workgroup per row block, one thread per row with its coordinates in registers; the
+// block's points pass through LDS in chunks of 64.
+constexpr int KTH_MAXK = 32;
+template <int DMAX>
+__global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_t *perm, uint32_t d, uint32_t k,
+                                                   uint32_t nrb, double *uR) {
+    __shared__ float sp[64 * DMAX];
+    __shared__ uint32_t sv[64];
+    __shared__ double red[4];
+    for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {
+        const uint32_t my = perm[(uint64_t)R * 256 + threadIdx.x];
+        const bool real = my != KNONE;
+        const double cnt = block_sum_f64(real ? 1.0 : 0.0, red);
+        float xr[DMAX];
+#pragma unroll
+        for (int t = 0; t < DMAX; t++) xr[t] = (real && (uint32_t)t < d) ? X[(uint64_t)my * d + t] : 0.0f;
+        float best[KTH_MAXK];
+#pragma unroll
+        for (int q = 0; q < KTH_MAXK; q++) best[q] = INFINITY;
+        float kcur = INFINITY;  // best[k - 1]
+        // window positions: this block's and the neighbour block's (when there is one)
+        const uint64_t own = (uint64_t)R * 256;
+        const uint64_t nb = nrb < 2 ? own : (R > 0 ? own - 256 : own + 256);
+        const uint32_t wlen = nrb < 2 ? 256u : 512u;
+        for (uint32_t c0 = 0; c0 < wlen; c0 += 64) {
+            const uint64_t base = c0 < 256 ? own + c0 : nb + (c0 - 256);
+            __syncthreads();
+            for (uint32_t e = threadIdx.x; e < 64 * DMAX; e += 256) {
+                const uint32_t c = e / DMAX, t = e % DMAX;
+                const uint32_t src = perm[base + c];
+                sp[e] = (src != KNONE && t < d) ? X[(uint64_t)src * d + t] : 0.0f;
+            }
+            if (threadIdx.x < 64) sv[threadIdx.x] = perm[base + threadIdx.x];
+            __syncthreads();
+            for (uint32_t c = 0; c < 64; c++) {
+                float s2 = 0.0f;
+#pragma unroll
+                for (int t = 0; t < DMAX; t++) {
+                    const float df = xr[t] - sp[c * DMAX + t];
+                    s2 = fmaf(df, df, s2);
+                }
+                const bool cand = real && sv[c] != KNONE && !(c0 < 256 && c0 + c == threadIdx.x) && s2 < kcur;
+                if (__any(cand)) {
+                    float v = cand ? s2 : INFINITY;
+#pragma unroll
+                    for (int q = 0; q < KTH_MAXK; q++) {
+                        const bool sw = v < best[q];
+                        const float tmp = best[q];
+                        best[q] = sw ? v : tmp;
+                        v = sw ? tmp : v;
+                    }
+#pragma unroll
+                    for (int q = 0; q < KTH_MAXK; q++)
+                        if ((uint32_t)q == k - 1) kcur = best[q];
+                }
+            }
+        }
+        const double m = block_max_f64(real ? (double)kcur : 0.0, red);
+        if (threadIdx.x == 0) {
+            if (cnt < 1.0) uR[R] = -1.0;
+            else if (!(m < INFINITY)) uR[R] = INFINITY;  // some row has k other points or fewer in the window
+            else uR[R] = sqrt(m * (1.0 + 1e-4)) + 1e-30;
+        }
+    }
+}
+
+// Grouped layout: group g (sorted positions start[g] .. start[g + 1] of perm) occupies layout
+// positions poff[g] .. poff[g] + size, padded to whole row blocks; other positions are KNONE.
+__global__ void k_layout_fill(const uint32_t *sorted_keys, const uint32_t *perm, const uint32_t *start,
+                              const uint32_t *poff, uint32_t n, uint32_t *layout) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t g = sorted_keys[p];
+        layout[poff[g] + (p - start[g])] = perm[p];
+    }
+}
+
+__global__ void k_padded_sizes(const uint32_t *start, uint32_t C, uint32_t *psz) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= C; g += gridDim.x * blockDim.x)
+        psz[g] = g < C ? (start[g + 1] - start[g] + 255u) / 256u * 256u : 0u;
+}
+
+static bool knn_prune_enabled(uint32_t n) {
+    const char *e = std::getenv("SCGPU_KNN_PRUNE");
+    if (e && e[0] == '0') return false;
+    if (e && e[0] == '1') return n >= 2 * (uint32_t)KNN_BM;
+    return n >= 16384;
+}
+
+// Layout of the pruned search: points grouped by k-means (C centroids, a few Lloyd
+// iterations), stable in the points' own order within a group, every group starting on a row
+// block; returns the layout's length (a multiple of 256), layout[j] = point or KNONE.
+static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf<uint32_t> &layout, hipStream_t s) {
+    const uint32_t C = std::min<uint32_t>(n, std::max<uint32_t>(16u, std::min<uint32_t>(2048u, n / 512u)));
+    DevBuf<uint32_t> perm;
+    constexpr int ITERS = 2;
+    DevBuf<float> cen((size_t)C * d), cnorm(C);
+    DevBuf<uint32_t> keys(n), start(C + 1);
+    perm.resize(n);
+    hipLaunchKernelGGL(k_kmeans_init, dim3(grid_for((uint64_t)C * d, 256, 4096)), dim3(256), 0, s, d_X, n, d, C,
+                       cen.get());
+    KERNEL_OK();
+    const int kb = std::max(1, bits_for(C));
+    for (int it = 0; it <= ITERS; it++) {
+        hipLaunchKernelGGL(k_kmeans_norms, dim3(grid_for(C, 256, 1024)), dim3(256), 0, s, cen.get(), d, C, cnorm.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_kmeans_assign, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, n, d, cen.get(),
+                           cnorm.get(), C, keys.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_knn_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, perm.get(), n);
+        KERNEL_OK();
+        radix_sort_pairs(keys, perm, n, kb, s);
+        hipLaunchKernelGGL(k_key_starts_u32, dim3(grid_for(C + 1, 256, 1024)), dim3(256), 0, s, keys.get(), n, C,
+                           start.get());
+        KERNEL_OK();
+        if (it == ITERS) break;
+        hipLaunchKernelGGL(k_kmeans_update, dim3(std::min<uint32_t>(C, 8192)), dim3(256), 0, s, d_X, d, perm.get(),
+                           start.get(), C, cen.get());
+        KERNEL_OK();
+    }
+    DevBuf<uint32_t> psz(C + 1), poff(C + 1);
+    hipLaunchKernelGGL(k_padded_sizes, dim3(grid_for(C + 1, 256, 1024)), dim3(256), 0, s, start.get(), C, psz.get());
+    KERNEL_OK();
+    exclusive_scan_u32(psz.get(), poff.get(), C + 1, s);
+    uint32_t rows = 0;
+    to_host(&rows, poff.get() + C, 1, s);
+    sync_stream(s);
+    layout.resize(rows);
+    hipLaunchKernelGGL(k_fill_u32_knn, dim3(grid_for(rows, 256, 4096)), dim3(256), 0, s, layout.get(), rows, KNONE);
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_layout_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, keys.get(), perm.get(),
+                       start.get(), poff.get(), n, layout.get());
+    KERNEL_OK();
+    return rows;
+}
+
 template <bool BF16, int NKC, int KM>
 static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
-                           const float *ny, uint32_t n, uint32_t npad, uint32_t cps, uint32_t *cidx, float *cs,
-                           hipStream_t s) {
+                           const float *ny, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
+                           const uint32_t *boff, uint32_t *cidx, float *cs, hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
-    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs);
+    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, cps, blist,
+                       boff, cidx, cs);
     KERNEL_OK();
 }
 
 template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
-                        const float *ny, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, uint32_t *cidx,
-                        float *cs, hipStream_t s) {
+                        const float *ny, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
+                        const uint32_t *boff, uint32_t *cidx, float *cs, hipStream_t s) {
     switch (KM) {
-        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs, s); break;
-        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs, s); break;
-        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, ny, n, npad, cps, cidx, cs, s); break;
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, cidx, cs, s); break;
     }
 }
 
@@ -597,16 +1016,30 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     // needs one candidate beyond the k-th)
     uint32_t K2 = 16;
     while (K2 < k + 1 && K2 < (uint32_t)KNN_MAXK2) K2 *= 2;
+    hipEvent_t e0, e1, e2, e3;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventCreate(&e2));
+    HIP_OK(hipEventCreate(&e3));
+    HIP_OK(hipEventRecord(e0, s));
+    // pruned search: the grouped layout (rows: its length, whole row blocks per group)
+    const bool prune = knn_prune_enabled(n);
+    DevBuf<uint32_t> layout;
+    const uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, s) : npad;
     // column splits: the count whose last wave of workgroups is fullest (time ~ waves / splits),
     // with a small charge per split for the extra candidates it sends to the rescore
-    const uint32_t row_blocks = npad / KNN_BM;
+    const uint32_t row_blocks = rows / KNN_BM;
     int dev = 0, cus = 256;
     HIP_OK(hipGetDevice(&dev));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const uint32_t slots = (uint32_t)cus * ((K2 == 16 && nkc <= 2) ? 2u : 1u);  // workgroups resident at once
-    uint32_t splits = 1;
+    // at least two splits where the list allows it: a single list holds the k + 1 nearest
+    // points only, and the certificate then needs the (k + 1)-th to stand clear of the k-th by the
+    // error bound, which fails for most rows of dense clusters (every such row costs an FP64 scan)
+    const uint32_t min_splits = 2 * K2 <= (uint32_t)KNN_MAXCAND ? 2u : 1u;
+    uint32_t splits = min_splits;
     double best = 1e30;
-    for (uint32_t sp = 1; sp <= 8 && sp * K2 <= (uint32_t)KNN_MAXCAND; sp++) {
+    for (uint32_t sp = min_splits; sp <= 8 && sp * K2 <= (uint32_t)KNN_MAXCAND; sp++) {
         const double waves = (double)((row_blocks * sp + slots - 1) / slots);
         const double cost = waves / sp + 0.02 * sp;
         if (cost < best) {
@@ -617,51 +1050,115 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     // the online tuner measures the alternatives when the grid is small enough for the last
     // wave of workgroups to matter (the result is exact for every split count)
     std::vector<uint32_t> cand{splits};
-    if (row_blocks < 4 * slots)
-        for (uint32_t sp = splits > 2 ? splits - 2 : 1; sp <= splits + 2 && sp <= 8 && sp * K2 <= (uint32_t)KNN_MAXCAND; sp++)
+    if (prune) {
+        // the pruned lists vary in length between row blocks: more splits balance the work
+        cand.clear();
+        for (uint32_t sp : {4u, 2u, 8u, 6u})
+            if (sp * K2 <= (uint32_t)KNN_MAXCAND) cand.push_back(sp);
+        if (cand.empty()) cand.push_back(min_splits);
+    } else if (row_blocks < 4 * slots) {
+        for (uint32_t sp = std::max(min_splits, splits > 2 ? splits - 2 : 1u); sp <= splits + 2 && sp <= 8 && sp * K2 <= (uint32_t)KNN_MAXCAND; sp++)
             if (sp != splits) cand.push_back(sp);
+    }
     const std::string tkey = "knn_tile/rb2^" + std::to_string(bits_for(row_blocks)) + "/kc" + std::to_string(nkc) +
-                             "/km" + std::to_string(K2) + (bf16 ? "/bf16" : "/fp32");
+                             "/km" + std::to_string(K2) + (bf16 ? "/bf16" : "/fp32") + (prune ? "/pruned" : "");
     const TuneChoice tc = tune_enabled() && cand.size() > 1 ? tune_pick(tkey, (int)cand.size(), 1) : TuneChoice{0, false};
     splits = cand[tc.index];
-    uint32_t cps = (npad + splits - 1) / splits;
+    if (const char *e = std::getenv("SCGPU_KNN_SPLITS")) {  // fixed split count (measurements)
+        const uint32_t f = (uint32_t)std::atoi(e);
+        if (f >= 1 && f <= 8 && f * K2 <= (uint32_t)KNN_MAXCAND) splits = f;
+    }
+    uint32_t cps = (rows + splits - 1) / splits;
     cps = (cps + KNN_TN - 1) / KNN_TN * KNN_TN;
-    splits = (npad + cps - 1) / cps;
-    DevBuf<unsigned short> Xh((size_t)npad * Kp), Xl((size_t)npad * Kp);
-    DevBuf<float> Xf((size_t)npad * Kp), ny(npad);
+    if (!prune) splits = (rows + cps - 1) / cps;
+    DevBuf<unsigned short> Xh((size_t)rows * Kp), Xl((size_t)rows * Kp);
+    DevBuf<float> Xf((size_t)rows * Kp), ny(rows);
     DevBuf<double> nx64(npad);
     DevBuf<unsigned int> ymax(1);
     HIP_OK(hipMemsetAsync(ymax.get(), 0, sizeof(unsigned int), s));
-    hipEvent_t e0, e1, e2, e3;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    HIP_OK(hipEventCreate(&e2));
-    HIP_OK(hipEventCreate(&e3));
-    HIP_OK(hipEventRecord(e0, s));
-    const uint64_t total = (uint64_t)npad * Kp;
-    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_X, n, d, Kp, total, Xh.get(),
-                       Xl.get(), Xf.get());
+    // pruned search: block spheres, in-block k-th distance bounds, each row block's list of
+    // column blocks
+    const uint32_t TN = nkc <= 2 ? 64u : 32u;  // KnnTileShape<NKC>::TN
+    DevBuf<uint32_t> blist, boff;
+    uint64_t list_total = 0;
+    if (prune) {
+        const uint32_t ncb = rows / TN;
+        DevBuf<double> cR((size_t)row_blocks * d), rR(row_blocks), uR(row_blocks), cB((size_t)ncb * d), rB(ncb);
+        DevBuf<uint32_t> nR(row_blocks), nB(ncb), cnt(row_blocks + 1);
+        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
+                           layout.get(), rows, d, (uint32_t)KNN_BM, row_blocks, cR.get(), rR.get(), nR.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(ncb, 16384)), dim3(256), 0, s, d_X, layout.get(),
+                           rows, d, TN, ncb, cB.get(), rB.get(), nB.get());
+        KERNEL_OK();
+        if (d <= 32)
+            hipLaunchKernelGGL(k_block_kth<32>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
+                               layout.get(), d, k, row_blocks, uR.get());
+        else if (d <= 64)
+            hipLaunchKernelGGL(k_block_kth<64>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
+                               layout.get(), d, k, row_blocks, uR.get());
+        else
+            hipLaunchKernelGGL(k_block_kth<128>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
+                               layout.get(), d, k, row_blocks, uR.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_fill_u32_knn, dim3(1), dim3(64), 0, s, cnt.get() + row_blocks, 1u, 0u);
+        KERNEL_OK();
+        const dim3 ngrid(std::min<uint32_t>(row_blocks, 16384));
+        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB.get(), rB.get(),
+                           nB.get(), d, row_blocks, ncb, (uint32_t)KNN_BM / TN, cnt.get(), (const uint32_t *)nullptr,
+                           (uint32_t *)nullptr);
+        KERNEL_OK();
+        boff.resize(row_blocks + 1);
+        exclusive_scan_u32(cnt.get(), boff.get(), row_blocks + 1, s);
+        uint32_t lt = 0;
+        to_host(&lt, boff.get() + row_blocks, 1, s);
+        sync_stream(s);
+        list_total = lt;
+        blist.resize(std::max<uint64_t>(list_total, 1));
+        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB.get(), rB.get(),
+                           nB.get(), d, row_blocks, ncb, (uint32_t)KNN_BM / TN, (uint32_t *)nullptr,
+                           (const uint32_t *)boff.get(), blist.get());
+        KERNEL_OK();
+    }
+    const uint32_t *pm = prune ? layout.get() : nullptr;
+    const uint32_t *bl = prune ? blist.get() : nullptr;
+    const uint32_t *bo = prune ? boff.get() : nullptr;
+    const uint64_t total = (uint64_t)rows * Kp;
+    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_X, pm, n, d, Kp, total,
+                       Xh.get(), Xl.get(), Xf.get());
     KERNEL_OK();
-    hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(npad, 256, 8192)), dim3(256), 0, s, d_X, n, d, npad, ny.get(),
+    hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, s, d_X, pm, n, d, rows, ny.get(),
                        nx64.get(), ymax.get());
     KERNEL_OK();
-    DevBuf<uint32_t> cidx((size_t)n * splits * K2);
-    DevBuf<float> cs((size_t)n * splits * K2);
+    // candidate lists per layout row (the pruned layout has padding rows)
+    const uint32_t crow = prune ? rows : n;
+    DevBuf<uint32_t> cidx((size_t)crow * splits * K2);
+    DevBuf<float> cs((size_t)crow * splits * K2);
     HIP_OK(hipEventRecord(e1, s));
     dim3 grid(row_blocks, splits);
     const unsigned short *xh = Xh.get(), *xl = Xl.get();
     const float *xf = Xf.get();
     switch (nkc * 2 + (bf16 ? 0 : 1)) {
-        case 2: launch_tile<true, 1>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        case 3: launch_tile<false, 1>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        case 4: launch_tile<true, 2>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        case 5: launch_tile<false, 2>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        case 6: launch_tile<true, 3>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        case 7: launch_tile<false, 3>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        case 8: launch_tile<true, 4>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
-        default: launch_tile<false, 4>(grid, xh, xl, xf, ny.get(), n, npad, K2, cps, cidx.get(), cs.get(), s); break;
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
     }
     HIP_OK(hipEventRecord(e2, s));
+    if (prune) {
+        // candidates back to the points' own rows and indices
+        DevBuf<uint32_t> cidx2((size_t)n * splits * K2);
+        DevBuf<float> cs2((size_t)n * splits * K2);
+        hipLaunchKernelGGL(k_knn_remap, dim3(grid_for((uint64_t)rows * splits * K2, 256, 16384)), dim3(256), 0, s, pm,
+                           rows, splits * K2, cidx.get(), cs.get(), cidx2.get(), cs2.get());
+        KERNEL_OK();
+        std::swap(cidx, cidx2);
+        std::swap(cs, cs2);
+    }
     unsigned int ymax_bits = 0;
     to_host(&ymax_bits, ymax.get(), 1, s);
     sync_stream(s);
@@ -709,7 +1206,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         st->rows_fallback = nf;
         st->tile_kernel_ms = t_tile;
         st->total_ms = t_all;
-        st->flops = 2.0 * (double)n * (double)n * (double)d;
+        // pairs the tile computed (padding included); the brute force computes npad^2
+        st->pairs_computed = prune ? (double)list_total * TN * KNN_BM : (double)npad * (double)npad;
+        st->flops = 2.0 * st->pairs_computed * (double)d;
     }
 }
 

@@ -85,6 +85,7 @@ class ScKnnStats(C.Structure):
         ("tile_kernel_ms", C.c_double),
         ("total_ms", C.c_double),
         ("flops", C.c_double),
+        ("pairs_computed", C.c_double),
     ]
 
 

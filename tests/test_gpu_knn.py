@@ -22,6 +22,51 @@ def test_knn_exact_vs_oracle(precision, n, d, k):
     print(f"{precision} n={n} d={d} k={k}: fallback rows {st['rows_fallback']}, tile {st['tile_kernel_ms']:.3f} ms")
 
 
+@pytest.fixture
+def knn_prune(monkeypatch):
+    def set_mode(v):
+        monkeypatch.setenv("SCGPU_KNN_PRUNE", v)
+    return set_mode
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("n,d,k,comps,spread", [
+    (1500, 50, 15, 10, 16.0),   # a few row blocks, clusters far apart: most column blocks skipped
+    (5000, 30, 7, 40, 9.0),     # clusters closer together than their radius
+    (4000, 8, 10, 1, 1.0),      # one Gaussian blob: hardly anything skipped
+])
+def test_knn_pruned_exact_vs_oracle(knn_prune, precision, n, d, k, comps, spread):
+    # block pruning (k-means grouped layout, bounding spheres) keeps the brute-force result:
+    # index sets and FP64 distances bit-exact, and equal to the unpruned search
+    x, _ = W.gaussian_mixture(n, d, comps, spread, seed=n + 7 * d)
+    knn_prune("1")
+    idx, d2, st = scgpu.knn(x, k, precision=precision, return_stats=True)
+    knn_prune("0")
+    idx0, d20, st0 = scgpu.knn(x, k, precision=precision, return_stats=True)
+    assert np.array_equal(idx, idx0) and np.array_equal(d2.view(np.uint64), d20.view(np.uint64))
+    rows = np.random.default_rng(1).choice(n, 300, replace=False)
+    oi, od = O.knn_bruteforce(x, k, rows=rows)
+    assert np.array_equal(idx[rows], oi)
+    assert np.array_equal(d2[rows].view(np.uint64), od.view(np.uint64))
+    print(f"pruned {precision} n={n} d={d}: pairs {st['pairs_computed']:.3g} of {st0['pairs_computed']:.3g}, "
+          f"fallback rows {st['rows_fallback']}")
+
+
+def test_knn_pruned_ties_duplicates_default_size():
+    # above the default pruning size: duplicated points and exact ties across groups
+    rng = np.random.default_rng(5)
+    base, _ = W.gaussian_mixture(20000, 16, 25, 16.0, seed=9)
+    x = base.copy()
+    x[1000:1300] = x[5000:5300]                      # exact duplicates far apart in input order
+    x[7000:7400] = np.round(x[7000:7400])            # rounded rows: many equal distances
+    idx, d2, st = scgpu.knn(x, 9, return_stats=True)
+    assert st["pairs_computed"] < 0.75 * 20224 ** 2  # the pruned path ran and skipped work
+    rows = np.concatenate([np.arange(1000, 1010), np.arange(5000, 5010), np.arange(7000, 7020),
+                           rng.choice(20000, 150, replace=False)])
+    oi, od = O.knn_bruteforce(x, 9, rows=rows)
+    assert np.array_equal(idx[rows], oi) and np.array_equal(d2[rows], od)
+
+
 def test_knn_ties_and_duplicates():
     rng = np.random.default_rng(1)
     x = rng.integers(0, 3, size=(500, 4)).astype(np.float32)  # many exact ties
