@@ -765,62 +765,104 @@ __global__ void __launch_bounds__(256) k_block_spheres(const float *X, const uin
     }
 }
 
-// Column blocks each row block needs, in the order of a sweep that starts at the row block's
-// own columns and wraps around: B is skipped when |cR - cB| - rR - rB > uR, uR bounding every
-// row's k-th nearest distance (k_block_kth; infinite when the block holds too few points).
-// Count pass (blist null): counts[R]; fill pass: blist[boff[R] + rank] = B, in sweep order.
+// Column blocks each row block needs, in sweep order (starting at the columns of the first of
+// the workgroup's row blocks and wrapping around): B is skipped when |cR - cB| - rR - rB > uR,
+// uR bounding every row's k-th nearest distance (k_block_kth; +inf: the block needs every
+// column block, negative: the block holds no points). Each workgroup decides RPW row blocks, so
+// one read of a column block's centre serves all of them. The centres' distance is evaluated in
+// f32 from f32 centres: rounding the centres moves it by at most 2^-24 (|cR| + |cB|) and the
+// f32 sum of non-negative squares by a relative (d + 2) 2^-24, so the skip test first takes
+// 1e-5 relatively and 1e-6 (|cR| + |cB|) absolutely off the distance. Count pass (blist null):
+// counts[R]; fill pass: blist[boff[R] + rank] = B, in sweep order.
+constexpr int RPW = 4;
 __global__ void __launch_bounds__(256) k_need_lists(const double *cR, const double *rR, const double *uR,
-                                                    const double *cB, const double *rB, const uint32_t *nB,
+                                                    const float *cB, const double *rB, const uint32_t *nB,
                                                     uint32_t d, uint32_t nrb, uint32_t ncb, uint32_t bstep,
                                                     uint32_t *counts, const uint32_t *boff, uint32_t *blist) {
-    __shared__ double sc[128];
-    __shared__ uint32_t s_w[4];
+    __shared__ float sc[RPW][128];
+    __shared__ float snr[RPW];
+    __shared__ uint32_t s_w[RPW][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {
+    for (uint32_t R0 = blockIdx.x * RPW; R0 < nrb; R0 += gridDim.x * RPW) {
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < d; t += 256) sc[t] = cR[(uint64_t)R * d + t];
+        for (uint32_t e = threadIdx.x; e < RPW * d; e += 256) {
+            const uint32_t r = e / d, t = e % d;
+            sc[r][t] = R0 + r < nrb ? (float)cR[(uint64_t)(R0 + r) * d + t] : 0.0f;
+        }
         __syncthreads();
-        const double rr = rR[R], ur = uR[R];
-        const bool all = !(ur < INFINITY);
-        const uint32_t B0 = (uint32_t)(((uint64_t)R * bstep) % ncb);
-        uint32_t total = 0;
-        if (ur >= 0.0) {  // negative: the block holds no points
-            for (uint32_t t0 = 0; t0 < ncb; t0 += 256) {  // workgroup-uniform
-                const uint32_t t = t0 + threadIdx.x;
-                bool need = false;
-                uint32_t B = 0;
-                if (t < ncb) {
-                    B = B0 + t;
-                    if (B >= ncb) B -= ncb;
-                    if (nB[B] > 0) {
-                        if (all) {
-                            need = true;
-                        } else {
-                            double s2 = 0.0;
-                            for (uint32_t u = 0; u < d; u++) {
-                                const double df = sc[u] - cB[(uint64_t)B * d + u];
-                                s2 += df * df;
-                            }
-                            const double dist = sqrt(s2) * (1.0 - 1e-12);
-                            need = !(dist > (rr + rB[B] + ur) * (1.0 + 1e-12));
+        if (threadIdx.x < RPW) {
+            float q = 0.0f;
+            for (uint32_t t = 0; t < d; t++) q = fmaf(sc[threadIdx.x][t], sc[threadIdx.x][t], q);
+            snr[threadIdx.x] = sqrtf(q);
+        }
+        __syncthreads();
+        double lim[RPW];
+        bool all[RPW], none[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const bool ok = R0 + r < nrb;
+            const double ur = ok ? uR[R0 + r] : -1.0;
+            none[r] = !(ur >= 0.0);
+            all[r] = !none[r] && !(ur < INFINITY);
+            lim[r] = ok ? rR[R0 + r] + ur : 0.0;
+        }
+        uint32_t total[RPW] = {};
+        const uint32_t B0 = (uint32_t)(((uint64_t)R0 * bstep) % ncb);
+        for (uint32_t t0 = 0; t0 < ncb; t0 += 256) {  // workgroup-uniform
+            const uint32_t t = t0 + threadIdx.x;
+            uint32_t B = 0;
+            bool has = false;
+            float dist[RPW] = {};
+            float nb2 = 0.0f;
+            if (t < ncb) {
+                B = B0 + t;
+                if (B >= ncb) B -= ncb;
+                has = nB[B] > 0;
+                if (has) {
+                    float s2[RPW] = {};
+                    for (uint32_t u = 0; u < d; u++) {
+                        const float cb = cB[(uint64_t)B * d + u];
+                        nb2 = fmaf(cb, cb, nb2);
+#pragma unroll
+                        for (int r = 0; r < RPW; r++) {
+                            const float df = sc[r][u] - cb;
+                            s2[r] = fmaf(df, df, s2[r]);
                         }
                     }
+#pragma unroll
+                    for (int r = 0; r < RPW; r++) dist[r] = sqrtf(s2[r]);
                 }
+            }
+            const double rb = has ? rB[B] : 0.0;
+            const float nbn = sqrtf(nb2);
+#pragma unroll
+            for (int r = 0; r < RPW; r++) {
+                bool need = false;
+                if (has && !none[r])
+                    need = all[r] || !((double)dist[r] * (1.0 - 1e-5) - 1e-6 * ((double)snr[r] + (double)nbn) >
+                                         (lim[r] + rb) * (1.0 + 1e-12));
                 const uint64_t m = __ballot(need);
-                if (lane == 0) s_w[wave] = (uint32_t)__popcll(m);
+                if (lane == 0) s_w[r][wave] = (uint32_t)__popcll(m);
                 __syncthreads();
                 uint32_t before = 0;
-                for (int w = 0; w < wave; w++) before += s_w[w];
-                const uint32_t wtot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+                for (int w = 0; w < wave; w++) before += s_w[r][w];
+                const uint32_t wtot = s_w[r][0] + s_w[r][1] + s_w[r][2] + s_w[r][3];
                 if (blist && need)
-                    blist[boff[R] + total + before +
+                    blist[boff[R0 + r] + total[r] + before +
                           __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = B;
-                total += wtot;
+                total[r] += wtot;
                 __syncthreads();
             }
         }
-        if (!blist && threadIdx.x == 0) counts[R] = total;
+        if (!blist && threadIdx.x == 0)
+            for (int r = 0; r < RPW; r++)
+                if (R0 + r < nrb) counts[R0 + r] = total[r];
     }
+}
+
+__global__ void k_f64_to_f32(const double *in, uint64_t n, float *out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
 }
 
 // Candidate lists of layout row j (per_row entries; rows layout rows) -> the point's own row,
@@ -1023,9 +1065,15 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     HIP_OK(hipEventCreate(&e3));
     HIP_OK(hipEventRecord(e0, s));
     // pruned search: the grouped layout (rows: its length, whole row blocks per group)
-    const bool prune = knn_prune_enabled(n);
+    bool prune = knn_prune_enabled(n);
     DevBuf<uint32_t> layout;
-    const uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, s) : npad;
+    uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, s) : npad;
+    // the lists of column blocks are addressed with 32-bit offsets: with more (row block, column
+    // block) pairs than that the search runs unpruned
+    if (prune && (uint64_t)(rows / KNN_BM) * (rows / (nkc <= 2 ? 64u : 32u)) >= 0xFFFFFFFFull) {
+        prune = false;
+        rows = npad;
+    }
     // column splits: the count whose last wave of workgroups is fullest (time ~ waves / splits),
     // with a small charge per split for the extra candidates it sends to the rescore
     const uint32_t row_blocks = rows / KNN_BM;
@@ -1103,8 +1151,12 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         KERNEL_OK();
         hipLaunchKernelGGL(k_fill_u32_knn, dim3(1), dim3(64), 0, s, cnt.get() + row_blocks, 1u, 0u);
         KERNEL_OK();
-        const dim3 ngrid(std::min<uint32_t>(row_blocks, 16384));
-        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB.get(), rB.get(),
+        DevBuf<float> cB32((size_t)ncb * d);
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for((uint64_t)ncb * d, 256, 4096)), dim3(256), 0, s, cB.get(),
+                           (uint64_t)ncb * d, cB32.get());
+        KERNEL_OK();
+        const dim3 ngrid(std::min<uint32_t>((row_blocks + RPW - 1) / RPW, 16384));
+        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
                            nB.get(), d, row_blocks, ncb, (uint32_t)KNN_BM / TN, cnt.get(), (const uint32_t *)nullptr,
                            (uint32_t *)nullptr);
         KERNEL_OK();
@@ -1115,7 +1167,7 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         sync_stream(s);
         list_total = lt;
         blist.resize(std::max<uint64_t>(list_total, 1));
-        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB.get(), rB.get(),
+        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
                            nB.get(), d, row_blocks, ncb, (uint32_t)KNN_BM / TN, (uint32_t *)nullptr,
                            (const uint32_t *)boff.get(), blist.get());
         KERNEL_OK();
