@@ -250,9 +250,13 @@ def main():
     if wl["kind"] == "knn":
         tile_ms = float(np.mean([s["tile_kernel_ms"] for s in kstats]))
         flops = kstats[0]["flops"]
+        # flops = 2 d x the point pairs the tiles computed (block pruning skips pairs that provably
+        # hold no k-nearest neighbour; the brute force would compute n^2)
+        pairs = kstats[0].get("pairs_computed") or 0.0
         knn_roof = {"kernel": f"k_knn_tile<{args.precision}>", "bound": "mfma", "unit": "TFLOP/s",
                     "peak": BF16_PEAK_TFLOPS, "achieved": flops / (tile_ms / 1e3) / 1e12,
-                    "kernel_ms_per_step": tile_ms, "traffic": None}
+                    "kernel_ms_per_step": tile_ms, "traffic": None, "pairs_computed": pairs,
+                    "pairs_vs_brute_force": pairs / float(wl["n"]) ** 2 if pairs else None}
         knn_roof["frac"] = knn_roof["achieved"] / BF16_PEAK_TFLOPS
     traffic, traffic_src = measured_traffic(wl["workload"])
     if traffic:
