@@ -147,7 +147,7 @@ constexpr size_t knn_tile_lds_bytes() {
 template <int KM, int BCAP>
 __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[KM], float (&thr_r)[4][4],
                                                uint32_t (&cnt4)[4], uint32_t *sCnt, float *sThr, uint2 *sBuf,
-                                               int wave, int lane) {
+                                               int wave, int lane, float tseed) {
     const int rl = wave * 64 + lane;
     const int q = lane >> 4;
     if ((lane & 15) == 0) {
@@ -162,7 +162,7 @@ __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[K
         const uint2 e = p < c ? sBuf[rl * BCAP + p] : make_uint2(__float_as_uint(INFINITY), KNONE);
         list_insert_net<KM>(lv, li, __uint_as_float(e.x), e.y);
     }
-    sThr[rl] = lv[KM - 1];
+    sThr[rl] = fminf(lv[KM - 1], tseed);
     __threadfence_block();
 #pragma unroll
     for (int rt = 0; rt < 4; rt++) {
@@ -242,7 +242,7 @@ template <bool BF16, int NKC, int KM>
 __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, const float *ny, uint32_t n, uint32_t npad,
                                                   uint32_t cols_per_split, const uint32_t *blist, const uint32_t *boff,
-                                                  uint32_t *cand_idx, float *cand_s) {
+                                                  const float *seed_s, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
     constexpr int BCAP = KnnTileShape<NKC>::BCAP;
@@ -283,7 +283,10 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         ns = cend > cbeg ? (cend - cbeg) / TN : 0;
     }
 
-    sThr[tid] = INFINITY;
+    // a row's threshold starts at its seed (s-space bound above which no k-nearest point lies,
+    // pruned search) or +inf, and never rises above it
+    const float tseed = seed_s ? seed_s[wrow0 + lane] : INFINITY;
+    sThr[tid] = tseed;
 
     // A fragments of the wave's four row tiles, in registers for the whole sweep
     bf16x8 ah[4][NKC], al[4][NKC];
@@ -346,10 +349,11 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     }
     float thr_r[4][4];
     uint32_t cnt4[4];
+    __syncthreads();  // the seeded thresholds are in sThr
 #pragma unroll
     for (int rt = 0; rt < 4; rt++) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) thr_r[rt][i] = INFINITY;
+        for (int i = 0; i < 4; i++) thr_r[rt][i] = sThr[wave * 64 + rt * 16 + 4 * quarter + i];
         cnt4[rt] = 0;
     }
 
@@ -380,17 +384,17 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
                                                     al, af, acc1);
             bool need = knn_subtile_hits<BCAP>(acc0, sNy[sub * 16 + (lane & 15)], c0 + sub * 16 + (lane & 15), row0,
                                                rl_base, qmask, thr_r, cnt4, sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
             if (sub + 2 < TN / 16)
                 knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
                                                         ah, al, af, acc0);
             need = knn_subtile_hits<BCAP>(acc1, sNy[(sub + 1) * 16 + (lane & 15)], c0 + (sub + 1) * 16 + (lane & 15),
                                           row0, rl_base, qmask, thr_r, cnt4, sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
         }
     }
 #undef KNN_PREFETCH
-    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane);
+    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
     // write this split's candidate list: cand[(row * splits + split) * KM + t]
     const uint32_t my_row = wrow0 + lane;
     if (my_row < n) {
@@ -860,6 +864,26 @@ __global__ void __launch_bounds__(256) k_need_lists(const double *cR, const doub
     }
 }
 
+// Seed of every layout row's threshold in the tile's s = |y|^2 - 2 x.y space: s_exact of a point
+// within the row's k-th nearest distance is at most seed_d2 - |x|^2, and its approximation at most
+// eps above, so starting the threshold at seed_d2 - |x|^2 + 2.02 eps (rounded up) records every
+// k-nearest point, while a value at or above it belongs to a point strictly farther than the
+// row's k-th nearest (no candidate a certificate needs is lost).
+__global__ void k_seed_thresholds(const double *seed_d2, const float *ny, uint32_t rows, double eps_coef,
+                                  double eps_const, float *seed_s) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x) {
+        const double sd = seed_d2[j];
+        const double y = (double)ny[j];
+        float out = INFINITY;
+        if (sd < INFINITY && y < INFINITY) {
+            const double v = sd - y + 2.02 * (eps_coef * sqrt(y) + eps_const);
+            out = (float)v;
+            if ((double)out < v) out = nextafterf(out, INFINITY);
+        }
+        seed_s[j] = out;
+    }
+}
+
 __global__ void k_f64_to_f32(const double *in, uint64_t n, float *out) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         out[i] = (float)in[i];
@@ -889,11 +913,12 @@ __global__ void k_knn_remap(const uint32_t *perm, uint32_t rows, uint32_t per_ro
 // the stored f32 coordinates: every term is non-negative, so the sum's relative error is below
 // (d + 2) 2^-24 and uR inflates the largest k-th squared distance by 1e-4 before the square
 // root. One workgroup per row block, one thread per row with its coordinates in registers; the
-// block's points pass through LDS in chunks of 64.
+// block's points pass through LDS in chunks of 64. seed_d2[row] gets the row's own bound on its
+// k-th nearest squared distance (+inf when it has none).
 constexpr int KTH_MAXK = 32;
 template <int DMAX>
 __global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_t *perm, uint32_t d, uint32_t k,
-                                                   uint32_t nrb, double *uR) {
+                                                   uint32_t nrb, double *uR, double *seed_d2) {
     __shared__ float sp[64 * DMAX];
     __shared__ uint32_t sv[64];
     __shared__ double red[4];
@@ -945,6 +970,8 @@ __global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_
                 }
             }
         }
+        // the row's own bound on its k-th nearest squared distance (+inf: none)
+        seed_d2[(uint64_t)R * 256 + threadIdx.x] = (real && kcur < INFINITY) ? (double)kcur * (1.0 + 1e-4) : INFINITY;
         const double m = block_max_f64(real ? (double)kcur : 0.0, red);
         if (threadIdx.x == 0) {
             if (cnt < 1.0) uR[R] = -1.0;
@@ -1026,22 +1053,22 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
 template <bool BF16, int NKC, int KM>
 static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                            const float *ny, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
-                           const uint32_t *boff, uint32_t *cidx, float *cs, hipStream_t s) {
+                           const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
     hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, cps, blist,
-                       boff, cidx, cs);
+                       boff, seed_s, cidx, cs);
     KERNEL_OK();
 }
 
 template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                         const float *ny, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
-                        const uint32_t *boff, uint32_t *cidx, float *cs, hipStream_t s) {
+                        const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
     switch (KM) {
-        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, cidx, cs, s); break;
-        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, cidx, cs, s); break;
-        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, cidx, cs, s); break;
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
     }
 }
 
@@ -1128,8 +1155,10 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     // column blocks
     const uint32_t TN = nkc <= 2 ? 64u : 32u;  // KnnTileShape<NKC>::TN
     DevBuf<uint32_t> blist, boff;
+    DevBuf<double> seed_d2;
     uint64_t list_total = 0;
     if (prune) {
+        seed_d2.resize(rows);
         const uint32_t ncb = rows / TN;
         DevBuf<double> cR((size_t)row_blocks * d), rR(row_blocks), uR(row_blocks), cB((size_t)ncb * d), rB(ncb);
         DevBuf<uint32_t> nR(row_blocks), nB(ncb), cnt(row_blocks + 1);
@@ -1141,13 +1170,13 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         KERNEL_OK();
         if (d <= 32)
             hipLaunchKernelGGL(k_block_kth<32>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                               layout.get(), d, k, row_blocks, uR.get());
+                               layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
         else if (d <= 64)
             hipLaunchKernelGGL(k_block_kth<64>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                               layout.get(), d, k, row_blocks, uR.get());
+                               layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
         else
             hipLaunchKernelGGL(k_block_kth<128>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                               layout.get(), d, k, row_blocks, uR.get());
+                               layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
         KERNEL_OK();
         hipLaunchKernelGGL(k_fill_u32_knn, dim3(1), dim3(64), 0, s, cnt.get() + row_blocks, 1u, 0u);
         KERNEL_OK();
@@ -1182,6 +1211,26 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, s, d_X, pm, n, d, rows, ny.get(),
                        nx64.get(), ymax.get());
     KERNEL_OK();
+    // the error bound of the approximate values (needed by the tile's seeded thresholds and the
+    // rescore): |s_approx - s| <= norm error + 2 |dot error|, doubled for safety; see the header
+    unsigned int ymax_bits = 0;
+    to_host(&ymax_bits, ymax.get(), 1, s);
+    sync_stream(s);
+    float ymaxf;
+    std::memcpy(&ymaxf, &ymax_bits, 4);
+    double Y = (double)ymaxf;
+    const double u24 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -16);
+    double dot_coef = bf16 ? (3.01 * u16 + 3.03 * (3.0 * Kp) * u24) : (2.02 * (double)Kp * u24);
+    double eps_coef = 2.0 * (2.0 * dot_coef * Y) + std::ldexp(1.0, -20) * 2.0 * Y;
+    double eps_const = 2.0 * (1.01 * d * u24 * Y * Y) + std::ldexp(1.0, -20) * Y * Y;
+    DevBuf<float> seed_s;
+    if (prune) {
+        seed_s.resize(rows);
+        hipLaunchKernelGGL(k_seed_thresholds, dim3(grid_for(rows, 256, 4096)), dim3(256), 0, s, seed_d2.get(),
+                           ny.get(), rows, eps_coef, eps_const, seed_s.get());
+        KERNEL_OK();
+    }
+    const float *sd = prune ? seed_s.get() : nullptr;
     // candidate lists per layout row (the pruned layout has padding rows)
     const uint32_t crow = prune ? rows : n;
     DevBuf<uint32_t> cidx((size_t)crow * splits * K2);
@@ -1191,14 +1240,14 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const unsigned short *xh = Xh.get(), *xl = Xl.get();
     const float *xf = Xf.get();
     switch (nkc * 2 + (bf16 ? 0 : 1)) {
-        case 2: launch_tile<true, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        case 3: launch_tile<false, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        case 4: launch_tile<true, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        case 5: launch_tile<false, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        case 6: launch_tile<true, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        case 7: launch_tile<false, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        case 8: launch_tile<true, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
-        default: launch_tile<false, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, cidx.get(), cs.get(), s); break;
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
     }
     HIP_OK(hipEventRecord(e2, s));
     if (prune) {
@@ -1211,17 +1260,6 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         std::swap(cidx, cidx2);
         std::swap(cs, cs2);
     }
-    unsigned int ymax_bits = 0;
-    to_host(&ymax_bits, ymax.get(), 1, s);
-    sync_stream(s);
-    float ymaxf;
-    std::memcpy(&ymaxf, &ymax_bits, 4);
-    double Y = (double)ymaxf;
-    const double u24 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -16);
-    // |s_approx - s| <= norm error + 2 |dot error|, doubled for safety; see the header comment
-    double dot_coef = bf16 ? (3.01 * u16 + 3.03 * (3.0 * Kp) * u24) : (2.02 * (double)Kp * u24);
-    double eps_coef = 2.0 * (2.0 * dot_coef * Y) + std::ldexp(1.0, -20) * 2.0 * Y;
-    double eps_const = 2.0 * (1.01 * d * u24 * Y * Y) + std::ldexp(1.0, -20) * Y * Y;
     DevBuf<uint32_t> fb(n);
     DevBuf<unsigned int> nfb(1);
     DevBuf<unsigned long long> nres(1);
