@@ -451,7 +451,9 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
             sI[wave][e] = jv[h];
         }
         // a split whose list is full (its last entry is a point) may hide closer points; the
-        // smallest last value over those splits bounds every point not in the lists
+        // smallest last value over those splits bounds every point not in the lists. A list that
+        // is not full hides only points at or above the row's seeded threshold (pruned search),
+        // which are strictly farther than its k-th nearest point
         float last = INFINITY;
         bool full = false;
         if ((uint32_t)lane < splits) {
