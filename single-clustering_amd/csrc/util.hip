@@ -192,21 +192,44 @@ void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream
     HIP_OK(hipcub::DeviceScan::ExclusiveSum(t.get(), tmp, in, out, (int)n, s));
 }
 
+// rocprim sorts up to 2^20 items of 4-byte or wider keys by block sort + merge passes (about
+// 2 log2(n / 1024) launches), whatever the key width. The keys here are group ids or packed
+// ids of few bits: up to 24 bits the onesweep radix sort needs at most three digit passes
+// (8 bits each) plus a histogram and a scan launch, and is used for every size above one
+// block. Both orders are stable, so the result is the same.
+using ShortKeySort = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                rocprim::default_config, 0>;
+
+// Stable sort of n pairs by the low end_bit bits of the keys, ping-ponging between (keys,
+// vals) and (k2, v2); returns true when the result is in (k2, v2).
+template <typename K, typename V>
+static bool sort_pairs_db(K *keys, V *vals, K *k2, V *v2, uint64_t n, int end_bit, hipStream_t s) {
+    size_t tmp = 0;
+    if (end_bit <= 24) {
+        rocprim::double_buffer<K> kb(keys, k2);
+        rocprim::double_buffer<V> vb(vals, v2);
+        HIP_OK(rocprim::radix_sort_pairs<ShortKeySort>(nullptr, tmp, kb, vb, n, 0u, (unsigned)end_bit, s));
+        DevBuf<unsigned char> t(tmp);
+        HIP_OK(rocprim::radix_sort_pairs<ShortKeySort>(t.get(), tmp, kb, vb, n, 0u, (unsigned)end_bit, s));
+        return kb.current() != keys;  // keys and values always move together
+    }
+    hipcub::DoubleBuffer<K> kb(keys, k2);
+    hipcub::DoubleBuffer<V> vb(vals, v2);
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)n, 0, end_bit, s));
+    DevBuf<unsigned char> t(tmp);
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, kb, vb, (int)n, 0, end_bit, s));
+    return kb.Current() != keys;
+}
+
 template <typename K, typename V>
 static void sort_pairs(K *keys, V *vals, uint64_t n, int end_bit, hipStream_t s) {
     if (n <= 1) return;
     DevBuf<K> k2(n);
     DevBuf<V> v2(n);
-    hipcub::DoubleBuffer<K> kb(keys, k2.get());
-    hipcub::DoubleBuffer<V> vb(vals, v2.get());
-    size_t tmp = 0;
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)n, 0, end_bit, s));
-    DevBuf<unsigned char> t(tmp);
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, kb, vb, (int)n, 0, end_bit, s));
-    if (kb.Current() != keys)
-        HIP_OK(hipMemcpyAsync(keys, kb.Current(), n * sizeof(K), hipMemcpyDeviceToDevice, s));
-    if (vb.Current() != vals)
-        HIP_OK(hipMemcpyAsync(vals, vb.Current(), n * sizeof(V), hipMemcpyDeviceToDevice, s));
+    if (sort_pairs_db(keys, vals, k2.get(), v2.get(), n, end_bit, s)) {
+        HIP_OK(hipMemcpyAsync(keys, k2.get(), n * sizeof(K), hipMemcpyDeviceToDevice, s));
+        HIP_OK(hipMemcpyAsync(vals, v2.get(), n * sizeof(V), hipMemcpyDeviceToDevice, s));
+    }
     // the temporaries return to the pool; keep them alive until the copies above are queued
     // (stream order guarantees later users of the recycled blocks run after the copies)
 }
@@ -222,14 +245,10 @@ static void sort_pairs_bufs(DevBuf<K> &keys, DevBuf<V> &vals, uint64_t n, int en
     if (n <= 1) return;
     DevBuf<K> k2(n);
     DevBuf<V> v2(n);
-    hipcub::DoubleBuffer<K> kb(keys.get(), k2.get());
-    hipcub::DoubleBuffer<V> vb(vals.get(), v2.get());
-    size_t tmp = 0;
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)n, 0, end_bit, s));
-    DevBuf<unsigned char> t(tmp);
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, kb, vb, (int)n, 0, end_bit, s));
-    if (kb.Current() != keys.get()) std::swap(keys, k2);
-    if (vb.Current() != vals.get()) std::swap(vals, v2);
+    if (sort_pairs_db(keys.get(), vals.get(), k2.get(), v2.get(), n, end_bit, s)) {
+        std::swap(keys, k2);
+        std::swap(vals, v2);
+    }
 }
 
 void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<float> &vals, uint64_t n, int end_bit, hipStream_t s) {
