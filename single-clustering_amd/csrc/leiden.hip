@@ -330,26 +330,48 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         loop = a.self_loop[v];
         // refinement: the node leaves its singleton, which becomes empty (refine.rs:71-74)
         if (REFINE) rbase = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
-        for (uint32_t s = lane; s < tsize; s += G) {
-            uint32_t c = keys[s];
-            if (c == EMPTY_KEY) continue;
-            long long wfx = vals[s];
-            if (c == own) {
-                wown = wfx;
-                continue;
+        // SB slots per lane at a time: their community totals are gathered with independent
+        // loads (a slot that is no candidate reads the vertex's own community, a valid index)
+        // before any is scored; every candidate is scored exactly as one at a time, and the
+        // argmax does not depend on the order
+        constexpr int SB = 4;
+        for (uint32_t s0 = lane; s0 < tsize; s0 += (uint32_t)SB * G) {
+            uint32_t cs[SB];
+            long long wv[SB], Wv[SB], Kv[SB];
+#pragma unroll
+            for (int j = 0; j < SB; j++) {
+                const uint32_t s = s0 + (uint32_t)j * G;
+                cs[j] = s < tsize ? keys[s] : EMPTY_KEY;
+                wv[j] = cs[j] != EMPTY_KEY ? vals[s] : 0;
             }
-            ds.c++;
-            double Wc = from_fx(a.Wfx[c], vp.sW);
-            if (a.has_max && !(Wc + na <= a.max_w)) continue;
-            double Kc = a.objective == 0 ? from_fx(a.Kfx[c], vp.sK) : 0.0;
-            double gain = delta_insert(a.objective, vp, from_fx(wfx, vp.sw), loop, k, na, Kc, Wc);
-            if (REFINE) {
-                double rel = gain - rbase;
-                if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
-                double score = vp.theta > 0.0 ? rel / vp.theta + gumbel(sp.seed, sp.round, v - vp.lo, c - vp.lo) : rel;
-                better(bs, bc, score, c);
-            } else {
-                better(bs, bc, gain, c);
+#pragma unroll
+            for (int j = 0; j < SB; j++) {
+                const uint32_t ci = (cs[j] != EMPTY_KEY && cs[j] != own) ? cs[j] : own;
+                Wv[j] = a.Wfx[ci];
+                Kv[j] = a.objective == 0 ? a.Kfx[ci] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < SB; j++) {
+                const uint32_t c = cs[j];
+                if (c == EMPTY_KEY) continue;
+                if (c == own) {
+                    wown = wv[j];
+                    continue;
+                }
+                ds.c++;
+                double Wc = from_fx(Wv[j], vp.sW);
+                if (a.has_max && !(Wc + na <= a.max_w)) continue;
+                double Kc = a.objective == 0 ? from_fx(Kv[j], vp.sK) : 0.0;
+                double gain = delta_insert(a.objective, vp, from_fx(wv[j], vp.sw), loop, k, na, Kc, Wc);
+                if (REFINE) {
+                    double rel = gain - rbase;
+                    if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
+                    double score =
+                        vp.theta > 0.0 ? rel / vp.theta + gumbel(sp.seed, sp.round, v - vp.lo, c - vp.lo) : rel;
+                    better(bs, bc, score, c);
+                } else {
+                    better(bs, bc, gain, c);
+                }
             }
         }
     }
