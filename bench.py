@@ -26,6 +26,7 @@ import numpy as np  # noqa: E402
 METRIC = "Leiden edges/s, kNN points/s on 1 MI355X; learned vs plain at equal modularity"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 
 def log(msg):
@@ -161,6 +162,10 @@ def main():
                     help="untimed steps (the online autotuner measures its candidates during these)")
     ap.add_argument("--config", default="knn_leiden", choices=["knn_leiden", "small", "large", "filter", "batch"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--objective", default=None, choices=["rb", "cpm"],
+                    help="objective for the configuration's Leiden run (default: its own, Rb)")
+    ap.add_argument("--resolution", type=float, default=None,
+                    help="resolution with --objective (default 1.0 for rb, 0.05 for cpm)")
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--knn-cpu-rows", type=int, default=0,
@@ -196,6 +201,11 @@ def main():
     log(f"[bench] workload {wl['workload']} ready")
     if wl["kind"] == "batch":
         return bench_batch(args, wl, scgpu)
+    if args.objective:
+        res = args.resolution if args.resolution is not None else (1.0 if args.objective == "rb" else 0.05)
+        wl["objective"] = (args.objective, res)
+        if args.objective != "rb" or res != 1.0:
+            wl["workload"] += f"_{args.objective}{res:g}"
     obj_name, res = wl["objective"]
     cfg = scgpu.LeidenConfig(objective=scgpu.ObjectiveKind(obj_name, res), seed=42)
 
@@ -253,11 +263,12 @@ def main():
         # flops = 2 d x the point pairs the tiles computed (block pruning skips pairs that provably
         # hold no k-nearest neighbour; the brute force would compute n^2)
         pairs = kstats[0].get("pairs_computed") or 0.0
+        peak = BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_MFMA_PEAK_TFLOPS
         knn_roof = {"kernel": f"k_knn_tile<{args.precision}>", "bound": "mfma", "unit": "TFLOP/s",
-                    "peak": BF16_PEAK_TFLOPS, "achieved": flops / (tile_ms / 1e3) / 1e12,
+                    "peak": peak, "achieved": flops / (tile_ms / 1e3) / 1e12,
                     "kernel_ms_per_step": tile_ms, "traffic": None, "pairs_computed": pairs,
                     "pairs_vs_brute_force": pairs / float(wl["n"]) ** 2 if pairs else None}
-        knn_roof["frac"] = knn_roof["achieved"] / BF16_PEAK_TFLOPS
+        knn_roof["frac"] = knn_roof["achieved"] / peak
     traffic, traffic_src = measured_traffic(wl["workload"])
     if traffic:
         if knn_roof and "k_knn_tile" in traffic and args.precision == "bf16":
@@ -374,6 +385,12 @@ def main():
                "sample": f"{len(times)} full leiden() runs of the C oracle on the same {n}-node graph, "
                          f"median {t_cpu:.3f} s, host {model} ({ncpu} logical CPUs visible)",
                "modularity": q_cpu, "ari_vs_gpu": O.ari(c.labels(), ol), "n_clusters": int(onc)}
+        if obj_name != "rb" or res != 1.0:
+            # the run's own objective (Rb::quality / Cpm::quality), both partitions on the CPU graph
+            qg = O.quality(og, c.labels(), obj_name, res)
+            qc = O.quality(og, ol, obj_name, res)
+            cpu["objective_quality"] = {"objective": obj_name, "resolution": res, "gpu": qg, "cpu": qc,
+                                        "rel_diff": (qg - qc) / abs(qc) if qc else None}
 
     # CPU baseline of the kNN half: the kd-tree restatement of the reference's exact path
     # (kiddo nearest_n(k + 1) on min(64, threads) workers) on a bounded sample of query rows of
@@ -424,7 +441,7 @@ def main():
         "dtype": "f64 (delta-quality), int64 fixed-point totals, f32 weights; kNN " +
                  ("split-bf16 MFMA + fp64 rescore" if args.precision == "bf16" else "fp32 MFMA + fp64 rescore"),
         "data": "synthetic",
-        "config": {"workload": wl["workload"], "nodes": n, "edges": m, "leiden": "Rb resolution 1.0, seed 42",
+        "config": {"workload": wl["workload"], "nodes": n, "edges": m, "leiden": f"{'Rb' if obj_name == 'rb' else 'CPM'} resolution {res:g}, seed 42",
                    "parallelism": "single GPU"},
         "leiden_ms": tl * 1e3,
         "graph_build_ms": tk * 1e3,

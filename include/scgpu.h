@@ -153,6 +153,14 @@ sc_status sc_graph_from_csr_parts_device(const uint64_t *d_node_ptrs, uint64_t n
                                          uint64_t nnz, const double *d_node_weights,
                                          sc_graph **out, sc_error *err);
 
+/* The real-data example's reader (examples/real_data.rs:22-44) + from_csr_parts (:129-136):
+ * <dir>/indptr.bin (u64), indices.bin (u32) and data.bin (f32), raw little-endian, the
+ * connectivities matrix as tools/export_h5ad.py:106-108 writes it; unit node weights. A file
+ * that cannot be read is SC_INVALID_ARGUMENT, file lengths that do not fit together are
+ * SC_INVALID_CSR (both before any device work); the arrays are then checked as
+ * sc_graph_from_csr_parts checks them. */
+sc_status sc_graph_from_csr_export(const char *dir, sc_graph **out, sc_error *err);
+
 /* CSRNetwork::from_edges_with_node_weights (csr_network.rs:120-167). Each undirected edge once;
  * (v, v, w) is a self-loop. node_weights may be NULL. */
 sc_status sc_graph_from_edges(uint64_t n_nodes, const uint64_t *edge_from, const uint64_t *edge_to,

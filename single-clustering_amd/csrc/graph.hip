@@ -492,7 +492,63 @@ static sc_status build_from_csr(const uint64_t *ptrs, uint64_t n, const uint32_t
     }
 }
 
+// ---- scanpy CSR export --------------------------------------------------------------------
+// <dir>/indptr.bin (u64), indices.bin (u32) and data.bin (f32), raw little-endian, as the
+// export tool writes a connectivities matrix (tools/export_h5ad.py:106-108) and the real-data
+// example reads it (examples/real_data.rs:22-44, 129-136).
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "the export files are little-endian");
+
+static std::vector<unsigned char> read_whole_file(const std::string &path) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw ScError(SC_INVALID_ARGUMENT, path + ": cannot open");
+    std::vector<unsigned char> buf;
+    unsigned char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    const bool bad = std::ferror(f) != 0;
+    std::fclose(f);
+    if (bad) throw ScError(SC_INVALID_ARGUMENT, path + ": read error");
+    return buf;
+}
+
+template <typename T>
+static std::vector<T> read_array_file(const std::string &path) {
+    const std::vector<unsigned char> raw = read_whole_file(path);
+    if (raw.size() % sizeof(T))
+        throw ScError(SC_INVALID_CSR, "malformed CSR input: " + path + " holds " + std::to_string(raw.size()) +
+                                          " bytes, not a whole number of " + std::to_string(sizeof(T)) +
+                                          "-byte entries");
+    std::vector<T> out(raw.size() / sizeof(T));
+    if (!out.empty()) std::memcpy(out.data(), raw.data(), raw.size());
+    return out;
+}
+
 extern "C" {
+
+sc_status sc_graph_from_csr_export(const char *dir, sc_graph **out, sc_error *err) {
+    std::vector<uint64_t> ptrs;
+    std::vector<uint32_t> nbrs;
+    std::vector<float> wts;
+    try {
+        if (!dir || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        const std::string d(dir);
+        ptrs = read_array_file<uint64_t>(d + "/indptr.bin");
+        nbrs = read_array_file<uint32_t>(d + "/indices.bin");
+        wts = read_array_file<float>(d + "/data.bin");
+        if (ptrs.empty())
+            throw ScError(SC_INVALID_CSR, "malformed CSR input: " + d + "/indptr.bin is empty (it holds n + 1 row offsets)");
+        if (nbrs.size() != wts.size())
+            throw ScError(SC_INVALID_CSR, "malformed CSR input: indices.bin has " + std::to_string(nbrs.size()) +
+                                              " entries but data.bin has " + std::to_string(wts.size()));
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    }
+    // the row offsets, ids and weights are checked as from_csr_parts checks them
+    return build_from_csr(ptrs.data(), ptrs.size() - 1, nbrs.data(), wts.data(), nbrs.size(), nullptr, false, out,
+                          err);
+}
 
 sc_status sc_graph_from_csr_parts(const uint64_t *node_ptrs, uint64_t n_nodes,
                                   const uint32_t *neighbors, const float *weights, uint64_t nnz,

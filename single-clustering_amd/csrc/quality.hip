@@ -49,9 +49,25 @@ __global__ void __launch_bounds__(256) k_vertex_terms(const uint64_t *labels, ui
     for (uint64_t v = blockIdx.x * (uint64_t)NG + threadIdx.x / G; v < n; v += (uint64_t)gridDim.x * NG) {
         const uint64_t c = labels[v];
         double acc = 0.0;
-        for (uint64_t e = ptrs[v] + lane; e < ptrs[v + 1]; e += G) {
-            uint64_t u = nbrs[e];
-            if (labels[u] == c && (u == v || v < u)) acc += (double)wts[e];
+        // EB entries per lane in flight (neighbour ids, then their labels, as independent
+        // loads); each lane still adds its entries e, e + G, e + 2G, ... in that order
+        constexpr int EB = 4;
+        const uint64_t end = ptrs[v + 1];
+        for (uint64_t e0 = ptrs[v] + lane; e0 < end; e0 += (uint64_t)EB * G) {
+            uint64_t u[EB];
+            float w[EB];
+#pragma unroll
+            for (int j = 0; j < EB; j++) {
+                const uint64_t e = e0 + (uint64_t)j * G;
+                u[j] = e < end ? nbrs[e] : v;
+                w[j] = e < end ? wts[e] : 0.0f;
+            }
+            bool same[EB];
+#pragma unroll
+            for (int j = 0; j < EB; j++) same[j] = labels[u[j]] == c;
+#pragma unroll
+            for (int j = 0; j < EB; j++)
+                if (e0 + (uint64_t)j * G < end && same[j] && (u[j] == v || v < u[j])) acc += (double)w[j];
         }
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);

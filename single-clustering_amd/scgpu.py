@@ -104,6 +104,7 @@ SIGNATURES = {
     "sc_leiden_config_default": ([], ScLeidenConfig),
     "sc_graph_from_csr_parts": ([u64p, C.c_uint64, u32p, f32p, C.c_uint64, f64p, C.POINTER(_GP),
                                  C.POINTER(ScError)], C.c_int),
+    "sc_graph_from_csr_export": ([C.c_char_p, C.POINTER(_GP), C.POINTER(ScError)], C.c_int),
     "sc_graph_from_csr_parts_device": ([C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
                                         C.c_void_p, C.POINTER(_GP), C.POINTER(ScError)], C.c_int),
     "sc_graph_from_edges": ([C.c_uint64, u64p, u64p, f64p, C.c_uint64, f64p, C.POINTER(_GP),
@@ -362,6 +363,17 @@ class CSRNetwork:
         st = _lib.sc_graph_from_csr_parts(_p(node_ptrs, u64p), n, _p(neighbors, u32p), _p(weights, f32p),
                                           len(neighbors), None if nw is None else _p(nw, f64p),
                                           C.byref(h), C.byref(err))
+        _check(st, err)
+        return cls(h)
+
+    @classmethod
+    def from_csr_export(cls, directory):
+        """The real-data example's reader (examples/real_data.rs:22-44, 129-136): the
+        connectivities matrix as `indptr.bin` (u64), `indices.bin` (u32) and `data.bin` (f32),
+        raw little-endian, read by the library and handed to from_csr_parts."""
+        h = C.c_void_p()
+        err = ScError()
+        st = _lib.sc_graph_from_csr_export(os.fsencode(os.fspath(directory)), C.byref(h), C.byref(err))
         _check(st, err)
         return cls(h)
 
@@ -645,6 +657,15 @@ def knn_graph(data, k, precision="bf16", mode=0, return_stats=False):
     if return_stats:
         return g, {k2: getattr(st, k2) for k2, _ in ScKnnStats._fields_}
     return g
+
+
+def write_csr_export(directory, node_ptrs, neighbors, weights):
+    """Writes a CSR matrix as the export tool does (tools/export_h5ad.py:106-108): indptr.bin
+    (u64), indices.bin (u32) and data.bin (f32), raw little-endian, in `directory`."""
+    os.makedirs(directory, exist_ok=True)
+    np.ascontiguousarray(node_ptrs).astype("<u8").tofile(os.path.join(directory, "indptr.bin"))
+    np.ascontiguousarray(neighbors).astype("<u4").tofile(os.path.join(directory, "indices.bin"))
+    np.ascontiguousarray(weights).astype("<f4").tofile(os.path.join(directory, "data.bin"))
 
 
 def knn_gaussian_csr(idx, d2, mode=0):
