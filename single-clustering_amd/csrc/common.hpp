@@ -281,8 +281,10 @@ struct sc_graph {
 namespace scgpu {
 // Builds the cached aggregates of a graph whose rows are already sorted, merged and free of
 // zero weights (CSRNetwork::finish's second half, csr_network.rs:313-351): strengths,
-// self-loops, total weight, edge count, and the degree-sum check.
-void graph_finish_sorted(sc_graph &g, hipStream_t s);
+// self-loops, total weight, edge count, and the degree-sum check. count_entries: g.nnz is not
+// known on the host yet and is read from ptrs[n] with the other row statistics (the nbrs/wts
+// buffers may then be larger than the entries they hold).
+void graph_finish_sorted(sc_graph &g, hipStream_t s, bool count_entries = false);
 // Full finish from arbitrary rows: sort, merge duplicates in f64, drop zero sums, compact.
 void graph_finish_unsorted(sc_graph &g, hipStream_t s);
 }  // namespace scgpu

@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_row_stats(const uint64_t *ptrs, const u
     if (threadIdx.x < NUM_BUCKETS && s_bc[threadIdx.x]) atomicAdd(&bucket_counts[threadIdx.x], s_bc[threadIdx.x]);
 }
 
-void graph_finish_sorted(sc_graph &g, hipStream_t s) {
+void graph_finish_sorted(sc_graph &g, hipStream_t s, bool count_entries) {
     uint64_t n = g.n;
     g.strength.resize(n);
     g.self_loop.resize(n);
@@ -270,10 +270,13 @@ void graph_finish_sorted(sc_graph &g, hipStream_t s) {
     }
     unsigned int h[NW] = {0};
     to_host(h, cnt.get(), NW, s);
+    uint64_t entries = g.nnz;
+    if (count_entries) to_host(&entries, g.ptrs.get() + n, 1, s);
     const double *in[4] = {row_total.get(), g.strength.get(), g.strength.get(), g.node_w.get()};
     const bool is_max[4] = {false, false, true, false};
     double r[4];
     device_reduce4_f64(in, is_max, n, r, s);  // synchronises the stream
+    g.nnz = entries;
     unsigned long long ec = 0;
     std::memcpy(&ec, h, sizeof(ec));
     g.edge_count = ec;

@@ -928,47 +928,66 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
     const DecideArgs &a = A.d;
     const int lane = threadIdx.x & 63;
     unsigned long long skipped = 0;
+    // the workgroup's shares of all buckets are walked as one range, so a thread's loads for
+    // positions of different buckets are in flight together; appends stay per bucket
+    uint32_t lo[NUM_BUCKETS], cum[NUM_BUCKETS + 1];
+    cum[0] = 0;
+#pragma unroll
     for (int k = 0; k < NUM_BUCKETS; k++) {
-        uint32_t lo, hi;
-        bucket_share(A, k, lo, hi);
-        for (uint32_t i0 = lo; i0 < hi; i0 += SWEEP_BS) {  // workgroup-uniform bounds
-            const uint32_t i = i0 + threadIdx.x;
-            bool take = false;
-            uint32_t v = 0;
-            float x[MLP_NF] = {};
-            if (i < hi) {
-                v = A.lists[i];
-                const VParams vp = vparams(a, v);
-                take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
-                if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
-                if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
+        uint32_t hi;
+        bucket_share(A, k, lo[k], hi);
+        cum[k + 1] = cum[k] + (hi - lo[k]);
+    }
+    const uint32_t total = cum[NUM_BUCKETS];
+    for (uint32_t j0 = 0; j0 < total; j0 += SWEEP_BS) {  // workgroup-uniform bounds
+        const uint32_t j = j0 + threadIdx.x;
+        // bucket of range position j: the last k with cum[k] <= j (empty buckets are passed over)
+        int kb = 0;
+        uint32_t i = 0;
+#pragma unroll
+        for (int k = 0; k < NUM_BUCKETS; k++)
+            if (j >= cum[k]) {
+                kb = k;
+                i = lo[k] + (j - cum[k]);
             }
-            // learned filter: a vertex predicted to stay is not decided now; every lane of the
-            // wave reaches the wave-collective inference
-            if (FILTER && filter_on && __any(take)) {
-                const float z = mlp_forward_wave(A.filter_pk, x);
-                if (take && z < A.filter_thr) {
-                    // it leaves the active set like a vertex that decided to stay (a
-                    // neighbour's move re-activates it), marked as skipped
-                    take = false;
-                    skipped++;
-                    a.active[v] = A.filter_judge == 2 ? 2 : 0;
-                    A.f_mark[v] = 1;
-                } else if (take) {
-                    A.f_mark[v] = 0;
-                }
+        bool take = false;
+        uint32_t v = 0;
+        float x[MLP_NF] = {};
+        if (j < total) {
+            v = A.lists[i];
+            const VParams vp = vparams(a, v);
+            take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
+            if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
+            if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
+        }
+        // learned filter: a vertex predicted to stay is not decided now; every lane of the
+        // wave reaches the wave-collective inference
+        if (FILTER && filter_on && __any(take)) {
+            const float z = mlp_forward_wave(A.filter_pk, x);
+            if (take && z < A.filter_thr) {
+                // it leaves the active set like a vertex that decided to stay (a
+                // neighbour's move re-activates it), marked as skipped
+                take = false;
+                skipped++;
+                a.active[v] = A.filter_judge == 2 ? 2 : 0;
+                A.f_mark[v] = 1;
+            } else if (take) {
+                A.f_mark[v] = 0;
             }
-            const uint64_t m = __ballot(take);
-            if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                unsigned int basei = 0;
-                if (lane == leader) basei = atomicAdd(&s_wc[k], (unsigned int)__popcll(m));
-                basei = __shfl(basei, leader);
-                if (take) {
-                    const uint32_t below =
-                        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    A.work[lo + basei + below] = i - A.bstart[k];
-                }
+        }
+        // one wave-aggregated append per bucket the wave's positions touch
+#pragma unroll
+        for (int k = 0; k < NUM_BUCKETS; k++) {
+            const uint64_t m = __ballot(take && kb == k);
+            if (!m) continue;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            unsigned int basei = 0;
+            if (lane == leader) basei = atomicAdd(&s_wc[k], (unsigned int)__popcll(m));
+            basei = __shfl(basei, leader);
+            if (take && kb == k) {
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                A.work[lo[k] + basei + below] = i - A.bstart[k];
             }
         }
     }
@@ -2639,19 +2658,18 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
                        kscan.get(), key2.get(), np, ng, a0.get(), b0.get(), cnt_a.get(), row_len.get());
     KERNEL_OK();
     exclusive_scan_u64(row_len.get(), out->ptrs.get(), (uint64_t)ng + 1, s);
-    uint64_t total = 0;
-    to_host(&total, out->ptrs.get() + ng, 1, s);
-    sync_stream(s);
-    out->nnz = total;
-    out->nbrs.resize(total);
-    out->wts.resize(total);
+    // a pair gives at most two entries (its row and, for g < h, the mirrored one), so 2 np
+    // bounds the coarse graph without waiting for its count; graph_finish_sorted reads the count
+    // with the other row statistics
+    out->nbrs.resize(2 * (uint64_t)np);
+    out->wts.resize(2 * (uint64_t)np);
     if (np) {
         hipLaunchKernelGGL(k_fill_rows, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, pa.get(), pb.get(), pw.get(),
                            keep.get(), kscan.get(), key2.get(), hw.get(), np, sent, a0.get(), b0.get(), cnt_a.get(),
                            out->ptrs.get(), out->nbrs.get(), out->wts.get());
         KERNEL_OK();
     }
-    graph_finish_sorted(*out, s);
+    graph_finish_sorted(*out, s, true);
     return out;
 }
 
