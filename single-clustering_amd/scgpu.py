@@ -659,6 +659,57 @@ def knn_graph(data, k, precision="bf16", mode=0, return_stats=False):
     return g
 
 
+class DeviceArray:
+    """A host array copied once into device memory (hipMalloc + a synchronous hipMemcpy through
+    the HIP runtime the library itself loads), for the entry points that take device-resident
+    input. `ptr` is the device address; free() releases it."""
+
+    _hip = None
+
+    def __init__(self, host):
+        if DeviceArray._hip is None:
+            hip = C.CDLL("libamdhip64.so")
+            hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            hip.hipFree.argtypes = [C.c_void_p]
+            DeviceArray._hip = hip
+        host = np.ascontiguousarray(host)
+        self.shape, self.dtype, self.nbytes = host.shape, host.dtype, host.nbytes
+        p = C.c_void_p()
+        if DeviceArray._hip.hipMalloc(C.byref(p), max(self.nbytes, 1)) != 0:
+            raise MemoryError(f"hipMalloc of {self.nbytes} bytes failed")
+        self.ptr = p.value
+        if self.nbytes and DeviceArray._hip.hipMemcpy(p, host.ctypes.data, self.nbytes, 1) != 0:  # host to device
+            self.free()
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    def free(self):
+        if self.ptr:
+            DeviceArray._hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def knn_graph_device(d_data, n, d, k, precision="bf16", mode=0, return_stats=False):
+    """knn_graph for points already resident in device memory: d_data is the device address of
+    n x d row-major f32 (for example a torch tensor's data_ptr(), complete before the call)."""
+    opts = ScKnnOpts(0 if precision == "bf16" else 1, int(mode))
+    h = C.c_void_p()
+    st = ScKnnStats()
+    err = ScError()
+    _check(_lib.sc_knn_graph_device(C.c_void_p(int(d_data)), int(n), int(d), int(k), C.byref(opts), C.byref(h),
+                                    C.byref(st), C.byref(err)), err)
+    g = CSRNetwork(h)
+    if return_stats:
+        return g, {k2: getattr(st, k2) for k2, _ in ScKnnStats._fields_}
+    return g
+
+
 def write_csr_export(directory, node_ptrs, neighbors, weights):
     """Writes a CSR matrix as the export tool does (tools/export_h5ad.py:106-108): indptr.bin
     (u64), indices.bin (u32) and data.bin (f32), raw little-endian, in `directory`."""

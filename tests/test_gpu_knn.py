@@ -87,3 +87,17 @@ def test_gaussian_graph_vs_oracle(mode):
     p, n_, w = g.to_csr_parts()
     assert np.array_equal(p, op) and np.array_equal(n_, on) and np.allclose(w, ow, rtol=1e-6, atol=0)
     g.validate_symmetry()
+
+
+@pytest.mark.parametrize("n,d,k", [(3000, 30, 9), (20000, 16, 7)])
+def test_knn_graph_device_input_equals_host_input(n, d, k):
+    # points resident in device memory give the same graph as points handed over from the host
+    # (the second size runs the block-pruned search)
+    x, _ = W.gaussian_mixture(n, d, 8, 9.0, seed=n)
+    xd = scgpu.DeviceArray(x.astype(np.float32))
+    g1, st1 = scgpu.knn_graph_device(xd.ptr, n, d, k, return_stats=True)
+    g0, st0 = scgpu.knn_graph(x, k, return_stats=True)
+    xd.free()
+    for a, b in zip(g1.to_csr_parts(), g0.to_csr_parts()):
+        assert np.array_equal(a, b)
+    assert st1["pairs_computed"] == st0["pairs_computed"]
