@@ -413,7 +413,22 @@ __device__ __forceinline__ double exact_d2(const float *X, uint32_t d, uint32_t 
     const float *a = X + (uint64_t)i * d;
     const float *b = X + (uint64_t)j * d;
     double s = 0.0;
-    for (uint32_t t = 0; t < d; t++) {
+    uint32_t t = 0;
+    // eight coordinates of both points are loaded together, then added in order t, t + 1, ...
+    for (; t + 8 <= d; t += 8) {
+        float av[8], bv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            av[q] = a[t + q];
+            bv[q] = b[t + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const double diff = (double)av[q] - (double)bv[q];
+            s = __dadd_rn(s, __dmul_rn(diff, diff));
+        }
+    }
+    for (; t < d; t++) {
         double diff = (double)a[t] - (double)b[t];
         s = __dadd_rn(s, __dmul_rn(diff, diff));
     }
@@ -1129,8 +1144,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     std::vector<uint32_t> cand{splits};
     if (prune) {
         // the pruned lists vary in length between row blocks: more splits balance the work
+        // (two splits leave the certificate too little margin: rows fall back to the FP64 scan)
         cand.clear();
-        for (uint32_t sp : {4u, 2u, 8u, 6u})
+        for (uint32_t sp : {4u, 8u, 6u})
             if (sp * K2 <= (uint32_t)KNN_MAXCAND) cand.push_back(sp);
         if (cand.empty()) cand.push_back(min_splits);
     } else if (row_blocks < 4 * slots) {
