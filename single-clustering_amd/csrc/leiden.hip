@@ -137,8 +137,11 @@ struct VParams {
     bool frozen;
 };
 
+// BATCH = false: a run without a job table, whose parameters are the same for every vertex (the
+// persistent sweeps are compiled for it separately, so they stay scalar)
+template <bool BATCH = true>
 __device__ __forceinline__ VParams vparams(const DecideArgs &a, uint32_t v) {
-    if (!a.vjob) return VParams{a.gamma, a.four_m, a.threshold, a.theta, a.sK, a.sW, a.sw, 0u, false};
+    if (!BATCH || !a.vjob) return VParams{a.gamma, a.four_m, a.threshold, a.theta, a.sK, a.sW, a.sw, 0u, false};
     const JobParams &p = a.jobs[a.vjob[v]];
     return VParams{p.gamma, p.four_m, p.threshold, p.theta, p.sK, p.sW, p.sw, p.lo, p.frozen != 0u};
 }
@@ -249,7 +252,7 @@ struct DecideStats {
 // the bucket's vertex list; sel, when given, holds the positions (within blist) of this step's
 // vertices, which are then already known to be active and in the step's subset. base is the
 // index (into sel, or blist) of the unit's first vertex. Every thread of the block calls this.
-template <int BS, int G, int T, bool REFINE, bool GTAB>
+template <int BS, int G, int T, bool REFINE, bool GTAB, bool BATCH = true>
 __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParams &sp, const uint32_t *blist,
                                             const uint32_t *sel, uint32_t len, uint64_t base, uint32_t *s_keys,
                                             long long *s_vals, double *r_s, uint32_t *r_c, long long *r_w,
@@ -260,7 +263,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
     bool valid = idx < len;
     uint32_t pos = valid ? (sel ? sel[idx] : (uint32_t)idx) : 0u;
     uint32_t v = valid ? blist[pos] : 0u;
-    const VParams vp = vparams(a, v);
+    const VParams vp = vparams<BATCH>(a, v);
     if (valid && !sel) {
         valid = a.active[v] != 0 && !vp.frozen && subset_of(sp.seed, v - vp.lo, a.P) == sp.sub;
         if (REFINE) valid = valid && a.memb[v] == v && a.csize[v] == 1;
@@ -922,7 +925,7 @@ __device__ __forceinline__ void bucket_share(const SweepArgs &A, int k, uint32_t
 
 // Positions (relative to their bucket) of this workgroup's share that decide in this step,
 // stored at the start of the share's own range of work[]; s_wc[k] gets their count.
-template <bool REFINE, bool FILTER>
+template <bool REFINE, bool FILTER, bool BATCH>
 __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t it,
                                               bool filter_on, unsigned int *s_wc) {
     const DecideArgs &a = A.d;
@@ -955,7 +958,7 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
         float x[MLP_NF] = {};
         if (j < total) {
             v = A.lists[i];
-            const VParams vp = vparams(a, v);
+            const VParams vp = vparams<BATCH>(a, v);
             take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
             if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
             if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
@@ -1011,7 +1014,7 @@ template <> struct BucketShape<5> { static constexpr int G = 256, T = 2048; };
 template <> struct BucketShape<6> { static constexpr int G = 256, T = 4096; };
 template <> struct BucketShape<7> { static constexpr int G = 256, T = 1; };
 
-template <bool REFINE>
+template <bool REFINE, bool BATCH>
 __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParams &sp, const unsigned int *s_wc,
                                              uint32_t *s_keys, long long *s_vals, double *r_s, uint32_t *r_c,
                                              long long *r_w, DecideStats &ds) {
@@ -1030,7 +1033,7 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
             uint32_t lo, hi;                                                                          \
             bucket_share(A, BK, lo, hi);                                                              \
             for (uint64_t base = 0; base < len; base += NG)                                           \
-                decide_unit<SWEEP_BS, G, T, REFINE, BK == GTAB_BUCKET>(A.d, sp, A.lists + A.bstart[BK],  \
+                decide_unit<SWEEP_BS, G, T, REFINE, BK == GTAB_BUCKET, BATCH>(A.d, sp, A.lists + A.bstart[BK],  \
                                                                        A.work + lo,                   \
                                                              len, base, s_keys, s_vals, r_s, r_c, r_w, ds); \
         }                                                                                             \
@@ -1046,8 +1049,9 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
 #undef SC_DECIDE_BUCKET
 }
 
+template <bool BATCH>
 __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint64_t from, uint64_t to) {
-    const VParams vp = vparams(A.d, (uint32_t)v);
+    const VParams vp = vparams<BATCH>(A.d, (uint32_t)v);
     const long long kf = to_fx(A.d.strength[v], vp.sK), wf = to_fx(A.d.node_w[v], vp.sW);
     atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
     atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)(-kf));
@@ -1073,7 +1077,7 @@ __global__ void k_sweep_collect(unsigned int *ctl, unsigned int *bc_words, uint3
     for (uint32_t i = threadIdx.x; i < n_bc_words; i += blockDim.x) bc_words[i] = 0;
 }
 
-template <bool REFINE, bool FILTER>
+template <bool REFINE, bool FILTER, bool BATCH>
 __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3))) k_sweep(SweepArgs A) {
     __shared__ uint32_t s_keys[SWEEP_TAB];
     __shared__ long long s_vals[SWEEP_TAB];
@@ -1124,7 +1128,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
     if (REFINE) {
         // singletons: every community gets exactly one vertex, no two atomics share an address
         for (uint64_t v = gtid; v < n; v += gsize) {
-            const VParams vp = vparams(a, (uint32_t)v);
+            const VParams vp = vparams<BATCH>(a, (uint32_t)v);
             atomicAdd((unsigned long long *)&A.Kfx[v], (unsigned long long)to_fx(a.strength[v], vp.sK));
             atomicAdd((unsigned long long *)&A.Wfx[v], (unsigned long long)to_fx(a.node_w[v], vp.sW));
             atomicAdd(&A.csize[v], 1u);
@@ -1147,7 +1151,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         __syncthreads();
         for (uint64_t v = gtid; v < n; v += gsize) {
             const uint32_t c = a.memb[v];
-            const VParams vp = vparams(a, (uint32_t)v);
+            const VParams vp = vparams<BATCH>(a, (uint32_t)v);
             const long long kf = to_fx(a.strength[v], vp.sK), wf = to_fx(a.node_w[v], vp.sW);
             uint32_t h = hash32(c) & (PT - 1);
             bool done = false;
@@ -1209,19 +1213,24 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             if (threadIdx.x < NUM_BUCKETS) s_wc[threadIdx.x] = 0;
             if (threadIdx.x == 0) s_mc = 0;
             __syncthreads();
+            const long long t_dec = A.trace ? wall_clock64() : 0;
             // compact and decide the workgroup's own share of every bucket
-            sweep_compact<REFINE, FILTER>(A, seed, sub, it, filter_on, s_wc);
+            sweep_compact<REFINE, FILTER, BATCH>(A, seed, sub, it, filter_on, s_wc);
             StepParams sp{seed, sub, it, step, &s_mc, REFINE ? nullptr : A.mstep, mv_list};
-            sweep_decide<REFINE>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
+            sweep_decide<REFINE, BATCH>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
             __syncthreads();
             const unsigned int nm = s_mc;
             if (threadIdx.x == 0) {
                 if (A.trace && step < A.max_steps) {
-                    // traced runs: deciders and movers of every step
+                    // traced runs: deciders and movers of every step, and the largest and summed
+                    // compact + decide time of the workgroups (clock ticks)
                     unsigned int dec = 0;
                     for (int b = 0; b < NUM_BUCKETS; b++) dec += s_wc[b];
                     if (dec) atomicAdd(&A.wcnt[(uint64_t)step * NUM_BUCKETS + 0], dec);
                     if (nm) atomicAdd(&A.wcnt[(uint64_t)step * NUM_BUCKETS + 1], nm);
+                    const unsigned int dt = (unsigned int)min(wall_clock64() - t_dec, 0x7FFFFFFFll);
+                    atomicMax(&A.wcnt[(uint64_t)step * NUM_BUCKETS + 2], dt);
+                    atomicAdd(&A.wcnt[(uint64_t)step * NUM_BUCKETS + 3], dt);
                 }
                 if (!REFINE) {
                     if (nm) A.mcnt[step] = 1u;  // someone chose to move: not converged
@@ -1265,7 +1274,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                     } else {
                         const uint64_t from = REFINE ? v : a.memb[v];
                         A.memb[v] = (uint32_t)t;
-                        move_totals(A, v, from, t);
+                        move_totals<BATCH>(A, v, from, t);
                         applied++;
                         if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                     }
@@ -1593,7 +1602,7 @@ static bool graph_sweeps() { return sweep_mode() == 2; }
 // Workgroups of a sweep launch: all must be resident at once (cooperative launch); small
 // levels use fewer, which makes each grid barrier cheaper.
 // Workgroups of a sweep launch that can be resident at once.
-template <bool REFINE, bool FILTER = false>
+template <bool REFINE, bool FILTER = false, bool BATCH = false>
 static uint32_t sweep_resident() {
     static thread_local int resident[64] = {0};
     int dev = 0;
@@ -1601,16 +1610,16 @@ static uint32_t sweep_resident() {
     int &r = resident[dev & 63];
     if (!r) {
         int per = 0, cus = 0;
-        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep<REFINE, FILTER>, SWEEP_BS, 0));
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep<REFINE, FILTER, BATCH>, SWEEP_BS, 0));
         HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         r = std::max(1, per) * std::max(1, cus);
     }
     return (uint32_t)r;
 }
 
-template <bool REFINE, bool FILTER = false>
+template <bool REFINE, bool FILTER = false, bool BATCH = false>
 static uint32_t sweep_grid(uint32_t n, uint64_t nnz) {
-    const uint32_t r = sweep_resident<REFINE, FILTER>();
+    const uint32_t r = sweep_resident<REFINE, FILTER, BATCH>();
     static const long long forced = [] {
         const char *e = std::getenv("SCGPU_SWEEP_BLOCKS");
         return e ? std::atoll(e) : 0ll;
@@ -1666,16 +1675,21 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     // k_sweep_collect clears them for the next one
     // workgroup count: the heuristic first, then a few alternatives the online tuner measures
     // (any count gives identical labels, so exploring never changes a result)
-    const uint32_t heur = filtered ? sweep_grid<false, true>(n, nnz) : sweep_grid<REFINE>(n, nnz);
+    // a run without a job table uses the sweep compiled for uniform parameters
+    const bool batch = A.d.vjob != nullptr;
+    const uint32_t heur = filtered ? sweep_grid<false, true>(n, nnz)
+                                   : (batch ? sweep_grid<REFINE, false, true>(n, nnz) : sweep_grid<REFINE>(n, nnz));
     std::vector<uint32_t> grids{heur};
     {
-        const uint32_t res = filtered ? sweep_resident<false, true>() : sweep_resident<REFINE>();
+        const uint32_t res = filtered ? sweep_resident<false, true>()
+                                      : (batch ? sweep_resident<REFINE, false, true>() : sweep_resident<REFINE>());
         const uint32_t lo = std::max<uint32_t>(8u, heur / 4);
         for (uint32_t c : {heur / 2, heur * 2, res})
             if (c >= lo && c <= res && std::find(grids.begin(), grids.end(), c) == grids.end()) grids.push_back(c);
     }
-    const std::string tkey = std::string(REFINE ? "sweep/refine" : (filtered ? "sweep/move-filtered" : "sweep/move")) + "/n2^" +
-                             std::to_string(bits_for(n)) + "/e2^" + std::to_string(bits_for(nnz));
+    const std::string tkey = std::string(REFINE ? "sweep/refine" : (filtered ? "sweep/move-filtered" : "sweep/move")) +
+                             (batch ? "-batch" : "") + "/n2^" + std::to_string(bits_for(n)) + "/e2^" +
+                             std::to_string(bits_for(nnz));
     const TuneChoice tc = tune_enabled() ? tune_pick(tkey, (int)grids.size()) : TuneChoice{0, false};
     const uint32_t grid = grids[tc.index];
     // each workgroup keeps its movers in its own slice of the mover array: its shares of the
@@ -1704,7 +1718,8 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         HIP_OK(hipEventRecord(ts.t0, s));
     }
     if (ctx.timer.on && !REFINE) ctx.timer.begin(s, ev);
-    const void *kern = filtered ? (const void *)k_sweep<false, true> : (const void *)k_sweep<REFINE, false>;
+    const void *kern = filtered ? (const void *)k_sweep<false, true, false>
+                                : (batch ? (const void *)k_sweep<REFINE, false, true> : (const void *)k_sweep<REFINE, false, false>);
     HIP_OK(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(SWEEP_BS), args, 0, s));
     if (ctx.timer.on && !REFINE) ctx.timer.end(s, ev);
     if (sample) {
@@ -1746,6 +1761,13 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
         line += " deciders/movers:";
         for (uint32_t i = 0; i < ns; i++)
             line += " " + std::to_string(wc[(size_t)i * NUM_BUCKETS]) + "/" + std::to_string(wc[(size_t)i * NUM_BUCKETS + 1]);
+        line += " decide us max/mean over workgroups:";
+        for (uint32_t i = 0; i < ns; i++) {
+            char buf[48];
+            std::snprintf(buf, sizeof(buf), " %.1f/%.1f", wc[(size_t)i * NUM_BUCKETS + 2] / 100.0,
+                          wc[(size_t)i * NUM_BUCKETS + 3] / 100.0 / grid);
+            line += buf;
+        }
         std::fprintf(stderr, "%s\n", line.c_str());
     }
 }
