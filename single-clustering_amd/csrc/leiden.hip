@@ -1683,9 +1683,12 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     {
         const uint32_t res = filtered ? sweep_resident<false, true>()
                                       : (batch ? sweep_resident<REFINE, false, true>() : sweep_resident<REFINE>());
+        // candidates stay within 4x of the heuristic: on a small level every count measures
+        // about the same (launch-bound), but each step's barrier and compaction grow with it
         const uint32_t lo = std::max<uint32_t>(8u, heur / 4);
+        const uint64_t hi = std::min<uint64_t>(res, 4ull * heur);
         for (uint32_t c : {heur / 2, heur * 2, res})
-            if (c >= lo && c <= res && std::find(grids.begin(), grids.end(), c) == grids.end()) grids.push_back(c);
+            if (c >= lo && c <= hi && std::find(grids.begin(), grids.end(), c) == grids.end()) grids.push_back(c);
     }
     const std::string tkey = std::string(REFINE ? "sweep/refine" : (filtered ? "sweep/move-filtered" : "sweep/move")) +
                              (batch ? "-batch" : "") + "/n2^" + std::to_string(bits_for(n)) + "/e2^" +
