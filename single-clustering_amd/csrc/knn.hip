@@ -932,8 +932,9 @@ __global__ void k_knn_remap(const uint32_t *perm, uint32_t rows, uint32_t per_ro
 // root. One workgroup per row block, one thread per row with its coordinates in registers; the
 // block's points pass through LDS in chunks of 64. seed_d2[row] gets the row's own bound on its
 // k-th nearest squared distance (+inf when it has none).
-constexpr int KTH_MAXK = 32;
-template <int DMAX>
+// KK: entries of the per-row insertion network (KK >= k; k is at most FB_MAXK).
+static_assert(FB_MAXK <= 32, "the widest insertion network holds 32 entries");
+template <int DMAX, int KK>
 __global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_t *perm, uint32_t d, uint32_t k,
                                                    uint32_t nrb, double *uR, double *seed_d2) {
     __shared__ float sp[64 * DMAX];
@@ -946,9 +947,9 @@ __global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_
         float xr[DMAX];
 #pragma unroll
         for (int t = 0; t < DMAX; t++) xr[t] = (real && (uint32_t)t < d) ? X[(uint64_t)my * d + t] : 0.0f;
-        float best[KTH_MAXK];
+        float best[KK];
 #pragma unroll
-        for (int q = 0; q < KTH_MAXK; q++) best[q] = INFINITY;
+        for (int q = 0; q < KK; q++) best[q] = INFINITY;
         float kcur = INFINITY;  // best[k - 1]
         // window positions: this block's and the neighbour block's (when there is one)
         const uint64_t own = (uint64_t)R * 256;
@@ -975,14 +976,14 @@ __global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_
                 if (__any(cand)) {
                     float v = cand ? s2 : INFINITY;
 #pragma unroll
-                    for (int q = 0; q < KTH_MAXK; q++) {
+                    for (int q = 0; q < KK; q++) {
                         const bool sw = v < best[q];
                         const float tmp = best[q];
                         best[q] = sw ? v : tmp;
                         v = sw ? tmp : v;
                     }
 #pragma unroll
-                    for (int q = 0; q < KTH_MAXK; q++)
+                    for (int q = 0; q < KK; q++)
                         if ((uint32_t)q == k - 1) kcur = best[q];
                 }
             }
@@ -1186,15 +1187,17 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(ncb, 16384)), dim3(256), 0, s, d_X, layout.get(),
                            rows, d, TN, ncb, cB.get(), rB.get(), nB.get());
         KERNEL_OK();
-        if (d <= 32)
-            hipLaunchKernelGGL(k_block_kth<32>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                               layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
-        else if (d <= 64)
-            hipLaunchKernelGGL(k_block_kth<64>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                               layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
-        else
-            hipLaunchKernelGGL(k_block_kth<128>, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                               layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
+        // the coordinates held per row (DMAX >= d) and the insertion network (KK >= k) are sized
+        // to the search; the first k entries of the network are the same for any KK >= k
+        const dim3 kg(std::min<uint32_t>(row_blocks, 16384));
+        auto kth = [&](auto kern) {
+            hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
+        };
+        const bool k16 = k <= 16;
+        if (d <= 32) kth(k16 ? k_block_kth<32, 16> : k_block_kth<32, 32>);
+        else if (d <= 56) kth(k16 ? k_block_kth<56, 16> : k_block_kth<56, 32>);
+        else if (d <= 64) kth(k16 ? k_block_kth<64, 16> : k_block_kth<64, 32>);
+        else kth(k16 ? k_block_kth<128, 16> : k_block_kth<128, 32>);
         KERNEL_OK();
         hipLaunchKernelGGL(k_fill_u32_knn, dim3(1), dim3(64), 0, s, cnt.get() + row_blocks, 1u, 0u);
         KERNEL_OK();
