@@ -687,6 +687,52 @@ __global__ void __launch_bounds__(256) k_kmeans_assign(const float *X, uint32_t 
     }
 }
 
+// The same assignment with the point's coordinates in registers (DMAX >= d), loaded once instead
+// of once per centroid chunk; coordinates t >= d and absent centroids of the last chunk are
+// zero in LDS, and a zero term leaves the FMA chain unchanged, so the sums equal the kernel above.
+template <int DMAX>
+__global__ void __launch_bounds__(256) k_kmeans_assign_r(const float *X, uint32_t n, uint32_t d, const float *cen,
+                                                         const float *cnorm, uint32_t C, uint32_t *assign) {
+    __shared__ float sc[KMC * DMAX];
+    __shared__ float sn[KMC];
+    for (uint32_t i0 = blockIdx.x * 256u; i0 < n; i0 += gridDim.x * 256u) {  // workgroup-uniform
+        const uint32_t i = i0 + threadIdx.x;
+        const bool ok = i < n;
+        float xr[DMAX];
+#pragma unroll
+        for (int t = 0; t < DMAX; t++) xr[t] = (ok && (uint32_t)t < d) ? X[(uint64_t)i * d + t] : 0.0f;
+        float best = INFINITY;
+        uint32_t bid = 0;
+        for (uint32_t c0 = 0; c0 < C; c0 += KMC) {
+            const uint32_t cc = min((uint32_t)KMC, C - c0);
+            __syncthreads();
+            for (uint32_t e = threadIdx.x; e < (uint32_t)KMC * DMAX; e += 256) {
+                const uint32_t c = e / DMAX, t = e % DMAX;
+                sc[t * KMC + c] = (c < cc && t < d) ? cen[(uint64_t)(c0 + c) * d + t] : 0.0f;
+            }
+            if (threadIdx.x < KMC) sn[threadIdx.x] = threadIdx.x < cc ? cnorm[c0 + threadIdx.x] : 0.0f;
+            __syncthreads();
+            float acc[KMC];
+#pragma unroll
+            for (int c = 0; c < KMC; c++) acc[c] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < DMAX; t++) {
+#pragma unroll
+                for (int c = 0; c < KMC; c++) acc[c] = fmaf(xr[t], sc[t * KMC + c], acc[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < KMC; c++) {
+                const float dist = sn[c] - 2.0f * acc[c];
+                if ((uint32_t)c < cc && dist < best) {
+                    best = dist;
+                    bid = c0 + (uint32_t)c;
+                }
+            }
+        }
+        if (ok) assign[i] = bid;
+    }
+}
+
 // start[g] = first position of the sorted keys holding a key >= g (g = 0..G).
 __global__ void k_key_starts_u32(const uint32_t *keys, uint32_t n, uint32_t G, uint32_t *start) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
@@ -1038,8 +1084,15 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
     for (int it = 0; it <= ITERS; it++) {
         hipLaunchKernelGGL(k_kmeans_norms, dim3(grid_for(C, 256, 1024)), dim3(256), 0, s, cen.get(), d, C, cnorm.get());
         KERNEL_OK();
-        hipLaunchKernelGGL(k_kmeans_assign, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, n, d, cen.get(),
-                           cnorm.get(), C, keys.get());
+        const dim3 ag(grid_for(n, 256, 8192));
+        if (d <= 32)
+            hipLaunchKernelGGL(k_kmeans_assign_r<32>, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+        else if (d <= 56)
+            hipLaunchKernelGGL(k_kmeans_assign_r<56>, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+        else if (d <= 64)
+            hipLaunchKernelGGL(k_kmeans_assign_r<64>, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+        else
+            hipLaunchKernelGGL(k_kmeans_assign, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
         KERNEL_OK();
         hipLaunchKernelGGL(k_knn_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, perm.get(), n);
         KERNEL_OK();
