@@ -1339,7 +1339,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     DevBuf<unsigned long long> nres(1);
     HIP_OK(hipMemsetAsync(nfb.get(), 0, sizeof(unsigned int), s));
     HIP_OK(hipMemsetAsync(nres.get(), 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_knn_rescore, dim3(std::min<uint32_t>((n + RS_WAVES - 1) / RS_WAVES, 16384)),
+    // a few long-lived workgroups per CU walk the rows: launching one short workgroup per few rows
+    // left most of the machine idle waiting for workgroup creation
+    hipLaunchKernelGGL(k_knn_rescore, dim3(std::min<uint32_t>((n + RS_WAVES - 1) / RS_WAVES, 2048)),
                        dim3(64 * RS_WAVES), 0, s, d_X, n, d, k, K2, splits,
                        cidx.get(), cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fb.get(), nfb.get(),
                        nres.get());
