@@ -771,19 +771,39 @@ __device__ __forceinline__ double block_max_f64(double v, double *red) {
 __global__ void __launch_bounds__(256) k_kmeans_update(const float *X, uint32_t d, const uint32_t *perm,
                                                        const uint32_t *start, uint32_t C, float *cen) {
     __shared__ float part[256];
+    __shared__ uint32_t sidx[256];
     const uint32_t slices = 256 / d;  // d <= 128
     const uint32_t sl = threadIdx.x / d, t = threadIdx.x % d;
     for (uint32_t g = blockIdx.x; g < C; g += gridDim.x) {
         const uint32_t b = start[g], e = start[g + 1];
         if (b == e) continue;  // workgroup-uniform
+        // slice sl sums the group's positions q = sl, sl + slices, sl + 2 slices, ... in order,
+        // the even-numbered ones into acc and the odd-numbered ones into acc2; the point indices
+        // pass through LDS 256 at a time, so eight coordinate loads are in flight per batch
         float acc = 0.0f, acc2 = 0.0f;
-        if (sl < slices) {
-            uint32_t j = b + sl;
-            for (; j + slices < e; j += 2 * slices) {  // two loads in flight
-                acc += X[(uint64_t)perm[j] * d + t];
-                acc2 += X[(uint64_t)perm[j + slices] * d + t];
+        for (uint32_t c0 = 0; c0 < e - b; c0 += 256) {  // workgroup-uniform
+            const uint32_t cl = min(256u, e - b - c0);
+            __syncthreads();
+            if (threadIdx.x < cl) sidx[threadIdx.x] = perm[b + c0 + threadIdx.x];
+            __syncthreads();
+            if (sl < slices) {
+                uint32_t p = c0 + (sl + slices - c0 % slices) % slices;  // first position of the slice
+                uint32_t q = (p - sl) / slices;
+                for (; p < c0 + cl; p += 8 * slices, q += 8) {
+                    float v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const uint32_t pp = p + (uint32_t)u * slices;
+                        v[u] = pp < c0 + cl ? X[(uint64_t)sidx[pp - c0] * d + t] : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        if (p + (uint32_t)u * slices >= c0 + cl) break;
+                        if ((q + (uint32_t)u) & 1u) acc2 += v[u];
+                        else acc += v[u];
+                    }
+                }
             }
-            if (j < e) acc += X[(uint64_t)perm[j] * d + t];
         }
         acc += acc2;
         __syncthreads();
