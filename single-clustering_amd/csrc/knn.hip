@@ -1438,18 +1438,20 @@ __device__ __forceinline__ float gauss_w(double si, double sj, double d2) {
     return (float)(sqrt(num / den) * exp(-d2 / den));
 }
 
-// Emits (row << 32 | col, w) entries. mode 0: both directions of the unordered pair, weight from
-// the canonical (min, max) order; duplicates from the two endpoints' lists collapse later.
-// mode 1: only pairs i <= j from row i's list with w > 1e-14, both directions (:131-168).
+// Emits (row << nb | col, w) entries, nb = bits_for(n) (so the key has 2 nb bits and the
+// all-ones sentinel `sent` is above every real key). mode 0: both directions of the unordered
+// pair, weight from the canonical (min, max) order; duplicates from the two endpoints' lists
+// collapse later. mode 1: only pairs i <= j from row i's list with w > 1e-14, both directions
+// (:131-168).
 __global__ void k_gauss_emit(const uint32_t *idx, const double *d2, const double *sigma, uint32_t n, uint32_t k,
-                             int mode, uint64_t *keys, float *vals) {
+                             int mode, uint32_t nb, uint64_t sent, uint64_t *keys, float *vals) {
     for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < (uint64_t)n * k;
          q += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t i = (uint32_t)(q / k);
         uint32_t j = idx[q];
         uint64_t o = 2 * q;
-        keys[o] = ~0ull;
-        keys[o + 1] = ~0ull;
+        keys[o] = sent;
+        keys[o + 1] = sent;
         vals[o] = 0.0f;
         vals[o + 1] = 0.0f;
         if (j == KNONE) continue;
@@ -1457,41 +1459,41 @@ __global__ void k_gauss_emit(const uint32_t *idx, const double *d2, const double
             uint32_t a = i < j ? i : j, b = i < j ? j : i;
             float w = gauss_w(sigma[a], sigma[b], d2[q]);
             if (!(w > 0.0f)) continue;
-            keys[o] = ((uint64_t)a << 32) | b;
+            keys[o] = ((uint64_t)a << nb) | b;
             vals[o] = w;
-            keys[o + 1] = ((uint64_t)b << 32) | a;
+            keys[o + 1] = ((uint64_t)b << nb) | a;
             vals[o + 1] = w;
         } else {
             if (!(i <= j)) continue;
             float w = gauss_w(sigma[i], sigma[j], d2[q]);
             if (!(w > 1e-14f)) continue;
-            keys[o] = ((uint64_t)i << 32) | j;
+            keys[o] = ((uint64_t)i << nb) | j;
             vals[o] = w;
             if (i != j) {
-                keys[o + 1] = ((uint64_t)j << 32) | i;
+                keys[o + 1] = ((uint64_t)j << nb) | i;
                 vals[o + 1] = w;
             }
         }
     }
 }
 
-__global__ void k_dedup_heads(const uint64_t *keys, uint64_t m, uint32_t *head) {
+__global__ void k_dedup_heads(const uint64_t *keys, uint64_t m, uint64_t sent, uint32_t *head) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x)
-        head[e] = (keys[e] != ~0ull && (e == 0 || keys[e] != keys[e - 1])) ? 1u : 0u;
+        head[e] = (keys[e] != sent && (e == 0 || keys[e] != keys[e - 1])) ? 1u : 0u;
 }
 
 // symmetrize by max: equal keys carry equal weights here, the max keeps that explicit
 __global__ void k_dedup_emit(const uint64_t *keys, const float *vals, const uint32_t *head, const uint32_t *pos,
-                             uint64_t m, uint32_t *nbrs, float *wts, unsigned long long *row_count) {
+                             uint64_t m, uint32_t nb, uint32_t *nbrs, float *wts, unsigned long long *row_count) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
         if (!head[e]) continue;
         uint64_t key = keys[e];
         float w = vals[e];
         for (uint64_t f = e + 1; f < m && keys[f] == key; f++) w = fmaxf(w, vals[f]);
         uint32_t o = pos[e];
-        nbrs[o] = (uint32_t)(key & 0xFFFFFFFFull);
+        nbrs[o] = (uint32_t)(key & ((1ull << nb) - 1));
         wts[o] = w;
-        atomicAdd(&row_count[key >> 32], 1ull);
+        atomicAdd(&row_count[key >> nb], 1ull);
     }
 }
 
@@ -1503,12 +1505,15 @@ std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const dou
     uint64_t m = 2ull * n * k;
     DevBuf<uint64_t> keys(m);
     DevBuf<float> vals(m);
+    // (row, col) keys of 2 nb bits: the sort needs ceil(2 nb / 8) digit passes, not 8
+    const uint32_t nb = (uint32_t)std::max(1, bits_for(n));
+    const uint64_t sent = 2 * nb >= 64 ? ~0ull : (1ull << (2 * nb)) - 1;
     hipLaunchKernelGGL(k_gauss_emit, dim3(grid_for((uint64_t)n * k, 256, 8192)), dim3(256), 0, s, d_idx, d_d2,
-                       sigma.get(), n, k, mode, keys.get(), vals.get());
+                       sigma.get(), n, k, mode, nb, sent, keys.get(), vals.get());
     KERNEL_OK();
-    radix_sort_pairs(keys, vals, m, 64, s);
+    radix_sort_pairs(keys, vals, m, (int)(2 * nb), s);
     DevBuf<uint32_t> head(m), pos(m);
-    hipLaunchKernelGGL(k_dedup_heads, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, keys.get(), m, head.get());
+    hipLaunchKernelGGL(k_dedup_heads, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, keys.get(), m, sent, head.get());
     KERNEL_OK();
     exclusive_scan_u32(head.get(), pos.get(), m, s);
     uint32_t lp = 0, lh = 0;
@@ -1524,7 +1529,7 @@ std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const dou
     DevBuf<unsigned long long> rc(n + 1);
     HIP_OK(hipMemsetAsync(rc.get(), 0, (n + 1) * sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_dedup_emit, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, keys.get(), vals.get(), head.get(),
-                       pos.get(), m, g->nbrs.get(), g->wts.get(), rc.get());
+                       pos.get(), m, nb, g->nbrs.get(), g->wts.get(), rc.get());
     KERNEL_OK();
     g->ptrs.resize(n + 1);
     exclusive_scan_u64(reinterpret_cast<uint64_t *>(rc.get()), g->ptrs.get(), n + 1, s);
