@@ -191,8 +191,8 @@ def main():
                           "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see the sharding paragraph of DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(kNN would shard by query rows, batched runs by job, Leiden by vertex range)", file=sys.stderr)
         sys.exit(2)
 
     import scgpu
