@@ -34,6 +34,8 @@ def knn_prune(monkeypatch):
     (1500, 50, 15, 10, 16.0),   # a few row blocks, clusters far apart: most column blocks skipped
     (5000, 30, 7, 40, 9.0),     # clusters closer together than their radius
     (4000, 8, 10, 1, 1.0),      # one Gaussian blob: hardly anything skipped
+    (3000, 60, 20, 12, 9.0),    # k > 16 (32-entry bound network), 57-64 coordinates in registers
+    (2500, 100, 25, 8, 12.0),   # more than 64 coordinates (the generic assignment and bound kernels)
 ])
 def test_knn_pruned_exact_vs_oracle(knn_prune, precision, n, d, k, comps, spread):
     # block pruning (k-means grouped layout, bounding spheres) keeps the brute-force result:
