@@ -1063,10 +1063,16 @@ __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint
 
 __global__ void k_stamp_clock(long long *out) { *out = wall_clock64(); }
 
-// After a sweep: its iterations, moves and barrier-timeout flag join the run's counters, and the
-// barrier words and result words are zeroed for the next sweep (no memset between sweeps).
+// After a sweep: its iterations, moves, decide statistics and barrier-timeout flag join the run's
+// counters, and the barrier words, result words and statistics are zeroed for the next sweep (no
+// memset between sweeps).
 __global__ void k_sweep_collect(unsigned int *ctl, unsigned int *bc_words, uint32_t n_bc_words, uint32_t err_word,
-                                unsigned long long *run, unsigned long long *run_err) {
+                                unsigned long long *run, unsigned long long *run_err, unsigned long long *stat,
+                                unsigned long long *run_stat) {
+    if (threadIdx.x >= 64 && threadIdx.x < 67) {
+        run_stat[threadIdx.x - 64] += stat[threadIdx.x - 64];
+        stat[threadIdx.x - 64] = 0;
+    }
     if (threadIdx.x == 0) {
         run[0] += ctl[2];
         run[1] += ctl[3];
@@ -1735,7 +1741,8 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     hipLaunchKernelGGL(k_sweep_collect, dim3(1), dim3(256), 0, s, A.ctl, reinterpret_cast<unsigned int *>(A.bc),
                        (uint32_t)(sizeof(BarrierCtl) / sizeof(unsigned int)),
                        (uint32_t)(offsetof(BarrierCtl, err) / sizeof(unsigned int)),
-                       ctx.counters.get() + RUN_SWEEP + (REFINE ? 2 : 0), ctx.counters.get() + RUN_ERR);
+                       ctx.counters.get() + RUN_SWEEP + (REFINE ? 2 : 0), ctx.counters.get() + RUN_ERR, ls.stat.get(),
+                       ctx.counters.get() + RUN_STAT + (REFINE ? 3 : 0));
     KERNEL_OK();
     if (sweep_trace_on()) {
         sync_stream(s);
@@ -1870,8 +1877,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         a.goff = b.goff.get();
     }
     if (persistent_sweeps() || ctx.vjob || ctx.fopt) {
-        run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz);
-        harvest_stats(ls, ctx, true);
+        run_sweep<false>(a, b, ls, ctx, level_seed, memb, L.nnz);  // its collect kernel takes the statistics
         return;
     }
     init_totals(L, memb, ls, sK, sW, s);
@@ -1971,8 +1977,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
         a.goff = b.goff.get();
     }
     if (persistent_sweeps() || ctx.vjob) {
-        run_sweep<true>(a, b, ls, ctx, level_seed, rmemb, L.nnz);
-        harvest_stats(ls, ctx, false);
+        run_sweep<true>(a, b, ls, ctx, level_seed, rmemb, L.nnz);  // its collect kernel takes the statistics
         return;
     }
     hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rmemb, n);
