@@ -7,6 +7,10 @@
  *
  * Graph handles are immutable after construction and device-resident; concurrent sc_leiden
  * calls on one handle are safe (each call owns its workspace and stream).
+ *
+ * Size limits: fewer than 2^32 - 1 nodes and fewer than 2^32 - 1 stored adjacency entries per
+ * graph (for a batch: per union of all jobs; for kNN graphs: 2 * n * k). Larger inputs return
+ * SC_TOO_MANY_NODES or SC_INVALID_ARGUMENT before any device work.
  */
 #ifndef SCGPU_H
 #define SCGPU_H
@@ -116,6 +120,11 @@ typedef struct sc_knn_stats {
 const char *sc_version(void);
 /* Releases cached device memory; optional. */
 void sc_shutdown(void);
+/* No counterpart in the reference (diagnostic). Device bytes held by the calling thread's
+ * allocator: blocks in use (live) and blocks cached for reuse. A block allocated on one thread
+ * and released on another (a graph built on one thread and freed on another) leaves the
+ * allocating thread's pool and is returned to the device at once. Either pointer may be NULL. */
+void sc_pool_bytes(uint64_t *live_out, uint64_t *cached_out);
 /* Records device time of the hot kernels into sc_leiden_stats (adds HIP events around each
  * local-move decide launch). Also switched on by the environment variable SCGPU_PROFILE=1. */
 void sc_set_profiling(int32_t enable);

@@ -39,22 +39,37 @@ struct ScError {
 
 void fill_error(sc_error *err, const ScError &e);
 
+// Call only inside a catch (...) block of an extern "C" entry point: maps the exception in
+// flight to an sc_status and fills err. ScError keeps its code; std::bad_alloc (host memory)
+// becomes SC_OUT_OF_MEMORY; anything else SC_DEVICE_ERROR with its text. No exception crosses
+// the C ABI.
+sc_status status_from_exception(sc_error *err) noexcept;
+
 // ---- caching device allocator -----------------------------------------------------------
 // hipMalloc is slow and synchronizing; blocks are cached by rounded size and reused. A block
 // is returned to the cache only after the owning call has synchronized its stream.
 class DevicePool {
   public:
+    // The calling thread's pool.
     static DevicePool &get();
     void *alloc(size_t bytes);
-    void release(void *p);
+    // Returns a block to its owner: cached for reuse when the calling thread owns the pool,
+    // hipFree'd at once otherwise.
+    static void return_block(DevicePool *owner, void *p);
     void trim();
-    size_t cached_bytes() const { return cached_; }
+    size_t cached_bytes();
+    size_t live_bytes();
+    // Frees every block, cached or live (process-exit teardown only).
+    void release_all();
 
   private:
+    void release(void *p);
+    void release_foreign(void *p);
     std::mutex mu_;
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
     size_t cached_ = 0;
+    size_t live_bytes_ = 0;
 };
 
 template <typename T>
@@ -64,21 +79,28 @@ class DevBuf {
     explicit DevBuf(size_t n) { resize(n); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_), pool_(o.pool_) { o.p_ = nullptr; o.n_ = 0; o.pool_ = nullptr; }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { reset(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            reset();
+            p_ = o.p_; n_ = o.n_; pool_ = o.pool_;
+            o.p_ = nullptr; o.n_ = 0; o.pool_ = nullptr;
+        }
         return *this;
     }
     ~DevBuf() { reset(); }
     void resize(size_t n) {
         reset();
+        DevicePool &pool = DevicePool::get();
+        p_ = static_cast<T *>(pool.alloc((n ? n : 1) * sizeof(T)));
         n_ = n;
-        p_ = static_cast<T *>(DevicePool::get().alloc((n ? n : 1) * sizeof(T)));
+        pool_ = &pool;
     }
     void reset() {
-        if (p_) DevicePool::get().release(p_);
+        if (p_) DevicePool::return_block(pool_, p_);
         p_ = nullptr;
         n_ = 0;
+        pool_ = nullptr;
     }
     T *get() const { return p_; }
     size_t size() const { return n_; }
@@ -87,6 +109,7 @@ class DevBuf {
   private:
     T *p_ = nullptr;
     size_t n_ = 0;
+    DevicePool *pool_ = nullptr;  // the pool (of the allocating thread) the block belongs to
 };
 
 // Stream owned by the calling thread (default priority), created lazily.
@@ -152,7 +175,13 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 65535u * 16u
     return (unsigned)g;
 }
 
-// Device-wide helpers implemented with hipcub in util.hip.
+// Stored entries of any graph on the device stay below this bound: entry positions, run
+// indices and scans over entries are 32-bit. check_entry_count throws SC_INVALID_ARGUMENT
+// for larger graphs before any device work.
+constexpr uint64_t MAX_ENTRIES = 0xFFFFFFFFull;
+void check_entry_count(uint64_t nnz);
+
+// Device-wide helpers implemented with rocprim in util.hip.
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s);
 void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s);
 void radix_sort_pairs_u64_f32(uint64_t *keys, float *vals, uint64_t n, int end_bit, hipStream_t s);

@@ -478,8 +478,10 @@ static sc_status build_from_csr(const uint64_t *ptrs, uint64_t n, const uint32_t
         return SC_INVALID_ARGUMENT;
     }
     *out = nullptr;
-    sc_graph *g = new sc_graph();
+    sc_graph *g = nullptr;
     try {
+        check_entry_count(nnz);
+        g = new sc_graph();
         hipStream_t s = thread_stream();
         upload_graph(*g, ptrs, n, nbrs, wts, nnz, node_w, device_src, s);
         validate_csr_device(g->ptrs.get(), n, g->nbrs.get(), g->wts.get(), nnz,
@@ -488,10 +490,9 @@ static sc_status build_from_csr(const uint64_t *ptrs, uint64_t n, const uint32_t
         sync_stream(s);
         *out = g;
         return SC_OK;
-    } catch (const ScError &e) {
+    } catch (...) {
         delete g;
-        fill_error(err, e);
-        return e.code;
+        return status_from_exception(err);
     }
 }
 
@@ -544,9 +545,8 @@ sc_status sc_graph_from_csr_export(const char *dir, sc_graph **out, sc_error *er
         if (nbrs.size() != wts.size())
             throw ScError(SC_INVALID_CSR, "malformed CSR input: indices.bin has " + std::to_string(nbrs.size()) +
                                               " entries but data.bin has " + std::to_string(wts.size()));
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
     // the row offsets, ids and weights are checked as from_csr_parts checks them
     return build_from_csr(ptrs.data(), ptrs.size() - 1, nbrs.data(), wts.data(), nbrs.size(), nullptr, false, out,
@@ -628,6 +628,7 @@ sc_status sc_graph_from_edges(uint64_t n, const uint64_t *from, const uint64_t *
         std::vector<uint64_t> ptrs(n + 1, 0);
         for (uint64_t v = 0; v < n; v++) ptrs[v + 1] = ptrs[v] + counts[v];
         uint64_t slots = ptrs[n];
+        check_entry_count(slots);
         std::vector<uint32_t> nbrs(slots);
         std::vector<float> wts(slots);
         std::vector<uint64_t> cur(ptrs.begin(), ptrs.end() - 1);
@@ -649,9 +650,8 @@ sc_status sc_graph_from_edges(uint64_t n, const uint64_t *from, const uint64_t *
         }
         *out = g;
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -680,9 +680,8 @@ sc_status sc_graph_to_csr_parts(const sc_graph *g, uint64_t *node_ptrs, uint32_t
         if (strengths) to_host(strengths, g->strength.get(), g->n, s);
         sync_stream(s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -715,9 +714,8 @@ sc_status sc_graph_validate_symmetry(const sc_graph *g, sc_error *err) {
         e.payload.edge_v = u;
         e.payload.weight = w;
         throw e;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 

@@ -1565,9 +1565,8 @@ sc_status sc_knn(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc
         if (d2_out) to_host(d2_out, d2.get(), (size_t)n * k, s);
         sync_stream(s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -1576,6 +1575,7 @@ sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k, co
     try {
         if (!data || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
         if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        check_entry_count(2ull * n * k);  // the Gaussian assembly's entries, before any search
         *out = nullptr;
         hipStream_t s = thread_stream();
         DevBuf<float> X((size_t)n * d);
@@ -1586,9 +1586,8 @@ sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k, co
         std::unique_ptr<sc_graph> g = gaussian_graph_device(idx.get(), d2.get(), (uint32_t)n, k, knn_mode(opts), s);
         *out = g.release();
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -1597,6 +1596,7 @@ sc_status sc_knn_graph_device(const float *d_data, uint64_t n, uint32_t d, uint3
     try {
         if (!d_data || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
         if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        check_entry_count(2ull * n * k);  // the Gaussian assembly's entries, before any search
         *out = nullptr;
         hipStream_t s = thread_stream();
         DevBuf<uint32_t> idx((size_t)n * k);
@@ -1605,9 +1605,8 @@ sc_status sc_knn_graph_device(const float *d_data, uint64_t n, uint32_t d, uint3
         std::unique_ptr<sc_graph> g = gaussian_graph_device(idx.get(), d2.get(), (uint32_t)n, k, knn_mode(opts), s);
         *out = g.release();
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -1618,6 +1617,7 @@ sc_status sc_knn_gaussian_csr(const uint32_t *idx, const double *d2, uint64_t n,
         if (!idx || !d2 || !node_ptrs || !neighbors || !weights || !nnz_out)
             throw ScError(SC_INVALID_ARGUMENT, "null argument");
         if (k == 0 || k > 32) throw ScError(SC_INVALID_ARGUMENT, "k must be in [1, 32]");
+        check_entry_count(2ull * n * k);
         for (uint64_t q = 0; q < n * k; q++)
             if (idx[q] != KNONE && (uint64_t)idx[q] >= n) {
                 ScError e(SC_NODE_INDEX_OUT_OF_RANGE, "neighbour index " + std::to_string(idx[q]) +
@@ -1638,9 +1638,8 @@ sc_status sc_knn_gaussian_csr(const uint32_t *idx, const double *d2, uint64_t n,
         sync_stream(s);
         *nnz_out = g->nnz;
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 

@@ -3228,9 +3228,8 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
         }
         t_stats = st;
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -3273,9 +3272,8 @@ sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, con
         }
         t_stats = st;
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -3301,6 +3299,7 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
             E += graphs[g]->nnz * n_resolutions;
         }
         if (N > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes in the batch");
+        check_entry_count(E);
         if (N && !labels_out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
         hipStream_t s = thread_stream();
         // the union graph: job j = graph j / R at resolution j % R, vertices and entries in job order
@@ -3381,9 +3380,8 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
         }
         t_stats = st;
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -3424,9 +3422,8 @@ sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *la
         sync_stream(s);
         t_stats = st;
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -3468,9 +3465,8 @@ sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph
         sync_stream(s);
         *out = c.release();
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 

@@ -207,9 +207,8 @@ sc_status sc_mlp_grad(const float *w, const float *X, const uint8_t *y, uint64_t
         to_host(grad_out, dg.get(), MLP_PARAMS, s);
         sync_stream(s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -252,9 +251,8 @@ sc_status sc_mlp_train(const float *X, const uint8_t *y, uint64_t n, uint32_t ep
         to_host(w_inout, dw.get(), MLP_PARAMS, s);
         sync_stream(s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -283,9 +281,8 @@ sc_status sc_mlp_predict(const float *w, const float *X, uint64_t n, int32_t bf1
         to_host(out, dout.get(), n, s);
         sync_stream(s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 

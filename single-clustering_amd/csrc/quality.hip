@@ -369,7 +369,7 @@ double sc_modularity(const sc_graph *g, const uint64_t *labels, double resolutio
         double q = quality_device(*g, d.get(), 0, resolution, s);
         sync_stream(s);
         return q;
-    } catch (const ScError &) {
+    } catch (...) {
         stage_drop();
         return NAN;
     }
@@ -386,9 +386,8 @@ sc_status sc_quality(const sc_graph *g, const uint64_t *labels, int32_t objectiv
         *out = quality_device(*g, d.get(), objective == SC_RB ? 1 : 2, resolution, s);
         sync_stream(s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 
@@ -407,9 +406,8 @@ sc_status sc_disconnected_community_count(const sc_graph *g, const uint64_t *lab
         to_device(d.get(), l32.data(), n, s);
         *out = disconnected_count_device(*g, d.get(), n ? maxl + 1 : 0, s);
         return SC_OK;
-    } catch (const ScError &e) {
-        fill_error(err, e);
-        return e.code;
+    } catch (...) {
+        return status_from_exception(err);
     }
 }
 

@@ -7,7 +7,7 @@
 #include <chrono>
 #include <cstring>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/rocprim.hpp>
 
 #include <atomic>
 #include <cstdlib>
@@ -30,11 +30,128 @@ void fill_error(sc_error *err, const ScError &e) {
     std::snprintf(err->msg, sizeof(err->msg), "%s", e.msg.c_str());
 }
 
+sc_status status_from_exception(sc_error *err) noexcept {
+    try {
+        throw;
+    } catch (const ScError &e) {
+        fill_error(err, e);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        fill_error(err, ScError(SC_OUT_OF_MEMORY, "host memory allocation failed"));
+        return SC_OUT_OF_MEMORY;
+    } catch (const std::exception &x) {
+        fill_error(err, ScError(SC_DEVICE_ERROR, std::string("internal error: ") + x.what()));
+        return SC_DEVICE_ERROR;
+    } catch (...) {
+        fill_error(err, ScError(SC_DEVICE_ERROR, "internal error: unknown exception"));
+        return SC_DEVICE_ERROR;
+    }
+}
+
+// ---- process-wide registry of the library's HIP objects ------------------------------------
+// Streams, pinned staging buffers and device pools are created per host thread. They are
+// registered here so that a thread's own objects are released when the thread ends, and so that
+// everything still held is released once, in order, by an atexit handler that runs before the
+// HIP runtime's own teardown (it is registered after the runtime's first use). After that
+// teardown no HIP call is made: blocks released later are dropped.
+namespace {
+struct Registry {
+    std::mutex mu;
+    std::vector<DevicePool *> pools;
+    std::vector<hipStream_t> streams;
+    std::vector<void *> pinned;
+    bool torn_down = false;
+};
+Registry &registry() {
+    static Registry *r = new Registry();  // never destroyed: used by the atexit handler
+    return *r;
+}
+std::once_flag g_atexit_once;
+
+void teardown_at_exit() {
+    Registry &r = registry();
+    std::vector<DevicePool *> pools;
+    std::vector<hipStream_t> streams;
+    std::vector<void *> pinned;
+    {
+        std::lock_guard<std::mutex> lk(r.mu);
+        if (r.torn_down) return;
+        r.torn_down = true;
+        pools.swap(r.pools);
+        streams.swap(r.streams);
+        pinned.swap(r.pinned);
+    }
+    (void)hipDeviceSynchronize();
+    for (DevicePool *p : pools) p->release_all();
+    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+    for (void *b : pinned) (void)hipHostFree(b);
+    (void)hipGetLastError();
+}
+
+// Called after each successful creation of a HIP object: the handler then runs before the
+// runtime's own exit-time teardown, which was registered when the runtime started.
+void ensure_teardown_registered() {
+    std::call_once(g_atexit_once, [] { std::atexit(teardown_at_exit); });
+}
+
+bool torn_down() {
+    Registry &r = registry();
+    std::lock_guard<std::mutex> lk(r.mu);
+    return r.torn_down;
+}
+
+template <typename T>
+void unregister(std::vector<T> &v, T x) {
+    for (size_t i = 0; i < v.size(); i++)
+        if (v[i] == x) { v[i] = v.back(); v.pop_back(); return; }
+}
+
+// Ends with its host thread (before the process-exit handlers for the main thread): releases
+// the thread's cached blocks, its stream and its staging buffer.
+struct ThreadObjects {
+    DevicePool *pool = nullptr;
+    hipStream_t stream = nullptr;
+    char *stage = nullptr;
+    ~ThreadObjects() {
+        Registry &r = registry();
+        bool live;
+        {
+            std::lock_guard<std::mutex> lk(r.mu);
+            live = !r.torn_down;
+            if (live) {
+                if (stream) unregister(r.streams, stream);
+                if (stage) unregister(r.pinned, (void *)stage);
+            }
+        }
+        if (!live) return;
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (stage) (void)hipHostFree(stage);
+        // the pool object stays (blocks of live graphs still name it as their owner); its cached
+        // blocks are freed now
+        if (pool) pool->trim();
+        (void)hipGetLastError();
+    }
+};
+thread_local ThreadObjects t_objs;
+}  // namespace
+
 // One pool per host thread: every block is recycled on the stream of the thread that freed it,
-// so stream order alone protects reuse (work of another thread never sees a recycled block).
+// so stream order alone protects reuse (work of another thread never sees a recycled block). A
+// block freed on another thread than the one that allocated it (a graph handle built on one
+// thread and freed on another) is hipFree'd at once instead: hipFree waits for the device, so
+// no queued work of the owning thread can still use it.
 DevicePool &DevicePool::get() {
-    thread_local DevicePool *pool = new DevicePool();  // never destroyed: outlives teardown
-    return *pool;
+    DevicePool *&p = t_objs.pool;
+    if (!p) {
+        p = new DevicePool();  // never destroyed: blocks of live graphs keep a pointer to it
+        Registry &r = registry();
+        std::lock_guard<std::mutex> lk(r.mu);
+        r.pools.push_back(p);
+    }
+    return *p;
 }
 
 static size_t round_size(size_t b) {
@@ -60,6 +177,7 @@ void *DevicePool::alloc(size_t bytes) {
             free_.erase(it);
             cached_ -= have;
             live_[p] = have;
+            live_bytes_ += have;
             return p;
         }
     }
@@ -70,8 +188,10 @@ void *DevicePool::alloc(size_t bytes) {
         trim();
         HIP_OK(hipMalloc(&p, sz));
     }
+    ensure_teardown_registered();
     std::lock_guard<std::mutex> lk(mu_);
     live_[p] = sz;
+    live_bytes_ += sz;
     return p;
 }
 
@@ -81,7 +201,19 @@ void DevicePool::release(void *p) {
     if (it == live_.end()) return;
     free_.emplace(it->second, p);
     cached_ += it->second;
+    live_bytes_ -= it->second;
     live_.erase(it);
+}
+
+void DevicePool::release_foreign(void *p) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = live_.find(p);
+        if (it == live_.end()) return;
+        live_bytes_ -= it->second;
+        live_.erase(it);
+    }
+    if (!torn_down()) (void)hipFree(p);
 }
 
 void DevicePool::trim() {
@@ -91,9 +223,35 @@ void DevicePool::trim() {
     cached_ = 0;
 }
 
+void DevicePool::release_all() {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    for (auto &kv : live_) (void)hipFree(kv.first);
+    free_.clear();
+    live_.clear();
+    cached_ = 0;
+    live_bytes_ = 0;
+}
+
+void DevicePool::return_block(DevicePool *owner, void *p) {
+    if (!p || !owner) return;
+    if (torn_down()) return;  // the exit handler freed every block already
+    if (owner == &get()) owner->release(p);
+    else owner->release_foreign(p);
+}
+
+size_t DevicePool::cached_bytes() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cached_;
+}
+
+size_t DevicePool::live_bytes() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return live_bytes_;
+}
+
 namespace {
 struct HostStage {
-    char *buf = nullptr;
     bool failed = false;
     size_t used = 0;
     hipStream_t stream = nullptr;
@@ -106,7 +264,7 @@ constexpr size_t STAGE_MAX_COPY = 4096;
 void stage_flush(hipStream_t s) {
     HostStage &st = t_stage;
     if (st.pending.empty() || st.stream != s) return;
-    for (auto &p : st.pending) std::memcpy(std::get<0>(p), st.buf + std::get<1>(p), std::get<2>(p));
+    for (auto &p : st.pending) std::memcpy(std::get<0>(p), t_objs.stage + std::get<1>(p), std::get<2>(p));
     st.pending.clear();
     st.used = 0;
     st.stream = nullptr;
@@ -116,18 +274,23 @@ void stage_flush(hipStream_t s) {
 bool stage_d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
     HostStage &st = t_stage;
     if (bytes > STAGE_MAX_COPY || st.failed) return false;
-    if (!st.buf) {
-        if (hipHostMalloc((void **)&st.buf, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+    if (!t_objs.stage) {
+        char *b = nullptr;
+        if (hipHostMalloc((void **)&b, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
-            st.buf = nullptr;
             st.failed = true;
             return false;
         }
+        t_objs.stage = b;
+        ensure_teardown_registered();
+        Registry &r = registry();
+        std::lock_guard<std::mutex> lk(r.mu);
+        r.pinned.push_back(b);
     }
     if (!st.pending.empty() && st.stream != s) return false;
     const size_t off = (st.used + 15) & ~size_t(15);
     if (off + bytes > STAGE_BYTES) return false;
-    HIP_OK(hipMemcpyAsync(st.buf + off, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(t_objs.stage + off, src, bytes, hipMemcpyDeviceToHost, s));
     st.pending.emplace_back(dst, off, bytes);
     st.used = off + bytes;
     st.stream = s;
@@ -156,8 +319,14 @@ void sync_stream(hipStream_t s) {
 }
 
 hipStream_t thread_stream() {
-    thread_local hipStream_t s = nullptr;
-    if (!s) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipStream_t &s = t_objs.stream;
+    if (!s) {
+        HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ensure_teardown_registered();
+        Registry &r = registry();
+        std::lock_guard<std::mutex> lk(r.mu);
+        r.streams.push_back(s);
+    }
     return s;
 }
 
@@ -175,21 +344,30 @@ bool profiling_enabled() {
 
 void set_profiling(int on) { g_profiling.store(on ? 1 : 0); }
 
-// ---- hipcub wrappers ----------------------------------------------------------------------
-void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s) {
+// ---- rocprim wrappers ----------------------------------------------------------------------
+// Item counts are passed as size_t (rocprim's own type), so counts of 2^31 and more are not
+// truncated. The callers' 32-bit outputs (run indices, positions) stay in range because every
+// graph is limited to fewer than 2^32 - 1 stored entries (check_entry_count).
+template <typename T>
+static void exclusive_scan(const T *in, T *out, uint64_t n, hipStream_t s) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    HIP_OK(rocprim::exclusive_scan(nullptr, tmp, in, out, T(0), (size_t)n, rocprim::plus<T>(), s));
     DevBuf<unsigned char> t(tmp);
-    HIP_OK(hipcub::DeviceScan::ExclusiveSum(t.get(), tmp, in, out, (int)n, s));
+    HIP_OK(rocprim::exclusive_scan(t.get(), tmp, in, out, T(0), (size_t)n, rocprim::plus<T>(), s));
 }
 
-void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s) {
-    if (n == 0) return;
-    size_t tmp = 0;
-    HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
-    DevBuf<unsigned char> t(tmp);
-    HIP_OK(hipcub::DeviceScan::ExclusiveSum(t.get(), tmp, in, out, (int)n, s));
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s) { exclusive_scan(in, out, n, s); }
+void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s) { exclusive_scan(in, out, n, s); }
+
+void check_entry_count(uint64_t nnz) {
+    if (nnz >= (uint64_t)MAX_ENTRIES) {
+        ScError e(SC_INVALID_ARGUMENT, "a graph of " + std::to_string(nnz) +
+                                           " stored entries exceeds the device limit of 2^32 - 2 entries");
+        e.payload.got = nnz;
+        e.payload.expected = (uint64_t)MAX_ENTRIES - 1;
+        throw e;
+    }
 }
 
 // rocprim sorts up to 2^20 items of 4-byte or wider keys by block sort + merge passes (about
@@ -208,17 +386,17 @@ static bool sort_pairs_db(K *keys, V *vals, K *k2, V *v2, uint64_t n, int end_bi
     if (end_bit <= 24) {
         rocprim::double_buffer<K> kb(keys, k2);
         rocprim::double_buffer<V> vb(vals, v2);
-        HIP_OK(rocprim::radix_sort_pairs<ShortKeySort>(nullptr, tmp, kb, vb, n, 0u, (unsigned)end_bit, s));
+        HIP_OK(rocprim::radix_sort_pairs<ShortKeySort>(nullptr, tmp, kb, vb, (size_t)n, 0u, (unsigned)end_bit, s));
         DevBuf<unsigned char> t(tmp);
-        HIP_OK(rocprim::radix_sort_pairs<ShortKeySort>(t.get(), tmp, kb, vb, n, 0u, (unsigned)end_bit, s));
+        HIP_OK(rocprim::radix_sort_pairs<ShortKeySort>(t.get(), tmp, kb, vb, (size_t)n, 0u, (unsigned)end_bit, s));
         return kb.current() != keys;  // keys and values always move together
     }
-    hipcub::DoubleBuffer<K> kb(keys, k2);
-    hipcub::DoubleBuffer<V> vb(vals, v2);
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)n, 0, end_bit, s));
+    rocprim::double_buffer<K> kb(keys, k2);
+    rocprim::double_buffer<V> vb(vals, v2);
+    HIP_OK(rocprim::radix_sort_pairs(nullptr, tmp, kb, vb, (size_t)n, 0u, (unsigned)end_bit, s));
     DevBuf<unsigned char> t(tmp);
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, kb, vb, (int)n, 0, end_bit, s));
-    return kb.Current() != keys;
+    HIP_OK(rocprim::radix_sort_pairs(t.get(), tmp, kb, vb, (size_t)n, 0u, (unsigned)end_bit, s));
+    return kb.current() != keys;
 }
 
 template <typename K, typename V>
@@ -431,6 +609,12 @@ void sc_set_profiling(int32_t enable) { scgpu::set_profiling(enable); }
 void sc_shutdown(void) {
     (void)hipDeviceSynchronize();
     scgpu::DevicePool::get().trim();
+}
+
+void sc_pool_bytes(uint64_t *live_out, uint64_t *cached_out) {
+    scgpu::DevicePool &p = scgpu::DevicePool::get();
+    if (live_out) *live_out = p.live_bytes();
+    if (cached_out) *cached_out = p.cached_bytes();
 }
 
 }

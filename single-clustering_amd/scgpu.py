@@ -99,6 +99,7 @@ _GP = C.c_void_p
 SIGNATURES = {
     "sc_version": ([], C.c_char_p),
     "sc_shutdown": ([], None),
+    "sc_pool_bytes": ([u64p, u64p], None),
     "sc_set_profiling": ([C.c_int32], None),
     "sc_set_sweep_mode": ([C.c_int32], C.c_int),
     "sc_leiden_config_default": ([], ScLeidenConfig),
@@ -218,6 +219,13 @@ def lib():
 
 def version():
     return _lib.sc_version().decode()
+
+
+def pool_bytes():
+    """(live, cached) device bytes of the calling thread's allocator (diagnostic)."""
+    live, cached = C.c_uint64(), C.c_uint64()
+    _lib.sc_pool_bytes(C.byref(live), C.byref(cached))
+    return live.value, cached.value
 
 
 def set_profiling(on):

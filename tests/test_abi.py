@@ -43,3 +43,31 @@ def test_default_config_mirrors_reference_default():
     assert (c.objective, c.resolution, c.has_seed, c.seed) == (0, 1.0, 1, 42)
     assert (c.n_iterations, c.max_levels, c.tolerance) == (2, 64, 1e-9)
     assert (c.refine, c.refine_randomness, c.patience, c.has_max_community_weight) == (1, 0.01, 2, 0)
+
+
+def test_entry_count_guard_before_device_work():
+    """Graphs of 2^32 - 1 or more stored entries are refused on the host (entry positions and
+    scans over entries are 32-bit on the device); no array is read and no GPU is touched."""
+    import ctypes as C
+
+    import numpy as np
+    import scgpu
+
+    L = scgpu.lib()
+    ptrs = np.zeros(2, np.uint64)
+    nbrs = np.zeros(1, np.uint32)
+    wts = np.zeros(1, np.float32)
+    for nnz in (1 << 32, (1 << 32) - 1):
+        h = C.c_void_p()
+        err = scgpu.ScError()
+        st = L.sc_graph_from_csr_parts(ptrs.ctypes.data_as(scgpu.u64p), 1, nbrs.ctypes.data_as(scgpu.u32p),
+                                       wts.ctypes.data_as(scgpu.f32p), nnz, None, C.byref(h), C.byref(err))
+        assert st == 101 and err.code == 101 and not h.value
+        assert err.got == nnz and err.expected == (1 << 32) - 2
+        assert b"2^32 - 2" in err.msg
+    # kNN graphs hold 2 n k entries before merging: refused before the search
+    x = np.zeros((1, 1), np.float32)
+    h = C.c_void_p()
+    err = scgpu.ScError()
+    st = L.sc_knn_graph(x.ctypes.data_as(scgpu.f32p), 1 << 31, 1, 2, None, C.byref(h), None, C.byref(err))
+    assert st == 101 and err.got == 1 << 33

@@ -249,3 +249,40 @@ def test_csr_export_reader_and_real_data_driver(tmp_path):
         assert 0.0 <= r["nmi"] <= 1.0 + 1e-12 and r["ours_k"] > 0
     assert rows[1]["ari"] > 0.9  # six well-separated components at resolution 1
     assert lines[2] == "exhaustive symmetry check: OK" and lines[-1].startswith("best agreement")
+
+
+def test_graph_freed_on_another_thread_returns_its_memory():
+    """A handle built on one thread and freed on another: its blocks leave the building thread's
+    allocator (live bytes back to where they were) and are not cached there (they go back to the
+    device), so nothing leaks into a pool that may never run again."""
+    import threading
+
+    x, _ = W.gaussian_mixture(3000, 8, 5, 9.0, seed=1)
+    box = {}
+    built, freed = threading.Event(), threading.Event()
+
+    def worker():
+        box["before"] = scgpu.pool_bytes()
+        box["g"] = scgpu.knn_graph(x, 6)
+        box["with_graph"] = scgpu.pool_bytes()
+        built.set()
+        freed.wait(60)
+        box["after"] = scgpu.pool_bytes()
+
+    t = threading.Thread(target=worker)
+    t.start()
+    assert built.wait(120)
+    g = box.pop("g")
+    assert g.node_count() == 3000
+    del g                                  # sc_graph_free on the main thread
+    freed.set()
+    t.join(60)
+    (live0, _), (live1, cached1), (live2, cached2) = box["before"], box["with_graph"], box["after"]
+    assert live1 > live0                    # the graph's arrays were live in the worker's pool
+    assert live2 == live0                   # released from the worker's pool ...
+    assert cached2 == cached1               # ... and not cached there
+    # the main thread's own pool is untouched by the foreign release
+    m_live, m_cached = scgpu.pool_bytes()
+    g2 = scgpu.knn_graph(x, 6)
+    del g2
+    assert scgpu.pool_bytes()[0] == m_live
