@@ -273,6 +273,18 @@ double sc_modularity(const sc_graph *g, const uint64_t *labels, double resolutio
 sc_status sc_quality(const sc_graph *g, const uint64_t *labels, int32_t objective,
                      double resolution, double *out, sc_error *err);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc). Same result as sc_quality, bit for bit (the
+ * same kernels and summation order), but d_labels is a device pointer (n entries) and every
+ * working buffer lives in one file-scope device array of the library (~32 MiB per GPU, held
+ * until the process ends): the call makes no hipMalloc / hipFree and creates no stream. It runs
+ * on the null stream and synchronizes before returning. Limit: labels below 2^20 (1,048,576
+ * communities); a larger label returns SC_INVALID_ARGUMENT (payload got = the label) before the
+ * per-community work. Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several host threads) race and must be serialized by the caller. */
+sc_status sc_quality_noalloc(const sc_graph *g, const uint64_t *d_labels, int32_t objective,
+                             double resolution, double *out, sc_error *err);
+
 /* disconnected_community_count (tests/common/mod.rs:171-194): communities with more than one
  * connected component using same-label edges only. */
 sc_status sc_disconnected_community_count(const sc_graph *g, const uint64_t *labels,

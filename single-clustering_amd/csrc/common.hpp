@@ -192,7 +192,14 @@ void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int en
 void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<float> &vals, uint64_t n, int end_bit, hipStream_t s);
 void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals, uint64_t n, int end_bit, hipStream_t s);
 void radix_sort_pairs(DevBuf<uint32_t> &keys, DevBuf<uint64_t> &vals, uint64_t n, int end_bit, hipStream_t s);
-// Deterministic sum of n doubles (fixed two-level tree).
+// Deterministic sum of n doubles (fixed two-level tree: RED_BLOCKS strided partial sums of
+// RED_THREADS threads, then one block).
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+constexpr uint64_t RED_PARTIALS = RED_BLOCKS + 1;
+// The same sum (identical association order) into caller-provided partials of RED_PARTIALS
+// doubles, n >= 1: the result is left in part[RED_BLOCKS]; no allocation, no synchronization.
+void device_sum_f64_async(const double *in, uint64_t n, double *part, hipStream_t s);
 double device_sum_f64(const double *in, uint64_t n, hipStream_t s);
 double device_max_f64(const double *in, uint64_t n, hipStream_t s);
 // The same four fixed-shape reductions as device_sum_f64 / device_max_f64 (identical

@@ -354,11 +354,106 @@ uint64_t disconnected_count_device(const sc_graph &g, const uint32_t *d_labels32
     return out;
 }
 
+// ---- allocation-free quality ----------------------------------------------------------------
+// A variant of sc_quality for callers that cannot allocate device memory. Its working memory is
+// one file-scope device array per GPU, laid out as: the largest label (8 bytes), the partials
+// of the fixed-shape sum, the per-community fixed-point totals (in, K, W: 3 x 8 bytes per
+// community) and the per-community terms (8 bytes per community).
+constexpr uint64_t NOALLOC_MAX_COMMUNITIES = 1ull << 20;
+constexpr size_t NA_MAX_OFF = 0;
+constexpr size_t NA_PART_OFF = 256;
+constexpr size_t NA_ACC_OFF = (NA_PART_OFF + RED_PARTIALS * sizeof(double) + 255) / 256 * 256;
+constexpr size_t NA_TERM_OFF = NA_ACC_OFF + 3 * NOALLOC_MAX_COMMUNITIES * sizeof(long long);
+constexpr size_t NA_BYTES = NA_TERM_OFF + NOALLOC_MAX_COMMUNITIES * sizeof(double);  // ~32 MiB
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Device address of this GPU's copy of the array (the array's name is a host-side stand-in and
+// must not reach a kernel): looked up once per device, with that device current.
+static unsigned char *scratch_address() {
+    static std::mutex mu;
+    static unsigned char *addr[64] = {nullptr};
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw ScError(SC_DEVICE_ERROR, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev] = static_cast<unsigned char *>(p);
+    }
+    return addr[dev];
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// quality_device's computation (same kernels, grids and summation order, so the result is
+// bit-identical) with every buffer in the scratch array. Zeroing is done by kernels, the only
+// copies are two 8-byte readbacks into host memory, and the stream is synchronized twice.
+static double quality_noalloc(const sc_graph &g, const uint64_t *d_labels, int mode, double gamma, hipStream_t s) {
+    const uint64_t n = g.n;
+    if (n == 0) return 0.0;
+    unsigned char *base = scratch_address();
+    unsigned long long *mx = reinterpret_cast<unsigned long long *>(base + NA_MAX_OFF);
+    double *part = reinterpret_cast<double *>(base + NA_PART_OFF);
+    long long *acc = reinterpret_cast<long long *>(base + NA_ACC_OFF);
+    double *term = reinterpret_cast<double *>(base + NA_TERM_OFF);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, mx, (uint64_t)1);
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_max_label, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d_labels, n, mx);
+    KERNEL_OK();
+    unsigned long long max_label = 0;
+    HIP_OK(hipMemcpyAsync(&max_label, mx, sizeof(max_label), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (max_label >= NOALLOC_MAX_COMMUNITIES) {
+        ScError e(SC_INVALID_ARGUMENT, "label " + std::to_string(max_label) +
+                                           " exceeds the allocation-free limit: labels must be below 2^20 (1048576)");
+        e.payload.got = max_label;
+        e.payload.expected = NOALLOC_MAX_COMMUNITIES - 1;
+        throw e;
+    }
+    const uint64_t nl = max_label + 1;
+    const int s_in = fixed_scale_for(g.total_weight + 1e-300);
+    const int sK = fixed_scale_for(2.0 * g.total_weight + 1e-300);
+    const int sW = fixed_scale_for(g.total_node_weight + 1e-300);
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(3 * nl, 256, 4096)), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long *>(acc), 3 * nl);
+    KERNEL_OK();
+    hipLaunchKernelGGL((k_vertex_terms<8>), dim3(grid_for(n, 32, 1024)), dim3(256), 0, s, d_labels, n, g.ptrs.get(),
+                       g.nbrs.get(), g.wts.get(), g.strength.get(), g.node_w.get(), s_in, sK, sW, acc, acc + nl,
+                       acc + 2 * nl);
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_community_terms, dim3(grid_for(nl, 256, 4096)), dim3(256), 0, s, acc, acc + nl,
+                       acc + 2 * nl, nl, s_in, sK, sW, mode, gamma, g.total_weight, term);
+    KERNEL_OK();
+    device_sum_f64_async(term, nl, part, s);
+    double q = 0.0;
+    HIP_OK(hipMemcpyAsync(&q, part + RED_BLOCKS, sizeof(q), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return q;
+}
+
 }  // namespace scgpu
 
 using namespace scgpu;
 
 extern "C" {
+
+sc_status sc_quality_noalloc(const sc_graph *g, const uint64_t *d_labels, int32_t objective, double resolution,
+                             double *out, sc_error *err) {
+    try {
+        if (!g || !out || (!d_labels && g->n)) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        if (objective != SC_RB && objective != SC_CPM) throw ScError(SC_INVALID_ARGUMENT, "unknown objective");
+        // the default (null) stream: nothing is created on first use
+        *out = quality_noalloc(*g, d_labels, objective == SC_RB ? 1 : 2, resolution, hipStream_t(0));
+        return SC_OK;
+    } catch (...) {
+        return status_from_exception(err);
+    }
+}
 
 double sc_modularity(const sc_graph *g, const uint64_t *labels, double resolution) {
     try {

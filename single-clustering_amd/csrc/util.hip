@@ -448,8 +448,6 @@ void radix_sort_pairs_u32_u32(uint32_t *keys, uint32_t *vals, uint64_t n, int en
 // ---- deterministic reductions -----------------------------------------------------------
 // Fixed shape: 1024 blocks x 256 threads of strided partial sums, then one block. The
 // association order depends only on n, so results are bit-reproducible.
-constexpr int RED_BLOCKS = 1024;
-constexpr int RED_THREADS = 256;
 
 template <bool MAX>
 __global__ void __launch_bounds__(RED_THREADS) k_reduce_stage(const double *in, uint64_t n, double *out) {
@@ -473,22 +471,30 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_stage(const double *in, 
 }
 
 template <bool MAX>
+static void reduce_f64_launch(const double *in, uint64_t n, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, in, n, part);
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(1), dim3(RED_THREADS), 0, s, part, (uint64_t)RED_BLOCKS,
+                       part + RED_BLOCKS);
+    KERNEL_OK();
+}
+
+template <bool MAX>
 static double reduce_f64(const double *in, uint64_t n, hipStream_t s) {
     if (n == 0) {
         sync_stream(s);  // like the general path: the stream is synchronized on return
         return MAX ? -INFINITY : 0.0;
     }
-    DevBuf<double> part(RED_BLOCKS + 1);
-    hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, in, n,
-                       part.get());
-    KERNEL_OK();
-    hipLaunchKernelGGL(k_reduce_stage<MAX>, dim3(1), dim3(RED_THREADS), 0, s, part.get(),
-                       (uint64_t)RED_BLOCKS, part.get() + RED_BLOCKS);
-    KERNEL_OK();
+    DevBuf<double> part(RED_PARTIALS);
+    reduce_f64_launch<MAX>(in, n, part.get(), s);
     double r = 0.0;
     to_host(&r, part.get() + RED_BLOCKS, 1, s);
     sync_stream(s);
     return r;
+}
+
+void device_sum_f64_async(const double *in, uint64_t n, double *part, hipStream_t s) {
+    reduce_f64_launch<false>(in, n, part, s);
 }
 
 struct Reduce4Args {

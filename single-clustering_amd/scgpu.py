@@ -139,6 +139,7 @@ SIGNATURES = {
     "sc_leiden_last_stats": ([C.POINTER(ScLeidenStats)], C.c_int),
     "sc_modularity": ([_GP, u64p, C.c_double], C.c_double),
     "sc_quality": ([_GP, u64p, C.c_int32, C.c_double, f64p, C.POINTER(ScError)], C.c_int),
+    "sc_quality_noalloc": ([_GP, C.c_void_p, C.c_int32, C.c_double, f64p, C.POINTER(ScError)], C.c_int),
     "sc_disconnected_community_count": ([_GP, u64p, u64p, C.POINTER(ScError)], C.c_int),
     "sc_knn": ([f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(ScKnnOpts), u32p, f64p,
                 C.POINTER(ScKnnStats), C.POINTER(ScError)], C.c_int),
