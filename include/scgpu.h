@@ -8,7 +8,7 @@
  * Graph handles are immutable after construction and device-resident; concurrent sc_leiden
  * calls on one handle are safe (each call owns its workspace and stream).
  *
- * Size limits: fewer than 2^32 - 1 nodes and fewer than 2^32 - 1 stored adjacency entries per
+ * Size limits: fewer than 2^31 nodes and fewer than 2^32 - 1 stored adjacency entries per
  * graph (for a batch: per union of all jobs; for kNN graphs: 2 * n * k). Larger inputs return
  * SC_TOO_MANY_NODES or SC_INVALID_ARGUMENT before any device work.
  */
@@ -198,6 +198,12 @@ sc_status sc_graph_validate_symmetry(const sc_graph *g, sc_error *err);
  * for every node (VectorGrouping::from_assignments after normalization, grouping.rs:154-172). */
 sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph **out,
                              sc_error *err);
+
+/* Partition::renumber_into_grouping / VectorGrouping::from_assignments (partition.rs:265-306,
+ * grouping.rs:154-172, 226-254): the ids in use compacted to 0..n_groups in ascending order, on
+ * the device with the compaction every level of sc_leiden uses. labels below 2^32 - 1. */
+sc_status sc_renumber(const uint64_t *labels, uint64_t n, uint64_t *out, uint64_t *n_groups_out,
+                      sc_error *err);
 
 /* ---- clustering ------------------------------------------------------------------------ */
 /* leiden (leiden/mod.rs:168-204). labels_out: n entries, consecutive from 0 (Clustering). */

@@ -1060,6 +1060,21 @@ double or_local_move_once(const or_graph *g, uint64_t *membership, int objective
 /* Metrics: disconnected_community_count (tests/common/mod.rs:171-194), nmi (:133-168),
  * ARI (examples/real_data.rs:50-75).                                                          */
 
+uint64_t or_normalize_groups(const uint64_t *labels, uint64_t n, uint64_t *out) {
+    /* group_count = max + 1 (from_assignments, grouping.rs:154-172) */
+    uint64_t group_count = 0;
+    for (uint64_t v = 0; v < n; v++)
+        if (labels[v] + 1 > group_count) group_count = labels[v] + 1;
+    /* sizes, then ascending new ids for the groups in use (normalize_groups, :226-254) */
+    uint64_t *sizes = (uint64_t *)calloc(group_count ? group_count : 1, sizeof(uint64_t));
+    for (uint64_t v = 0; v < n; v++) sizes[labels[v]]++;
+    uint64_t next = 0;
+    for (uint64_t c = 0; c < group_count; c++) sizes[c] = sizes[c] ? next++ : UINT64_MAX;
+    for (uint64_t v = 0; v < n; v++) out[v] = sizes[labels[v]];
+    free(sizes);
+    return next;
+}
+
 uint64_t or_disconnected_community_count(const or_graph *g, const uint64_t *labels) {
     uint64_t n = g->n, nc = 0;
     for (uint64_t v = 0; v < n; v++) if (labels[v] + 1 > nc) nc = labels[v] + 1;

@@ -105,6 +105,10 @@ double or_local_move_once(const or_graph *g, uint64_t *membership, int objective
                           double resolution, uint64_t seed, int has_max, double max_w);
 
 uint64_t or_disconnected_community_count(const or_graph *g, const uint64_t *labels);
+/* VectorGrouping::from_assignments + normalize_groups (grouping.rs:154-172, 226-254), which
+ * Partition::renumber_into_grouping (partition.rs:265-306) feeds: ids in use compacted to
+ * 0..count in ascending order. Returns count. */
+uint64_t or_normalize_groups(const uint64_t *labels, uint64_t n, uint64_t *out);
 double or_nmi(const uint64_t *a, const uint64_t *b, uint64_t n);
 double or_ari(const uint64_t *a, const uint64_t *b, uint64_t n);
 

@@ -86,6 +86,8 @@ def lib():
         L.or_local_move_once.restype = C.c_double
         L.or_disconnected_community_count.argtypes = [gp, u64p]
         L.or_disconnected_community_count.restype = C.c_uint64
+        L.or_normalize_groups.argtypes = [u64p, C.c_uint64, u64p]
+        L.or_normalize_groups.restype = C.c_uint64
         L.or_nmi.argtypes = [u64p, u64p, C.c_uint64]
         L.or_nmi.restype = C.c_double
         L.or_ari.argtypes = [u64p, u64p, C.c_uint64]
@@ -256,6 +258,14 @@ def local_move_once(g, membership, objective="rb", resolution=1.0, seed=0, max_w
 def disconnected_community_count(g, labels):
     labels = np.ascontiguousarray(labels, dtype=np.uint64)
     return lib().or_disconnected_community_count(g._g, _p(labels, u64p))
+
+
+def normalize_groups(labels):
+    """VectorGrouping::from_assignments normalization: (ids compacted ascending, count)."""
+    labels = np.ascontiguousarray(labels, dtype=np.uint64)
+    out = np.zeros(len(labels), dtype=np.uint64)
+    k = lib().or_normalize_groups(_p(labels, u64p), len(labels), _p(out, u64p))
+    return out, k
 
 
 def nmi(a, b):

@@ -180,6 +180,12 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 65535u * 16u
 // for larger graphs before any device work.
 constexpr uint64_t MAX_ENTRIES = 0xFFFFFFFFull;
 void check_entry_count(uint64_t nnz);
+// Nodes (points, jobs' vertices together) of any device run: below 2^31, so 32-bit vertex
+// loops (index + grid stride) never wrap and community ids, which span [0, 2n), stay below
+// 2^32 - 1. Above u32::MAX the reference's TooManyNodes message, between the two the device
+// limit; both SC_TOO_MANY_NODES with n_nodes set.
+constexpr uint64_t MAX_NODES = (1ull << 31) - 1;
+void check_node_count(uint64_t n);
 
 // Device-wide helpers implemented with rocprim in util.hip.
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s);

@@ -59,12 +59,7 @@ static uint64_t first_bad(void (*launch)(unsigned long long *, hipStream_t), hip
 static void validate_csr_device(const uint64_t *d_ptrs, uint64_t n, const uint32_t *d_nbrs,
                                 const float *d_wts, uint64_t nnz, const double *d_nw,
                                 hipStream_t s) {
-    if (n > 0xFFFFFFFFull) {
-        ScError e(SC_TOO_MANY_NODES, "graph has " + std::to_string(n) +
-                                         " nodes, exceeding the 4294967295 addressable by u32 adjacency");
-        e.payload.n_nodes = n;
-        throw e;
-    }
+    check_node_count(n);
     uint64_t last = 0;
     to_host(&last, d_ptrs + n, 1, s);
     sync_stream(s);
@@ -480,6 +475,7 @@ static sc_status build_from_csr(const uint64_t *ptrs, uint64_t n, const uint32_t
     *out = nullptr;
     sc_graph *g = nullptr;
     try {
+        check_node_count(n);
         check_entry_count(nnz);
         g = new sc_graph();
         hipStream_t s = thread_stream();
@@ -587,11 +583,7 @@ sc_status sc_graph_from_edges(uint64_t n, const uint64_t *from, const uint64_t *
                 }
             }
         }
-        if (n > 0xFFFFFFFFull) {
-            ScError e(SC_TOO_MANY_NODES, "too many nodes");
-            e.payload.n_nodes = n;
-            throw e;
-        }
+        check_node_count(n);
         std::vector<uint64_t> counts(n, 0);
         for (uint64_t i = 0; i < m; i++) {
             uint64_t a = from[i], b = to[i];

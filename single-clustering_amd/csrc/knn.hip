@@ -1554,7 +1554,7 @@ sc_status sc_knn(const float *data, uint64_t n, uint32_t d, uint32_t k, const sc
                  double *d2_out, sc_knn_stats *stats, sc_error *err) {
     try {
         if (!data) throw ScError(SC_INVALID_ARGUMENT, "null data");
-        if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        check_node_count(n);
         hipStream_t s = thread_stream();
         DevBuf<float> X((size_t)n * d);
         to_device(X.get(), data, (size_t)n * d, s);
@@ -1574,7 +1574,7 @@ sc_status sc_knn_graph(const float *data, uint64_t n, uint32_t d, uint32_t k, co
                        sc_knn_stats *stats, sc_error *err) {
     try {
         if (!data || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
-        if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        check_node_count(n);
         check_entry_count(2ull * n * k);  // the Gaussian assembly's entries, before any search
         *out = nullptr;
         hipStream_t s = thread_stream();
@@ -1595,7 +1595,7 @@ sc_status sc_knn_graph_device(const float *d_data, uint64_t n, uint32_t d, uint3
                               sc_graph **out, sc_knn_stats *stats, sc_error *err) {
     try {
         if (!d_data || !out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
-        if (n > 0xFFFFFFF0ull) throw ScError(SC_TOO_MANY_NODES, "too many points");
+        check_node_count(n);
         check_entry_count(2ull * n * k);  // the Gaussian assembly's entries, before any search
         *out = nullptr;
         hipStream_t s = thread_stream();
@@ -1617,6 +1617,7 @@ sc_status sc_knn_gaussian_csr(const uint32_t *idx, const double *d2, uint64_t n,
         if (!idx || !d2 || !node_ptrs || !neighbors || !weights || !nnz_out)
             throw ScError(SC_INVALID_ARGUMENT, "null argument");
         if (k == 0 || k > 32) throw ScError(SC_INVALID_ARGUMENT, "k must be in [1, 32]");
+        check_node_count(n);
         check_entry_count(2ull * n * k);
         for (uint64_t q = 0; q < n * k; q++)
             if (idx[q] != KNONE && (uint64_t)idx[q] >= n) {

@@ -75,6 +75,12 @@ struct DecideArgs {
     uint32_t *movers;
     unsigned int *n_movers;
     uint32_t *targeted_cnt;
+    // local moving without a weight cap (null otherwise): joiner admission (join_admitted).
+    // jsum / jmin hold two sets of jstride entries, used by alternating steps
+    long long *jsum;
+    uint32_t *jmin;
+    double *mgain;
+    uint64_t jstride;
     // objective
     int objective;
     double gamma;
@@ -153,6 +159,33 @@ __device__ __forceinline__ double delta_insert(int objective, const VParams &p, 
         return (w_c + loop) - p.gamma * k * (2.0 * Kc + k) / p.four_m;
     }
     return (w_c + loop) - p.gamma * na * (2.0 * Wc + na - 1.0) / 2.0;
+}
+
+// Joiner admission (local moving without a weight cap). Every decision of a sub-sweep reads one
+// snapshot, so the joiners of one community each assume it unchanged; if they all land, its
+// strength (Rb) or node weight (CPM) grows by theirs. The sequential reference sees each earlier
+// joiner land (local_move.rs:58-119), so a flood of joiners that each gain on the snapshot but
+// lose together cannot happen there. A mover whose gain stays above the threshold when every
+// other joiner of its target lands moves; of the remaining joiners of a community only the lowest
+// vertex id moves (its snapshot gain holds if no one else lands), the others stay active and
+// decide again. The joiners' sums are fixed point (exact, order independent) and the lowest id is
+// an atomicMin, so admission is deterministic.
+__device__ __forceinline__ long long join_amount(const DecideArgs &a, const VParams &vp, double k, double na) {
+    return a.objective == 0 ? to_fx(k, vp.sK) : to_fx(na, vp.sW);
+}
+
+__device__ __forceinline__ bool join_admitted(const DecideArgs &a, const VParams &vp, uint32_t v, uint32_t t,
+                                              uint32_t set) {
+    const uint64_t idx = (uint64_t)set * a.jstride + t;
+    if (a.jmin[idx] == v) return true;
+    const double k = a.strength[v], na = a.node_w[v];
+    const long long others = a.jsum[idx] - join_amount(a, vp, k, na);
+    double extra;
+    if (a.objective == 0)
+        extra = vp.four_m == 0.0 ? 0.0 : vp.gamma * k * (2.0 * from_fx(others, vp.sK)) / vp.four_m;
+    else
+        extra = vp.gamma * na * from_fx(others, vp.sW);
+    return a.mgain[v] - extra > vp.threshold;
 }
 
 // Features of an active vertex for the learned filter, from the state every decision reads
@@ -242,6 +275,7 @@ struct StepParams {
     unsigned int *n_movers;  // mover count: global, or the workgroup's LDS counter with mv_list
     uint32_t *mstep;         // step stamp of accepted movers (persistent local moving), else null
     uint32_t *mv_list;       // persistent sweeps: the workgroup's own mover list (else a.movers)
+    uint32_t *mv_tgt;        // persistent local moving: the movers' targets, beside mv_list (else null)
 };
 
 struct DecideStats {
@@ -412,10 +446,13 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
             double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, vp.sK) : 0.0;
             double Wown = from_fx(a.Wfx[own] - wfxv, vp.sW);
             double base_q = delta_insert(a.objective, vp, from_fx(wown, vp.sw), loop, k, na, Kown, Wown);
-            // the empty community: this vertex's own id, when no one holds it
-            if (a.csize[v] == 0 && (!a.has_max || na <= a.max_w)) {
+            // the empty community (local_move.rs:93-100): this vertex's own id when no one holds
+            // it, else its private id n + v (community ids span [0, 2n), so a vertex whose own
+            // id names a community -- its own or another -- can still leave to an empty one)
+            const uint32_t e = a.csize[v] == 0 ? v : (a.csize[a.n + v] == 0 ? a.n + v : NONE);
+            if (e != NONE && (!a.has_max || na <= a.max_w)) {
                 double ge = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
-                better(bs, bc, ge, v);
+                better(bs, bc, ge, e);
             }
             bool accept = bc != NONE && bc != own && bs - base_q > vp.threshold;
             if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
@@ -441,6 +478,12 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
             if (accept) {
                 a.target[v] = bc;
                 if (sp.mstep) sp.mstep[v] = sp.step;
+                if (a.jsum) {
+                    a.mgain[v] = bs - base_q;
+                    const uint64_t idx = (uint64_t)(sp.step & 1u) * a.jstride + bc;
+                    atomicAdd((unsigned long long *)&a.jsum[idx], (unsigned long long)join_amount(a, vp, k, na));
+                    atomicMin(&a.jmin[idx], v);
+                }
                 push = true;
             }
         } else if (bc != NONE) {
@@ -457,10 +500,12 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         unsigned int slot0 = 0;
         if ((int)(threadIdx.x & 63) == leader) slot0 = atomicAdd(sp.n_movers, (unsigned int)__popcll(pm));
         slot0 = __shfl(slot0, leader);
-        if (push)
-            (sp.mv_list ? sp.mv_list : a.movers)[slot0 + __builtin_amdgcn_mbcnt_hi(
-                                                             (uint32_t)(pm >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] = v;
+        if (push) {
+            const unsigned int slot = slot0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+            (sp.mv_list ? sp.mv_list : a.movers)[slot] = v;
+            if (sp.mv_tgt) sp.mv_tgt[slot] = bc;
+        }
     }
     group_sync<G, GTAB>();
 }
@@ -522,10 +567,12 @@ struct ApplyArgs {
     unsigned int *n_moved;
     const uint32_t *claim;  // with a community-weight cap: lowest mover id per target, else null
     const uint32_t *iter_dev;  // graph-replayed rounds: the round index in device memory, else null
+    uint8_t *admitted;      // local moving without a cap: 1 if the mover was admitted (join_admitted)
 };
 
-// Local moving: every decided mover moves (targets were checked against the snapshot).
-__global__ void k_apply_move(ApplyArgs a) {
+// Local moving: a decided mover moves if its target admits it (the lowest claim with a weight
+// cap, join_admitted without; set 0 of the joiner sums, which k_clear_joins resets afterwards).
+__global__ void k_apply_move(ApplyArgs a, DecideArgs d) {
     unsigned int cnt = *a.n_movers;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint64_t v = a.movers[i];
@@ -533,6 +580,14 @@ __global__ void k_apply_move(ApplyArgs a) {
         if (a.claim && a.claim[t] != (uint32_t)v) {
             a.active[v] = 1;  // another vertex claimed the capped community first: retry later
             continue;
+        }
+        if (d.jsum) {
+            const bool go = join_admitted(d, vparams<true>(d, (uint32_t)v), (uint32_t)v, (uint32_t)t, 0u);
+            a.admitted[v] = go ? 1 : 0;
+            if (!go) {
+                a.active[v] = 1;  // the target is crowded this sub-sweep: decide again later
+                continue;
+            }
         }
         const uint64_t old = a.memb[v];
         a.memb[v] = (uint32_t)t;
@@ -591,6 +646,17 @@ __global__ void k_claim(const uint32_t *movers, const unsigned int *count, const
     }
 }
 
+// Resets the joiner sums and lowest ids of this sub-sweep's targets (set 0) for the next one.
+__global__ void k_clear_joins(const uint32_t *movers, const unsigned int *count, const uint32_t *target,
+                              long long *jsum, uint32_t *jmin) {
+    unsigned int cnt = *count;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t t = target[movers[i]];
+        jsum[t] = 0;
+        jmin[t] = 0xFFFFFFFFu;
+    }
+}
+
 // Re-activation after moves. Local moving: neighbours not in the mover's new community
 // (local_move.rs:114-118). Refinement: neighbours inside the same working community. With a
 // weight cap (claim != null) a local mover whose claim lost did not move and activates nothing.
@@ -599,13 +665,15 @@ __global__ void __launch_bounds__(256) k_activate(const uint32_t *list, const un
                                                   const uint64_t *ptrs, const uint32_t *nbrs,
                                                   const uint32_t *memb, const uint32_t *constraint,
                                                   uint8_t *active, const uint32_t *target = nullptr,
-                                                  const uint32_t *claim = nullptr) {
+                                                  const uint32_t *claim = nullptr,
+                                                  const uint8_t *admitted = nullptr) {
     constexpr int NG = 256 / G;
     unsigned int cnt = *count;
     const int lane = threadIdx.x % G;
     for (unsigned int i = blockIdx.x * NG + threadIdx.x / G; i < cnt; i += gridDim.x * NG) {
         uint32_t v = list[i];
         if (claim && claim[target[v]] != v) continue;
+        if (admitted && !admitted[v]) continue;  // not admitted: it did not move
         uint32_t mv = REFINE ? constraint[v] : memb[v];
         for (uint64_t e = ptrs[v] + lane; e < ptrs[v + 1]; e += G) {
             uint32_t u = nbrs[e];
@@ -794,6 +862,7 @@ struct SweepArgs {
     unsigned int *ccnt;     // [max_steps] vertices that moved (refinement)
     uint32_t *mstep;        // step stamp of accepted movers (local moving)
     uint32_t mv_stride;     // each workgroup's mover list: d.movers + blockIdx.x * mv_stride
+    uint32_t *mv_tgt;       // local moving with joiner admission: the movers' targets, same layout
     uint32_t *claim;        // lowest mover id per capped target, or null without a weight cap
     BarrierCtl *bc;         // grid barrier words (zeroed before the launch)
     unsigned int *ctl;      // [0] steps run, [2] iterations, [3] moves
@@ -1115,6 +1184,20 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             a.tround[v] = NONE;
             a.targeted_cnt[v] = 0;
         } else {
+            // the vertex's private empty community n + v starts empty
+            A.Kfx[n + v] = 0;
+            A.Wfx[n + v] = 0;
+            A.csize[n + v] = 0;
+            if (capped) A.claim[n + v] = NONE;
+            if (a.jsum) {
+#pragma unroll
+                for (uint64_t set = 0; set < 2; set++) {
+                    a.jsum[set * a.jstride + v] = 0;
+                    a.jsum[set * a.jstride + n + v] = 0;
+                    a.jmin[set * a.jstride + v] = NONE;
+                    a.jmin[set * a.jstride + n + v] = NONE;
+                }
+            }
             A.mstep[v] = NONE;
             if (a.f_margin) {
                 a.f_margin[v] = 0.0f;
@@ -1192,6 +1275,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
     const uint32_t gslot = threadIdx.x / SWEEP_ACT_G;
     constexpr uint32_t NGA = SWEEP_BS / SWEEP_ACT_G;
     uint32_t *const mv_list = a.movers + (uint64_t)blockIdx.x * A.mv_stride;  // this workgroup's movers
+    uint32_t *const mv_tgt = A.mv_tgt ? A.mv_tgt + (uint64_t)blockIdx.x * A.mv_stride : nullptr;
     if (threadIdx.x == 0) s_mc = 0;
     uint32_t step = 0, it = 0;
     bool ok = true;
@@ -1204,6 +1288,18 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         const uint64_t seed = REFINE ? a.seed : mix64(A.level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
         const uint32_t P = REFINE ? 1u : a.P;
         for (uint32_t sub = 0; sub < P && ok; sub++, step++) {
+            // joiner admission alternates two sets of sums by step: the set the previous step used
+            // is reset here from this workgroup's movers of that step; this step writes the other
+            // set, and the next step's decide follows the barriers of this one
+            if (!REFINE && mv_tgt && step > 0) {
+                const unsigned int pm = s_mc;
+                const uint64_t off = (uint64_t)((step - 1) & 1u) * a.jstride;
+                for (uint32_t i = threadIdx.x; i < pm; i += SWEEP_BS) {
+                    const uint32_t t = mv_tgt[i];
+                    a.jsum[off + t] = 0;
+                    a.jmin[off + t] = NONE;
+                }
+            }
             // clear the claims and join counts this workgroup's movers of the previous step
             // left behind; every workgroup finishes that before anyone decides again
             if (step > 0 && (capped || REFINE)) {
@@ -1222,7 +1318,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             const long long t_dec = A.trace ? wall_clock64() : 0;
             // compact and decide the workgroup's own share of every bucket
             sweep_compact<REFINE, FILTER, BATCH>(A, seed, sub, it, filter_on, s_wc);
-            StepParams sp{seed, sub, it, step, &s_mc, REFINE ? nullptr : A.mstep, mv_list};
+            StepParams sp{seed, sub, it, step, &s_mc, REFINE ? nullptr : A.mstep, mv_list, REFINE ? nullptr : mv_tgt};
             sweep_decide<REFINE, BATCH>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
             __syncthreads();
             const unsigned int nm = s_mc;
@@ -1271,8 +1367,10 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                     const uint64_t tt = a.target[t];
                     const uint32_t cl = capped ? A.claim[t] : (uint32_t)v;
                     go = ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == it) & (tt == v))) & (cl == (uint32_t)v);
+                } else if (capped) {
+                    go = A.claim[t] == (uint32_t)v;
                 } else {
-                    go = !capped || A.claim[t] == (uint32_t)v;
+                    go = !a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v, (uint32_t)t, step & 1u);
                 }
                 if (lane == 0) {
                     if (!go) {
@@ -1297,7 +1395,13 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                         if (REFINE) {
                             act = a.constraint[u] == mv;
                         } else {
-                            const uint32_t cu = A.mstep[u] == step ? a.target[u] : a.memb[u];
+                            // a neighbour that decided to move this step is in its target if
+                            // admitted (the same test its own workgroup applies), else unmoved
+                            uint32_t cu = a.memb[u];
+                            if (A.mstep[u] == step) {
+                                const uint32_t tu = a.target[u];
+                                if (!a.jsum || join_admitted(a, vparams<BATCH>(a, u), u, tu, step & 1u)) cu = tu;
+                            }
                             act = cu != (uint32_t)t;
                             if (a.f_nmv) atomicAdd(&a.f_nmv[u], 1u);
                         }
@@ -1427,6 +1531,13 @@ struct LevelState {
     DevBuf<uint8_t> f_mark;
     // persistent sweeps: compacted step lists, mover stamps, step counters + barrier + control
     DevBuf<uint32_t> work, mstep;
+    // joiner admission (local moving without a cap): sums and lowest ids per target (two sets),
+    // the movers' snapshot gain margins, their targets beside the persistent mover lists, and
+    // (multi-launch) whether each mover was admitted
+    DevBuf<long long> jsum;
+    DevBuf<uint32_t> jmin, mv_tgt;
+    DevBuf<double> mgain;
+    DevBuf<uint8_t> admitted;
     DevBuf<unsigned int> sweep;
     DevBuf<unsigned char> sweep_ctl;
 };
@@ -1562,17 +1673,19 @@ static DecideArgs base_args(const sc_graph &L, LevelState &ls, const sc_leiden_c
     return a;
 }
 
+// Community-indexed arrays span [0, 2n): ids below n are vertex ids (a vertex's own id is its
+// first empty community), n + v is vertex v's private empty community.
 static void alloc_level(LevelState &ls, uint32_t n, hipStream_t s) {
-    ls.Kfx.resize(n);
-    ls.Wfx.resize(n);
-    ls.csize.resize(n);
+    ls.Kfx.resize(2 * (size_t)n);
+    ls.Wfx.resize(2 * (size_t)n);
+    ls.csize.resize(2 * (size_t)n);
     ls.active.resize(n);
     ls.target.resize(n);
     ls.tround.resize(n);
     ls.movers.resize(n);
     ls.moved.resize(n);
     ls.targeted.resize(n);
-    ls.claim.resize(n);
+    ls.claim.resize(2 * (size_t)n);
     ls.counters.resize(4);
     ls.stat.resize(4);
     HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
@@ -1707,6 +1820,10 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     if (ls.movers.size() < stride * grid) ls.movers.resize(stride * grid);
     A.d.movers = ls.movers.get();
     A.mv_stride = (uint32_t)stride;
+    if (!REFINE && A.d.jsum) {
+        if (ls.mv_tgt.size() < stride * grid) ls.mv_tgt.resize(stride * grid);
+        A.mv_tgt = ls.mv_tgt.get();
+    }
     constexpr uint32_t TRACE_CAP = 2048;
     DevBuf<long long> trace;
     if (sweep_trace_on()) {
@@ -1784,9 +1901,9 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
 
 static void init_totals(const sc_graph &L, const uint32_t *memb, LevelState &ls, int sK, int sW, hipStream_t s) {
     uint32_t n = (uint32_t)L.n;
-    HIP_OK(hipMemsetAsync(ls.Kfx.get(), 0, n * sizeof(long long), s));
-    HIP_OK(hipMemsetAsync(ls.Wfx.get(), 0, n * sizeof(long long), s));
-    HIP_OK(hipMemsetAsync(ls.csize.get(), 0, n * sizeof(uint32_t), s));
+    HIP_OK(hipMemsetAsync(ls.Kfx.get(), 0, 2 * (size_t)n * sizeof(long long), s));
+    HIP_OK(hipMemsetAsync(ls.Wfx.get(), 0, 2 * (size_t)n * sizeof(long long), s));
+    HIP_OK(hipMemsetAsync(ls.csize.get(), 0, 2 * (size_t)n * sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_init_totals, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, memb, L.strength.get(),
                        L.node_w.get(), n, sK, sW, ls.Kfx.get(), ls.Wfx.get(), ls.csize.get());
     KERNEL_OK();
@@ -1871,6 +1988,20 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     }
     a.memb = memb;
     a.P = MOVE_SUBSWEEPS;
+    const bool capped = ctx.cfg->has_max_community_weight != 0;
+    if (!capped) {
+        // joiner admission: two sets of 2n sums / lowest ids (alternating persistent steps; the
+        // multi-launch iterations use set 0 and clear it after every sub-sweep)
+        if (ls.jsum.size() < 4 * (size_t)n) {
+            ls.jsum.resize(4 * (size_t)n);
+            ls.jmin.resize(4 * (size_t)n);
+        }
+        if (ls.mgain.size() < n) ls.mgain.resize(n);
+        a.jsum = ls.jsum.get();
+        a.jmin = ls.jmin.get();
+        a.mgain = ls.mgain.get();
+        a.jstride = 2 * (uint64_t)n;
+    }
     if (b.start[GTAB_BUCKET] < b.start[GTAB_BUCKET + 1]) {
         a.gkeys = b.gkeys.get();
         a.gvals = b.gvals.get();
@@ -1883,6 +2014,11 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     init_totals(L, memb, ls, sK, sW, s);
     hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
     KERNEL_OK();
+    if (!capped) {
+        HIP_OK(hipMemsetAsync(ls.jsum.get(), 0, 2 * (size_t)n * sizeof(long long), s));
+        HIP_OK(hipMemsetAsync(ls.jmin.get(), 0xFF, 2 * (size_t)n * sizeof(uint32_t), s));
+        if (ls.admitted.size() < n) ls.admitted.resize(n);
+    }
     ApplyArgs ap{};
     ap.movers = ls.movers.get();
     ap.n_movers = ls.counters.get();
@@ -1896,8 +2032,8 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     ap.sK = sK;
     ap.sW = sW;
     ap.active = ls.active.get();
-    const bool capped = ctx.cfg->has_max_community_weight != 0;
     ap.claim = capped ? ls.claim.get() : nullptr;
+    ap.admitted = capped ? nullptr : ls.admitted.get();
     DevBuf<unsigned int> iter_moves(MOVE_SUBSWEEPS);
     const bool use_graph = graph_sweeps();
     DevBuf<uint32_t> iter_dev;
@@ -1918,18 +2054,24 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
             fill_u32(iter_moves.get() + sub, 1, 0u, s);
             decide_all<false>(a, b, s, (ctx.timer.on && !use_graph) ? &ctx.timer : nullptr);
             if (capped) {
-                fill_u32(ls.claim.get(), n, 0xFFFFFFFFu, s);
+                fill_u32(ls.claim.get(), 2 * (uint64_t)n, 0xFFFFFFFFu, s);
                 hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), iter_moves.get() + sub,
                                    ls.target.get(), ls.claim.get());
                 KERNEL_OK();
             }
-            hipLaunchKernelGGL(k_apply_move, dim3(1024), dim3(256), 0, s, ap);
+            hipLaunchKernelGGL(k_apply_move, dim3(1024), dim3(256), 0, s, ap, a);
             KERNEL_OK();
             hipLaunchKernelGGL((k_activate<16, false>), dim3(1024), dim3(256), 0, s, ls.movers.get(),
                                iter_moves.get() + sub, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)memb,
                                (const uint32_t *)nullptr, ls.active.get(), (const uint32_t *)ls.target.get(),
-                               (const uint32_t *)(capped ? ls.claim.get() : nullptr));
+                               (const uint32_t *)(capped ? ls.claim.get() : nullptr),
+                               (const uint8_t *)(capped ? nullptr : ls.admitted.get()));
             KERNEL_OK();
+            if (!capped) {
+                hipLaunchKernelGGL(k_clear_joins, dim3(1024), dim3(256), 0, s, ls.movers.get(), iter_moves.get() + sub,
+                                   ls.target.get(), ls.jsum.get(), ls.jmin.get());
+                KERNEL_OK();
+            }
         }
         if (use_graph) {
             hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(1), 0, s, iter_dev.get());
@@ -2022,7 +2164,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
         fill_u32(ls.targeted.get(), n, 0u, s);
         decide_all<true>(a, b, s, nullptr);
         if (capped) {
-            fill_u32(ls.claim.get(), n, 0xFFFFFFFFu, s);
+            fill_u32(ls.claim.get(), 2 * (uint64_t)n, 0xFFFFFFFFu, s);
             hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), ls.counters.get(),
                                ls.target.get(), ls.claim.get());
             KERNEL_OK();
@@ -2812,6 +2954,33 @@ static std::vector<uint64_t> batch_ranges(const BatchState &B) {
     return r;
 }
 
+// After local moving of a batched run, community ids span [0, 2N): c < N is a vertex id of its
+// job's range [lo, hi), N + v is vertex v's private empty community. Each job's ids are mapped
+// in order into [2 lo, 2 hi) (vertex ids to lo + c, private ids to hi + v), so a job's ids are
+// contiguous and ordered as in a standalone run of that job ([0, n_j), then n_j + v), and the
+// ascending compactions that follow number every job's communities as its standalone run does.
+__global__ void k_job_contiguous_ids(uint32_t *memb, uint32_t n, const uint32_t *vjob, const uint64_t *ranges) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t c = memb[v];
+        const bool priv = c >= n;
+        const uint32_t base = priv ? c - n : c;
+        const uint32_t j = vjob[base];
+        memb[v] = (uint32_t)(priv ? ranges[j + 1] + base : ranges[j] + base);
+    }
+}
+
+static void batch_job_contiguous_ids(const BatchState &B, uint32_t *memb, const uint32_t *vjob, uint32_t n,
+                                     hipStream_t s) {
+    if (n == 0) return;
+    const std::vector<uint64_t> r = batch_ranges(B);
+    DevBuf<uint64_t> d_r(B.J + 1);
+    to_device(d_r.get(), r.data(), B.J + 1, s);
+    hipLaunchKernelGGL(k_job_contiguous_ids, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, memb, n, vjob,
+                       (const uint64_t *)d_r.get());
+    KERNEL_OK();
+    sync_stream(s);  // r is a host temporary
+}
+
 // Per-level parameters of every job, from the level graph's per-job totals (the same values
 // and reductions run_once derives from a graph of that job alone).
 static void batch_level_params(BatchState &B, const sc_graph &L, const sc_leiden_config &cfg, hipStream_t s) {
@@ -2943,6 +3112,8 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
                          (unsigned long long)L.nnz, (unsigned long long)L.max_degree);
         pc.lap(0);
         local_move(L, memb.get(), ls, b, ctx, level_seed, sK, sW, sw);
+        // community ids now span [0, 2n); a batched run lays each job's ids out contiguously
+        if (B) batch_job_contiguous_ids(*B, memb.get(), vjob_l.get(), n, s);
         pc.lap(1);
         uint32_t ng;
         if (cfg.refine) {
@@ -2950,7 +3121,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
             pc.lap(2);
             ng = renumber(rmemb.get(), n, n, groups.get(), s);
         } else {
-            ng = renumber(memb.get(), n, n, groups.get(), s);
+            ng = renumber(memb.get(), n, 2 * n, groups.get(), s);
         }
         pc.lap(3);
         std::vector<uint32_t> jng;
@@ -2976,7 +3147,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         hipLaunchKernelGGL(k_scatter_coarse_memb, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, groups.get(),
                            memb.get(), n, rmemb.get());
         KERNEL_OK();
-        renumber(rmemb.get(), ng, n, memb.get(), s, false);
+        renumber(rmemb.get(), ng, 2 * n, memb.get(), s, false);
         if (B) batch_rebase(*B, memb.get(), vjob_l.get(), ng, s);
         hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, groups.get(), node_at.get(), n0);
         KERNEL_OK();
@@ -2987,7 +3158,7 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
     ctx.vjob = nullptr;
     ctx.jobs = nullptr;
     uint32_t n = (uint32_t)level->n;
-    uint32_t nc = renumber(memb.get(), n, n, rmemb.get(), s);
+    uint32_t nc = renumber(memb.get(), n, 2 * n, rmemb.get(), s);
     // labels[v] = final community of v's node at the last level
     HIP_OK(hipMemcpyAsync(labels, node_at.get(), n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, rmemb.get(), labels, n0);
@@ -3182,7 +3353,7 @@ sc_status sc_leiden_trace(const sc_graph *g, const sc_leiden_config *cfg, uint64
                           uint8_t *y_out, uint64_t *n_out, sc_error *err) {
     try {
         if (!g || !cfg || !n_out || (cap && (!x_out || !y_out))) throw ScError(SC_INVALID_ARGUMENT, "null argument");
-        if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
+        check_node_count(g->n);
         hipStream_t s = thread_stream();
         const uint32_t n = (uint32_t)g->n;
         DevBuf<float> tx(std::max<uint64_t>(cap, 1) * MLP_NF);
@@ -3241,7 +3412,7 @@ sc_status sc_leiden_filtered(const sc_graph *g, const sc_leiden_config *cfg, con
             throw ScError(SC_INVALID_ARGUMENT, "null argument");
         if (!(threshold > 0.0 && threshold < 1.0)) throw ScError(SC_INVALID_ARGUMENT, "threshold must be in (0, 1)");
         if (judge < 0 || judge > 2) throw ScError(SC_INVALID_ARGUMENT, "judge must be 0, 1 or 2");
-        if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
+        check_node_count(g->n);
         hipStream_t s = thread_stream();
         const uint32_t n = (uint32_t)g->n;
         DevBuf<float> w(MLP_PARAMS);
@@ -3298,7 +3469,7 @@ sc_status sc_leiden_batch(const sc_graph *const *graphs, uint64_t n_graphs, cons
             N += graphs[g]->n * n_resolutions;
             E += graphs[g]->nnz * n_resolutions;
         }
-        if (N > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes in the batch");
+        check_node_count(N);
         check_entry_count(E);
         if (N && !labels_out) throw ScError(SC_INVALID_ARGUMENT, "null argument");
         hipStream_t s = thread_stream();
@@ -3407,7 +3578,7 @@ sc_status sc_leiden(const sc_graph *g, const sc_leiden_config *cfg, uint64_t *la
                     sc_error *err) {
     try {
         if (!g || !cfg || !n_clusters_out || (!labels_out && g->n)) throw ScError(SC_INVALID_ARGUMENT, "null argument");
-        if (g->n > 0xFFFFFFFEull) throw ScError(SC_TOO_MANY_NODES, "too many nodes");
+        check_node_count(g->n);
         hipStream_t s = thread_stream();
         uint32_t n = (uint32_t)g->n;
         DevBuf<uint32_t> labels(n);
@@ -3464,6 +3635,33 @@ sc_status sc_graph_aggregate(const sc_graph *g, const uint64_t *groups, sc_graph
         }
         sync_stream(s);
         *out = c.release();
+        return SC_OK;
+    } catch (...) {
+        return status_from_exception(err);
+    }
+}
+
+sc_status sc_renumber(const uint64_t *labels, uint64_t n, uint64_t *out, uint64_t *n_groups_out, sc_error *err) {
+    try {
+        if ((!labels || !out) && n) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        check_node_count(n);
+        uint64_t maxl = 0;
+        for (uint64_t v = 0; v < n; v++) maxl = std::max(maxl, labels[v]);
+        if (n && maxl >= 0xFFFFFFFFull) throw ScError(SC_INVALID_ARGUMENT, "labels must be below 2^32 - 1");
+        uint32_t ng = 0;
+        if (n) {
+            std::vector<uint32_t> l32(n);
+            for (uint64_t v = 0; v < n; v++) l32[v] = (uint32_t)labels[v];
+            hipStream_t s = thread_stream();
+            DevBuf<uint32_t> d(n);
+            to_device(d.get(), l32.data(), n, s);
+            // the device compaction every level of leiden() uses, in place
+            ng = renumber(d.get(), (uint32_t)n, (uint32_t)(maxl + 1), d.get(), s);
+            to_host(l32.data(), d.get(), n, s);
+            sync_stream(s);
+            for (uint64_t v = 0; v < n; v++) out[v] = l32[v];
+        }
+        if (n_groups_out) *n_groups_out = ng;
         return SC_OK;
     } catch (...) {
         return status_from_exception(err);

@@ -360,6 +360,16 @@ static void exclusive_scan(const T *in, T *out, uint64_t n, hipStream_t s) {
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s) { exclusive_scan(in, out, n, s); }
 void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s) { exclusive_scan(in, out, n, s); }
 
+void check_node_count(uint64_t n) {
+    if (n <= MAX_NODES) return;
+    ScError e(SC_TOO_MANY_NODES,
+              n > 0xFFFFFFFFull
+                  ? "graph has " + std::to_string(n) + " nodes, exceeding the 4294967295 addressable by u32 adjacency"
+                  : "graph has " + std::to_string(n) + " nodes, exceeding the device limit of 2147483647 (2^31 - 1) nodes");
+    e.payload.n_nodes = n;
+    throw e;
+}
+
 void check_entry_count(uint64_t nnz) {
     if (nnz >= (uint64_t)MAX_ENTRIES) {
         ScError e(SC_INVALID_ARGUMENT, "a graph of " + std::to_string(nnz) +

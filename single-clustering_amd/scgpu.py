@@ -139,6 +139,7 @@ SIGNATURES = {
     "sc_leiden_last_stats": ([C.POINTER(ScLeidenStats)], C.c_int),
     "sc_modularity": ([_GP, u64p, C.c_double], C.c_double),
     "sc_quality": ([_GP, u64p, C.c_int32, C.c_double, f64p, C.POINTER(ScError)], C.c_int),
+    "sc_renumber": ([u64p, C.c_uint64, u64p, u64p, C.POINTER(ScError)], C.c_int),
     "sc_quality_noalloc": ([_GP, C.c_void_p, C.c_int32, C.c_double, f64p, C.POINTER(ScError)], C.c_int),
     "sc_disconnected_community_count": ([_GP, u64p, u64p, C.POINTER(ScError)], C.c_int),
     "sc_knn": ([f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(ScKnnOpts), u32p, f64p,
@@ -627,6 +628,16 @@ def quality(graph, labels, objective):
     _check(_lib.sc_quality(graph.handle, _p(labels, u64p), objective.code, objective.resolution, C.byref(out),
                            C.byref(err)), err)
     return out.value
+
+
+def renumber(labels):
+    """Partition::renumber_into_grouping: (ids compacted ascending on the device, count)."""
+    labels = np.ascontiguousarray(labels, dtype=np.uint64)
+    out = np.zeros(len(labels), dtype=np.uint64)
+    k = C.c_uint64(0)
+    err = ScError()
+    _check(_lib.sc_renumber(_p(labels, u64p), len(labels), _p(out, u64p), C.byref(k), C.byref(err)), err)
+    return out, k.value
 
 
 def disconnected_community_count(graph, labels):

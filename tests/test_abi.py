@@ -69,5 +69,9 @@ def test_entry_count_guard_before_device_work():
     x = np.zeros((1, 1), np.float32)
     h = C.c_void_p()
     err = scgpu.ScError()
-    st = L.sc_knn_graph(x.ctypes.data_as(scgpu.f32p), 1 << 31, 1, 2, None, C.byref(h), None, C.byref(err))
+    st = L.sc_knn_graph(x.ctypes.data_as(scgpu.f32p), 1 << 30, 1, 4, None, C.byref(h), None, C.byref(err))
     assert st == 101 and err.got == 1 << 33
+    # and more points than the device's node limit (2^31 - 1): TooManyNodes
+    err = scgpu.ScError()
+    st = L.sc_knn_graph(x.ctypes.data_as(scgpu.f32p), 1 << 31, 1, 1, None, C.byref(h), None, C.byref(err))
+    assert st == 8 and err.n_nodes == 1 << 31

@@ -459,3 +459,17 @@ def test_rand_f64_stream_first_values():
     assert f[0] == float(((w[1] << np.uint64(32)) | w[0]) >> np.uint64(11)) * 2.0 ** -53
     assert f[1] == float(((w[3] << np.uint64(32)) | w[2]) >> np.uint64(11)) * 2.0 ** -53
     assert np.all((f >= 0) & (f < 1))
+
+
+def test_normalize_groups_known_answer_and_definition():
+    """partition.rs:673-682: ids {5, 2} compact to [1, 1, 1, 0, 0, 0] (ascending, 2 groups); on
+    random sparse ids the result is each id's rank among the distinct ids (numpy's unique)."""
+    out, k = O.normalize_groups([5, 5, 5, 2, 2, 2])
+    assert list(out) == [1, 1, 1, 0, 0, 0] and k == 2
+    rng = np.random.default_rng(8)
+    for n, hi in [(1, 1), (50, 7), (1000, 10 ** 6), (5000, 10 ** 7)]:
+        ids = rng.integers(0, hi, n, dtype=np.uint64) if hi > 1 else np.zeros(n, np.uint64)
+        out, k = O.normalize_groups(ids)
+        uniq, inv = np.unique(ids, return_inverse=True)
+        assert k == len(uniq) and np.array_equal(out, inv.astype(np.uint64))
+    assert O.normalize_groups(np.zeros(0, np.uint64))[1] == 0
