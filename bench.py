@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: kNN Gaussian-connectivity graph + Leiden on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on): 100,000 points x 50 dims
-from a 30-component isotropic Gaussian mixture (centres ~ N(0, 16 I), sd 1, proportions ~
-Dirichlet(2)), exact k = 15 neighbours, Gaussian connectivities with sigma_i = distance to the
-k-th neighbour, symmetrized by max, then Leiden (Rb, resolution 1.0, seed 42). One step =
-kNN graph build (device resident) + Leiden on that graph.
+Default workload (the one every target of the metric is quoted on): 8,000,000 points x 30 dims
+from a 400-component isotropic Gaussian mixture (centres ~ N(0, 9 I), sd 1, proportions ~
+Dirichlet(1)), exact k = 7 neighbours, Gaussian connectivities with sigma_i = distance to the
+k-th neighbour, symmetrized by max (~48M undirected edges), then Leiden (Rb, resolution 1.0,
+seed 42). The graph is built once on the device from points resident in HBM; one step = one
+leiden() on that resident graph. kNN graph builds are timed separately (after the online
+tuner's exploration builds), and the learned active-vertex filter is compared with plain Leiden
+on the same graph.
 
-Prints one JSON line. `value` = Leiden edges/s (undirected edges of the level-0 graph / Leiden
-wall time); kNN points/s is reported beside it. Other workloads: --config small (planted
-partition, 10k nodes) and --config large (4M x 30, k = 7).
+Prints one JSON line. `value` = Leiden edges/s (undirected edges of the level-0 graph / mean
+leiden() wall time over the timed steps); kNN points/s is reported beside it. Other workloads:
+--config knn_leiden (100k x 50, k = 15), large (4M x 30, k = 7), small (planted partition, 10k
+nodes) and batch (64 graphs x 8 resolutions).
 """
 import argparse
 import json
@@ -49,7 +53,8 @@ def cpu_info():
 def measured_traffic(workload):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated
     against a 1 GiB copy), from the committed profile summary of this workload, or None."""
-    path = os.path.join(ROOT, "profiles", f"{workload.replace('knn_leiden_100k_d50_k15', 'knn_leiden_100k')}_hbm_traffic.md")
+    short = {"knn_leiden_100k_d50_k15": "knn_leiden_100k", "knn_leiden_8M_d30_k7": "knn_leiden_8m"}
+    path = os.path.join(ROOT, "profiles", f"{short.get(workload, workload)}_hbm_traffic.md")
     out = {}
     try:
         with open(path) as f:
@@ -101,14 +106,26 @@ def bench_batch(args, wl, scgpu):
     for i in range(args.warmup):
         scgpu.leiden_batch(graphs, res, cfg)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    times = []
+    times, stats = [], []
     jobs = None
+    scgpu.set_profiling(True)
     for i in range(args.steps):
         t0 = time.perf_counter()
         jobs = scgpu.leiden_batch(graphs, res, cfg)
         times.append(time.perf_counter() - t0)
+        stats.append(scgpu.last_stats())
         log(f"[bench] step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms for {len(jobs)} jobs")
+    scgpu.set_profiling(False)
     t = float(np.mean(times))
+    move_ms = float(np.mean([s["move_kernel_ms"] for s in stats]))
+    move_bytes = float(np.mean([s["move_kernel_bytes"] for s in stats]))
+    launches = float(np.mean([s["move_kernel_launches"] for s in stats]))
+    roof = {"kernel": "k_sweep<local moving, batched> (persistent; all jobs of a level in one launch)",
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": (move_bytes / (move_ms / 1e3) / 1e9) if move_ms > 0 else None,
+            "kernel_ms_per_step": move_ms, "launches_per_step": launches,
+            "bytes_per_launch": move_bytes / launches if launches else None, "traffic": None}
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if roof["achieved"] else None
     # the same jobs as a loop of standalone calls (untimed for the metric; shows what batching buys)
     t0 = time.perf_counter()
     same = 0
@@ -149,24 +166,72 @@ def bench_batch(args, wl, scgpu):
                    "jobs": len(jobs), "nodes_per_resolution": wl["n"], "edges_all_jobs": m_total,
                    "leiden": "Rb, seed 42, defaults", "parallelism": "single GPU, one union graph"},
         "loop_of_standalone_calls_ms": t_loop * 1e3, "batch_speedup_vs_loop": t_loop / t,
-        "jobs_identical_to_standalone": same, "leiden_stats": st, "cpu_baseline": cpu,
+        "jobs_identical_to_standalone": same, "leiden_stats": st, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _pinned_cores(count):
+    try:
+        cores = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = list(range(os.cpu_count() or 1))
+    return cores[-count:] if len(cores) >= count else cores
+
+
+def cpu_leiden_reps(og, reps, obj_name, res, seconds_hint):
+    """The C oracle's single-threaded leiden() `reps` times on the same graph, each rep in its own
+    thread pinned to its own host core (ctypes releases the GIL, so the reps run side by side).
+    Returns (times, labels of the first rep, n_clusters, cores used)."""
+    import threading
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    cores = _pinned_cores(reps)
+    times = [None] * reps
+    out = {}
+
+    def work(i):
+        try:
+            os.sched_setaffinity(0, {cores[i % len(cores)]})  # this thread only (Linux)
+        except (AttributeError, OSError):
+            pass
+        t0 = time.perf_counter()
+        l, nc = O.leiden(og, seed=42, objective=obj_name, resolution=res)
+        times[i] = time.perf_counter() - t0
+        if i == 0:
+            out["labels"], out["nc"] = l, nc
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(reps)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    return times, out["labels"], out["nc"], [cores[i % len(cores)] for i in range(reps)]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=8,
-                    help="untimed steps (the online autotuner measures its candidates during these)")
-    ap.add_argument("--config", default="knn_leiden", choices=["knn_leiden", "small", "large", "filter", "batch"])
+    ap.add_argument("--steps", type=int, default=10, help="timed leiden() steps on the resident graph")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed leiden() steps (the online autotuner measures its candidates during these)")
+    ap.add_argument("--config", default="filter", choices=["filter", "knn_leiden", "small", "large", "batch"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--objective", default=None, choices=["rb", "cpm"],
                     help="objective for the configuration's Leiden run (default: its own, Rb)")
     ap.add_argument("--resolution", type=float, default=None,
                     help="resolution with --objective (default 1.0 for rb, 0.05 for cpm)")
-    ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
+    ap.add_argument("--knn-explore", type=int, default=3,
+                    help="untimed kNN graph builds first (the tuner measures its column-split candidates)")
+    ap.add_argument("--knn-builds", type=int, default=None,
+                    help="timed kNN graph builds (default 2 above 1M points, else 10)")
+    ap.add_argument("--cpu-reps", type=int, default=None,
+                    help="CPU-baseline leiden() reps, run side by side on pinned cores (default 2 above 1M "
+                         "nodes, else 5)")
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0,
+                    help="batch configuration: CPU-baseline jobs run until this many seconds have passed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--knn-cpu-rows", type=int, default=0,
                     help="query rows timed for the CPU kNN baseline (evenly spaced sample; default 8000, "
@@ -209,46 +274,57 @@ def main():
     obj_name, res = wl["objective"]
     cfg = scgpu.LeidenConfig(objective=scgpu.ObjectiveKind(obj_name, res), seed=42)
 
-    # the points are resident in HBM before the timed region (uploaded once here); the rate with
-    # the host-to-device copy inside is measured separately below
-    x_dev = scgpu.DeviceArray(np.ascontiguousarray(wl["x"], dtype=np.float32)) if wl["kind"] == "knn" else None
-
-    def step():
+    # ---- the graph: built on the device from points resident in HBM (uploaded once here) ----
+    t_knn, kstats, t_knn_host = [], [], None
+    if wl["kind"] == "knn":
+        n_pts, d_pts = wl["x"].shape
+        x_dev = scgpu.DeviceArray(np.ascontiguousarray(wl["x"], dtype=np.float32))
+        g = None
+        for i in range(max(args.knn_explore, 1)):
+            t0 = time.perf_counter()
+            g = scgpu.knn_graph_device(x_dev.ptr, n_pts, d_pts, wl["k"], precision=args.precision)
+            log(f"[bench] kNN graph build {i + 1}/{args.knn_explore} (untimed, tuner exploring): "
+                f"{(time.perf_counter() - t0) * 1e3:.1f} ms")
+        builds = args.knn_builds or (2 if wl["n"] > 1_000_000 else 10)
+        scgpu.set_profiling(True)
+        for i in range(builds):
+            del g
+            t0 = time.perf_counter()
+            g, kst = scgpu.knn_graph_device(x_dev.ptr, n_pts, d_pts, wl["k"], precision=args.precision,
+                                            return_stats=True)
+            t_knn.append(time.perf_counter() - t0)
+            kstats.append(kst)
+            log(f"[bench] kNN graph build {i + 1}/{builds}: {t_knn[-1] * 1e3:.1f} ms "
+                f"(tile {kst['tile_kernel_ms']:.1f} ms)")
+        scgpu.set_profiling(False)
+        x_dev.free()
+        if wl["n"] <= 1_000_000:
+            # the same build from host points (the upload inside: the PCIe-inclusive rate)
+            th = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                scgpu.knn_graph(wl["x"], wl["k"], precision=args.precision)
+                th.append(time.perf_counter() - t0)
+            t_knn_host = float(np.mean(th))
+    else:
         t0 = time.perf_counter()
-        if wl["kind"] == "knn":
-            g, kst = scgpu.knn_graph_device(x_dev.ptr, wl["x"].shape[0], wl["x"].shape[1], wl["k"],
-                                            precision=args.precision, return_stats=True)
-        else:
-            g, kst = scgpu.CSRNetwork.from_csr_parts(*wl["csr"]), None
-        t1 = time.perf_counter()
-        c = scgpu.leiden(g, cfg)
-        t2 = time.perf_counter()
-        return g, c, kst, scgpu.last_stats(), t1 - t0, t2 - t1
+        g = scgpu.CSRNetwork.from_csr_parts(*wl["csr"])
+        t_knn.append(time.perf_counter() - t0)
 
+    # ---- the timed steps: leiden() on the resident graph ----
     for i in range(args.warmup):
-        step()
+        c = scgpu.leiden(g, cfg)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
     scgpu.set_profiling(True)
-    t_knn, t_leiden, lstats, kstats = [], [], [], []
-    g = c = None
+    t_leiden, lstats = [], []
+    c = None
     for i in range(args.steps):
-        g, c, kst, lst, a, b = step()
-        t_knn.append(a)
-        t_leiden.append(b)
-        lstats.append(lst)
-        kstats.append(kst)
-        log(f"[bench] step {i + 1}/{args.steps}: graph {a * 1e3:.2f} ms, leiden {b * 1e3:.2f} ms, "
-            f"{c.n_clusters()} clusters")
+        t0 = time.perf_counter()
+        c = scgpu.leiden(g, cfg)
+        t_leiden.append(time.perf_counter() - t0)
+        lstats.append(scgpu.last_stats())
+        log(f"[bench] step {i + 1}/{args.steps}: leiden {t_leiden[-1] * 1e3:.2f} ms, {c.n_clusters()} clusters")
     scgpu.set_profiling(False)
-    t_knn_host = None
-    if wl["kind"] == "knn":
-        # the same graph build from host points (the upload inside: the PCIe-inclusive rate)
-        th = []
-        for _ in range(2 if wl["n"] <= 1_000_000 else 1):
-            t0 = time.perf_counter()
-            scgpu.knn_graph(wl["x"], wl["k"], precision=args.precision)
-            th.append(time.perf_counter() - t0)
-        t_knn_host = float(np.mean(th))
 
     m = g.edge_count()
     n = g.node_count()
@@ -257,7 +333,7 @@ def main():
     edges_per_s = m / tl
     q = scgpu.modularity(g, c.labels(), 1.0)
 
-    # roofline of the dominant kernel (time summed over the timed steps)
+    # roofline of the step's dominant kernel: the local-moving sweeps (device time summed per step)
     move_ms = float(np.mean([s["move_kernel_ms"] for s in lstats]))
     move_bytes = float(np.mean([s["move_kernel_bytes"] for s in lstats]))
     launches = float(np.mean([s["move_kernel_launches"] for s in lstats]))
@@ -273,15 +349,16 @@ def main():
     knn_roof = None
     if wl["kind"] == "knn":
         tile_ms = float(np.mean([s["tile_kernel_ms"] for s in kstats]))
-        flops = kstats[0]["flops"]
+        flops = kstats[-1]["flops"]
         # flops = 2 d x the point pairs the tiles computed (block pruning skips pairs that provably
         # hold no k-nearest neighbour; the brute force would compute n^2)
-        pairs = kstats[0].get("pairs_computed") or 0.0
+        pairs = kstats[-1].get("pairs_computed") or 0.0
         peak = BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_MFMA_PEAK_TFLOPS
         knn_roof = {"kernel": f"k_knn_tile<{args.precision}>", "bound": "mfma", "unit": "TFLOP/s",
                     "peak": peak, "achieved": flops / (tile_ms / 1e3) / 1e12,
-                    "kernel_ms_per_step": tile_ms, "traffic": None, "pairs_computed": pairs,
-                    "pairs_vs_brute_force": pairs / float(wl["n"]) ** 2 if pairs else None}
+                    "kernel_ms_per_build": tile_ms, "traffic": None, "pairs_computed": pairs,
+                    "pairs_vs_brute_force": pairs / float(wl["n"]) ** 2 if pairs else None,
+                    "brute_force_equivalent_tflops": 2.0 * float(wl["n"]) ** 2 * wl["d"] / (tile_ms / 1e3) / 1e12}
         knn_roof["frac"] = knn_roof["achieved"] / peak
     traffic, traffic_src = measured_traffic(wl["workload"])
     if traffic:
@@ -292,12 +369,7 @@ def main():
             leiden_roof["traffic_per_step"] = traffic["k_sweep_move_per_step"]
             leiden_roof["traffic"] = traffic["k_sweep_move_per_step"] / max(launches, 1.0)
             leiden_roof["traffic_source"] = traffic_src
-    dominant = leiden_roof
-    if knn_roof and knn_roof["kernel_ms_per_step"] > leiden_roof["kernel_ms_per_step"]:
-        dominant = knn_roof
 
-    # learned active-vertex filter vs plain Leiden on the same graph (trained on traces from
-    # graphs of the same distribution built with other seeds)
     # the three launch modes of local moving / refinement on the same graph (identical labels)
     sweep_modes = None
     if not args.no_sweep_modes:
@@ -315,15 +387,18 @@ def main():
         scgpu.set_sweep_mode("persistent")
         log("[bench] sweep modes: " + ", ".join(f"{k} {v['leiden_ms']:.2f} ms" for k, v in sweep_modes.items()))
 
+    # learned active-vertex filter vs plain Leiden on the same graph (trained on traces from
+    # graphs of the same distribution built with other seeds)
     learned = None
     if not args.no_learned and wl["kind"] == "knn":
         import workloads as W
 
         t0 = time.perf_counter()
         Xs, ys, gts = [], [], []
-        n_tr = min(wl["n"], 1_000_000 if args.config in ("large", "filter") else 100_000)
+        big = args.config in ("large", "filter")
+        n_tr = min(wl["n"], 1_000_000 if big else 100_000)
         for sd in (101, 102):
-            if args.config in ("large", "filter"):
+            if big:
                 xt, _ = W.gaussian_mixture(n_tr, 30, 200, 9.0, sd=1.0, alpha=1.0, seed=sd)
             else:
                 xt, _ = W.gaussian_mixture(n_tr, wl["d"], 30, 16.0, sd=1.0, alpha=2.0, seed=sd)
@@ -350,14 +425,15 @@ def main():
                     ts.append(time.perf_counter() - t1)
                 dq = (scgpu.modularity(gc, cc.labels(), 1.0) - q_ref) / abs(q_ref)
                 calib[str(cand)] = {"ms": 1e3 * min(ts), "modularity_rel_diff": dq}
-            ok = [c for c in calib if calib[c]["modularity_rel_diff"] >= -1e-6]
-            thr = float(min(ok, key=lambda c: calib[c]["ms"])) if ok else 0.02
+            ok = [c2 for c2 in calib if calib[c2]["modularity_rel_diff"] >= -1e-6]
+            thr = float(min(ok, key=lambda c2: calib[c2]["ms"])) if ok else 0.02
+
         # timed: the filter with its full-sweep judge (every level ends in a converged plain
         # local-moving state); beside it the judge over the skipped vertices only, and no judge
         # (the result then rests on the model alone)
         def timed(judge):
             ts = []
-            for i in range(max(args.steps, 1) + 1):
+            for i in range(min(max(args.steps, 1), 5) + 1):
                 t1 = time.perf_counter()
                 cl, st = scgpu.leiden_filtered(g, w, thr, cfg, judge=judge)
                 if i:  # first run is a warm-up
@@ -376,7 +452,8 @@ def main():
             ", ".join(f"{j} judge {r['leiden_ms']:.2f} ms Q {r['modularity']:.10f}" for j, r in runs.items()) +
             f" (plain {tl * 1e3:.2f} ms, Q {q:.10f}); skip-error rate {full['skip_error_rate']:.4f}")
 
-    # CPU baseline: the oracle's single-threaded Leiden on the same graph, bounded sample
+    # CPU baseline: the oracle's single-threaded Leiden on the same graph, reps side by side on
+    # pinned cores (each rep uses one core)
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -384,27 +461,27 @@ def main():
 
         ptrs, nbrs, wts = g.to_csr_parts()
         og = O.Graph.from_csr_parts(ptrs, nbrs, wts)
-        times = []
-        t_start = time.perf_counter()
-        ol = None
-        while len(times) < 5 and (time.perf_counter() - t_start) < args.cpu_baseline_seconds:
-            t0 = time.perf_counter()
-            ol, onc = O.leiden(og, seed=42, objective=obj_name, resolution=res)
-            times.append(time.perf_counter() - t0)
-            log(f"[bench] cpu baseline rep {len(times)}: {times[-1]:.3f} s")
+        reps = args.cpu_reps or (2 if n > 1_000_000 else 5)
+        log(f"[bench] cpu baseline: {reps} reps of the C oracle's leiden() side by side ...")
+        times, ol, onc, cores = cpu_leiden_reps(og, reps, obj_name, res, None)
         t_cpu = float(np.median(times))
+        mad = float(np.median(np.abs(np.array(times) - t_cpu)))
+        log(f"[bench] cpu baseline reps: " + ", ".join(f"{t:.3f} s" for t in times))
         model, ncpu = cpu_info()
         q_cpu = O.modularity(og, ol, 1.0)
         cpu = {"value": m / t_cpu, "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": f"{len(times)} full leiden() runs of the C oracle on the same {n}-node graph, "
-                         f"median {t_cpu:.3f} s, host {model} ({ncpu} logical CPUs visible)",
-               "modularity": q_cpu, "ari_vs_gpu": O.ari(c.labels(), ol), "n_clusters": int(onc)}
+               "sample": f"{reps} full leiden() runs of the C oracle (oracle/oracle.c) on the same {n}-node graph, "
+                         f"side by side, each on its own pinned host core ({cores}); median {t_cpu:.3f} s, "
+                         f"MAD {mad:.3f} s; host {model} ({ncpu} logical CPUs visible)",
+               "rep_seconds": times, "modularity": q_cpu, "ari_vs_gpu": O.ari(c.labels(), ol),
+               "n_clusters": int(onc)}
         if obj_name != "rb" or res != 1.0:
             # the run's own objective (Rb::quality / Cpm::quality), both partitions on the CPU graph
             qg = O.quality(og, c.labels(), obj_name, res)
             qc = O.quality(og, ol, obj_name, res)
             cpu["objective_quality"] = {"objective": obj_name, "resolution": res, "gpu": qg, "cpu": qc,
                                         "rel_diff": (qg - qc) / abs(qc) if qc else None}
+        del og
 
     # CPU baseline of the kNN half: the kd-tree restatement of the reference's exact path
     # (kiddo nearest_n(k + 1) on min(64, threads) workers) on a bounded sample of query rows of
@@ -448,26 +525,28 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": (tk + tl) * 1e3,
+        "ms_per_step": tl * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (delta-quality), int64 fixed-point totals, f32 weights; kNN " +
                  ("split-bf16 MFMA + fp64 rescore" if args.precision == "bf16" else "fp32 MFMA + fp64 rescore"),
         "data": "synthetic",
-        "config": {"workload": wl["workload"], "nodes": n, "edges": m, "leiden": f"{'Rb' if obj_name == 'rb' else 'CPM'} resolution {res:g}, seed 42",
-                   "parallelism": "single GPU"},
+        "config": {"workload": wl["workload"], "nodes": n, "edges": m,
+                   "leiden": f"{'Rb' if obj_name == 'rb' else 'CPM'} resolution {res:g}, seed 42",
+                   "step": "one leiden() on the device-resident graph", "parallelism": "single GPU"},
         "leiden_ms": tl * 1e3,
         "graph_build_ms": tk * 1e3,
+        "knn_builds_timed": len(t_knn) if wl["kind"] == "knn" else 0,
         "knn_points_per_s": (wl["n"] / tk) if wl["kind"] == "knn" else None,
         "knn_points_per_s_host_input": (wl["n"] / t_knn_host) if t_knn_host else None,
         "graph_build_ms_host_input": t_knn_host * 1e3 if t_knn_host else None,
         "modularity": q,
         "n_clusters": c.n_clusters(),
         "leiden_stats": lstats[-1],
-        "knn_stats": kstats[-1],
-        "roofline": dominant,
-        "roofline_other": knn_roof if dominant is leiden_roof else leiden_roof,
+        "knn_stats": kstats[-1] if kstats else None,
+        "roofline": leiden_roof,
+        "roofline_knn": knn_roof,
         "sweep_modes": sweep_modes,
         "learned_filter": learned,
         "cpu_baseline": cpu,

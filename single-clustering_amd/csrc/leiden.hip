@@ -1929,19 +1929,22 @@ static void finish_run_stats(RunCtx &ctx) {
     unsigned long long h[RUN_COUNTERS];
     to_host(h, ctx.counters.get(), RUN_COUNTERS, ctx.s);
     sync_stream(ctx.s);
+    ctx.st.move_iterations += h[RUN_SWEEP + 0];
+    ctx.st.moves += h[RUN_SWEEP + 1];
+    ctx.st.refine_rounds += h[RUN_SWEEP + 2];
     for (int ph = 0; ph < 2; ph++) {
         const unsigned long long *c = h + RUN_STAT + 3 * ph;
         ctx.st.vertex_visits += c[0];
         ctx.st.edge_visits += c[1];
-        // minimum unique bytes: neighbour id + weight + membership per entry, 32 B per vertex,
-        // one 8-byte community total per candidate scored
-        const double bytes = 12.0 * (double)c[1] + 32.0 * (double)c[0] + 8.0 * (double)c[2];
+        // minimum unique bytes: neighbour id + weight + membership per entry (refinement also
+        // reads the neighbour's working community: 4 more), 32 B per vertex, one 8-byte
+        // community total per candidate scored, and per move two communities' two totals
+        // (32 B; refinement's moves are counted by the persistent sweeps only)
+        double bytes = (ph == 0 ? 12.0 : 16.0) * (double)c[1] + 32.0 * (double)c[0] + 8.0 * (double)c[2];
+        bytes += 32.0 * (double)(ph == 0 ? ctx.st.moves : h[RUN_SWEEP + 3]);
         ctx.st.algorithmic_bytes += bytes;
         if (ph == 0) ctx.move_bytes += bytes;
     }
-    ctx.st.move_iterations += h[RUN_SWEEP + 0];
-    ctx.st.moves += h[RUN_SWEEP + 1];
-    ctx.st.refine_rounds += h[RUN_SWEEP + 2];
     for (auto &pm : ctx.phase_marks) {
         float ms = 0.0f;
         HIP_OK(hipEventElapsedTime(&ms, pm.second.first, pm.second.second));

@@ -103,3 +103,18 @@ def test_knn_graph_device_input_equals_host_input(n, d, k):
     for a, b in zip(g1.to_csr_parts(), g0.to_csr_parts()):
         assert np.array_equal(a, b)
     assert st1["pairs_computed"] == st0["pairs_computed"]
+
+
+def test_gaussian_mode1_by_hand_on_the_gpu():
+    """The hand-worked six-point line of the oracle's mode-1 test, end to end on the GPU (kNN,
+    sigma, emission, CSR): the same pairs and weights."""
+    from test_oracle import LINE_PAIRS_MODE1, LINE_POINTS, LINE_SIGMA_MEDIAN, _gauss
+
+    g = scgpu.knn_graph(LINE_POINTS, 4, mode=1)
+    p, nb, w = g.to_csr_parts()
+    got = {(i, int(nb[e])): float(w[e]) for i in range(6) for e in range(p[i], p[i + 1])}
+    assert set(got) == LINE_PAIRS_MODE1 | {(b, a) for a, b in LINE_PAIRS_MODE1}
+    x = LINE_POINTS[:, 0].astype(np.float64)
+    for (i, j), wt in got.items():
+        ref = np.float32(_gauss(LINE_SIGMA_MEDIAN[i], LINE_SIGMA_MEDIAN[j], (x[i] - x[j]) ** 2))
+        assert abs(wt - ref) <= 1e-6 * ref, (i, j, wt, ref)

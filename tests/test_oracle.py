@@ -473,3 +473,47 @@ def test_normalize_groups_known_answer_and_definition():
         uniq, inv = np.unique(ids, return_inverse=True)
         assert k == len(uniq) and np.array_equal(out, inv.astype(np.uint64))
     assert O.normalize_groups(np.zeros(0, np.uint64))[1] == 0
+
+
+# ---- reference-compatible Gaussian connectivity (mode 1), by hand ------------------------------
+LINE_POINTS = np.array([[0.0], [1.0], [3.0], [7.0], [12.0], [20.0]], dtype=np.float32)
+# each row's 4 nearest (self excluded), worked out by hand from the coordinates above
+LINE_KNN = [[1, 2, 3, 4], [0, 2, 3, 4], [1, 0, 3, 4], [2, 4, 1, 0], [3, 5, 2, 1], [4, 3, 2, 1]]
+# sigma = sqrt of the upper median (index len/2 of the sorted nonzero squared distances,
+# neighborhood/mod.rs:105-124): row 0's d^2 are 1, 9, 49, 144 -> 49 -> 7; and so on
+LINE_SIGMA_MEDIAN = [7.0, 6.0, 4.0, 6.0, 9.0, 17.0]
+# a pair {i < j} is emitted iff j is among row i's neighbours (the i <= j rule, :131-166);
+# the union of both rows' lists would also hold {1,5}, {2,5}, {3,5}
+LINE_PAIRS_MODE1 = {(0, 1), (0, 2), (0, 3), (0, 4), (1, 2), (1, 3), (1, 4), (2, 3), (2, 4), (3, 4), (4, 5)}
+
+
+def _gauss(si, sj, d2):
+    den = si * si + sj * sj
+    return math.sqrt(2.0 * si * sj / den) * math.exp(-d2 / den)
+
+
+def test_gaussian_mode1_sigma_and_emission_by_hand():
+    """The crate's weighting (neighborhood/mod.rs:105-168) on six points of a line: the median
+    rule, the lower-row emission and the weights, against values worked out by hand. A lower
+    median, the k-th distance or union emission each change the expected CSR."""
+    idx, d2 = O.knn_bruteforce(LINE_POINTS, 4)
+    assert [list(r) for r in idx] == LINE_KNN
+    ptrs, nbrs, wts = O.knn_gaussian(idx, d2, mode=1)
+    got = {}
+    for i in range(6):
+        for e in range(ptrs[i], ptrs[i + 1]):
+            got[(i, int(nbrs[e]))] = float(wts[e])
+    want_pairs = LINE_PAIRS_MODE1 | {(b, a) for a, b in LINE_PAIRS_MODE1}
+    assert set(got) == want_pairs
+    x = LINE_POINTS[:, 0].astype(np.float64)
+    for (i, j), w in got.items():
+        w_ref = _gauss(LINE_SIGMA_MEDIAN[i], LINE_SIGMA_MEDIAN[j], (x[i] - x[j]) ** 2)
+        assert w == np.float32(w_ref), (i, j, w, w_ref)
+    # mode 0 (the benchmark's rule): sigma = the k-th distance, union of both lists
+    ptrs0, nbrs0, wts0 = O.knn_gaussian(idx, d2, mode=0)
+    assert ptrs0[-1] == 2 * (len(LINE_PAIRS_MODE1) + 3)
+    sig_k = [12.0, 11.0, 9.0, 7.0, 11.0, 19.0]
+    for i in range(6):
+        for e in range(ptrs0[i], ptrs0[i + 1]):
+            j = int(nbrs0[e])
+            assert wts0[e] == np.float32(_gauss(sig_k[i], sig_k[j], (x[i] - x[j]) ** 2))
