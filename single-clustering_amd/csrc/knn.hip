@@ -54,21 +54,47 @@ __device__ __forceinline__ float bf16_bits_to_f32(unsigned short b) {
     return __uint_as_float(((uint32_t)b) << 16);
 }
 
-// One thread per element of the padded npad x Kp layout: split-BF16 copies and the f32 copy
-// (coalesced; padding rows and columns are zero). With perm, row j of the layout is point
-// perm[j], or padding where perm[j] == KNONE (the grouped layout of the pruned search, whose
-// rows number npad with perm); without it rows j < n are the points.
-__global__ void k_knn_prep(const float *X, const uint32_t *perm, uint32_t n, uint32_t d, uint32_t Kp, uint64_t total,
-                           unsigned short *Xh, unsigned short *Xl, float *Xf) {
+// One thread per element of the padded npad x Kp layout, coalesced: split-BF16 copies (bf16
+// path) or the f32 copy (fp32 path). Row j holds point j's coordinates, then its squared norm
+// ny[j] in the two slots t = d, d + 1 (bf16: hi and lo parts of the f32 norm, lo copy zero; f32:
+// the norm and zero), zeros beyond. As the B operand of the tile (column points) a row's dot
+// product with A = (-2 x, 1, 1, 0 ...) is the s-value |y|^2 - 2 x.y itself; padding rows hold
+// zeros and an infinite norm, so they never pass a threshold. With perm, row j of the layout is
+// point perm[j], or padding where perm[j] == KNONE (the grouped layout of the pruned search,
+// whose rows number npad with perm); without it rows j < n are the points.
+__global__ void k_knn_prep(const float *X, const uint32_t *perm, const float *ny, uint32_t n, uint32_t d, uint32_t Kp,
+                           uint64_t total, unsigned short *Xh, unsigned short *Xl, float *Xf) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t j = (uint32_t)(e / Kp), t = (uint32_t)(e % Kp);
         const uint32_t src = perm ? perm[j] : (j < n ? j : KNONE);
-        const float v = (src != KNONE && t < d) ? X[(uint64_t)src * d + t] : 0.0f;
-        const unsigned short h = bf16_bits_rn(v);
-        Xh[e] = h;
-        Xl[e] = bf16_bits_rn(v - bf16_bits_to_f32(h));
-        Xf[e] = v;
+        if (t < d) {
+            const float v = src != KNONE ? X[(uint64_t)src * d + t] : 0.0f;
+            if (Xh) {
+                const unsigned short h = bf16_bits_rn(v);
+                Xh[e] = h;
+                Xl[e] = bf16_bits_rn(v - bf16_bits_to_f32(h));
+            } else {
+                Xf[e] = v;
+            }
+        } else if (t < d + 2) {
+            const float y = ny[j];  // +inf for padding rows
+            if (Xh) {
+                const unsigned short h = bf16_bits_rn(y);
+                const bool finite = y < INFINITY;
+                Xh[e] = t == d ? h : (finite ? bf16_bits_rn(y - bf16_bits_to_f32(h)) : (unsigned short)0);
+                Xl[e] = 0;
+            } else {
+                Xf[e] = t == d ? y : 0.0f;
+            }
+        } else {
+            if (Xh) {
+                Xh[e] = 0;
+                Xl[e] = 0;
+            } else {
+                Xf[e] = 0.0f;
+            }
+        }
     }
 }
 
@@ -133,12 +159,25 @@ template <int NKC> struct KnnTileShape {
 };
 
 template <bool BF16, int NKC>
-constexpr size_t knn_tile_lds_bytes() {
+constexpr size_t knn_tile_stage_bytes() {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
     constexpr size_t STRIDE = BF16 ? (KP + 8) : (KP + 4);
-    constexpr size_t STAGE = BF16 ? 2 * TN * STRIDE * 2 : TN * STRIDE * 4;
-    return STAGE + TN * 4 + KNN_BM * 4 * 2 + (size_t)KNN_BM * KnnTileShape<NKC>::BCAP * 8;
+    return BF16 ? 2 * TN * STRIDE * 2 : TN * STRIDE * 4;
+}
+
+// Two stage buffers (one barrier per stage: a stage is written while the previous one is read)
+// where they fit in 64 KB next to the thresholds and hit buffers; otherwise one.
+template <bool BF16, int NKC>
+constexpr int knn_tile_stage_bufs() {
+    constexpr size_t rest = KNN_BM * 4 * 2 + (size_t)KNN_BM * KnnTileShape<NKC>::BCAP * 8;
+    return 2 * knn_tile_stage_bytes<BF16, NKC>() + rest <= 64 * 1024 ? 2 : 1;
+}
+
+template <bool BF16, int NKC>
+constexpr size_t knn_tile_lds_bytes() {
+    return knn_tile_stage_bufs<BF16, NKC>() * knn_tile_stage_bytes<BF16, NKC>() + KNN_BM * 4 * 2 +
+           (size_t)KNN_BM * KnnTileShape<NKC>::BCAP * 8;
 }
 
 // Row-owner merge: lane r folds row r's buffered hits into its sorted list, clears the buffer
@@ -209,7 +248,7 @@ __device__ __forceinline__ void knn_subtile_mfma(const unsigned short *sBh, cons
 // Byte i of cnt4[rt] counts the buffered hits of row 16 rt + i of this lane's quarter.
 // Returns true when some row could overflow its buffer on the next sub-tile.
 template <int BCAP>
-__device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float nyc, uint32_t col, uint32_t row0,
+__device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], uint32_t col, uint32_t row0,
                                                  uint32_t rl_base, uint64_t qmask, const float (&thr_r)[4][4],
                                                  uint32_t (&cnt4)[4], uint2 *sBuf) {
     bool need = false;
@@ -217,7 +256,7 @@ __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float ny
     for (int rt = 0; rt < 4; rt++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float sv = __builtin_fmaf(-2.0f, acc[rt][i], nyc);  // nyc - 2 acc, one rounding
+            const float sv = acc[rt][i];  // |y|^2 - 2 x.y (the norm is folded into the product)
             bool hit = sv < thr_r[rt][i];
             if (__ballot(hit)) {
                 const uint32_t rl = rl_base + rt * 16 + i;
@@ -240,24 +279,22 @@ __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], float ny
 // block's list of column blocks of TN points: blist[boff[row block] .. boff[row block + 1]).
 template <bool BF16, int NKC, int KM>
 __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
-                                                  const float *Xf, const float *ny, uint32_t n, uint32_t npad,
+                                                  const float *Xf, uint32_t d, uint32_t n, uint32_t npad,
                                                   uint32_t cols_per_split, const uint32_t *blist, const uint32_t *boff,
                                                   const float *seed_s, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
     constexpr int BCAP = KnnTileShape<NKC>::BCAP;
     constexpr int STRIDE = BF16 ? (KP + 8) : (KP + 4);            // staged point stride (elements)
-    constexpr int STAGE_BYTES = BF16 ? 2 * TN * STRIDE * 2 : TN * STRIDE * 4;
+    constexpr int STAGE_BYTES = (int)knn_tile_stage_bytes<BF16, NKC>();
+    constexpr int NBUF = knn_tile_stage_bufs<BF16, NKC>();
     constexpr int CHUNKS = BF16 ? 2 * TN * KP / 8 : TN * KP / 4;  // 16-byte chunks per stage
     constexpr int CPT = CHUNKS / 256;                             // chunks per thread
     static_assert(CHUNKS % 256 == 0, "stage must split evenly over the workgroup");
     static_assert(STAGE_BYTES % 16 == 0, "threshold array must stay 16-byte aligned");
 
     extern __shared__ unsigned char smem[];
-    unsigned short *sBh = reinterpret_cast<unsigned short *>(smem);
-    float *sBf = reinterpret_cast<float *>(smem);
-    float *sNy = reinterpret_cast<float *>(smem + STAGE_BYTES);
-    float *sThr = sNy + TN;                                       // per block row
+    float *sThr = reinterpret_cast<float *>(smem + NBUF * STAGE_BYTES);  // per block row
     uint32_t *sCnt = reinterpret_cast<uint32_t *>(sThr + KNN_BM); // per block row
     uint2 *sBuf = reinterpret_cast<uint2 *>(sCnt + KNN_BM);       // [row][BCAP] (s bits, column)
 
@@ -288,7 +325,9 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     const float tseed = seed_s ? seed_s[wrow0 + lane] : INFINITY;
     sThr[tid] = tseed;
 
-    // A fragments of the wave's four row tiles, in registers for the whole sweep
+    // A fragments of the wave's four row tiles, in registers for the whole sweep: the query row
+    // as (-2 x, 1, 1, 0, ...) against the column rows (y, |y|^2 hi, |y|^2 lo, 0, ...), so each
+    // product is s = |y|^2 - 2 x.y (scaling by -2 is exact in bf16 and f32)
     bf16x8 ah[4][NKC], al[4][NKC];
     float af[4][8 * NKC];
 #pragma unroll
@@ -298,11 +337,32 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         for (int kc = 0; kc < NKC; kc++) {
             if (BF16) {
                 const uint64_t off = (uint64_t)arow * KP + kc * 32 + 8 * quarter;
-                ah[rt][kc] = *reinterpret_cast<const bf16x8 *>(Xh + off);
-                al[rt][kc] = *reinterpret_cast<const bf16x8 *>(Xl + off);
+                union {
+                    bf16x8 v;
+                    unsigned short u[8];
+                } h, l;
+                h.v = *reinterpret_cast<const bf16x8 *>(Xh + off);
+                l.v = *reinterpret_cast<const bf16x8 *>(Xl + off);
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const uint32_t t = kc * 32 + 8 * quarter + e;
+                    if (t < d) {
+                        h.u[e] = bf16_bits_rn(-2.0f * bf16_bits_to_f32(h.u[e]));
+                        l.u[e] = bf16_bits_rn(-2.0f * bf16_bits_to_f32(l.u[e]));
+                    } else {
+                        h.u[e] = t < d + 2 ? (unsigned short)0x3F80u : (unsigned short)0;  // 1.0 at the norm slots
+                        l.u[e] = 0;
+                    }
+                }
+                ah[rt][kc] = h.v;
+                al[rt][kc] = l.v;
             } else {
 #pragma unroll
-                for (int q = 0; q < 8; q++) af[rt][kc * 8 + q] = Xf[(uint64_t)arow * KP + kc * 32 + q * 4 + quarter];
+                for (int q = 0; q < 8; q++) {
+                    const uint32_t t = kc * 32 + q * 4 + quarter;
+                    const float v = Xf[(uint64_t)arow * KP + t];
+                    af[rt][kc * 8 + q] = t < d ? -2.0f * v : (t == d ? 1.0f : 0.0f);
+                }
             }
         }
     }
@@ -325,8 +385,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         }
     }
     u32x4 pre[CPT];
-    float ny_pre = 0.0f;
-    // loads stage C0 of the points into pre[] / ny_pre (a macro keeps the offsets in VGPRs)
+    // loads stage C0 of the points into pre[] (a macro keeps the offsets in VGPRs)
 #define KNN_PREFETCH(C0)                                                                                   \
     do {                                                                                                   \
         _Pragma("unroll") for (int c = 0; c < CPT; c++) {                                                  \
@@ -337,7 +396,6 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
                 pre[c] = *reinterpret_cast<const u32x4 *>(Xf + (uint64_t)(C0) * KP + src_off[c]);          \
             }                                                                                              \
         }                                                                                                  \
-        if (tid < TN) ny_pre = ny[(C0) + tid];                                                             \
     } while (0)
 
     float lv[KM];
@@ -361,17 +419,29 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     // block-local row of this lane's value i of row tile rt, minus 16 rt + i
     const uint32_t rl_base = wave * 64 + 4 * quarter;
     const uint64_t qmask = 0xFFFFull << (16 * quarter);
-    if (ns) KNN_PREFETCH(blist ? blist[jb] * TN : cbeg);
+    // first column of stage i; with a list, its index is loaded two stages ahead (used one stage
+    // later for the prefetch), so no stage waits for a list entry
+    auto stage_col = [&](uint32_t i) -> uint32_t { return blist ? blist[jb + i * jstep] * TN : cbeg + i * TN; };
+    uint32_t c0_cur = ns ? stage_col(0) : 0u;
+    uint32_t c0_nxt = ns > 1 ? stage_col(1) : 0u;
+    if (ns) KNN_PREFETCH(c0_cur);
     for (uint32_t st = 0; st < ns; st++) {
-        const uint32_t c0 = blist ? blist[jb + st * jstep] * TN : cbeg + st * TN;
-        __syncthreads();  // previous stage fully consumed
+        const uint32_t c0 = c0_cur;
+        const uint32_t c0n = c0_nxt;
+        if (st + 2 < ns) c0_nxt = stage_col(st + 2);
+        // with one buffer, wait until the previous stage is consumed; with two, the barrier of
+        // the previous stage already ordered the reads of the buffer written here
+        if (NBUF == 1) __syncthreads();
+        const uint32_t boff_e = NBUF == 2 ? (st & 1u) * (STAGE_BYTES / (BF16 ? 2 : 4)) : 0u;
+        unsigned short *const sBh = reinterpret_cast<unsigned short *>(smem) + (BF16 ? boff_e : 0u);
+        float *const sBf = reinterpret_cast<float *>(smem) + (BF16 ? 0u : boff_e);
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (BF16) *reinterpret_cast<u32x4 *>(sBh + dst_off[c]) = pre[c];
             else *reinterpret_cast<u32x4 *>(sBf + dst_off[c]) = pre[c];
         }
-        if (tid < TN) sNy[tid] = ny_pre;
-        if (st + 1 < ns) KNN_PREFETCH(blist ? blist[jb + (st + 1) * jstep] * TN : c0 + TN);  // lands while this stage computes
+        if (st + 1 < ns) KNN_PREFETCH(c0n);  // lands while this stage computes
+        c0_cur = c0n;
         __syncthreads();
 
         // two accumulator sets: the products of sub-tile t + 1 are issued before the threshold
@@ -382,14 +452,14 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         for (int sub = 0; sub < TN / 16; sub += 2) {
             knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 1) * (16u * STRIDE) + lane_base, ah,
                                                     al, af, acc1);
-            bool need = knn_subtile_hits<BCAP>(acc0, sNy[sub * 16 + (lane & 15)], c0 + sub * 16 + (lane & 15), row0,
-                                               rl_base, qmask, thr_r, cnt4, sBuf);
+            bool need = knn_subtile_hits<BCAP>(acc0, c0 + sub * 16 + (lane & 15), row0, rl_base, qmask, thr_r, cnt4,
+                                               sBuf);
             if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
             if (sub + 2 < TN / 16)
                 knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
                                                         ah, al, af, acc0);
-            need = knn_subtile_hits<BCAP>(acc1, sNy[(sub + 1) * 16 + (lane & 15)], c0 + (sub + 1) * 16 + (lane & 15),
-                                          row0, rl_base, qmask, thr_r, cnt4, sBuf);
+            need = knn_subtile_hits<BCAP>(acc1, c0 + (sub + 1) * 16 + (lane & 15), row0, rl_base, qmask, thr_r, cnt4,
+                                          sBuf);
             if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
         }
     }
@@ -1143,33 +1213,34 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
 
 template <bool BF16, int NKC, int KM>
 static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
-                           const float *ny, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
+                           uint32_t d, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
                            const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
-    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, ny, n, npad, cps, blist,
+    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, d, n, npad, cps, blist,
                        boff, seed_s, cidx, cs);
     KERNEL_OK();
 }
 
 template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
-                        const float *ny, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
+                        uint32_t d, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
                         const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
     switch (KM) {
-        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
-        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
-        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, ny, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
     }
 }
 
 void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precision, uint32_t *d_idx,
                 double *d_d2, sc_knn_stats *st, hipStream_t s) {
     if (k == 0 || k > FB_MAXK) throw ScError(SC_INVALID_ARGUMENT, "k must be in [1, 32]");
-    if (d == 0 || d > 128) throw ScError(SC_INVALID_ARGUMENT, "d must be in [1, 128]");
+    if (d == 0 || d > 126) throw ScError(SC_INVALID_ARGUMENT, "d must be in [1, 126]");
     if (n < 2) throw ScError(SC_INVALID_ARGUMENT, "need at least 2 points");
     const bool bf16 = precision == 0;
-    const uint32_t nkc = (d + 31) / 32;
+    // coordinates, then the two slots of the folded squared norm (k_knn_prep), padded to 32
+    const uint32_t nkc = (d + 2 + 31) / 32;
     const uint32_t Kp = 32 * nkc;
     const uint32_t npad = (n + KNN_BM - 1) / KNN_BM * KNN_BM;
     // candidates kept per row and split: the smallest of 16, 32, 64 above k (the certificate
@@ -1238,8 +1309,8 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     uint32_t cps = (rows + splits - 1) / splits;
     cps = (cps + KNN_TN - 1) / KNN_TN * KNN_TN;
     if (!prune) splits = (rows + cps - 1) / cps;
-    DevBuf<unsigned short> Xh((size_t)rows * Kp), Xl((size_t)rows * Kp);
-    DevBuf<float> Xf((size_t)rows * Kp), ny(rows);
+    DevBuf<unsigned short> Xh(bf16 ? (size_t)rows * Kp : 0), Xl(bf16 ? (size_t)rows * Kp : 0);
+    DevBuf<float> Xf(bf16 ? 0 : (size_t)rows * Kp), ny(rows);
     DevBuf<double> nx64(npad);
     DevBuf<unsigned int> ymax(1);
     HIP_OK(hipMemsetAsync(ymax.get(), 0, sizeof(unsigned int), s));
@@ -1299,11 +1370,11 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const uint32_t *bl = prune ? blist.get() : nullptr;
     const uint32_t *bo = prune ? boff.get() : nullptr;
     const uint64_t total = (uint64_t)rows * Kp;
-    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_X, pm, n, d, Kp, total,
-                       Xh.get(), Xl.get(), Xf.get());
-    KERNEL_OK();
     hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, s, d_X, pm, n, d, rows, ny.get(),
                        nx64.get(), ymax.get());
+    KERNEL_OK();
+    hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_X, pm, (const float *)ny.get(),
+                       n, d, Kp, total, bf16 ? Xh.get() : nullptr, bf16 ? Xl.get() : nullptr, bf16 ? nullptr : Xf.get());
     KERNEL_OK();
     // the error bound of the approximate values (needed by the tile's seeded thresholds and the
     // rescore): |s_approx - s| <= norm error + 2 |dot error|, doubled for safety; see the header
@@ -1313,10 +1384,19 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     float ymaxf;
     std::memcpy(&ymaxf, &ymax_bits, 4);
     double Y = (double)ymaxf;
+    // The tile's value is one FP32-accumulated sum over the Kp slots of the products (-2 x_t)(y_t)
+    // (bf16: the three split products) and 1 * (|y|^2 hi) + 1 * (|y|^2 lo). Its error against the
+    // exact s = |y|^2 - 2 x.y: the split truncation of the dot part (3 * 2^-16 |x||y|, doubled by
+    // the factor 2), the f32 norm (d * 2^-24 |y|^2), the bf16 hi + lo split of the norm (2^-16
+    // |y|^2) and the accumulation (3 Kp + 2 terms of magnitude at most 2 |x||y| or |y|^2, taken
+    // at three times 2^-24 per term, whatever order and rounding the matrix core's accumulation
+    // uses), all with |y| <= Y; the whole bound is doubled for safety.
     const double u24 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -16);
-    double dot_coef = bf16 ? (3.01 * u16 + 3.03 * (3.0 * Kp) * u24) : (2.02 * (double)Kp * u24);
-    double eps_coef = 2.0 * (2.0 * dot_coef * Y) + std::ldexp(1.0, -20) * 2.0 * Y;
-    double eps_const = 2.0 * (1.01 * d * u24 * Y * Y) + std::ldexp(1.0, -20) * Y * Y;
+    const double terms = bf16 ? 3.0 * Kp + 2.0 : (double)Kp + 1.0;
+    const double acc_coef = 3.03 * terms * u24;  // 3x the round-to-nearest bound per addition
+    const double split_coef = bf16 ? 3.01 * u16 : 0.0;
+    double eps_coef = 2.0 * (2.0 * (split_coef + acc_coef) * Y);
+    double eps_const = 2.0 * ((1.01 * d * u24 + (bf16 ? u16 : 0.0) + acc_coef) * Y * Y);
     DevBuf<float> seed_s;
     if (prune) {
         seed_s.resize(rows);
@@ -1334,14 +1414,14 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const unsigned short *xh = Xh.get(), *xl = Xl.get();
     const float *xf = Xf.get();
     switch (nkc * 2 + (bf16 ? 0 : 1)) {
-        case 2: launch_tile<true, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 3: launch_tile<false, 1>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 4: launch_tile<true, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 5: launch_tile<false, 2>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 6: launch_tile<true, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 7: launch_tile<false, 3>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 8: launch_tile<true, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        default: launch_tile<false, 4>(grid, xh, xl, xf, ny.get(), crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
     }
     HIP_OK(hipEventRecord(e2, s));
     if (prune) {
