@@ -251,6 +251,15 @@ template <int BCAP>
 __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], uint32_t col, uint32_t row0,
                                                  uint32_t rl_base, uint64_t qmask, const float (&thr_r)[4][4],
                                                  uint32_t (&cnt4)[4], uint2 *sBuf) {
+    // hits are rare once the thresholds have settled: one wave-wide test of all 16 values first
+    // (one branch per sub-tile instead of one per value), the per-value pass only when some
+    // lane has a hit
+    bool any = false;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) any |= acc[rt][i] < thr_r[rt][i];
+    if (!__ballot(any)) return false;
     bool need = false;
 #pragma unroll
     for (int rt = 0; rt < 4; rt++)
