@@ -284,12 +284,13 @@ __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], uint32_t
 }
 
 // Column stages: the split's contiguous range of cols_per_split points, or, for the pruned
-// search (blist non-null), every gridDim.y-th entry, from the split's index on, of this row
+// search (blist non-null), every splits-th entry, from the split's index on, of this row
 // block's list of column blocks of TN points: blist[boff[row block] .. boff[row block + 1]).
 template <bool BF16, int NKC, int KM>
 __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, uint32_t d, uint32_t n, uint32_t npad,
-                                                  uint32_t cols_per_split, const uint32_t *blist, const uint32_t *boff,
+                                                  uint32_t cols_per_split, uint32_t row_blocks, uint32_t splits,
+                                                  const uint32_t *blist, const uint32_t *boff,
                                                   const float *seed_s, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
@@ -307,22 +308,30 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     uint32_t *sCnt = reinterpret_cast<uint32_t *>(sThr + KNN_BM); // per block row
     uint2 *sBuf = reinterpret_cast<uint2 *>(sCnt + KNN_BM);       // [row][BCAP] (s bits, column)
 
+    // XCD-aware order: workgroup w runs on XCD w mod 8, and each XCD takes a contiguous range
+    // of the (split, row block) space, so the workgroups resident on one XCD at a time are
+    // neighbouring row blocks of one split, whose column lists largely coincide: their column
+    // stages are shared through that XCD's L2
+    const uint32_t nwg = row_blocks * splits;
+    const uint32_t chunk = (nwg + 7u) / 8u;
+    const uint32_t lg = (blockIdx.x & 7u) * chunk + (blockIdx.x >> 3);
+    if (lg >= nwg) return;  // grid padding (the whole workgroup leaves before any barrier)
+    const uint32_t split = lg / row_blocks, rb = lg % row_blocks;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
     const int quarter = lane >> 4;
-    const uint32_t row0 = blockIdx.x * KNN_BM;
+    const uint32_t row0 = rb * KNN_BM;
     const uint32_t wrow0 = row0 + wave * 64;
-    const uint32_t split = blockIdx.y;
     uint32_t cbeg = 0, ns = 0, jb = 0;
-    const uint32_t jstep = blist ? gridDim.y : 1u;
+    const uint32_t jstep = blist ? splits : 1u;
     if (blist) {
-        // splits take every gridDim.y-th block of the list: each split sees a sample of the near
+        // splits take every splits-th block of the list: each split sees a sample of the near
         // blocks, so no split's list is filled by the nearest points alone (which would leave
         // the certificate no margin)
-        const uint32_t L0 = boff[blockIdx.x], L = boff[blockIdx.x + 1] - L0;
+        const uint32_t L0 = boff[rb], L = boff[rb + 1] - L0;
         jb = L0 + split;
-        ns = L > split ? (L - split + gridDim.y - 1) / gridDim.y : 0u;
+        ns = L > split ? (L - split + splits - 1) / splits : 0u;
     } else {
         cbeg = split * cols_per_split;
         const uint32_t cend = min(npad, cbeg + cols_per_split);
@@ -477,7 +486,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     // write this split's candidate list: cand[(row * splits + split) * KM + t]
     const uint32_t my_row = wrow0 + lane;
     if (my_row < n) {
-        const uint64_t o = ((uint64_t)my_row * gridDim.y + split) * KM;
+        const uint64_t o = ((uint64_t)my_row * splits + split) * KM;
 #pragma unroll
         for (int p = 0; p < KM; p++) {
             cand_idx[o + p] = li[p];
@@ -1226,7 +1235,9 @@ static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned s
                            const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
-    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), grid, dim3(256), lds, s, Xh, Xl, Xf, d, n, npad, cps, blist,
+    const uint32_t row_blocks = grid.x, splits = grid.y;
+    const dim3 g1(8 * ((row_blocks * splits + 7) / 8));  // one dimension; the kernel maps it XCD-aware
+    hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), g1, dim3(256), lds, s, Xh, Xl, Xf, d, n, npad, cps, row_blocks, splits, blist,
                        boff, seed_s, cidx, cs);
     KERNEL_OK();
 }
