@@ -176,6 +176,21 @@ sc_status sc_graph_from_edges(uint64_t n_nodes, const uint64_t *edge_from, const
                               const double *edge_weight, uint64_t n_edges,
                               const double *node_weights, sc_graph **out, sc_error *err);
 
+/* CSRNetwork::from_csr_matrix / from_csr_matrix_with_node_weights (csr_network.rs:360-391): a
+ * sparse adjacency matrix (n_rows x n_rows, CSR arrays: row_ptrs[n_rows + 1], usize column
+ * indices and f64 values, nnz entries) read as symmetric: only entries with row <= col, zeros
+ * skipped, a diagonal entry is a self-loop; then from_edges_with_node_weights. node_weights may
+ * be NULL (unit weights). Row offsets that do not start at 0, end at nnz or decrease are
+ * SC_INVALID_CSR (a CsrMatrix cannot hold them). */
+sc_status sc_graph_from_csr_matrix(uint64_t n_rows, const uint64_t *row_ptrs, const uint64_t *col_indices,
+                                   const double *values, uint64_t nnz, const double *node_weights,
+                                   sc_graph **out, sc_error *err);
+
+/* CSRNetwork::to_csr_matrix (csr_network.rs:610-618): row_ptrs[n + 1], col_indices[nnz] (usize)
+ * and values[nnz] (f64) of the stored adjacency. Any output pointer may be NULL. */
+sc_status sc_graph_to_csr_matrix(const sc_graph *g, uint64_t *row_ptrs, uint64_t *col_indices, double *values,
+                                 sc_error *err);
+
 void sc_graph_free(sc_graph *g);
 
 /* Accessors (csr_network.rs:407-508). */

@@ -647,6 +647,57 @@ sc_status sc_graph_from_edges(uint64_t n, const uint64_t *from, const uint64_t *
     }
 }
 
+// CSRNetwork::from_csr_matrix_with_node_weights (csr_network.rs:360-391): the triplets of a
+// sparse matrix in row order, upper triangle and diagonal only (row <= col), zero entries
+// skipped, become an edge list for from_edges_with_node_weights. The matrix arrives as CSR
+// arrays (usize column indices, f64 values); its row offsets are checked as a CsrMatrix's are.
+sc_status sc_graph_from_csr_matrix(uint64_t n_rows, const uint64_t *row_ptrs, const uint64_t *col_indices,
+                                   const double *values, uint64_t nnz, const double *node_weights, sc_graph **out,
+                                   sc_error *err) {
+    std::vector<uint64_t> a, b;
+    std::vector<double> w;
+    try {
+        if (!out || !row_ptrs || (nnz && (!col_indices || !values))) throw ScError(SC_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        if (row_ptrs[0] != 0 || row_ptrs[n_rows] != nnz)
+            throw ScError(SC_INVALID_CSR, "malformed CSR input: row offsets must start at 0 and end at the entry count");
+        for (uint64_t r = 0; r < n_rows; r++)
+            if (row_ptrs[r + 1] < row_ptrs[r])
+                throw ScError(SC_INVALID_CSR, "malformed CSR input: row offsets must be non-decreasing");
+        for (uint64_t r = 0; r < n_rows; r++)
+            for (uint64_t e = row_ptrs[r]; e < row_ptrs[r + 1]; e++)
+                if (r <= col_indices[e] && values[e] != 0.0) {
+                    a.push_back(r);
+                    b.push_back(col_indices[e]);
+                    w.push_back(values[e]);
+                }
+    } catch (...) {
+        return status_from_exception(err);
+    }
+    return sc_graph_from_edges(n_rows, a.data(), b.data(), w.data(), a.size(), node_weights, out, err);
+}
+
+// CSRNetwork::to_csr_matrix (csr_network.rs:610-618): the stored adjacency as a symmetric CSR
+// matrix with usize column indices and f64 values. Any output pointer may be NULL.
+sc_status sc_graph_to_csr_matrix(const sc_graph *g, uint64_t *row_ptrs, uint64_t *col_indices, double *values,
+                                 sc_error *err) {
+    try {
+        if (!g) throw ScError(SC_INVALID_ARGUMENT, "null graph");
+        std::vector<uint32_t> nb(col_indices ? g->nnz : 0);
+        std::vector<float> wt(values ? g->nnz : 0);
+        hipStream_t s = thread_stream();
+        if (row_ptrs) to_host(row_ptrs, g->ptrs.get(), g->n + 1, s);
+        if (col_indices) to_host(nb.data(), g->nbrs.get(), g->nnz, s);
+        if (values) to_host(wt.data(), g->wts.get(), g->nnz, s);
+        sync_stream(s);
+        for (uint64_t e = 0; col_indices && e < g->nnz; e++) col_indices[e] = nb[e];
+        for (uint64_t e = 0; values && e < g->nnz; e++) values[e] = (double)wt[e];
+        return SC_OK;
+    } catch (...) {
+        return status_from_exception(err);
+    }
+}
+
 void sc_graph_free(sc_graph *g) {
     if (!g) return;
     (void)hipStreamSynchronize(thread_stream());

@@ -110,6 +110,9 @@ SIGNATURES = {
                                         C.c_void_p, C.POINTER(_GP), C.POINTER(ScError)], C.c_int),
     "sc_graph_from_edges": ([C.c_uint64, u64p, u64p, f64p, C.c_uint64, f64p, C.POINTER(_GP),
                              C.POINTER(ScError)], C.c_int),
+    "sc_graph_from_csr_matrix": ([C.c_uint64, u64p, u64p, f64p, C.c_uint64, f64p, C.POINTER(_GP), C.POINTER(ScError)],
+                                 C.c_int),
+    "sc_graph_to_csr_matrix": ([_GP, u64p, u64p, f64p, C.POINTER(ScError)], C.c_int),
     "sc_graph_free": ([_GP], None),
     "sc_graph_node_count": ([_GP], C.c_uint64),
     "sc_graph_edge_count": ([_GP], C.c_uint64),
@@ -405,6 +408,47 @@ class CSRNetwork:
                                       None if nw is None else _p(nw, f64p), C.byref(h), C.byref(err))
         _check(st, err)
         return cls(h)
+
+    @classmethod
+    def from_csr_matrix(cls, matrix, node_weights=None):
+        """from_csr_matrix / from_csr_matrix_with_node_weights (csr_network.rs:360-391): a square
+        sparse matrix (scipy.sparse CSR, or a (row_ptrs, col_indices, values) tuple) read as
+        symmetric (entries with row <= col, zeros skipped)."""
+        if isinstance(matrix, tuple):
+            indptr, indices, data = matrix
+            n = len(indptr) - 1
+        else:
+            indptr, indices, data = matrix.indptr, matrix.indices, matrix.data
+            n = matrix.shape[0]
+        indptr = np.ascontiguousarray(indptr, dtype=np.uint64)
+        indices = np.ascontiguousarray(indices, dtype=np.uint64)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        nw = None
+        if node_weights is not None:
+            nw = np.ascontiguousarray(node_weights, dtype=np.float64)
+            if len(nw) != n:
+                err = ScError()
+                err.code = 5
+                err.got = len(nw)
+                err.expected = n
+                err.msg = f"node weight vector has length {len(nw)}, expected {n}".encode()
+                raise ClusteringError(err)
+        h = C.c_void_p()
+        err = ScError()
+        st = _lib.sc_graph_from_csr_matrix(n, _p(indptr, u64p), _p(indices, u64p), _p(data, f64p), len(indices),
+                                           None if nw is None else _p(nw, f64p), C.byref(h), C.byref(err))
+        _check(st, err)
+        return cls(h)
+
+    def to_csr_matrix(self):
+        """to_csr_matrix (csr_network.rs:610-618): (row_ptrs u64, col_indices u64, values f64)."""
+        n, e = self.node_count(), self.nnz()
+        ptrs = np.zeros(n + 1, dtype=np.uint64)
+        cols = np.zeros(max(e, 1), dtype=np.uint64)
+        vals = np.zeros(max(e, 1), dtype=np.float64)
+        err = ScError()
+        _check(_lib.sc_graph_to_csr_matrix(self._h, _p(ptrs, u64p), _p(cols, u64p), _p(vals, f64p), C.byref(err)), err)
+        return ptrs, cols[:e], vals[:e]
 
     def node_count(self):
         return _lib.sc_graph_node_count(self._h)

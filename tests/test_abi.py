@@ -75,3 +75,19 @@ def test_entry_count_guard_before_device_work():
     err = scgpu.ScError()
     st = L.sc_knn_graph(x.ctypes.data_as(scgpu.f32p), 1 << 31, 1, 1, None, C.byref(h), None, C.byref(err))
     assert st == 8 and err.n_nodes == 1 << 31
+
+
+def test_from_csr_matrix_checks_row_offsets_on_the_host():
+    """A matrix whose row offsets a CsrMatrix could not hold is InvalidCsr before any device work."""
+    import ctypes as C
+
+    import numpy as np
+    import scgpu
+
+    ptrs = np.array([0, 2, 1], np.uint64)
+    cols = np.array([1, 0], np.uint64)
+    vals = np.array([1.0, 1.0])
+    h, err = C.c_void_p(), scgpu.ScError()
+    st = scgpu.lib().sc_graph_from_csr_matrix(2, ptrs.ctypes.data_as(scgpu.u64p), cols.ctypes.data_as(scgpu.u64p),
+                                              vals.ctypes.data_as(scgpu.f64p), 2, None, C.byref(h), C.byref(err))
+    assert st == 9 and not h.value

@@ -286,3 +286,51 @@ def test_graph_freed_on_another_thread_returns_its_memory():
     g2 = scgpu.knn_graph(x, 6)
     del g2
     assert scgpu.pool_bytes()[0] == m_live
+
+
+def test_from_csr_matrix_reads_the_upper_triangle():
+    """from_csr_matrix (csr_network.rs:360-391): entries with row <= col, zeros skipped, the
+    diagonal a self-loop, then from_edges_with_node_weights -- bit-identical to from_edges of
+    that list and to the oracle's; to_csr_matrix (:610-618) returns the stored adjacency widened."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(21)
+    for trial in range(6):
+        n = int(rng.integers(1, 300))
+        dense = sp.random(n, n, density=min(1.0, 6.0 / n), random_state=trial, format="coo")
+        vals = np.where(rng.random(dense.nnz) < 0.1, 0.0, rng.random(dense.nnz) * 3)
+        upper = sp.coo_matrix((vals, (dense.row, dense.col)), shape=(n, n))
+        # a symmetric matrix, and (trial odd) an unsymmetric one: only the upper part counts
+        m = (upper + upper.T).tocsr() if trial % 2 == 0 else upper.tocsr()
+        m.sort_indices()
+        nw = rng.random(n) + 0.5 if trial == 3 else None
+        g = scgpu.CSRNetwork.from_csr_matrix(m, node_weights=nw)
+        coo = m.tocoo()
+        keep = (coo.row <= coo.col) & (coo.data != 0)
+        order = np.lexsort((coo.col[keep], coo.row[keep]))  # triplet order: rows, then columns
+        edges = [(int(a), int(b), float(w)) for a, b, w in
+                 zip(coo.row[keep][order], coo.col[keep][order], coo.data[keep][order])]
+        ref = scgpu.CSRNetwork.from_edges(n, edges, node_weights=nw)
+        for x, y in zip(g.to_csr_parts(), ref.to_csr_parts()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(g.strengths(), ref.strengths())
+        assert_same_graph(g, O.Graph.from_edges(n, edges, node_weights=nw))
+        p, c, v = g.to_csr_matrix()
+        p2, c2, v2 = g.to_csr_parts()
+        assert np.array_equal(p, p2) and np.array_equal(c, c2.astype(np.uint64))
+        assert np.array_equal(v, v2.astype(np.float64))
+
+
+@pytest.mark.parametrize("ptrs,cols,vals,variant", [
+    ([0, 1, 2], [1, 5], [1.0, 1.0], "NodeIndexOutOfRange"),
+    ([0, 1, 2], [1, 0], [-1.0, -1.0], "NegativeWeight"),
+    ([0, 2, 1], [1, 0], [1.0, 1.0], "InvalidCsr"),
+    ([1, 1, 2], [1, 0], [1.0, 1.0], "InvalidCsr"),
+])
+def test_from_csr_matrix_rejects_malformed_input(ptrs, cols, vals, variant):
+    with pytest.raises(scgpu.ClusteringError) as e:
+        scgpu.CSRNetwork.from_csr_matrix((ptrs, cols, vals))
+    assert e.value.variant == variant
+    with pytest.raises(scgpu.ClusteringError) as e:
+        scgpu.CSRNetwork.from_csr_matrix(([0, 1, 2], [1, 0], [1.0, 1.0]), node_weights=[1.0])
+    assert e.value.variant == "NodeWeightLengthMismatch"
