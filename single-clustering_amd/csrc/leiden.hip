@@ -1354,8 +1354,9 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 }
                 if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
             }
-            // apply this workgroup's movers; re-activation is fused with it where it cannot race
-            // with the moves
+            // apply this workgroup's movers; refinement's re-activation is fused with it (it reads
+            // the working communities, which do not change here); local moving re-activates after
+            // all moves landed (below), like the multi-launch path
             unsigned int applied = 0;
             for (uint32_t i = gslot; i < nm; i += NGA) {
                 const uint64_t v = mv_list[i];
@@ -1383,29 +1384,12 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                         if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                     }
                 }
-                if (go && (REFINE || !capped)) {
-                    // refinement: neighbours in the same working community (refine.rs); local
-                    // moving: neighbours whose community after this step differs from t
-                    // (local_move.rs:114-118) -- a mover's is its target, read from the stamp
-                    const uint32_t mv = REFINE ? a.constraint[v] : 0u;
+                if (REFINE && go) {
+                    // refinement: neighbours in the same working community (refine.rs)
+                    const uint32_t mv = a.constraint[v];
                     for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
                         const uint32_t u = a.nbrs[e];
-                        if (u == v) continue;
-                        bool act;
-                        if (REFINE) {
-                            act = a.constraint[u] == mv;
-                        } else {
-                            // a neighbour that decided to move this step is in its target if
-                            // admitted (the same test its own workgroup applies), else unmoved
-                            uint32_t cu = a.memb[u];
-                            if (A.mstep[u] == step) {
-                                const uint32_t tu = a.target[u];
-                                if (!a.jsum || join_admitted(a, vparams<BATCH>(a, u), u, tu, step & 1u)) cu = tu;
-                            }
-                            act = cu != (uint32_t)t;
-                            if (a.f_nmv) atomicAdd(&a.f_nmv[u], 1u);
-                        }
-                        if (act) a.active[u] = 1;
+                        if (u != v && a.constraint[u] == mv) a.active[u] = 1;
                     }
                 }
             }
@@ -1417,12 +1401,16 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 }
             }
             if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
-            if (!REFINE && capped) {
-                // local moving with a cap: activate after all moves landed
+            if (!REFINE) {
+                // local moving: after all moves landed, a mover that moved re-activates its
+                // neighbours outside its new community (local_move.rs:114-118)
                 for (uint32_t i = gslot; i < nm; i += NGA) {
                     const uint64_t v = mv_list[i];
                     const uint32_t t = a.target[v];
-                    if (A.claim[t] != (uint32_t)v) continue;
+                    const bool moved_v = capped ? A.claim[t] == (uint32_t)v
+                                                : (!a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v,
+                                                                            t, step & 1u));
+                    if (!moved_v) continue;
                     for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
                         const uint32_t u = a.nbrs[e];
                         if (u == v) continue;
@@ -1864,9 +1852,9 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     if (sweep_trace_on()) {
         sync_stream(s);
         // barriers: 2 prologue, then per step: decide, apply; a cleanup first (refinement, cap);
-        // with a cap also claims and, in local moving, activation. Phase times in us.
+        // with a cap also claims; local moving also activation. Phase times in us.
         const bool cp = A.claim != nullptr;
-        const uint32_t per = 2u + (cp ? 1u : 0u) + ((!REFINE && cp) ? 1u : 0u);
+        const uint32_t per = 2u + (cp ? 1u : 0u) + (!REFINE ? 1u : 0u);
         const uint32_t cleanups = (REFINE || cp) && ctl[0] ? ctl[0] - 1u : 0u;
         const uint32_t nb = std::min<uint32_t>(TRACE_CAP, 2 + ctl[0] * per + cleanups);
         std::vector<long long> h(nb + 1);
