@@ -437,16 +437,33 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     // block-local row of this lane's value i of row tile rt, minus 16 rt + i
     const uint32_t rl_base = wave * 64 + 4 * quarter;
     const uint64_t qmask = 0xFFFFull << (16 * quarter);
-    // first column of stage i; with a list, its index is loaded two stages ahead (used one stage
-    // later for the prefetch), so no stage waits for a list entry
-    auto stage_col = [&](uint32_t i) -> uint32_t { return blist ? blist[jb + i * jstep] * TN : cbeg + i * TN; };
-    uint32_t c0_cur = ns ? stage_col(0) : 0u;
-    uint32_t c0_nxt = ns > 1 ? stage_col(1) : 0u;
-    if (ns) KNN_PREFETCH(c0_cur);
+    // List entries arrive in batches of 64 stages, one entry per lane (lst_cur: stages
+    // [64 b, 64 b + 64) of the current batch b, lst_nxt: the batch after it, loaded a whole batch
+    // ahead); a stage's first column is read out of the batch register with a lane read. A list
+    // entry loaded per stage would be waited for at the next stage's prefetch, which exposes one
+    // memory latency per stage.
+    auto list_batch = [&](uint32_t b) -> uint32_t {
+        const uint32_t i = 64u * b + (uint32_t)lane;
+        return i < ns ? blist[jb + i * jstep] : 0u;
+    };
+    uint32_t lst_cur = 0, lst_nxt = 0;
+    if (blist && ns) {
+        lst_cur = list_batch(0);
+        lst_nxt = list_batch(1);
+    }
+    // first column of stage i, for i in the current batch (nxt false) or the next (nxt true)
+    auto stage_col = [&](uint32_t i, bool nxt) -> uint32_t {
+        if (!blist) return cbeg + i * TN;
+        return (uint32_t)__builtin_amdgcn_readlane((int)(nxt ? lst_nxt : lst_cur), (int)(i & 63u)) * TN;
+    };
+    if (ns) KNN_PREFETCH(stage_col(0, false));
     for (uint32_t st = 0; st < ns; st++) {
-        const uint32_t c0 = c0_cur;
-        const uint32_t c0n = c0_nxt;
-        if (st + 2 < ns) c0_nxt = stage_col(st + 2);
+        if (blist && st && (st & 63u) == 0) {
+            lst_cur = lst_nxt;
+            lst_nxt = list_batch((st >> 6) + 1);
+        }
+        const uint32_t c0 = stage_col(st, false);
+        const uint32_t c0n = st + 1 < ns ? stage_col(st + 1, ((st + 1) & 63u) == 0) : 0u;
         // with one buffer, wait until the previous stage is consumed; with two, the barrier of
         // the previous stage already ordered the reads of the buffer written here
         if (NBUF == 1) __syncthreads();
@@ -459,7 +476,6 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
             else *reinterpret_cast<u32x4 *>(sBf + dst_off[c]) = pre[c];
         }
         if (st + 1 < ns) KNN_PREFETCH(c0n);  // lands while this stage computes
-        c0_cur = c0n;
         __syncthreads();
 
         // two accumulator sets: the products of sub-tile t + 1 are issued before the threshold
@@ -940,98 +956,184 @@ __global__ void __launch_bounds__(256) k_block_spheres(const float *X, const uin
     }
 }
 
-// Column blocks each row block needs, in sweep order (starting at the columns of the first of
-// the workgroup's row blocks and wrapping around): B is skipped when |cR - cB| - rR - rB > uR,
-// uR bounding every row's k-th nearest distance (k_block_kth; +inf: the block needs every
-// column block, negative: the block holds no points). Each workgroup decides RPW row blocks, so
-// one read of a column block's centre serves all of them. The centres' distance is evaluated in
-// f32 from f32 centres: rounding the centres moves it by at most 2^-24 (|cR| + |cB|) and the
-// f32 sum of non-negative squares by a relative (d + 2) 2^-24, so the skip test first takes
-// 1e-5 relatively and 1e-6 (|cR| + |cB|) absolutely off the distance. Count pass (blist null):
-// counts[R]; fill pass: blist[boff[R] + rank] = B, in sweep order.
-constexpr int RPW = 4;
-__global__ void __launch_bounds__(256) k_need_lists(const double *cR, const double *rR, const double *uR,
-                                                    const float *cB, const double *rB, const uint32_t *nB,
-                                                    uint32_t d, uint32_t nrb, uint32_t ncb, uint32_t bstep,
-                                                    uint32_t *counts, const uint32_t *boff, uint32_t *blist) {
-    __shared__ float sc[RPW][128];
-    __shared__ float snr[RPW];
-    __shared__ uint32_t s_w[RPW][4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t R0 = blockIdx.x * RPW; R0 < nrb; R0 += gridDim.x * RPW) {
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < RPW * d; e += 256) {
-            const uint32_t r = e / d, t = e % d;
-            sc[r][t] = R0 + r < nrb ? (float)cR[(uint64_t)(R0 + r) * d + t] : 0.0f;
+// Group of every row block: groups occupy whole row blocks of the layout, group g the positions
+// [poff[g], poff[g + 1]), so row block R belongs to the largest g with poff[g] <= 256 R (groups
+// without points have empty ranges and are passed over by the search).
+__global__ void k_rb_groups(const uint32_t *poff, uint32_t C, uint32_t nrb, uint32_t *rbg) {
+    for (uint32_t R = blockIdx.x * blockDim.x + threadIdx.x; R < nrb; R += gridDim.x * blockDim.x) {
+        const uint32_t p = R * (uint32_t)KNN_BM;
+        uint32_t lo = 0, hi = C;  // poff[lo] <= p < poff[hi] (poff[C] = layout rows > p)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (poff[mid] <= p) lo = mid;
+            else hi = mid;
         }
-        __syncthreads();
-        if (threadIdx.x < RPW) {
-            float q = 0.0f;
-            for (uint32_t t = 0; t < d; t++) q = fmaf(sc[threadIdx.x][t], sc[threadIdx.x][t], q);
-            snr[threadIdx.x] = sqrtf(q);
+        rbg[R] = lo;
+    }
+}
+
+// Squared radius of every group about its centre cen_g: r2bits[g] = the bits of the largest
+// |y - cen_g|^2 over the group's points, summed in f64 from the f32 values (relative error below
+// (d + 2) 2^-53; the readers inflate the radius by 1e-12). Non-negative doubles order like their
+// bit patterns, so the maximum is an integer atomic. A wave covers 64 consecutive positions of
+// one row block, hence of one group: one atomic per wave.
+__global__ void __launch_bounds__(256) k_group_radii(const float *X, const uint32_t *layout, uint32_t rows, uint32_t d,
+                                                     const float *cen, const uint32_t *rbg,
+                                                     unsigned long long *r2bits) {
+    for (uint32_t p0 = blockIdx.x * 256u; p0 < rows; p0 += gridDim.x * 256u) {  // rows: whole row blocks
+        const uint32_t p = p0 + threadIdx.x;
+        const uint32_t g = rbg[p / (uint32_t)KNN_BM];
+        const uint32_t y = layout[p];
+        double r2 = -1.0;
+        if (y != KNONE) {
+            r2 = 0.0;
+            for (uint32_t t = 0; t < d; t++) {
+                const double df = (double)X[(uint64_t)y * d + t] - (double)cen[(uint64_t)g * d + t];
+                r2 += df * df;
+            }
         }
-        __syncthreads();
-        double lim[RPW];
-        bool all[RPW], none[RPW];
 #pragma unroll
-        for (int r = 0; r < RPW; r++) {
-            const bool ok = R0 + r < nrb;
-            const double ur = ok ? uR[R0 + r] : -1.0;
-            none[r] = !(ur >= 0.0);
-            all[r] = !none[r] && !(ur < INFINITY);
-            lim[r] = ok ? rR[R0 + r] + ur : 0.0;
-        }
-        uint32_t total[RPW] = {};
-        const uint32_t B0 = (uint32_t)(((uint64_t)R0 * bstep) % ncb);
-        for (uint32_t t0 = 0; t0 < ncb; t0 += 256) {  // workgroup-uniform
-            const uint32_t t = t0 + threadIdx.x;
-            uint32_t B = 0;
-            bool has = false;
-            float dist[RPW] = {};
-            float nb2 = 0.0f;
-            if (t < ncb) {
-                B = B0 + t;
-                if (B >= ncb) B -= ncb;
-                has = nB[B] > 0;
-                if (has) {
-                    float s2[RPW] = {};
-                    for (uint32_t u = 0; u < d; u++) {
-                        const float cb = cB[(uint64_t)B * d + u];
-                        nb2 = fmaf(cb, cb, nb2);
+        for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
+        if ((threadIdx.x & 63) == 0 && r2 >= 0.0) atomicMax(&r2bits[g], (unsigned long long)__double_as_longlong(r2));
+    }
+}
+
+// f32 value at or above the (non-negative, possibly infinite) double v.
+__device__ __forceinline__ float f32_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+// Groups each row block needs, as a bit mask of mwords words per row block. Every point y of
+// group g has |x - y| >= |x - cen_g| - r_g, so g holds none of row x's k nearest points when
+// |x - cen_g| - r_g exceeds u_x, the row's bound on its k-th nearest distance (sqrt(seed_d2));
+// the block needs g when some row does. The bound is per row: a row block's bounding sphere
+// adds the block's radius to the group's, which in high dimensions is most of the distance
+// between neighbouring clusters. Distances are summed in f32 from the f32 coordinates and
+// centres, a relative error below (DMAX + 3) 2^-24 on the square; r_g and u_x are rounded up to
+// f32 and their sum inflated by 1e-6, and the test takes 2e-5 relatively off the distance. A
+// row without a bound (+inf) needs every group. One workgroup per row block, one thread per row
+// with its coordinates in registers; 64 centres at a time pass through LDS.
+constexpr uint32_t KNN_MAX_GROUPS = 2048;
+template <int DMAX>
+__global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32_t *layout, uint32_t d, uint32_t nrb,
+                                                    uint32_t C, const float *cen,
+                                                    const unsigned long long *r2bits, const double *seed_d2,
+                                                    uint32_t mwords, uint32_t *gmask) {
+    __shared__ float sc[64 * DMAX];
+    __shared__ float sr[64];
+    __shared__ uint32_t sm[KNN_MAX_GROUPS / 32];
+    const int lane = threadIdx.x & 63;
+    for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {
+        const uint64_t j = (uint64_t)R * KNN_BM + threadIdx.x;
+        const uint32_t y = layout[j];
+        const bool real = y != KNONE;
+        const double sd = real ? seed_d2[j] : 0.0;
+        float xr[DMAX];
 #pragma unroll
-                        for (int r = 0; r < RPW; r++) {
-                            const float df = sc[r][u] - cb;
-                            s2[r] = fmaf(df, df, s2[r]);
-                        }
+        for (int t = 0; t < DMAX; t++) xr[t] = (real && (uint32_t)t < d) ? X[(uint64_t)y * d + t] : 0.0f;
+        const float ux = real ? f32_up(sqrt(sd)) : 0.0f;
+        for (uint32_t w = threadIdx.x; w < mwords; w += 256) sm[w] = 0u;
+        const bool all = __syncthreads_or(real && !(sd < INFINITY)) != 0;
+        if (all) {
+            for (uint32_t w = threadIdx.x; w < mwords; w += 256)
+                sm[w] = (w + 1) * 32 <= C ? 0xFFFFFFFFu : ((1u << (C - w * 32)) - 1u);
+        } else {
+            for (uint32_t g0 = 0; g0 < C; g0 += 64) {
+                const uint32_t gn = min(64u, C - g0);
+                __syncthreads();
+                for (uint32_t e = threadIdx.x; e < 64 * DMAX; e += 256) {
+                    const uint32_t c = e / DMAX, t = e % DMAX;
+                    sc[e] = (c < gn && t < d) ? cen[(uint64_t)(g0 + c) * d + t] : 0.0f;
+                }
+                // (a group without points keeps radius 0; its layout range, hence its block list, is empty)
+                if (threadIdx.x < gn)
+                    sr[threadIdx.x] = f32_up(sqrt(__longlong_as_double((long long)r2bits[g0 + threadIdx.x])) * (1.0 + 1e-12));
+                __syncthreads();
+                for (uint32_t c = 0; c < gn; c++) {
+                    float s2 = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < DMAX; t++) {
+                        const float df = xr[t] - sc[c * DMAX + t];
+                        s2 = fmaf(df, df, s2);
                     }
-#pragma unroll
-                    for (int r = 0; r < RPW; r++) dist[r] = sqrtf(s2[r]);
+                    const float rg = sr[c];
+                    const bool need = real && !(sqrtf(s2) * (1.0f - 2e-5f) > (rg + ux) * (1.0f + 1e-6f));
+                    if (__ballot(need) && lane == 0) atomicOr(&sm[(g0 + c) >> 5], 1u << ((g0 + c) & 31u));
                 }
             }
-            const double rb = has ? rB[B] : 0.0;
-            const float nbn = sqrtf(nb2);
-#pragma unroll
-            for (int r = 0; r < RPW; r++) {
-                bool need = false;
-                if (has && !none[r])
-                    need = all[r] || !((double)dist[r] * (1.0 - 1e-5) - 1e-6 * ((double)snr[r] + (double)nbn) >
-                                         (lim[r] + rb) * (1.0 + 1e-12));
-                const uint64_t m = __ballot(need);
-                if (lane == 0) s_w[r][wave] = (uint32_t)__popcll(m);
-                __syncthreads();
-                uint32_t before = 0;
-                for (int w = 0; w < wave; w++) before += s_w[r][w];
-                const uint32_t wtot = s_w[r][0] + s_w[r][1] + s_w[r][2] + s_w[r][3];
-                if (blist && need)
-                    blist[boff[R0 + r] + total[r] + before +
-                          __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = B;
-                total[r] += wtot;
-                __syncthreads();
+        }
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < mwords; w += 256) gmask[(uint64_t)R * mwords + w] = sm[w];
+        __syncthreads();
+    }
+}
+
+// Column blocks each row block needs: the blocks of the groups in its mask (k_row_groups), its
+// own group first (its nearest points tighten the tile's thresholds early), then the others in
+// increasing order, each group's blocks in layout order. A block is further skipped when
+// |cR - cB| - rR - rB > uR (uR bounds every row's k-th nearest distance, k_block_kth; +inf: no
+// skip; negative: the row block holds no points and needs nothing). The centres' distance is
+// evaluated in f32 from f32 centres: rounding the centres moves it by at most 2^-24 (|cR| +
+// |cB|) and the f32 sum of non-negative squares by a relative (d + 2) 2^-24, so the skip test
+// first takes 1e-5 relatively and 1e-6 (|cR| + |cB|) absolutely off the distance. One wave per
+// row block. Count pass (blist null): counts[R]; fill pass: blist[boff[R] + rank] = B.
+__global__ void __launch_bounds__(256) k_group_lists(const double *cR, const double *rR, const double *uR,
+                                                     const float *cB, const double *rB, const uint32_t *nB,
+                                                     uint32_t d, uint32_t nrb, const uint32_t *poff,
+                                                     const uint32_t *rbg, uint32_t mwords, const uint32_t *gmask,
+                                                     uint32_t tn, uint32_t *counts, const uint32_t *boff,
+                                                     uint32_t *blist) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t R = blockIdx.x * 4u + wave; R < nrb; R += gridDim.x * 4u) {  // wave-uniform
+        const double ur = uR[R];
+        uint32_t total = 0;
+        if (ur >= 0.0) {
+            const bool all = !(ur < INFINITY);
+            const double lim = rR[R] + ur;
+            float nr2 = 0.0f;
+            for (uint32_t t = 0; t < d; t++) {
+                const float c = (float)cR[(uint64_t)R * d + t];
+                nr2 = fmaf(c, c, nr2);
+            }
+            const double nrn = (double)sqrtf(nr2);
+            auto group = [&](uint32_t g) {
+                const uint32_t b0 = poff[g] / tn, b1 = poff[g + 1] / tn;
+                for (uint32_t B0 = b0; B0 < b1; B0 += 64) {
+                    const uint32_t B = B0 + (uint32_t)lane;
+                    bool need = B < b1 && nB[B] > 0;
+                    if (need && !all) {
+                        float s2 = 0.0f, nb2 = 0.0f;
+                        for (uint32_t t = 0; t < d; t++) {
+                            const float cb = cB[(uint64_t)B * d + t];
+                            const float df = (float)cR[(uint64_t)R * d + t] - cb;
+                            s2 = fmaf(df, df, s2);
+                            nb2 = fmaf(cb, cb, nb2);
+                        }
+                        need = !((double)sqrtf(s2) * (1.0 - 1e-5) - 1e-6 * (nrn + (double)sqrtf(nb2)) >
+                                 (lim + rB[B]) * (1.0 + 1e-12));
+                    }
+                    const uint64_t m = __ballot(need);
+                    if (blist && need)
+                        blist[boff[R] + total +
+                              __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = B;
+                    total += (uint32_t)__popcll(m);
+                }
+            };
+            const uint32_t own = rbg[R];
+            group(own);
+            for (uint32_t w = 0; w < mwords; w++) {
+                uint32_t word = gmask[(uint64_t)R * mwords + w];
+                if (w == (own >> 5)) word &= ~(1u << (own & 31u));
+                while (word) {
+                    const uint32_t bit = (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    group(w * 32u + bit);
+                }
             }
         }
-        if (!blist && threadIdx.x == 0)
-            for (int r = 0; r < RPW; r++)
-                if (R0 + r < nrb) counts[R0 + r] = total[r];
+        if (!blist && lane == 0) counts[R] = total;
     }
 }
 
@@ -1177,8 +1279,10 @@ static bool knn_prune_enabled(uint32_t n) {
 
 // Layout of the pruned search: points grouped by k-means (C centroids, a few Lloyd
 // iterations), stable in the points' own order within a group, every group starting on a row
-// block; returns the layout's length (a multiple of 256), layout[j] = point or KNONE.
-static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf<uint32_t> &layout, hipStream_t s) {
+// block; returns the layout's length (a multiple of 256), layout[j] = point or KNONE, and the
+// groups: gC of them, centres gcen (gC x d), layout offsets gpoff (gC + 1, whole row blocks).
+static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf<uint32_t> &layout,
+                                DevBuf<float> &gcen, DevBuf<uint32_t> &gpoff, uint32_t &gC, hipStream_t s) {
     const uint32_t C = std::min<uint32_t>(n, std::max<uint32_t>(16u, std::min<uint32_t>(2048u, n / 512u)));
     DevBuf<uint32_t> perm;
     constexpr int ITERS = 2;
@@ -1226,6 +1330,10 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
     hipLaunchKernelGGL(k_layout_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, keys.get(), perm.get(),
                        start.get(), poff.get(), n, layout.get());
     KERNEL_OK();
+    // the centres of the final assignment and the groups' layout offsets (the group-level pruning)
+    gcen = std::move(cen);
+    gpoff = std::move(poff);
+    gC = C;
     return rows;
 }
 
@@ -1276,7 +1384,10 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     // pruned search: the grouped layout (rows: its length, whole row blocks per group)
     bool prune = knn_prune_enabled(n);
     DevBuf<uint32_t> layout;
-    uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, s) : npad;
+    DevBuf<float> gcen;
+    DevBuf<uint32_t> gpoff;
+    uint32_t gC = 0;
+    uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, gcen, gpoff, gC, s) : npad;
     // the lists of column blocks are addressed with 32-bit offsets: with more (row block, column
     // block) pairs than that the search runs unpruned
     if (prune && (uint64_t)(rows / KNN_BM) * (rows / (nkc <= 2 ? 64u : 32u)) >= 0xFFFFFFFFull) {
@@ -1369,10 +1480,32 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for((uint64_t)ncb * d, 256, 4096)), dim3(256), 0, s, cB.get(),
                            (uint64_t)ncb * d, cB32.get());
         KERNEL_OK();
-        const dim3 ngrid(std::min<uint32_t>((row_blocks + RPW - 1) / RPW, 16384));
-        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
-                           nB.get(), d, row_blocks, ncb, (uint32_t)KNN_BM / TN, cnt.get(), (const uint32_t *)nullptr,
-                           (uint32_t *)nullptr);
+        // groups each row block needs (per-row bounds against the groups' spheres), then its list
+        // of column blocks within those groups
+        DevBuf<uint32_t> rbg(row_blocks);
+        hipLaunchKernelGGL(k_rb_groups, dim3(grid_for(row_blocks, 256, 4096)), dim3(256), 0, s, gpoff.get(), gC,
+                           row_blocks, rbg.get());
+        KERNEL_OK();
+        DevBuf<unsigned long long> r2bits(gC);
+        HIP_OK(hipMemsetAsync(r2bits.get(), 0, sizeof(unsigned long long) * gC, s));
+        hipLaunchKernelGGL(k_group_radii, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
+                           layout.get(), rows, d, gcen.get(), rbg.get(), r2bits.get());
+        KERNEL_OK();
+        if (gC == 0 || gC > KNN_MAX_GROUPS) throw ScError(SC_INVALID_ARGUMENT, "kNN grouping: group count out of range");
+        const uint32_t mwords = (gC + 31) / 32;
+        DevBuf<uint32_t> gmask((size_t)row_blocks * mwords);
+        auto rowg = [&](auto kern) {
+            hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks, gC, gcen.get(),
+                               r2bits.get(), seed_d2.get(), mwords, gmask.get());
+        };
+        if (d <= 32) rowg(k_row_groups<32>);
+        else if (d <= 64) rowg(k_row_groups<64>);
+        else rowg(k_row_groups<128>);
+        KERNEL_OK();
+        const dim3 ngrid(std::min<uint32_t>((row_blocks + 3) / 4, 16384));
+        hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
+                           nB.get(), d, row_blocks, gpoff.get(), rbg.get(), mwords, gmask.get(), TN, cnt.get(),
+                           (const uint32_t *)nullptr, (uint32_t *)nullptr);
         KERNEL_OK();
         boff.resize(row_blocks + 1);
         exclusive_scan_u32(cnt.get(), boff.get(), row_blocks + 1, s);
@@ -1381,8 +1514,8 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         sync_stream(s);
         list_total = lt;
         blist.resize(std::max<uint64_t>(list_total, 1));
-        hipLaunchKernelGGL(k_need_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
-                           nB.get(), d, row_blocks, ncb, (uint32_t)KNN_BM / TN, (uint32_t *)nullptr,
+        hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
+                           nB.get(), d, row_blocks, gpoff.get(), rbg.get(), mwords, gmask.get(), TN, (uint32_t *)nullptr,
                            (const uint32_t *)boff.get(), blist.get());
         KERNEL_OK();
     }
