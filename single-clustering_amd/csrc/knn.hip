@@ -285,12 +285,13 @@ __device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], uint32_t
 
 // Column stages: the split's contiguous range of cols_per_split points, or, for the pruned
 // search (blist non-null), every splits-th entry, from the split's index on, of this row
-// block's list of column blocks of TN points: blist[boff[row block] .. boff[row block + 1]).
+// block's list of column blocks of TN points: blist[boff[row block] .. + bcnt[row block]).
 template <bool BF16, int NKC, int KM>
 __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_tile(const unsigned short *Xh, const unsigned short *Xl,
                                                   const float *Xf, uint32_t d, uint32_t n, uint32_t npad,
                                                   uint32_t cols_per_split, uint32_t row_blocks, uint32_t splits,
-                                                  const uint32_t *blist, const uint32_t *boff,
+                                                  const uint32_t *blist, const uint32_t *boff, const uint32_t *bcnt,
+                                                  const uint32_t *rb_order,
                                                   const float *seed_s, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
@@ -308,15 +309,26 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     uint32_t *sCnt = reinterpret_cast<uint32_t *>(sThr + KNN_BM); // per block row
     uint2 *sBuf = reinterpret_cast<uint2 *>(sCnt + KNN_BM);       // [row][BCAP] (s bits, column)
 
-    // XCD-aware order: workgroup w runs on XCD w mod 8, and each XCD takes a contiguous range
-    // of the (split, row block) space, so the workgroups resident on one XCD at a time are
-    // neighbouring row blocks of one split, whose column lists largely coincide: their column
-    // stages are shared through that XCD's L2
     const uint32_t nwg = row_blocks * splits;
-    const uint32_t chunk = (nwg + 7u) / 8u;
-    const uint32_t lg = (blockIdx.x & 7u) * chunk + (blockIdx.x >> 3);
-    if (lg >= nwg) return;  // grid padding (the whole workgroup leaves before any barrier)
-    const uint32_t split = lg / row_blocks, rb = lg % row_blocks;
+    uint32_t split, rb;
+    if (rb_order) {
+        // pruned lists differ in length by the size of the clusters they cover: the row blocks
+        // run longest list first (rb_order), all splits of one row block together, so the
+        // shortest lists fill the end of the grid; consecutive workgroups go to different XCDs
+        const uint32_t lg = blockIdx.x;
+        if (lg >= nwg) return;  // grid padding (the whole workgroup leaves before any barrier)
+        rb = rb_order[lg / splits];
+        split = lg % splits;
+    } else {
+        // XCD-aware order: workgroup w runs on XCD w mod 8, and each XCD takes a contiguous
+        // range of the (split, row block) space, so the workgroups resident on one XCD at a time
+        // are neighbouring row blocks of one split, whose columns coincide
+        const uint32_t chunk = (nwg + 7u) / 8u;
+        const uint32_t lg = (blockIdx.x & 7u) * chunk + (blockIdx.x >> 3);
+        if (lg >= nwg) return;
+        split = lg / row_blocks;
+        rb = lg % row_blocks;
+    }
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
@@ -329,7 +341,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         // splits take every splits-th block of the list: each split sees a sample of the near
         // blocks, so no split's list is filled by the nearest points alone (which would leave
         // the certificate no margin)
-        const uint32_t L0 = boff[rb], L = boff[rb + 1] - L0;
+        const uint32_t L0 = boff[rb], L = bcnt[rb];
         jb = L0 + split;
         ns = L > split ? (L - split + splits - 1) / splits : 0u;
     } else {
@@ -1005,25 +1017,52 @@ __device__ __forceinline__ float f32_up(double v) {
     return f;
 }
 
+// Per-group data of the group-level pruning: centres transposed (cenT[t * C + g]) and their f32
+// norms, and the radius rounded up: r64 = sqrt(r2) (1 + 1e-12) (f64), rf = r64 rounded up to f32.
+__global__ void k_group_prep(const float *cen, uint32_t C, uint32_t d, const unsigned long long *r2bits, float *cenT,
+                             float *cn, double *r64, float *rf) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < C; g += gridDim.x * blockDim.x) {
+        float q = 0.0f;
+        for (uint32_t t = 0; t < d; t++) {
+            const float c = cen[(uint64_t)g * d + t];
+            cenT[(uint64_t)t * C + g] = c;
+            q = fmaf(c, c, q);
+        }
+        cn[g] = sqrtf(q);
+        // a group without points keeps radius 0; its layout range, hence its block list, is empty
+        const double r = sqrt(__longlong_as_double((long long)r2bits[g])) * (1.0 + 1e-12);
+        r64[g] = r;
+        rf[g] = f32_up(r);
+    }
+}
+
 // Groups each row block needs, as a bit mask of mwords words per row block. Every point y of
 // group g has |x - y| >= |x - cen_g| - r_g, so g holds none of row x's k nearest points when
 // |x - cen_g| - r_g exceeds u_x, the row's bound on its k-th nearest distance (sqrt(seed_d2));
 // the block needs g when some row does. The bound is per row: a row block's bounding sphere
 // adds the block's radius to the group's, which in high dimensions is most of the distance
-// between neighbouring clusters. Distances are summed in f32 from the f32 coordinates and
-// centres, a relative error below (DMAX + 3) 2^-24 on the square; r_g and u_x are rounded up to
-// f32 and their sum inflated by 1e-6, and the test takes 2e-5 relatively off the distance. A
-// row without a bound (+inf) needs every group. One workgroup per row block, one thread per row
-// with its coordinates in registers; 64 centres at a time pass through LDS.
+// between neighbouring clusters. The block's sphere (cR, rR, uR >= every u_x) first passes over
+// the groups no row can reach (|cR - cen_g| - rR - r_g > uR, the test of the column-block lists
+// with its f32 error terms), and the per-row test runs on the rest. Per-row distances are
+// summed in f32 from the f32 coordinates and centres, a relative error below (DMAX + 3) 2^-24 on
+// the square; r_g and u_x are rounded up to f32 and their sum inflated by 1e-6, and the test
+// takes 2e-5 relatively off the distance. A row without a bound (+inf) needs every group. One
+// workgroup per row block, one thread per row with its coordinates in registers; 64 centres at a
+// time pass through LDS.
 constexpr uint32_t KNN_MAX_GROUPS = 2048;
 template <int DMAX>
 __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32_t *layout, uint32_t d, uint32_t nrb,
-                                                    uint32_t C, const float *cen,
-                                                    const unsigned long long *r2bits, const double *seed_d2,
+                                                    uint32_t C, const float *cen, const float *cenT, const float *cn,
+                                                    const double *r64, const float *rf, const double *cR,
+                                                    const double *rR, const double *uR, const double *seed_d2,
                                                     uint32_t mwords, uint32_t *gmask) {
     __shared__ float sc[64 * DMAX];
     __shared__ float sr[64];
+    __shared__ float scr[DMAX];
     __shared__ uint32_t sm[KNN_MAX_GROUPS / 32];
+    __shared__ uint32_t sg[KNN_MAX_GROUPS];
+    __shared__ uint32_t s_ng;
+    __shared__ float s_nr;
     const int lane = threadIdx.x & 63;
     for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {
         const uint64_t j = (uint64_t)R * KNN_BM + threadIdx.x;
@@ -1035,21 +1074,41 @@ __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32
         for (int t = 0; t < DMAX; t++) xr[t] = (real && (uint32_t)t < d) ? X[(uint64_t)y * d + t] : 0.0f;
         const float ux = real ? f32_up(sqrt(sd)) : 0.0f;
         for (uint32_t w = threadIdx.x; w < mwords; w += 256) sm[w] = 0u;
+        for (uint32_t t = threadIdx.x; t < DMAX; t += 256) scr[t] = t < d ? (float)cR[(uint64_t)R * d + t] : 0.0f;
+        if (threadIdx.x == 0) s_ng = 0u;
         const bool all = __syncthreads_or(real && !(sd < INFINITY)) != 0;
+        const double ur = uR[R];
         if (all) {
             for (uint32_t w = threadIdx.x; w < mwords; w += 256)
                 sm[w] = (w + 1) * 32 <= C ? 0xFFFFFFFFu : ((1u << (C - w * 32)) - 1u);
-        } else {
-            for (uint32_t g0 = 0; g0 < C; g0 += 64) {
-                const uint32_t gn = min(64u, C - g0);
+        } else if (ur >= 0.0) {
+            if (threadIdx.x == 0) {
+                float q = 0.0f;
+                for (uint32_t t = 0; t < d; t++) q = fmaf(scr[t], scr[t], q);
+                s_nr = sqrtf(q);
+            }
+            __syncthreads();
+            const double lim = rR[R] + ur;
+            const double nrn = (double)s_nr;
+            for (uint32_t g = threadIdx.x; g < C; g += 256) {
+                float s2 = 0.0f;
+                for (uint32_t t = 0; t < d; t++) {
+                    const float df = scr[t] - cenT[(uint64_t)t * C + g];
+                    s2 = fmaf(df, df, s2);
+                }
+                const bool skip = (double)sqrtf(s2) * (1.0 - 1e-5) - 1e-6 * (nrn + (double)cn[g]) > (lim + r64[g]) * (1.0 + 1e-12);
+                if (!skip) sg[atomicAdd(&s_ng, 1u)] = g;
+            }
+            __syncthreads();
+            const uint32_t ng = s_ng;
+            for (uint32_t g0 = 0; g0 < ng; g0 += 64) {
+                const uint32_t gn = min(64u, ng - g0);
                 __syncthreads();
                 for (uint32_t e = threadIdx.x; e < 64 * DMAX; e += 256) {
                     const uint32_t c = e / DMAX, t = e % DMAX;
-                    sc[e] = (c < gn && t < d) ? cen[(uint64_t)(g0 + c) * d + t] : 0.0f;
+                    sc[e] = (c < gn && t < d) ? cen[(uint64_t)sg[g0 + c] * d + t] : 0.0f;
                 }
-                // (a group without points keeps radius 0; its layout range, hence its block list, is empty)
-                if (threadIdx.x < gn)
-                    sr[threadIdx.x] = f32_up(sqrt(__longlong_as_double((long long)r2bits[g0 + threadIdx.x])) * (1.0 + 1e-12));
+                if (threadIdx.x < gn) sr[threadIdx.x] = rf[sg[g0 + threadIdx.x]];
                 __syncthreads();
                 for (uint32_t c = 0; c < gn; c++) {
                     float s2 = 0.0f;
@@ -1058,9 +1117,11 @@ __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32
                         const float df = xr[t] - sc[c * DMAX + t];
                         s2 = fmaf(df, df, s2);
                     }
-                    const float rg = sr[c];
-                    const bool need = real && !(sqrtf(s2) * (1.0f - 2e-5f) > (rg + ux) * (1.0f + 1e-6f));
-                    if (__ballot(need) && lane == 0) atomicOr(&sm[(g0 + c) >> 5], 1u << ((g0 + c) & 31u));
+                    const bool need = real && !(sqrtf(s2) * (1.0f - 2e-5f) > (sr[c] + ux) * (1.0f + 1e-6f));
+                    if (__ballot(need) && lane == 0) {
+                        const uint32_t g = sg[g0 + c];
+                        atomicOr(&sm[g >> 5], 1u << (g & 31u));
+                    }
                 }
             }
         }
@@ -1070,50 +1131,26 @@ __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32
     }
 }
 
-// Column blocks each row block needs: the blocks of the groups in its mask (k_row_groups), its
-// own group first (its nearest points tighten the tile's thresholds early), then the others in
-// increasing order, each group's blocks in layout order. A block is further skipped when
-// |cR - cB| - rR - rB > uR (uR bounds every row's k-th nearest distance, k_block_kth; +inf: no
-// skip; negative: the row block holds no points and needs nothing). The centres' distance is
-// evaluated in f32 from f32 centres: rounding the centres moves it by at most 2^-24 (|cR| +
-// |cB|) and the f32 sum of non-negative squares by a relative (d + 2) 2^-24, so the skip test
-// first takes 1e-5 relatively and 1e-6 (|cR| + |cB|) absolutely off the distance. One wave per
-// row block. Count pass (blist null): counts[R]; fill pass: blist[boff[R] + rank] = B.
-__global__ void __launch_bounds__(256) k_group_lists(const double *cR, const double *rR, const double *uR,
-                                                     const float *cB, const double *rB, const uint32_t *nB,
-                                                     uint32_t d, uint32_t nrb, const uint32_t *poff,
-                                                     const uint32_t *rbg, uint32_t mwords, const uint32_t *gmask,
-                                                     uint32_t tn, uint32_t *counts, const uint32_t *boff,
-                                                     uint32_t *blist) {
+// Column blocks each row block needs: the blocks holding points of the groups in its mask
+// (k_row_groups), its own group first (its nearest points tighten the tile's thresholds early),
+// then the others in increasing order, each group's blocks in layout order. A group's points
+// fill the start of its layout range, so a block holds points when its first position does.
+// (Within the needed groups a further sphere test per column block removes well under 1% of
+// the pairs on clustered data, and is not made.) Row blocks without points (uR < 0) need nothing.
+// One wave per row block. Count pass (blist null): counts[R]; fill pass: blist[boff[R] + rank] = B.
+__global__ void __launch_bounds__(256) k_group_lists(const double *uR, const uint32_t *layout, uint32_t nrb,
+                                                     const uint32_t *poff, const uint32_t *rbg, uint32_t mwords,
+                                                     const uint32_t *gmask, uint32_t tn, uint32_t *counts,
+                                                     const uint32_t *boff, uint32_t *blist) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t R = blockIdx.x * 4u + wave; R < nrb; R += gridDim.x * 4u) {  // wave-uniform
-        const double ur = uR[R];
         uint32_t total = 0;
-        if (ur >= 0.0) {
-            const bool all = !(ur < INFINITY);
-            const double lim = rR[R] + ur;
-            float nr2 = 0.0f;
-            for (uint32_t t = 0; t < d; t++) {
-                const float c = (float)cR[(uint64_t)R * d + t];
-                nr2 = fmaf(c, c, nr2);
-            }
-            const double nrn = (double)sqrtf(nr2);
+        if (uR[R] >= 0.0) {
             auto group = [&](uint32_t g) {
                 const uint32_t b0 = poff[g] / tn, b1 = poff[g + 1] / tn;
                 for (uint32_t B0 = b0; B0 < b1; B0 += 64) {
                     const uint32_t B = B0 + (uint32_t)lane;
-                    bool need = B < b1 && nB[B] > 0;
-                    if (need && !all) {
-                        float s2 = 0.0f, nb2 = 0.0f;
-                        for (uint32_t t = 0; t < d; t++) {
-                            const float cb = cB[(uint64_t)B * d + t];
-                            const float df = (float)cR[(uint64_t)R * d + t] - cb;
-                            s2 = fmaf(df, df, s2);
-                            nb2 = fmaf(cb, cb, nb2);
-                        }
-                        need = !((double)sqrtf(s2) * (1.0 - 1e-5) - 1e-6 * (nrn + (double)sqrtf(nb2)) >
-                                 (lim + rB[B]) * (1.0 + 1e-12));
-                    }
+                    const bool need = B < b1 && layout[(uint64_t)B * tn] != KNONE;
                     const uint64_t m = __ballot(need);
                     if (blist && need)
                         blist[boff[R] + total +
@@ -1137,6 +1174,120 @@ __global__ void __launch_bounds__(256) k_group_lists(const double *cR, const dou
     }
 }
 
+// Column-block spheres for the per-row list filter, in f32: centre cB32 = cB rounded to f32
+// (moving it by at most 2^-24 |cB|), its squared norm (f32 sum) and norm, and the radius rounded up.
+__global__ void k_block_prep(const double *cB, const double *rB, uint32_t ncb, uint32_t d, float *cB32, float *rBf,
+                             float *nbn, float *nb2) {
+    for (uint32_t B = blockIdx.x * blockDim.x + threadIdx.x; B < ncb; B += gridDim.x * blockDim.x) {
+        float q = 0.0f;
+        for (uint32_t t = 0; t < d; t++) {
+            const float c = (float)cB[(uint64_t)B * d + t];
+            cB32[(uint64_t)B * d + t] = c;
+            q = fmaf(c, c, q);
+        }
+        nb2[B] = q;
+        nbn[B] = sqrtf(q);
+        rBf[B] = f32_up(rB[B]);
+    }
+}
+
+// Per-row filter of every row block's list of column blocks, in place: block B stays when some
+// row x of the block has |x - cB| - rB <= u_x (u_x: the row's bound on its k-th nearest
+// distance). The squared distance to cB32 is evaluated as |x|^2 + |cB32|^2 - 2 x.cB32 in f32
+// (one FMA per coordinate); its error is below (3 DMAX + 8) 2^-24 (|x|^2 + |cB32|^2) (the two
+// norms and the dot product, |x.c| <= (|x|^2 + |c|^2) / 2, and the final additions), which is
+// taken off before the square root, and the root's rounding and 1e-6 |cB32| (rounding of the
+// centre) off after; rB and u_x are rounded up to f32 and their sum inflated by 1e-6. A row
+// block with a row without a bound keeps its list. The list of row block R is blist[lbeg[R] .. lbeg[R] +
+// lcnt[R]); the kept blocks move to its front in order, and lkept[R] gets their count. One
+// workgroup per row block, one thread per row; 64 blocks at a time pass through LDS, read
+// before any of them is written back (a block is written at or before its own position).
+template <int DMAX>
+__global__ void __launch_bounds__(256) k_list_filter(const float *X, const uint32_t *layout, uint32_t d, uint32_t nrb,
+                                                     const double *seed_d2, const float *cB32, const float *rBf,
+                                                     const float *nbn, const float *nb2, const uint32_t *lbeg,
+                                                     const uint32_t *lcnt, uint32_t *blist, uint32_t *lkept) {
+    __shared__ float sc[64 * DMAX];
+    __shared__ float sr[64], sn[64], sq[64];
+    __shared__ uint32_t sB[64];
+    __shared__ uint32_t sf[4][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {
+        const uint64_t j = (uint64_t)R * KNN_BM + threadIdx.x;
+        const uint32_t y = layout[j];
+        const bool real = y != KNONE;
+        const double sd = real ? seed_d2[j] : 0.0;
+        float xr[DMAX];
+#pragma unroll
+        for (int t = 0; t < DMAX; t++) xr[t] = (real && (uint32_t)t < d) ? X[(uint64_t)y * d + t] : 0.0f;
+        const float ux = real ? f32_up(sqrt(sd)) : 0.0f;
+        float xn2 = 0.0f;
+#pragma unroll
+        for (int t = 0; t < DMAX; t++) xn2 = fmaf(xr[t], xr[t], xn2);
+        constexpr float ecoef = (float)(3 * DMAX + 8) / 16777216.0f;
+        const uint32_t L = lcnt[R], base = lbeg[R];
+        const bool all = __syncthreads_or(real && !(sd < INFINITY)) != 0;
+        if (all) {
+            if (threadIdx.x == 0) lkept[R] = L;
+            continue;  // workgroup-uniform
+        }
+        uint32_t kept = 0;  // wave 0's count (the only wave that writes)
+        for (uint32_t c0 = 0; c0 < L; c0 += 64) {
+            const uint32_t cn = min(64u, L - c0);
+            __syncthreads();
+            if (threadIdx.x < cn) {
+                const uint32_t B = blist[base + c0 + threadIdx.x];
+                sB[threadIdx.x] = B;
+                sr[threadIdx.x] = rBf[B];
+                sn[threadIdx.x] = nbn[B];
+                sq[threadIdx.x] = nb2[B];
+            }
+            __syncthreads();
+            for (uint32_t e = threadIdx.x; e < 64 * DMAX; e += 256) {
+                const uint32_t c = e / DMAX, t = e % DMAX;
+                sc[e] = (c < cn && t < d) ? cB32[(uint64_t)sB[c] * d + t] : 0.0f;
+            }
+            __syncthreads();
+            uint64_t keep = 0;  // bit c: some row of this wave needs block c (wave-uniform)
+            for (uint32_t c = 0; c < cn; c++) {
+                float dot = 0.0f;
+#pragma unroll
+                for (int t = 0; t < DMAX; t++) dot = fmaf(xr[t], sc[c * DMAX + t], dot);
+                const float nn = xn2 + sq[c];
+                const float lo2 = fmaf(-2.0f, dot, nn) - ecoef * nn;
+                const bool need =
+                    real && !(sqrtf(fmaxf(lo2, 0.0f)) * (1.0f - 1e-6f) - 1e-6f * sn[c] > (sr[c] + ux) * (1.0f + 1e-6f));
+                keep |= (uint64_t)(__ballot(need) != 0ull) << c;
+            }
+            if (lane == 0) {
+                sf[wave][0] = (uint32_t)keep;
+                sf[wave][1] = (uint32_t)(keep >> 32);
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t lo = sf[0][0] | sf[1][0] | sf[2][0] | sf[3][0];
+                const uint32_t hi = sf[0][1] | sf[1][1] | sf[2][1] | sf[3][1];
+                const bool k = (uint32_t)lane < cn && (((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1u) != 0u);
+                const uint64_t m = __ballot(k);
+                if (k)
+                    blist[base + kept +
+                          __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = sB[lane];
+                kept += (uint32_t)__popcll(m);
+            }
+        }
+        if (threadIdx.x == 0) lkept[R] = kept;
+        __syncthreads();
+    }
+}
+
+// Sort keys of the longest-list-first order: the complement of the list length, and the row block.
+__global__ void k_lpt_keys(const uint32_t *bcnt, uint32_t nrb, uint32_t *keys, uint32_t *ids) {
+    for (uint32_t R = blockIdx.x * blockDim.x + threadIdx.x; R < nrb; R += gridDim.x * blockDim.x) {
+        keys[R] = ~bcnt[R];
+        ids[R] = R;
+    }
+}
+
 // Seed of every layout row's threshold in the tile's s = |y|^2 - 2 x.y space: s_exact of a point
 // within the row's k-th nearest distance is at most seed_d2 - |x|^2, and its approximation at most
 // eps above, so starting the threshold at seed_d2 - |x|^2 + 2.02 eps (rounded up) records every
@@ -1155,11 +1306,6 @@ __global__ void k_seed_thresholds(const double *seed_d2, const float *ny, uint32
         }
         seed_s[j] = out;
     }
-}
-
-__global__ void k_f64_to_f32(const double *in, uint64_t n, float *out) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = (float)in[i];
 }
 
 // Candidate lists of layout row j (per_row entries; rows layout rows) -> the point's own row,
@@ -1257,11 +1403,44 @@ __global__ void __launch_bounds__(256) k_block_kth(const float *X, const uint32_
 
 // Grouped layout: group g (sorted positions start[g] .. start[g + 1] of perm) occupies layout
 // positions poff[g] .. poff[g] + size, padded to whole row blocks; other positions are KNONE.
-__global__ void k_layout_fill(const uint32_t *sorted_keys, const uint32_t *perm, const uint32_t *start,
+// The group of sorted position p is sorted_keys[p] >> shift.
+__global__ void k_layout_fill(const uint32_t *sorted_keys, uint32_t shift, const uint32_t *perm, const uint32_t *start,
                               const uint32_t *poff, uint32_t n, uint32_t *layout) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t g = sorted_keys[p];
+        const uint32_t g = sorted_keys[p] >> shift;
         layout[poff[g] + (p - start[g])] = perm[p];
+    }
+}
+
+// Distance of every point (sorted position p: point perm[p] of group keys[p]) to its group's
+// centre, f32, and their maximum (bits of a non-negative float order as unsigned integers).
+__global__ void k_group_dist(const float *X, uint32_t d, const float *cen, const uint32_t *keys, const uint32_t *perm,
+                             uint32_t n, float *dist, unsigned int *dmax) {
+    float m = 0.0f;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t y = perm[p], g = keys[p];
+        float q = 0.0f;
+        for (uint32_t t = 0; t < d; t++) {
+            const float df = X[(uint64_t)y * d + t] - cen[(uint64_t)g * d + t];
+            q = fmaf(df, df, q);
+        }
+        const float r = sqrtf(q);
+        dist[p] = r;
+        m = fmaxf(m, r);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(m));
+}
+
+// keys[p] = group << qbits | the distance quantised to qbits bits (of the largest): sorted, the
+// points of a group run from its centre outwards.
+__global__ void k_dist_keys(uint32_t *keys, const float *dist, const unsigned int *dmax, uint32_t n, uint32_t qbits) {
+    const float m = __uint_as_float(*dmax);
+    const float top = (float)((1u << qbits) - 1u);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const float f = m > 0.0f ? fminf(dist[p] / m * top, top) : 0.0f;
+        keys[p] = (keys[p] << qbits) | (uint32_t)f;
     }
 }
 
@@ -1293,6 +1472,10 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
                        cen.get());
     KERNEL_OK();
     const int kb = std::max(1, bits_for(C));
+    // distance bits of the final sort key: at most 20, so the quantiser's top value (2^qbits - 1)
+    // and every value below it are exact in f32 and the quantised distance never reaches the
+    // group bits
+    const uint32_t qbits = std::min<uint32_t>(20u, 32u - (uint32_t)kb);
     for (int it = 0; it <= ITERS; it++) {
         hipLaunchKernelGGL(k_kmeans_norms, dim3(grid_for(C, 256, 1024)), dim3(256), 0, s, cen.get(), d, C, cnorm.get());
         KERNEL_OK();
@@ -1312,7 +1495,21 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
         hipLaunchKernelGGL(k_key_starts_u32, dim3(grid_for(C + 1, 256, 1024)), dim3(256), 0, s, keys.get(), n, C,
                            start.get());
         KERNEL_OK();
-        if (it == ITERS) break;
+        if (it == ITERS) {
+            // within each group, points in increasing distance to the centre: a group's
+            // outliers share its last column blocks, so the spheres of its other blocks stay tight
+            DevBuf<float> gdist(n);
+            DevBuf<unsigned int> dmax(1);
+            HIP_OK(hipMemsetAsync(dmax.get(), 0, sizeof(unsigned int), s));
+            hipLaunchKernelGGL(k_group_dist, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, d, cen.get(), keys.get(),
+                               perm.get(), n, gdist.get(), dmax.get());
+            KERNEL_OK();
+            hipLaunchKernelGGL(k_dist_keys, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, keys.get(), gdist.get(),
+                               dmax.get(), n, qbits);
+            KERNEL_OK();
+            radix_sort_pairs(keys, perm, n, kb + (int)qbits, s);
+            break;
+        }
         hipLaunchKernelGGL(k_kmeans_update, dim3(std::min<uint32_t>(C, 8192)), dim3(256), 0, s, d_X, d, perm.get(),
                            start.get(), C, cen.get());
         KERNEL_OK();
@@ -1327,7 +1524,7 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
     layout.resize(rows);
     hipLaunchKernelGGL(k_fill_u32_knn, dim3(grid_for(rows, 256, 4096)), dim3(256), 0, s, layout.get(), rows, KNONE);
     KERNEL_OK();
-    hipLaunchKernelGGL(k_layout_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, keys.get(), perm.get(),
+    hipLaunchKernelGGL(k_layout_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, keys.get(), qbits, perm.get(),
                        start.get(), poff.get(), n, layout.get());
     KERNEL_OK();
     // the centres of the final assignment and the groups' layout offsets (the group-level pruning)
@@ -1340,24 +1537,26 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
 template <bool BF16, int NKC, int KM>
 static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                            uint32_t d, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
-                           const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
+                           const uint32_t *boff, const uint32_t *bcnt, const uint32_t *rbo, const float *seed_s,
+                           uint32_t *cidx, float *cs, hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
     const uint32_t row_blocks = grid.x, splits = grid.y;
     const dim3 g1(8 * ((row_blocks * splits + 7) / 8));  // one dimension; the kernel maps it XCD-aware
     hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), g1, dim3(256), lds, s, Xh, Xl, Xf, d, n, npad, cps, row_blocks, splits, blist,
-                       boff, seed_s, cidx, cs);
+                       boff, bcnt, rbo, seed_s, cidx, cs);
     KERNEL_OK();
 }
 
 template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                         uint32_t d, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
-                        const uint32_t *boff, const float *seed_s, uint32_t *cidx, float *cs, hipStream_t s) {
+                        const uint32_t *boff, const uint32_t *bcnt, const uint32_t *rbo, const float *seed_s,
+                        uint32_t *cidx, float *cs, hipStream_t s) {
     switch (KM) {
-        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
-        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
-        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, seed_s, cidx, cs, s); break;
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, cidx, cs, s); break;
     }
 }
 
@@ -1448,19 +1647,16 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     // pruned search: block spheres, in-block k-th distance bounds, each row block's list of
     // column blocks
     const uint32_t TN = nkc <= 2 ? 64u : 32u;  // KnnTileShape<NKC>::TN
-    DevBuf<uint32_t> blist, boff;
+    DevBuf<uint32_t> blist, boff, bcnt, rb_order;
     DevBuf<double> seed_d2;
     uint64_t list_total = 0;
     if (prune) {
         seed_d2.resize(rows);
         const uint32_t ncb = rows / TN;
-        DevBuf<double> cR((size_t)row_blocks * d), rR(row_blocks), uR(row_blocks), cB((size_t)ncb * d), rB(ncb);
-        DevBuf<uint32_t> nR(row_blocks), nB(ncb), cnt(row_blocks + 1);
+        DevBuf<double> cR((size_t)row_blocks * d), rR(row_blocks), uR(row_blocks);
+        DevBuf<uint32_t> nR(row_blocks), cnt(row_blocks + 1);
         hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
                            layout.get(), rows, d, (uint32_t)KNN_BM, row_blocks, cR.get(), rR.get(), nR.get());
-        KERNEL_OK();
-        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(ncb, 16384)), dim3(256), 0, s, d_X, layout.get(),
-                           rows, d, TN, ncb, cB.get(), rB.get(), nB.get());
         KERNEL_OK();
         // the coordinates held per row (DMAX >= d) and the insertion network (KK >= k) are sized
         // to the search; the first k entries of the network are the same for any KK >= k
@@ -1476,10 +1672,6 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         KERNEL_OK();
         hipLaunchKernelGGL(k_fill_u32_knn, dim3(1), dim3(64), 0, s, cnt.get() + row_blocks, 1u, 0u);
         KERNEL_OK();
-        DevBuf<float> cB32((size_t)ncb * d);
-        hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for((uint64_t)ncb * d, 256, 4096)), dim3(256), 0, s, cB.get(),
-                           (uint64_t)ncb * d, cB32.get());
-        KERNEL_OK();
         // groups each row block needs (per-row bounds against the groups' spheres), then its list
         // of column blocks within those groups
         DevBuf<uint32_t> rbg(row_blocks);
@@ -1493,18 +1685,24 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         KERNEL_OK();
         if (gC == 0 || gC > KNN_MAX_GROUPS) throw ScError(SC_INVALID_ARGUMENT, "kNN grouping: group count out of range");
         const uint32_t mwords = (gC + 31) / 32;
+        DevBuf<float> cenT((size_t)gC * d), gcn(gC), grf(gC);
+        DevBuf<double> gr64(gC);
+        hipLaunchKernelGGL(k_group_prep, dim3(grid_for(gC, 256, 64)), dim3(256), 0, s, gcen.get(), gC, d, r2bits.get(),
+                           cenT.get(), gcn.get(), gr64.get(), grf.get());
+        KERNEL_OK();
         DevBuf<uint32_t> gmask((size_t)row_blocks * mwords);
         auto rowg = [&](auto kern) {
-            hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks, gC, gcen.get(),
-                               r2bits.get(), seed_d2.get(), mwords, gmask.get());
+            hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks, gC, gcen.get(), cenT.get(),
+                               gcn.get(), gr64.get(), grf.get(), cR.get(), rR.get(), uR.get(), seed_d2.get(), mwords,
+                               gmask.get());
         };
         if (d <= 32) rowg(k_row_groups<32>);
         else if (d <= 64) rowg(k_row_groups<64>);
         else rowg(k_row_groups<128>);
         KERNEL_OK();
         const dim3 ngrid(std::min<uint32_t>((row_blocks + 3) / 4, 16384));
-        hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
-                           nB.get(), d, row_blocks, gpoff.get(), rbg.get(), mwords, gmask.get(), TN, cnt.get(),
+        hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, uR.get(), layout.get(), row_blocks, gpoff.get(),
+                           rbg.get(), mwords, gmask.get(), TN, cnt.get(),
                            (const uint32_t *)nullptr, (uint32_t *)nullptr);
         KERNEL_OK();
         boff.resize(row_blocks + 1);
@@ -1512,16 +1710,50 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         uint32_t lt = 0;
         to_host(&lt, boff.get() + row_blocks, 1, s);
         sync_stream(s);
-        list_total = lt;
-        blist.resize(std::max<uint64_t>(list_total, 1));
-        hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, cR.get(), rR.get(), uR.get(), cB32.get(), rB.get(),
-                           nB.get(), d, row_blocks, gpoff.get(), rbg.get(), mwords, gmask.get(), TN, (uint32_t *)nullptr,
+        blist.resize(std::max<uint64_t>(lt, 1));
+        hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, uR.get(), layout.get(), row_blocks, gpoff.get(),
+                           rbg.get(), mwords, gmask.get(), TN, (uint32_t *)nullptr,
                            (const uint32_t *)boff.get(), blist.get());
         KERNEL_OK();
+        // per-row filter of the lists against the column blocks' spheres (in place; bcnt: kept)
+        DevBuf<double> cB((size_t)ncb * d), rB(ncb);
+        DevBuf<uint32_t> nB(ncb);
+        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(ncb, 16384)), dim3(256), 0, s, d_X, layout.get(),
+                           rows, d, TN, ncb, cB.get(), rB.get(), nB.get());
+        KERNEL_OK();
+        DevBuf<float> cB32((size_t)ncb * d), rBf(ncb), nbn(ncb), nb2(ncb);
+        hipLaunchKernelGGL(k_block_prep, dim3(grid_for(ncb, 256, 4096)), dim3(256), 0, s, cB.get(), rB.get(), ncb, d,
+                           cB32.get(), rBf.get(), nbn.get(), nb2.get());
+        KERNEL_OK();
+        bcnt.resize(row_blocks + 1);
+        hipLaunchKernelGGL(k_fill_u32_knn, dim3(1), dim3(64), 0, s, bcnt.get() + row_blocks, 1u, 0u);
+        KERNEL_OK();
+        auto lfilt = [&](auto kern) {
+            hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks, seed_d2.get(), cB32.get(),
+                               rBf.get(), nbn.get(), nb2.get(), boff.get(), cnt.get(), blist.get(), bcnt.get());
+        };
+        if (d <= 32) lfilt(k_list_filter<32>);
+        else if (d <= 64) lfilt(k_list_filter<64>);
+        else lfilt(k_list_filter<128>);
+        KERNEL_OK();
+        // entries kept over all row blocks (the pairs the tile computes)
+        exclusive_scan_u32(bcnt.get(), cnt.get(), row_blocks + 1, s);
+        to_host(&lt, cnt.get() + row_blocks, 1, s);
+        // row blocks by decreasing list length (ties in row-block order) for the tile's grid
+        DevBuf<uint32_t> lkeys(row_blocks);
+        rb_order.resize(row_blocks);
+        hipLaunchKernelGGL(k_lpt_keys, dim3(grid_for(row_blocks, 256, 4096)), dim3(256), 0, s, bcnt.get(), row_blocks,
+                           lkeys.get(), rb_order.get());
+        KERNEL_OK();
+        radix_sort_pairs(lkeys, rb_order, row_blocks, 32, s);
+        sync_stream(s);
+        list_total = lt;
     }
     const uint32_t *pm = prune ? layout.get() : nullptr;
     const uint32_t *bl = prune ? blist.get() : nullptr;
     const uint32_t *bo = prune ? boff.get() : nullptr;
+    const uint32_t *bc = prune ? bcnt.get() : nullptr;
+    const uint32_t *ro = prune ? rb_order.get() : nullptr;
     const uint64_t total = (uint64_t)rows * Kp;
     hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, s, d_X, pm, n, d, rows, ny.get(),
                        nx64.get(), ymax.get());
@@ -1567,14 +1799,14 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const unsigned short *xh = Xh.get(), *xl = Xl.get();
     const float *xf = Xf.get();
     switch (nkc * 2 + (bf16 ? 0 : 1)) {
-        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
-        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, sd, cidx.get(), cs.get(), s); break;
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
     }
     HIP_OK(hipEventRecord(e2, s));
     if (prune) {
