@@ -984,14 +984,26 @@ __global__ void k_rb_groups(const uint32_t *poff, uint32_t C, uint32_t nrb, uint
     }
 }
 
-// Squared radius of every group about its centre cen_g: r2bits[g] = the bits of the largest
-// |y - cen_g|^2 over the group's points, summed in f64 from the f32 values (relative error below
-// (d + 2) 2^-53; the readers inflate the radius by 1e-12). Non-negative doubles order like their
-// bit patterns, so the maximum is an integer atomic. A wave covers 64 consecutive positions of
-// one row block, hence of one group: one atomic per wave.
+// Core of every group: its points run from the centre outwards (the layout's order), and its
+// first ceil(0.9 size / 64) blocks of 64 positions are its core, ending at cend[g]; the rest
+// (the farthest tenth, which holds the outliers that set the group's radius) is its tail.
+__global__ void k_group_core(const uint32_t *start, const uint32_t *poff, uint32_t C, uint32_t *cend) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < C; g += gridDim.x * blockDim.x) {
+        const uint32_t size = start[g + 1] - start[g];
+        const uint32_t ncore = min((9u * size + 639u) / 640u, (size + 63u) / 64u);
+        cend[g] = poff[g] + 64u * ncore;
+    }
+}
+
+// Squared radii of every group about its centre cen_g: r2bits[g] = the bits of the largest
+// |y - cen_g|^2 over the group's points, r2core[g] over its core's, summed in f64 from the f32
+// values (relative error below (d + 2) 2^-53; the readers inflate the radius by 1e-12).
+// Non-negative doubles order like their bit patterns, so the maximum is an integer atomic. A
+// wave covers 64 consecutive positions of one row block, hence of one group and of its core or
+// its tail: one atomic per wave and radius.
 __global__ void __launch_bounds__(256) k_group_radii(const float *X, const uint32_t *layout, uint32_t rows, uint32_t d,
-                                                     const float *cen, const uint32_t *rbg,
-                                                     unsigned long long *r2bits) {
+                                                     const float *cen, const uint32_t *rbg, const uint32_t *cend,
+                                                     unsigned long long *r2bits, unsigned long long *r2core) {
     for (uint32_t p0 = blockIdx.x * 256u; p0 < rows; p0 += gridDim.x * 256u) {  // rows: whole row blocks
         const uint32_t p = p0 + threadIdx.x;
         const uint32_t g = rbg[p / (uint32_t)KNN_BM];
@@ -1006,7 +1018,11 @@ __global__ void __launch_bounds__(256) k_group_radii(const float *X, const uint3
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
-        if ((threadIdx.x & 63) == 0 && r2 >= 0.0) atomicMax(&r2bits[g], (unsigned long long)__double_as_longlong(r2));
+        if ((threadIdx.x & 63) == 0 && r2 >= 0.0) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(r2);
+            atomicMax(&r2bits[g], b);
+            if (p < cend[g]) atomicMax(&r2core[g], b);
+        }
     }
 }
 
@@ -1018,9 +1034,11 @@ __device__ __forceinline__ float f32_up(double v) {
 }
 
 // Per-group data of the group-level pruning: centres transposed (cenT[t * C + g]) and their f32
-// norms, and the radius rounded up: r64 = sqrt(r2) (1 + 1e-12) (f64), rf = r64 rounded up to f32.
-__global__ void k_group_prep(const float *cen, uint32_t C, uint32_t d, const unsigned long long *r2bits, float *cenT,
-                             float *cn, double *r64, float *rf) {
+// norms, and the radii rounded up: r64 = sqrt(r2) (1 + 1e-12) (f64), rf = r64 rounded up to f32,
+// rcf the same for the core.
+__global__ void k_group_prep(const float *cen, uint32_t C, uint32_t d, const unsigned long long *r2bits,
+                             const unsigned long long *r2core, float *cenT, float *cn, double *r64, float *rf,
+                             float *rcf) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < C; g += gridDim.x * blockDim.x) {
         float q = 0.0f;
         for (uint32_t t = 0; t < d; t++) {
@@ -1033,13 +1051,16 @@ __global__ void k_group_prep(const float *cen, uint32_t C, uint32_t d, const uns
         const double r = sqrt(__longlong_as_double((long long)r2bits[g])) * (1.0 + 1e-12);
         r64[g] = r;
         rf[g] = f32_up(r);
+        rcf[g] = f32_up(sqrt(__longlong_as_double((long long)r2core[g])) * (1.0 + 1e-12));
     }
 }
 
-// Groups each row block needs, as a bit mask of mwords words per row block. Every point y of
-// group g has |x - y| >= |x - cen_g| - r_g, so g holds none of row x's k nearest points when
-// |x - cen_g| - r_g exceeds u_x, the row's bound on its k-th nearest distance (sqrt(seed_d2));
-// the block needs g when some row does. The bound is per row: a row block's bounding sphere
+// Groups each row block needs, as two bit masks of mwords words per row block: gmask[R][0] the
+// groups needed at all, gmask[R][1] those whose core is needed (a subset: the core's radius is
+// at most the group's). Every point y of group g has |x - y| >= |x - cen_g| - r_g, so g holds
+// none of row x's k nearest points when |x - cen_g| - r_g exceeds u_x, the row's bound on its
+// k-th nearest distance (sqrt(seed_d2)), and its core none when |x - cen_g| - rc_g does; the
+// block needs what some row needs. The bound is per row: a row block's bounding sphere
 // adds the block's radius to the group's, which in high dimensions is most of the distance
 // between neighbouring clusters. The block's sphere (cR, rR, uR >= every u_x) first passes over
 // the groups no row can reach (|cR - cen_g| - rR - r_g > uR, the test of the column-block lists
@@ -1053,13 +1074,13 @@ constexpr uint32_t KNN_MAX_GROUPS = 2048;
 template <int DMAX>
 __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32_t *layout, uint32_t d, uint32_t nrb,
                                                     uint32_t C, const float *cen, const float *cenT, const float *cn,
-                                                    const double *r64, const float *rf, const double *cR,
-                                                    const double *rR, const double *uR, const double *seed_d2,
-                                                    uint32_t mwords, uint32_t *gmask) {
+                                                    const double *r64, const float *rf, const float *rcf,
+                                                    const double *cR, const double *rR, const double *uR,
+                                                    const double *seed_d2, uint32_t mwords, uint32_t *gmask) {
     __shared__ float sc[64 * DMAX];
-    __shared__ float sr[64];
+    __shared__ float sr[64], src[64];
     __shared__ float scr[DMAX];
-    __shared__ uint32_t sm[KNN_MAX_GROUPS / 32];
+    __shared__ uint32_t sm[2 * (KNN_MAX_GROUPS / 32)];
     __shared__ uint32_t sg[KNN_MAX_GROUPS];
     __shared__ uint32_t s_ng;
     __shared__ float s_nr;
@@ -1073,14 +1094,14 @@ __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32
 #pragma unroll
         for (int t = 0; t < DMAX; t++) xr[t] = (real && (uint32_t)t < d) ? X[(uint64_t)y * d + t] : 0.0f;
         const float ux = real ? f32_up(sqrt(sd)) : 0.0f;
-        for (uint32_t w = threadIdx.x; w < mwords; w += 256) sm[w] = 0u;
+        for (uint32_t w = threadIdx.x; w < 2 * mwords; w += 256) sm[w] = 0u;
         for (uint32_t t = threadIdx.x; t < DMAX; t += 256) scr[t] = t < d ? (float)cR[(uint64_t)R * d + t] : 0.0f;
         if (threadIdx.x == 0) s_ng = 0u;
         const bool all = __syncthreads_or(real && !(sd < INFINITY)) != 0;
         const double ur = uR[R];
         if (all) {
             for (uint32_t w = threadIdx.x; w < mwords; w += 256)
-                sm[w] = (w + 1) * 32 <= C ? 0xFFFFFFFFu : ((1u << (C - w * 32)) - 1u);
+                sm[w] = sm[mwords + w] = (w + 1) * 32 <= C ? 0xFFFFFFFFu : ((1u << (C - w * 32)) - 1u);
         } else if (ur >= 0.0) {
             if (threadIdx.x == 0) {
                 float q = 0.0f;
@@ -1108,7 +1129,10 @@ __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32
                     const uint32_t c = e / DMAX, t = e % DMAX;
                     sc[e] = (c < gn && t < d) ? cen[(uint64_t)sg[g0 + c] * d + t] : 0.0f;
                 }
-                if (threadIdx.x < gn) sr[threadIdx.x] = rf[sg[g0 + threadIdx.x]];
+                if (threadIdx.x < gn) {
+                    sr[threadIdx.x] = rf[sg[g0 + threadIdx.x]];
+                    src[threadIdx.x] = rcf[sg[g0 + threadIdx.x]];
+                }
                 __syncthreads();
                 for (uint32_t c = 0; c < gn; c++) {
                     float s2 = 0.0f;
@@ -1117,37 +1141,40 @@ __global__ void __launch_bounds__(256) k_row_groups(const float *X, const uint32
                         const float df = xr[t] - sc[c * DMAX + t];
                         s2 = fmaf(df, df, s2);
                     }
-                    const bool need = real && !(sqrtf(s2) * (1.0f - 2e-5f) > (sr[c] + ux) * (1.0f + 1e-6f));
-                    if (__ballot(need) && lane == 0) {
+                    const float dl = sqrtf(s2) * (1.0f - 2e-5f);
+                    const bool need = real && !(dl > (sr[c] + ux) * (1.0f + 1e-6f));
+                    const bool need_core = real && !(dl > (src[c] + ux) * (1.0f + 1e-6f));
+                    const bool any = __ballot(need) != 0ull, any_core = __ballot(need_core) != 0ull;
+                    if (lane == 0 && any) {
                         const uint32_t g = sg[g0 + c];
                         atomicOr(&sm[g >> 5], 1u << (g & 31u));
+                        if (any_core) atomicOr(&sm[mwords + (g >> 5)], 1u << (g & 31u));
                     }
                 }
             }
         }
         __syncthreads();
-        for (uint32_t w = threadIdx.x; w < mwords; w += 256) gmask[(uint64_t)R * mwords + w] = sm[w];
+        for (uint32_t w = threadIdx.x; w < 2 * mwords; w += 256) gmask[(uint64_t)R * 2 * mwords + w] = sm[w];
         __syncthreads();
     }
 }
 
-// Column blocks each row block needs: the blocks holding points of the groups in its mask
-// (k_row_groups), its own group first (its nearest points tighten the tile's thresholds early),
-// then the others in increasing order, each group's blocks in layout order. A group's points
-// fill the start of its layout range, so a block holds points when its first position does.
-// (Within the needed groups a further sphere test per column block removes well under 1% of
-// the pairs on clustered data, and is not made.) Row blocks without points (uR < 0) need nothing.
-// One wave per row block. Count pass (blist null): counts[R]; fill pass: blist[boff[R] + rank] = B.
+// Column blocks each row block needs: the blocks holding points of the groups in its masks
+// (k_row_groups; a group's core blocks when its core is needed, its tail blocks when the group
+// is), its own group first (its nearest points tighten the tile's thresholds early), then the
+// others in increasing order, each group's blocks in layout order. A group's points fill the
+// start of its layout range, so a block holds points when its first position does. Row blocks
+// without points (uR < 0) need nothing. One wave per row block. Count pass (blist null):
+// counts[R]; fill pass: blist[boff[R] + rank] = B.
 __global__ void __launch_bounds__(256) k_group_lists(const double *uR, const uint32_t *layout, uint32_t nrb,
-                                                     const uint32_t *poff, const uint32_t *rbg, uint32_t mwords,
-                                                     const uint32_t *gmask, uint32_t tn, uint32_t *counts,
-                                                     const uint32_t *boff, uint32_t *blist) {
+                                                     const uint32_t *poff, const uint32_t *cend, const uint32_t *rbg,
+                                                     uint32_t mwords, const uint32_t *gmask, uint32_t tn,
+                                                     uint32_t *counts, const uint32_t *boff, uint32_t *blist) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t R = blockIdx.x * 4u + wave; R < nrb; R += gridDim.x * 4u) {  // wave-uniform
         uint32_t total = 0;
         if (uR[R] >= 0.0) {
-            auto group = [&](uint32_t g) {
-                const uint32_t b0 = poff[g] / tn, b1 = poff[g + 1] / tn;
+            auto range = [&](uint32_t b0, uint32_t b1) {
                 for (uint32_t B0 = b0; B0 < b1; B0 += 64) {
                     const uint32_t B = B0 + (uint32_t)lane;
                     const bool need = B < b1 && layout[(uint64_t)B * tn] != KNONE;
@@ -1158,10 +1185,15 @@ __global__ void __launch_bounds__(256) k_group_lists(const double *uR, const uin
                     total += (uint32_t)__popcll(m);
                 }
             };
+            const uint32_t *mall = gmask + (uint64_t)R * 2 * mwords, *mcore = mall + mwords;
+            auto group = [&](uint32_t g) {
+                if ((mcore[g >> 5] >> (g & 31u)) & 1u) range(poff[g] / tn, cend[g] / tn);
+                range(cend[g] / tn, poff[g + 1] / tn);
+            };
             const uint32_t own = rbg[R];
-            group(own);
+            if ((mall[own >> 5] >> (own & 31u)) & 1u) group(own);
             for (uint32_t w = 0; w < mwords; w++) {
-                uint32_t word = gmask[(uint64_t)R * mwords + w];
+                uint32_t word = mall[w];
                 if (w == (own >> 5)) word &= ~(1u << (own & 31u));
                 while (word) {
                     const uint32_t bit = (uint32_t)__builtin_ctz(word);
@@ -1459,9 +1491,11 @@ static bool knn_prune_enabled(uint32_t n) {
 // Layout of the pruned search: points grouped by k-means (C centroids, a few Lloyd
 // iterations), stable in the points' own order within a group, every group starting on a row
 // block; returns the layout's length (a multiple of 256), layout[j] = point or KNONE, and the
-// groups: gC of them, centres gcen (gC x d), layout offsets gpoff (gC + 1, whole row blocks).
+// groups: gC of them, centres gcen (gC x d), layout offsets gpoff (gC + 1, whole row blocks),
+// sizes gstart[g + 1] - gstart[g].
 static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf<uint32_t> &layout,
-                                DevBuf<float> &gcen, DevBuf<uint32_t> &gpoff, uint32_t &gC, hipStream_t s) {
+                                DevBuf<float> &gcen, DevBuf<uint32_t> &gpoff, DevBuf<uint32_t> &gstart, uint32_t &gC,
+                                hipStream_t s) {
     const uint32_t C = std::min<uint32_t>(n, std::max<uint32_t>(16u, std::min<uint32_t>(2048u, n / 512u)));
     DevBuf<uint32_t> perm;
     constexpr int ITERS = 2;
@@ -1530,6 +1564,7 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
     // the centres of the final assignment and the groups' layout offsets (the group-level pruning)
     gcen = std::move(cen);
     gpoff = std::move(poff);
+    gstart = std::move(start);
     gC = C;
     return rows;
 }
@@ -1584,9 +1619,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     bool prune = knn_prune_enabled(n);
     DevBuf<uint32_t> layout;
     DevBuf<float> gcen;
-    DevBuf<uint32_t> gpoff;
+    DevBuf<uint32_t> gpoff, gstart;
     uint32_t gC = 0;
-    uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, gcen, gpoff, gC, s) : npad;
+    uint32_t rows = prune ? knn_group_order(d_X, n, d, layout, gcen, gpoff, gstart, gC, s) : npad;
     // the lists of column blocks are addressed with 32-bit offsets: with more (row block, column
     // block) pairs than that the search runs unpruned
     if (prune && (uint64_t)(rows / KNN_BM) * (rows / (nkc <= 2 ? 64u : 32u)) >= 0xFFFFFFFFull) {
@@ -1678,23 +1713,28 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         hipLaunchKernelGGL(k_rb_groups, dim3(grid_for(row_blocks, 256, 4096)), dim3(256), 0, s, gpoff.get(), gC,
                            row_blocks, rbg.get());
         KERNEL_OK();
-        DevBuf<unsigned long long> r2bits(gC);
+        DevBuf<uint32_t> cend(gC);
+        hipLaunchKernelGGL(k_group_core, dim3(grid_for(gC, 256, 64)), dim3(256), 0, s, gstart.get(), gpoff.get(), gC,
+                           cend.get());
+        KERNEL_OK();
+        DevBuf<unsigned long long> r2bits(gC), r2core(gC);
         HIP_OK(hipMemsetAsync(r2bits.get(), 0, sizeof(unsigned long long) * gC, s));
+        HIP_OK(hipMemsetAsync(r2core.get(), 0, sizeof(unsigned long long) * gC, s));
         hipLaunchKernelGGL(k_group_radii, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
-                           layout.get(), rows, d, gcen.get(), rbg.get(), r2bits.get());
+                           layout.get(), rows, d, gcen.get(), rbg.get(), cend.get(), r2bits.get(), r2core.get());
         KERNEL_OK();
         if (gC == 0 || gC > KNN_MAX_GROUPS) throw ScError(SC_INVALID_ARGUMENT, "kNN grouping: group count out of range");
         const uint32_t mwords = (gC + 31) / 32;
-        DevBuf<float> cenT((size_t)gC * d), gcn(gC), grf(gC);
+        DevBuf<float> cenT((size_t)gC * d), gcn(gC), grf(gC), grc(gC);
         DevBuf<double> gr64(gC);
         hipLaunchKernelGGL(k_group_prep, dim3(grid_for(gC, 256, 64)), dim3(256), 0, s, gcen.get(), gC, d, r2bits.get(),
-                           cenT.get(), gcn.get(), gr64.get(), grf.get());
+                           r2core.get(), cenT.get(), gcn.get(), gr64.get(), grf.get(), grc.get());
         KERNEL_OK();
-        DevBuf<uint32_t> gmask((size_t)row_blocks * mwords);
+        DevBuf<uint32_t> gmask((size_t)row_blocks * 2 * mwords);
         auto rowg = [&](auto kern) {
             hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks, gC, gcen.get(), cenT.get(),
-                               gcn.get(), gr64.get(), grf.get(), cR.get(), rR.get(), uR.get(), seed_d2.get(), mwords,
-                               gmask.get());
+                               gcn.get(), gr64.get(), grf.get(), grc.get(), cR.get(), rR.get(), uR.get(), seed_d2.get(),
+                               mwords, gmask.get());
         };
         if (d <= 32) rowg(k_row_groups<32>);
         else if (d <= 64) rowg(k_row_groups<64>);
@@ -1702,7 +1742,7 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         KERNEL_OK();
         const dim3 ngrid(std::min<uint32_t>((row_blocks + 3) / 4, 16384));
         hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, uR.get(), layout.get(), row_blocks, gpoff.get(),
-                           rbg.get(), mwords, gmask.get(), TN, cnt.get(),
+                           cend.get(), rbg.get(), mwords, gmask.get(), TN, cnt.get(),
                            (const uint32_t *)nullptr, (uint32_t *)nullptr);
         KERNEL_OK();
         boff.resize(row_blocks + 1);
@@ -1712,7 +1752,7 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         sync_stream(s);
         blist.resize(std::max<uint64_t>(lt, 1));
         hipLaunchKernelGGL(k_group_lists, ngrid, dim3(256), 0, s, uR.get(), layout.get(), row_blocks, gpoff.get(),
-                           rbg.get(), mwords, gmask.get(), TN, (uint32_t *)nullptr,
+                           cend.get(), rbg.get(), mwords, gmask.get(), TN, (uint32_t *)nullptr,
                            (const uint32_t *)boff.get(), blist.get());
         KERNEL_OK();
         // per-row filter of the lists against the column blocks' spheres (in place; bcnt: kept)
