@@ -741,6 +741,15 @@ __global__ void k_knn_iota(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
 }
 
+// Every stride-th point of X (n_out of them), as rows of d values.
+__global__ void k_gather_rows(const float *X, uint32_t d, uint32_t stride, uint32_t n_out, float *out) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < (uint64_t)n_out * d;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / d, t = e % d;
+        out[e] = X[i * stride * d + t];
+    }
+}
+
 // Initial centroids: C points spread evenly over the input order; f32 squared norms.
 __global__ void k_kmeans_init(const float *X, uint32_t n, uint32_t d, uint32_t C, float *cen) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < (uint64_t)C * d;
@@ -1488,21 +1497,32 @@ static bool knn_prune_enabled(uint32_t n) {
     return n >= 16384;
 }
 
-// Layout of the pruned search: points grouped by k-means (C centroids, a few Lloyd
-// iterations), stable in the points' own order within a group, every group starting on a row
-// block; returns the layout's length (a multiple of 256), layout[j] = point or KNONE, and the
+// Layout of the pruned search: points grouped by k-means (C centroids, Lloyd iterations on a
+// sample, then one assignment of every point), within a group in increasing distance to its
+// centre (quantised; ties in the points' own order), every group starting on a row block; returns the layout's length (a multiple of 256), layout[j] = point or KNONE, and the
 // groups: gC of them, centres gcen (gC x d), layout offsets gpoff (gC + 1, whole row blocks),
 // sizes gstart[g + 1] - gstart[g].
 static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf<uint32_t> &layout,
                                 DevBuf<float> &gcen, DevBuf<uint32_t> &gpoff, DevBuf<uint32_t> &gstart, uint32_t &gC,
                                 hipStream_t s) {
     const uint32_t C = std::min<uint32_t>(n, std::max<uint32_t>(16u, std::min<uint32_t>(2048u, n / 512u)));
-    DevBuf<uint32_t> perm;
     constexpr int ITERS = 2;
+    // Lloyd iterations on a sample (every stride-th point, about 256 per centre), then one
+    // assignment of all points to the centres
+    const uint32_t stride = std::max<uint32_t>(1u, n / std::min<uint32_t>(n, C * 256u));
+    const uint32_t ns = (n + stride - 1) / stride;
+    DevBuf<float> Xs;
+    const float *Xp = d_X;
+    if (stride > 1) {
+        Xs.resize((size_t)ns * d);
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid_for((uint64_t)ns * d, 256, 8192)), dim3(256), 0, s, d_X, d, stride,
+                           ns, Xs.get());
+        KERNEL_OK();
+        Xp = Xs.get();
+    }
     DevBuf<float> cen((size_t)C * d), cnorm(C);
-    DevBuf<uint32_t> keys(n), start(C + 1);
-    perm.resize(n);
-    hipLaunchKernelGGL(k_kmeans_init, dim3(grid_for((uint64_t)C * d, 256, 4096)), dim3(256), 0, s, d_X, n, d, C,
+    DevBuf<uint32_t> start(C + 1);
+    hipLaunchKernelGGL(k_kmeans_init, dim3(grid_for((uint64_t)C * d, 256, 4096)), dim3(256), 0, s, Xp, ns, d, C,
                        cen.get());
     KERNEL_OK();
     const int kb = std::max(1, bits_for(C));
@@ -1510,43 +1530,52 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
     // and every value below it are exact in f32 and the quantised distance never reaches the
     // group bits
     const uint32_t qbits = std::min<uint32_t>(20u, 32u - (uint32_t)kb);
-    for (int it = 0; it <= ITERS; it++) {
+    // nearest centre of every point of P (np points) -> keys, then the points sorted by centre
+    // (perm, stable) and each centre's first position (start)
+    auto assign_sort = [&](const float *P, uint32_t np, DevBuf<uint32_t> &keys, DevBuf<uint32_t> &perm) {
         hipLaunchKernelGGL(k_kmeans_norms, dim3(grid_for(C, 256, 1024)), dim3(256), 0, s, cen.get(), d, C, cnorm.get());
         KERNEL_OK();
-        const dim3 ag(grid_for(n, 256, 8192));
+        const dim3 ag(grid_for(np, 256, 8192));
         if (d <= 32)
-            hipLaunchKernelGGL(k_kmeans_assign_r<32>, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+            hipLaunchKernelGGL(k_kmeans_assign_r<32>, ag, dim3(256), 0, s, P, np, d, cen.get(), cnorm.get(), C, keys.get());
         else if (d <= 56)
-            hipLaunchKernelGGL(k_kmeans_assign_r<56>, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+            hipLaunchKernelGGL(k_kmeans_assign_r<56>, ag, dim3(256), 0, s, P, np, d, cen.get(), cnorm.get(), C, keys.get());
         else if (d <= 64)
-            hipLaunchKernelGGL(k_kmeans_assign_r<64>, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+            hipLaunchKernelGGL(k_kmeans_assign_r<64>, ag, dim3(256), 0, s, P, np, d, cen.get(), cnorm.get(), C, keys.get());
         else
-            hipLaunchKernelGGL(k_kmeans_assign, ag, dim3(256), 0, s, d_X, n, d, cen.get(), cnorm.get(), C, keys.get());
+            hipLaunchKernelGGL(k_kmeans_assign, ag, dim3(256), 0, s, P, np, d, cen.get(), cnorm.get(), C, keys.get());
         KERNEL_OK();
-        hipLaunchKernelGGL(k_knn_iota, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, perm.get(), n);
+        hipLaunchKernelGGL(k_knn_iota, dim3(grid_for(np, 256, 4096)), dim3(256), 0, s, perm.get(), np);
         KERNEL_OK();
-        radix_sort_pairs(keys, perm, n, kb, s);
-        hipLaunchKernelGGL(k_key_starts_u32, dim3(grid_for(C + 1, 256, 1024)), dim3(256), 0, s, keys.get(), n, C,
+        radix_sort_pairs(keys, perm, np, kb, s);
+        hipLaunchKernelGGL(k_key_starts_u32, dim3(grid_for(C + 1, 256, 1024)), dim3(256), 0, s, keys.get(), np, C,
                            start.get());
         KERNEL_OK();
-        if (it == ITERS) {
-            // within each group, points in increasing distance to the centre: a group's
-            // outliers share its last column blocks, so the spheres of its other blocks stay tight
-            DevBuf<float> gdist(n);
-            DevBuf<unsigned int> dmax(1);
-            HIP_OK(hipMemsetAsync(dmax.get(), 0, sizeof(unsigned int), s));
-            hipLaunchKernelGGL(k_group_dist, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, d, cen.get(), keys.get(),
-                               perm.get(), n, gdist.get(), dmax.get());
+    };
+    {
+        DevBuf<uint32_t> skeys(ns), sperm(ns);
+        for (int it = 0; it < ITERS; it++) {
+            assign_sort(Xp, ns, skeys, sperm);
+            hipLaunchKernelGGL(k_kmeans_update, dim3(std::min<uint32_t>(C, 8192)), dim3(256), 0, s, Xp, d, sperm.get(),
+                               start.get(), C, cen.get());
             KERNEL_OK();
-            hipLaunchKernelGGL(k_dist_keys, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, keys.get(), gdist.get(),
-                               dmax.get(), n, qbits);
-            KERNEL_OK();
-            radix_sort_pairs(keys, perm, n, kb + (int)qbits, s);
-            break;
         }
-        hipLaunchKernelGGL(k_kmeans_update, dim3(std::min<uint32_t>(C, 8192)), dim3(256), 0, s, d_X, d, perm.get(),
-                           start.get(), C, cen.get());
+    }
+    DevBuf<uint32_t> keys(n), perm(n);
+    assign_sort(d_X, n, keys, perm);
+    {
+        // within each group, points in increasing distance to the centre: a group's outliers
+        // share its last column blocks, so the spheres of its other blocks stay tight
+        DevBuf<float> gdist(n);
+        DevBuf<unsigned int> dmax(1);
+        HIP_OK(hipMemsetAsync(dmax.get(), 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_group_dist, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, d_X, d, cen.get(), keys.get(),
+                           perm.get(), n, gdist.get(), dmax.get());
         KERNEL_OK();
+        hipLaunchKernelGGL(k_dist_keys, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, keys.get(), gdist.get(),
+                           dmax.get(), n, qbits);
+        KERNEL_OK();
+        radix_sort_pairs(keys, perm, n, kb + (int)qbits, s);
     }
     DevBuf<uint32_t> psz(C + 1), poff(C + 1);
     hipLaunchKernelGGL(k_padded_sizes, dim3(grid_for(C + 1, 256, 1024)), dim3(256), 0, s, start.get(), C, psz.get());
