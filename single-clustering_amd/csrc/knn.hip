@@ -561,7 +561,7 @@ constexpr int RS_WAVES = 4;
 __global__ void __launch_bounds__(64 * RS_WAVES)
 k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, uint32_t splits,
               const uint32_t *cand_idx, const float *cand_s, const double *nx64, double eps_coef, double eps_const,
-              uint32_t *idx_out, double *d2_out, uint32_t *fallback, unsigned int *n_fallback,
+              uint32_t *idx_out, double *d2_out, int fallback_all, uint32_t *fallback, unsigned int *n_fallback,
               unsigned long long *n_rescored) {
     __shared__ float sS[RS_WAVES][KNN_MAXCAND];
     __shared__ uint32_t sI[RS_WAVES][KNN_MAXCAND];
@@ -656,7 +656,7 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
         // certified when no full split can hide a point at or below the k-th distance
         bool ok = true;
         if (any_full) ok = cnt >= k && (nx64[i] + (double)amin - eps > kth);
-        if (!ok && lane == 0) {
+        if ((!ok || fallback_all) && lane == 0) {
             const unsigned int slot = atomicAdd(n_fallback, 1u);
             fallback[slot] = i;
         }
@@ -665,21 +665,28 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
     if (lane == 0 && local) atomicAdd(n_rescored, local);
 }
 
-// Exact FP64 scan of one row per workgroup (128 threads), for rows the certificate rejected.
+// Exact FP64 scan for the rows the certificate rejected: workgroup (r, w) (128 threads) scans
+// the w-th of W equal ranges of points for fallback row r and writes its k nearest, sorted by
+// (d2, index), to part[(r W + w) k ..]; k_knn_fallback_merge then merges the W lists of a row.
+// Ranges spread one row over many workgroups: a single row scanned by one workgroup waited on
+// ~n/128 dependent distance computations per thread.
 constexpr int FB_THREADS = 128;
 constexpr int FB_MAXK = 32;
 
 __global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uint32_t n, uint32_t d, uint32_t k,
-                                                             const uint32_t *rows, uint32_t *idx_out, double *d2_out) {
+                                                             const uint32_t *rows, uint32_t *pidx, double *pd2) {
     __shared__ double sd[FB_THREADS * FB_MAXK];
     __shared__ uint32_t si[FB_THREADS * FB_MAXK];
     __shared__ uint32_t head[FB_THREADS];
     const uint32_t i = rows[blockIdx.x];
+    const uint32_t W = gridDim.y, w = blockIdx.y;
+    const uint32_t chunk = (n + W - 1) / W;
+    const uint32_t lo = min(n, w * chunk), hi = min(n, lo + chunk);
     const int t = threadIdx.x;
     double *md = sd + t * FB_MAXK;
     uint32_t *mi = si + t * FB_MAXK;
     uint32_t cnt = 0;
-    for (uint32_t j = t; j < n; j += FB_THREADS) {
+    for (uint32_t j = lo + t; j < hi; j += FB_THREADS) {
         if (j == i) continue;
         double dd = exact_d2(X, d, i, j);
         if (cnt == k && !(dd < md[k - 1])) continue;  // j increases: ties keep the lower index
@@ -700,6 +707,7 @@ __global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uin
     head[t] = 0;
     __syncthreads();
     if (t == 0) {
+        const uint64_t o = ((uint64_t)blockIdx.x * W + w) * k;
         for (uint32_t out = 0; out < k; out++) {
             int best = -1;
             double bdv = INFINITY;
@@ -717,9 +725,70 @@ __global__ void __launch_bounds__(FB_THREADS) k_knn_fallback(const float *X, uin
                 }
             }
             if (best >= 0) head[best]++;
-            idx_out[(uint64_t)i * k + out] = bidx;
-            d2_out[(uint64_t)i * k + out] = bdv;
+            pidx[o + out] = bidx;
+            pd2[o + out] = bdv;
         }
+    }
+}
+
+// The k nearest of fallback row r from its W partial lists (each sorted by (d2, index), KNONE
+// past its end): a W-way merge by (d2, index), one wave per row. Lane l holds the heads of lists
+// l, l + 64, l + 128 and l + 192 in registers; per output the wave reduces the smallest head and
+// only the list it came from loads its next entry (the points of different lists differ, so
+// the winning head is unique).
+constexpr uint32_t FB_MAXW = 256;
+__device__ __forceinline__ bool fb_less(double a, uint32_t ai, double b, uint32_t bi) {
+    return a < b || (a == b && ai < bi);
+}
+__global__ void __launch_bounds__(64) k_knn_fallback_merge(uint32_t k, uint32_t W, const uint32_t *rows,
+                                                           const uint32_t *pidx, const double *pd2, uint32_t *idx_out,
+                                                           double *d2_out) {
+    constexpr int HQ = FB_MAXW / 64;
+    const uint32_t i = rows[blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * W * k;
+    const uint32_t lane = threadIdx.x;
+    double hv[HQ];
+    uint32_t hi[HQ], hp[HQ];
+    auto load = [&](int q) {
+        const uint32_t w = lane + 64u * (uint32_t)q;
+        const uint32_t ix = (w < W && hp[q] < k) ? pidx[base + (uint64_t)w * k + hp[q]] : KNONE;
+        hi[q] = ix;
+        hv[q] = ix != KNONE ? pd2[base + (uint64_t)w * k + hp[q]] : INFINITY;
+    };
+#pragma unroll
+    for (int q = 0; q < HQ; q++) {
+        hp[q] = 0;
+        load(q);
+    }
+    for (uint32_t out = 0; out < k; out++) {
+        double bv = hv[0];
+        uint32_t bi = hi[0];
+#pragma unroll
+        for (int q = 1; q < HQ; q++)
+            if (fb_less(hv[q], hi[q], bv, bi)) {
+                bv = hv[q];
+                bi = hi[q];
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(bv, o);
+            const uint32_t oi = __shfl_xor(bi, o);
+            if (fb_less(ov, oi, bv, bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            idx_out[(uint64_t)i * k + out] = bi;
+            d2_out[(uint64_t)i * k + out] = bi != KNONE ? bv : INFINITY;
+        }
+        if (bi == KNONE) continue;  // every list is exhausted (wave-uniform)
+#pragma unroll
+        for (int q = 0; q < HQ; q++)
+            if (hi[q] == bi && hv[q] == bv) {
+                hp[q]++;
+                load(q);
+            }
     }
 }
 
@@ -1893,12 +1962,16 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     DevBuf<unsigned long long> nres(1);
     HIP_OK(hipMemsetAsync(nfb.get(), 0, sizeof(unsigned int), s));
     HIP_OK(hipMemsetAsync(nres.get(), 0, sizeof(unsigned long long), s));
+    // SCGPU_KNN_FALLBACK_ALL=1 sends every row through the exact FP64 scan as well (a test of
+    // that path: its rows then replace the certified ones)
+    const char *fa = std::getenv("SCGPU_KNN_FALLBACK_ALL");
+    const int fallback_all = fa && fa[0] == '1' ? 1 : 0;
     // a few long-lived workgroups per CU walk the rows: launching one short workgroup per few rows
     // left most of the machine idle waiting for workgroup creation
     hipLaunchKernelGGL(k_knn_rescore, dim3(std::min<uint32_t>((n + RS_WAVES - 1) / RS_WAVES, 2048)),
                        dim3(64 * RS_WAVES), 0, s, d_X, n, d, k, K2, splits,
-                       cidx.get(), cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fb.get(), nfb.get(),
-                       nres.get());
+                       cidx.get(), cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fallback_all, fb.get(),
+                       nfb.get(), nres.get());
     KERNEL_OK();
     unsigned int nf = 0;
     unsigned long long nr = 0;
@@ -1906,7 +1979,15 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     to_host(&nr, nres.get(), 1, s);
     sync_stream(s);
     if (nf) {
-        hipLaunchKernelGGL(k_knn_fallback, dim3(nf), dim3(FB_THREADS), 0, s, d_X, n, d, k, fb.get(), d_idx, d_d2);
+        // about 2048 workgroups in all: up to 256 ranges per row, one range per row above 2048 rows
+        const uint32_t W = std::max<uint32_t>(1u, std::min<uint32_t>(FB_MAXW, 2048u / nf));
+        DevBuf<uint32_t> pidx((size_t)nf * W * k);
+        DevBuf<double> pd2((size_t)nf * W * k);
+        hipLaunchKernelGGL(k_knn_fallback, dim3(nf, W), dim3(FB_THREADS), 0, s, d_X, n, d, k, fb.get(), pidx.get(),
+                           pd2.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_knn_fallback_merge, dim3(nf), dim3(64), 0, s, k, W, fb.get(), pidx.get(), pd2.get(),
+                           d_idx, d_d2);
         KERNEL_OK();
     }
     HIP_OK(hipEventRecord(e3, s));

@@ -118,3 +118,17 @@ def test_gaussian_mode1_by_hand_on_the_gpu():
     for (i, j), wt in got.items():
         ref = np.float32(_gauss(LINE_SIGMA_MEDIAN[i], LINE_SIGMA_MEDIAN[j], (x[i] - x[j]) ** 2))
         assert abs(wt - ref) <= 1e-6 * ref, (i, j, wt, ref)
+
+
+@pytest.mark.parametrize("n,d,k", [(3000, 50, 15), (2500, 30, 7), (1200, 100, 20)])
+def test_knn_fp64_fallback_scan_vs_oracle(monkeypatch, n, d, k):
+    # every row through the exact FP64 scan that serves rows the certificate rejects: its index
+    # sets and distances must equal the brute force bit for bit, ties to the lower index
+    x, _ = W.gaussian_mixture(n, d, 8, 9.0, seed=3 * n + d)
+    x[5] = x[17]  # an exact duplicate: a tie at distance 0 for two rows
+    monkeypatch.setenv("SCGPU_KNN_FALLBACK_ALL", "1")
+    idx, d2, st = scgpu.knn(x, k, return_stats=True)
+    assert st["rows_fallback"] == n
+    oi, od = O.knn_bruteforce(x, k)
+    assert np.array_equal(idx, oi)
+    assert np.array_equal(d2.view(np.uint64), od.view(np.uint64))
