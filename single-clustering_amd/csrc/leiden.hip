@@ -384,7 +384,8 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
 #pragma unroll
             for (int j = 0; j < SB; j++) {
                 const uint32_t ci = (cs[j] != EMPTY_KEY && cs[j] != own) ? cs[j] : own;
-                Wv[j] = a.Wfx[ci];
+                // Rb reads community strengths only, CPM node weights only; the cap reads weights
+                Wv[j] = (a.objective != 0 || a.has_max) ? a.Wfx[ci] : 0;
                 Kv[j] = a.objective == 0 ? a.Kfx[ci] : 0;
             }
 #pragma unroll
@@ -444,7 +445,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
             long long kfx = a.objective == 0 ? to_fx(k, vp.sK) : 0;
             long long wfxv = to_fx(na, vp.sW);
             double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, vp.sK) : 0.0;
-            double Wown = from_fx(a.Wfx[own] - wfxv, vp.sW);
+            double Wown = (a.objective != 0 || a.has_max) ? from_fx(a.Wfx[own] - wfxv, vp.sW) : 0.0;
             double base_q = delta_insert(a.objective, vp, from_fx(wown, vp.sw), loop, k, na, Kown, Wown);
             // the empty community (local_move.rs:93-100): this vertex's own id when no one holds
             // it, else its private id n + v (community ids span [0, 2n), so a vertex whose own
@@ -1118,14 +1119,21 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
 #undef SC_DECIDE_BUCKET
 }
 
+// A mover's totals: community strengths for Rb, node weights for CPM or under a weight cap (the
+// only readers of either), sizes always (the empty-community and singleton rules).
 template <bool BATCH>
 __device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint64_t from, uint64_t to) {
     const VParams vp = vparams<BATCH>(A.d, (uint32_t)v);
-    const long long kf = to_fx(A.d.strength[v], vp.sK), wf = to_fx(A.d.node_w[v], vp.sW);
-    atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
-    atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)(-kf));
-    atomicAdd((unsigned long long *)&A.Wfx[to], (unsigned long long)wf);
-    atomicAdd((unsigned long long *)&A.Wfx[from], (unsigned long long)(-wf));
+    if (A.d.objective == 0) {
+        const long long kf = to_fx(A.d.strength[v], vp.sK);
+        atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
+        atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)(-kf));
+    }
+    if (A.d.objective != 0 || A.d.has_max) {
+        const long long wf = to_fx(A.d.node_w[v], vp.sW);
+        atomicAdd((unsigned long long *)&A.Wfx[to], (unsigned long long)wf);
+        atomicAdd((unsigned long long *)&A.Wfx[from], (unsigned long long)(-wf));
+    }
     atomicAdd(&A.csize[to], 1u);
     atomicSub(&A.csize[from], 1u);
 }
@@ -1407,10 +1415,9 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 for (uint32_t i = gslot; i < nm; i += NGA) {
                     const uint64_t v = mv_list[i];
                     const uint32_t t = a.target[v];
-                    const bool moved_v = capped ? A.claim[t] == (uint32_t)v
-                                                : (!a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v,
-                                                                            t, step & 1u));
-                    if (!moved_v) continue;
+                    // a mover that landed is in its target now; one that was not admitted (or lost
+                    // its claim) is still in its old community, which differs from its target
+                    if (a.memb[v] != t) continue;
                     for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
                         const uint32_t u = a.nbrs[e];
                         if (u == v) continue;
