@@ -1362,38 +1362,41 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 }
                 if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
             }
-            // apply this workgroup's movers; refinement's re-activation is fused with it (it reads
-            // the working communities, which do not change here); local moving re-activates after
-            // all moves landed (below), like the multi-launch path
-            unsigned int applied = 0;
-            for (uint32_t i = gslot; i < nm; i += NGA) {
-                const uint64_t v = mv_list[i];
-                const uint64_t t = a.target[v];
-                bool go;
+            // apply this workgroup's movers, one lane each; refinement then re-activates in the
+            // same phase (it reads the working communities, which do not change here), local
+            // moving after all moves landed (below), like the multi-launch path
+            auto admitted = [&](uint64_t v, uint64_t t) -> bool {
                 if (REFINE) {
                     const uint32_t tc = a.targeted_cnt[v];
                     const uint32_t tr = a.tround[t];
                     const uint64_t tt = a.target[t];
                     const uint32_t cl = capped ? A.claim[t] : (uint32_t)v;
-                    go = ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == it) & (tt == v))) & (cl == (uint32_t)v);
-                } else if (capped) {
-                    go = A.claim[t] == (uint32_t)v;
+                    return ((tc == 0u) | ((tc == 1u) & (t < v) & (tr == it) & (tt == v))) & (cl == (uint32_t)v);
+                }
+                if (capped) return A.claim[t] == (uint32_t)v;
+                return !a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v, (uint32_t)t, step & 1u);
+            };
+            unsigned int applied = 0;
+            for (uint32_t i = threadIdx.x; i < nm; i += SWEEP_BS) {
+                const uint64_t v = mv_list[i];
+                const uint64_t t = a.target[v];
+                if (!admitted(v, t)) {
+                    a.active[v] = 1;  // blocked: look again next step
                 } else {
-                    go = !a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v, (uint32_t)t, step & 1u);
+                    const uint64_t from = REFINE ? v : a.memb[v];
+                    A.memb[v] = (uint32_t)t;
+                    move_totals<BATCH>(A, v, from, t);
+                    applied++;
+                    if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                 }
-                if (lane == 0) {
-                    if (!go) {
-                        a.active[v] = 1;  // blocked: look again next step
-                    } else {
-                        const uint64_t from = REFINE ? v : a.memb[v];
-                        A.memb[v] = (uint32_t)t;
-                        move_totals<BATCH>(A, v, from, t);
-                        applied++;
-                        if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
-                    }
-                }
-                if (REFINE && go) {
-                    // refinement: neighbours in the same working community (refine.rs)
+            }
+            if (REFINE) {
+                // refinement: neighbours in the same working community (refine.rs), SWEEP_ACT_G
+                // lanes per mover; the admission test reads only what apply leaves unchanged
+                for (uint32_t i = gslot; i < nm; i += NGA) {
+                    const uint64_t v = mv_list[i];
+                    const uint64_t t = a.target[v];
+                    if (!admitted(v, t)) continue;
                     const uint32_t mv = a.constraint[v];
                     for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
                         const uint32_t u = a.nbrs[e];
