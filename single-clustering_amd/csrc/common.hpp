@@ -297,6 +297,22 @@ __host__ __device__ __forceinline__ int bucket_of(uint64_t d) {
     if (d <= 2048) return 6;
     return GTAB_BUCKET;
 }
+
+// Upper degree bounds of the LDS degree classes 0..6 (class 7 takes the rest). Each bound is at
+// most the class's capacity (the defaults, which bucket_of uses): a vertex may go to a class
+// with more lanes and a larger table than its degree needs, never to a smaller one, so any
+// bounds give identical decisions.
+struct BucketBounds {
+    uint32_t hi[NUM_BUCKETS - 1];
+};
+constexpr BucketBounds DEFAULT_BUCKET_BOUNDS{{8, 16, 32, 128, 512, 1024, 2048}};
+
+__host__ __device__ __forceinline__ int bucket_of(uint64_t d, const BucketBounds &b) {
+#pragma unroll
+    for (int k = 0; k < NUM_BUCKETS - 1; k++)
+        if (d <= b.hi[k]) return k;
+    return GTAB_BUCKET;
+}
 }  // namespace scgpu
 
 struct sc_graph {

@@ -746,12 +746,12 @@ struct IterGraph {
 };
 
 // ---- degree buckets (bucket_of: common.hpp) ---------------------------------------------
-__global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, uint32_t *counts) {
+__global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, BucketBounds bb, uint32_t *counts) {
     __shared__ uint32_t sc[NUM_BUCKETS];
     if (threadIdx.x < NUM_BUCKETS) sc[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-        atomicAdd(&sc[bucket_of(ptrs[v + 1] - ptrs[v])], 1u);
+        atomicAdd(&sc[bucket_of(ptrs[v + 1] - ptrs[v], bb)], 1u);
     __syncthreads();
     if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
 }
@@ -761,7 +761,7 @@ struct BucketStarts {
 };
 
 // Block-local counting, one global reservation per bucket per block, then placement.
-__global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, BucketStarts starts, uint32_t *cursor,
+__global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, BucketBounds bb, BucketStarts starts, uint32_t *cursor,
                               uint32_t *lists) {
     __shared__ uint32_t cnt[NUM_BUCKETS], base[NUM_BUCKETS];
     for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gridDim.x * blockDim.x) {
@@ -771,7 +771,7 @@ __global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, BucketStarts sta
         int b = 0;
         uint32_t local = 0;
         if (v < n) {
-            b = bucket_of(ptrs[v + 1] - ptrs[v]);
+            b = bucket_of(ptrs[v + 1] - ptrs[v], bb);
             local = atomicAdd(&cnt[b], 1u);
         }
         __syncthreads();
@@ -802,16 +802,24 @@ struct Buckets {
     DevBuf<long long> gvals;
 };
 
-static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
+static bool default_bounds(const BucketBounds &bb) {
+    for (int k = 0; k < NUM_BUCKETS - 1; k++)
+        if (bb.hi[k] != DEFAULT_BUCKET_BOUNDS.hi[k]) return false;
+    return true;
+}
+
+static void build_buckets(const sc_graph &L, Buckets &b, const BucketBounds &bb, hipStream_t s) {
     uint32_t n = (uint32_t)L.n;
+    for (int k = 0; k < NUM_BUCKETS - 1; k++)
+        if (bb.hi[k] > DEFAULT_BUCKET_BOUNDS.hi[k]) throw ScError(SC_INVALID_ARGUMENT, "degree class bound above its capacity");
     DevBuf<uint32_t> cnt(2 * NUM_BUCKETS + 2);
     HIP_OK(hipMemsetAsync(cnt.get(), 0, (2 * NUM_BUCKETS + 2) * sizeof(uint32_t), s));
     uint32_t h[NUM_BUCKETS];
-    if (L.has_bucket_counts) {
+    if (L.has_bucket_counts && default_bounds(bb)) {
         // counted while the graph was finished (read back with its other aggregates)
         for (int i = 0; i < NUM_BUCKETS; i++) h[i] = L.bucket_counts[i];
     } else {
-        hipLaunchKernelGGL(k_bucket_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, cnt.get());
+        hipLaunchKernelGGL(k_bucket_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, bb, cnt.get());
         KERNEL_OK();
         to_host(h, cnt.get(), NUM_BUCKETS, s);
         sync_stream(s);
@@ -821,7 +829,7 @@ static void build_buckets(const sc_graph &L, Buckets &b, hipStream_t s) {
     BucketStarts starts;
     for (int i = 0; i <= NUM_BUCKETS; i++) starts.s[i] = b.start[i];
     b.lists.resize(n);
-    hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, starts,
+    hipLaunchKernelGGL(k_bucket_fill, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, L.ptrs.get(), n, bb, starts,
                        cnt.get() + NUM_BUCKETS, b.lists.get());
     KERNEL_OK();
     uint32_t nb6 = h[GTAB_BUCKET];
@@ -3106,8 +3114,22 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
             ctx.vjob = vjob_l.get();
             ctx.jobs = B->params.get();
         }
+        // degree-class bounds: the defaults, then a few narrower sets the online tuner measures
+        // over this level's local moving and refinement (identical decisions under any of them)
+        static constexpr BucketBounds BOUNDS[] = {DEFAULT_BUCKET_BOUNDS,
+                                                  {{6, 12, 24, 96, 384, 768, 1536}},
+                                                  {{8, 16, 32, 64, 512, 1024, 2048}},
+                                                  {{4, 12, 32, 128, 512, 1024, 2048}}};
+        const std::string bkey = std::string(B ? "buckets-batch" : "buckets") + "/n2^" + std::to_string(bits_for(n)) +
+                                 "/e2^" + std::to_string(bits_for(L.nnz)) + (cfg.refine ? "" : "/louvain");
+        const TuneChoice btc = tune_enabled() && n >= 4096 ? tune_pick(bkey, 4, 1) : TuneChoice{0, false};
         Buckets b;
-        build_buckets(L, b, s);
+        build_buckets(L, b, BOUNDS[btc.index], s);
+        TuneSample bts{};
+        if (btc.exploring) {
+            bts = TuneSample{bkey, btc.index, ctx.events.next(), ctx.events.next()};
+            HIP_OK(hipEventRecord(bts.t0, s));
+        }
         if (sweep_trace_on())
             std::fprintf(stderr, "[level %llu] n=%u nnz=%llu max_degree=%llu\n", (unsigned long long)lvl, n,
                          (unsigned long long)L.nnz, (unsigned long long)L.max_degree);
@@ -3120,10 +3142,13 @@ static uint32_t run_once(const sc_graph &G0, uint32_t *labels, uint64_t pass_see
         if (cfg.refine) {
             refine(L, memb.get(), rmemb.get(), ls, b, ctx, level_seed, sK, sW, sw);
             pc.lap(2);
-            ng = renumber(rmemb.get(), n, n, groups.get(), s);
-        } else {
-            ng = renumber(memb.get(), n, 2 * n, groups.get(), s);
         }
+        if (btc.exploring) {
+            HIP_OK(hipEventRecord(bts.t1, s));
+            ctx.tune_samples.push_back(bts);
+        }
+        if (cfg.refine) ng = renumber(rmemb.get(), n, n, groups.get(), s);
+        else ng = renumber(memb.get(), n, 2 * n, groups.get(), s);
         pc.lap(3);
         std::vector<uint32_t> jng;
         if (B) {

@@ -21,7 +21,12 @@ def test_autotuner_explores_without_changing_results():
         assert np.array_equal(scgpu.leiden(g).labels(), first.labels())
     rep = scgpu.autotune_report()
     print(rep)
-    assert "sweep/move" in rep and "sweep/refine" in rep and "knn_tile" in rep
+    assert "sweep/move" in rep and "sweep/refine" in rep and "knn_tile" in rep and "buckets/" in rep
+    # the four degree-class bound sets of the first level have each been measured once, with
+    # identical labels throughout (asserted above)
+    bline = [l for l in rep.splitlines() if l.startswith("buckets/")][0]
+    btimes = [float(t) for t in bline.split(":")[1].split("->")[0].split()]
+    assert len(btimes) == 4 and all(t >= 0 for t in btimes)
     # every candidate of the kNN key has been measured by now
     line = [l for l in rep.splitlines() if l.startswith("knn_tile")][0]
     times = [float(t) for t in line.split(":")[1].split("->")[0].split()]
