@@ -481,9 +481,11 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
                 if (sp.mstep) sp.mstep[v] = sp.step;
                 if (a.jsum) {
                     a.mgain[v] = bs - base_q;
-                    const uint64_t idx = (uint64_t)(sp.step & 1u) * a.jstride + bc;
-                    atomicAdd((unsigned long long *)&a.jsum[idx], (unsigned long long)join_amount(a, vp, k, na));
-                    atomicMin(&a.jmin[idx], v);
+                    if (!sp.mv_tgt) {  // persistent sweeps sum a workgroup's joiners first
+                        const uint64_t idx = (uint64_t)(sp.step & 1u) * a.jstride + bc;
+                        atomicAdd((unsigned long long *)&a.jsum[idx], (unsigned long long)join_amount(a, vp, k, na));
+                        atomicMin(&a.jmin[idx], v);
+                    }
                 }
                 push = true;
             }
@@ -1020,55 +1022,86 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
         cum[k + 1] = cum[k] + (hi - lo[k]);
     }
     const uint32_t total = cum[NUM_BUCKETS];
-    for (uint32_t j0 = 0; j0 < total; j0 += SWEEP_BS) {  // workgroup-uniform bounds
-        const uint32_t j = j0 + threadIdx.x;
-        // bucket of range position j: the last k with cum[k] <= j (empty buckets are passed over)
-        int kb = 0;
-        uint32_t i = 0;
+    // U positions per thread and pass: their list entries, then their vertices' flags (and, in
+    // refinement, memberships and sizes) are loaded with independent loads before any is used
+    constexpr int U = 4;
+    for (uint32_t j0 = 0; j0 < total; j0 += SWEEP_BS * U) {  // workgroup-uniform bounds
+        int kb[U];
+        uint32_t ip[U], v[U];
+        bool in[U];
 #pragma unroll
-        for (int k = 0; k < NUM_BUCKETS; k++)
-            if (j >= cum[k]) {
-                kb = k;
-                i = lo[k] + (j - cum[k]);
-            }
-        bool take = false;
-        uint32_t v = 0;
-        float x[MLP_NF] = {};
-        if (j < total) {
-            v = A.lists[i];
-            const VParams vp = vparams<BATCH>(a, v);
-            take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == sub;
-            if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
-            if (FILTER && filter_on && take) vertex_features(a, vp, A.mstep, v, it, x);
+        for (int q = 0; q < U; q++) {
+            const uint32_t j = j0 + (uint32_t)q * SWEEP_BS + threadIdx.x;
+            // bucket of range position j: the last k with cum[k] <= j (empty buckets are passed over)
+            kb[q] = 0;
+            ip[q] = 0;
+#pragma unroll
+            for (int k = 0; k < NUM_BUCKETS; k++)
+                if (j >= cum[k]) {
+                    kb[q] = k;
+                    ip[q] = lo[k] + (j - cum[k]);
+                }
+            in[q] = j < total;
+            v[q] = in[q] ? A.lists[ip[q]] : 0u;
         }
-        // learned filter: a vertex predicted to stay is not decided now; every lane of the
-        // wave reaches the wave-collective inference
-        if (FILTER && filter_on && __any(take)) {
-            const float z = mlp_forward_wave(A.filter_pk, x);
-            if (take && z < A.filter_thr) {
-                // it leaves the active set like a vertex that decided to stay (a
-                // neighbour's move re-activates it), marked as skipped
-                take = false;
-                skipped++;
-                a.active[v] = A.filter_judge == 2 ? 2 : 0;
-                A.f_mark[v] = 1;
-            } else if (take) {
-                A.f_mark[v] = 0;
+        uint8_t f[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) f[q] = in[q] ? a.active[v[q]] : (uint8_t)0;
+        bool take[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            take[q] = false;
+            if (f[q] == 1) {
+                const VParams vp = vparams<BATCH>(a, v[q]);
+                take[q] = !vp.frozen && subset_of(seed, v[q] - vp.lo, a.P) == sub;
             }
         }
-        // one wave-aggregated append per bucket the wave's positions touch
+        if (REFINE) {
+            uint32_t m[U], cs[U];
 #pragma unroll
-        for (int k = 0; k < NUM_BUCKETS; k++) {
-            const uint64_t m = __ballot(take && kb == k);
-            if (!m) continue;
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            unsigned int basei = 0;
-            if (lane == leader) basei = atomicAdd(&s_wc[k], (unsigned int)__popcll(m));
-            basei = __shfl(basei, leader);
-            if (take && kb == k) {
-                const uint32_t below =
-                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                A.work[lo[k] + basei + below] = i - A.bstart[k];
+            for (int q = 0; q < U; q++) {
+                m[q] = take[q] ? a.memb[v[q]] : 0u;
+                cs[q] = take[q] ? a.csize[v[q]] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) take[q] = take[q] && m[q] == v[q] && cs[q] == 1u;
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            bool tq = take[q];
+            if (FILTER && filter_on) {
+                // learned filter: a vertex predicted to stay is not decided now; every lane of
+                // the wave reaches the wave-collective inference
+                float x[MLP_NF] = {};
+                if (tq) vertex_features(a, vparams<BATCH>(a, v[q]), A.mstep, v[q], it, x);
+                if (__any(tq)) {
+                    const float z = mlp_forward_wave(A.filter_pk, x);
+                    if (tq && z < A.filter_thr) {
+                        // it leaves the active set like a vertex that decided to stay (a
+                        // neighbour's move re-activates it), marked as skipped
+                        tq = false;
+                        skipped++;
+                        a.active[v[q]] = A.filter_judge == 2 ? 2 : 0;
+                        A.f_mark[v[q]] = 1;
+                    } else if (tq) {
+                        A.f_mark[v[q]] = 0;
+                    }
+                }
+            }
+            // one wave-aggregated append per bucket the wave's positions touch
+#pragma unroll
+            for (int k = 0; k < NUM_BUCKETS; k++) {
+                const uint64_t mk = __ballot(tq && kb[q] == k);
+                if (!mk) continue;
+                const int leader = __ffsll((unsigned long long)mk) - 1;
+                unsigned int basei = 0;
+                if (lane == leader) basei = atomicAdd(&s_wc[k], (unsigned int)__popcll(mk));
+                basei = __shfl(basei, leader);
+                if (tq && kb[q] == k) {
+                    const uint32_t below =
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+                    A.work[lo[k] + basei + below] = ip[q] - A.bstart[k];
+                }
             }
         }
     }
@@ -1127,23 +1160,64 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
 #undef SC_DECIDE_BUCKET
 }
 
-// A mover's totals: community strengths for Rb, node weights for CPM or under a weight cap (the
-// only readers of either), sizes always (the empty-community and singleton rules).
+// A workgroup's mover totals, summed per community in LDS before they go to global memory:
+// late steps send many movers into the same few large communities, and same-address device
+// atomics serialize. AGG_SLOTS keys with K, W and size sums (integers, so the totals do not
+// depend on the order); a community that finds no slot within a few probes is updated directly.
+constexpr uint32_t AGG_SLOTS = 1024;
+
+struct AggTable {
+    uint32_t *key;
+    long long *K, *W, *N;
+};
+
+__device__ __forceinline__ void agg_clear(const AggTable &t, long long n_init = 0) {
+    for (uint32_t i = threadIdx.x; i < AGG_SLOTS; i += blockDim.x) {
+        t.key[i] = EMPTY_KEY;
+        t.K[i] = 0;
+        t.W[i] = 0;
+        t.N[i] = n_init;
+    }
+}
+
+__device__ __forceinline__ void agg_add(const SweepArgs &A, const AggTable &t, uint32_t c, long long kf, long long wf,
+                                        int dn) {
+    uint32_t h = hash32(c) & (AGG_SLOTS - 1);
+    for (int p = 0; p < 8; p++) {
+        const uint32_t prev = atomicCAS(&t.key[h], EMPTY_KEY, c);
+        if (prev == EMPTY_KEY || prev == c) {
+            if (kf) atomicAdd((unsigned long long *)&t.K[h], (unsigned long long)kf);
+            if (wf) atomicAdd((unsigned long long *)&t.W[h], (unsigned long long)wf);
+            atomicAdd((unsigned long long *)&t.N[h], (unsigned long long)(long long)dn);
+            return;
+        }
+        h = (h + 1) & (AGG_SLOTS - 1);
+    }
+    if (kf) atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)kf);
+    if (wf) atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)wf);
+    atomicAdd(&A.csize[c], (unsigned int)dn);
+}
+
+// v moves from community `from` to `to`: its strength for Rb, node weight for CPM or under a
+// weight cap (the only readers of either), size always
 template <bool BATCH>
-__device__ __forceinline__ void move_totals(const SweepArgs &A, uint64_t v, uint64_t from, uint64_t to) {
+__device__ __forceinline__ void agg_move(const SweepArgs &A, const AggTable &t, uint64_t v, uint32_t from,
+                                         uint32_t to) {
     const VParams vp = vparams<BATCH>(A.d, (uint32_t)v);
-    if (A.d.objective == 0) {
-        const long long kf = to_fx(A.d.strength[v], vp.sK);
-        atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
-        atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)(-kf));
+    const long long kf = A.d.objective == 0 ? to_fx(A.d.strength[v], vp.sK) : 0;
+    const long long wf = (A.d.objective != 0 || A.d.has_max) ? to_fx(A.d.node_w[v], vp.sW) : 0;
+    agg_add(A, t, to, kf, wf, 1);
+    agg_add(A, t, from, -kf, -wf, -1);
+}
+
+__device__ __forceinline__ void agg_flush(const SweepArgs &A, const AggTable &t) {
+    for (uint32_t i = threadIdx.x; i < AGG_SLOTS; i += blockDim.x) {
+        const uint32_t c = t.key[i];
+        if (c == EMPTY_KEY) continue;
+        if (t.K[i]) atomicAdd((unsigned long long *)&A.Kfx[c], (unsigned long long)t.K[i]);
+        if (t.W[i]) atomicAdd((unsigned long long *)&A.Wfx[c], (unsigned long long)t.W[i]);
+        if (t.N[i]) atomicAdd(&A.csize[c], (unsigned int)t.N[i]);
     }
-    if (A.d.objective != 0 || A.d.has_max) {
-        const long long wf = to_fx(A.d.node_w[v], vp.sW);
-        atomicAdd((unsigned long long *)&A.Wfx[to], (unsigned long long)wf);
-        atomicAdd((unsigned long long *)&A.Wfx[from], (unsigned long long)(-wf));
-    }
-    atomicAdd(&A.csize[to], 1u);
-    atomicSub(&A.csize[from], 1u);
 }
 
 __global__ void k_stamp_clock(long long *out) { *out = wall_clock64(); }
@@ -1245,6 +1319,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         // vertex whose community finds no slot within a few probes is added directly
         constexpr uint32_t PT = 1024;
         static_assert(3 * PT <= SWEEP_TAB, "prologue table does not fit the sweep's LDS");
+        static_assert(3 * AGG_SLOTS <= SWEEP_TAB, "mover table does not fit the sweep's LDS");
         uint32_t *tk = s_keys;
         long long *tK = s_vals, *tW = s_vals + PT, *tN = s_vals + 2 * PT;
         for (uint32_t i = threadIdx.x; i < PT; i += SWEEP_BS) {
@@ -1338,6 +1413,41 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             sweep_decide<REFINE, BATCH>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
             __syncthreads();
             const unsigned int nm = s_mc;
+            if (!REFINE && mv_tgt) {
+                // joiner admission sums: the workgroup's joiners of each target are summed (and
+                // their lowest id taken) in LDS, then added with one atomic per target and array
+                const AggTable agg{s_keys, s_vals, s_vals + AGG_SLOTS, s_vals + 2 * AGG_SLOTS};
+                agg_clear(agg, (long long)NONE);  // N: the lowest joiner id
+                __syncthreads();
+                const uint64_t off = (uint64_t)(step & 1u) * a.jstride;
+                for (uint32_t i = threadIdx.x; i < nm; i += SWEEP_BS) {
+                    const uint32_t v = mv_list[i], t = mv_tgt[i];
+                    const VParams vp = vparams<BATCH>(a, v);
+                    const long long amt = join_amount(a, vp, a.strength[v], a.node_w[v]);
+                    uint32_t h = hash32(t) & (AGG_SLOTS - 1);
+                    bool done = false;
+                    for (int p = 0; p < 8 && !done; p++) {
+                        const uint32_t prev = atomicCAS(&agg.key[h], EMPTY_KEY, t);
+                        if (prev == EMPTY_KEY || prev == t) {
+                            atomicAdd((unsigned long long *)&agg.K[h], (unsigned long long)amt);
+                            atomicMin((unsigned long long *)&agg.N[h], (unsigned long long)v);
+                            done = true;
+                        }
+                        h = (h + 1) & (AGG_SLOTS - 1);
+                    }
+                    if (!done) {
+                        atomicAdd((unsigned long long *)&a.jsum[off + t], (unsigned long long)amt);
+                        atomicMin(&a.jmin[off + t], v);
+                    }
+                }
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < AGG_SLOTS; i += SWEEP_BS) {
+                    const uint32_t t = agg.key[i];
+                    if (t == EMPTY_KEY) continue;
+                    atomicAdd((unsigned long long *)&a.jsum[off + t], (unsigned long long)agg.K[i]);
+                    atomicMin(&a.jmin[off + t], (uint32_t)agg.N[i]);
+                }
+            }
             if (threadIdx.x == 0) {
                 if (A.trace && step < A.max_steps) {
                     // traced runs: deciders and movers of every step, and the largest and summed
@@ -1385,6 +1495,9 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 return !a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v, (uint32_t)t, step & 1u);
             };
             unsigned int applied = 0;
+            const AggTable agg{s_keys, s_vals, s_vals + AGG_SLOTS, s_vals + 2 * AGG_SLOTS};
+            agg_clear(agg);  // the decide tables are free: every wave has left the decide phase
+            __syncthreads();
             for (uint32_t i = threadIdx.x; i < nm; i += SWEEP_BS) {
                 const uint64_t v = mv_list[i];
                 const uint64_t t = a.target[v];
@@ -1393,11 +1506,13 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 } else {
                     const uint64_t from = REFINE ? v : a.memb[v];
                     A.memb[v] = (uint32_t)t;
-                    move_totals<BATCH>(A, v, from, t);
+                    agg_move<BATCH>(A, agg, v, (uint32_t)from, (uint32_t)t);
                     applied++;
                     if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                 }
             }
+            __syncthreads();
+            agg_flush(A, agg);
             if (REFINE) {
                 // refinement: neighbours in the same working community (refine.rs), SWEEP_ACT_G
                 // lanes per mover; the admission test reads only what apply leaves unchanged
