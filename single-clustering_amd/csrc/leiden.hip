@@ -283,9 +283,10 @@ struct DecideStats {
 };
 
 // One unit of a decide pass: NG = BS / G vertices of one degree bucket, G lanes each. blist is
-// the bucket's vertex list; sel, when given, holds the positions (within blist) of this step's
-// vertices, which are then already known to be active and in the step's subset. base is the
-// index (into sel, or blist) of the unit's first vertex. Every thread of the block calls this.
+// the bucket's vertex list; sel, when given, holds this step's vertices (their positions within
+// blist for the global-table bucket), which are then already known to be active and in the
+// step's subset. base is the index (into sel, or blist) of the unit's first vertex. Every thread
+// of the block calls this.
 template <int BS, int G, int T, bool REFINE, bool GTAB, bool BATCH = true>
 __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParams &sp, const uint32_t *blist,
                                             const uint32_t *sel, uint32_t len, uint64_t base, uint32_t *s_keys,
@@ -295,8 +296,9 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
     const int grp = threadIdx.x / G;
     const uint64_t idx = base + grp;
     bool valid = idx < len;
+    // sel holds vertices, or list positions for the global-table bucket (it needs goff[pos])
     uint32_t pos = valid ? (sel ? sel[idx] : (uint32_t)idx) : 0u;
-    uint32_t v = valid ? blist[pos] : 0u;
+    uint32_t v = valid ? ((sel && !GTAB) ? pos : blist[pos]) : 0u;
     const VParams vp = vparams<BATCH>(a, v);
     if (valid && !sel) {
         valid = a.active[v] != 0 && !vp.frozen && subset_of(sp.seed, v - vp.lo, a.P) == sp.sub;
@@ -367,23 +369,58 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         loop = a.self_loop[v];
         // refinement: the node leaves its singleton, which becomes empty (refine.rs:71-74)
         if (REFINE) rbase = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
-        // SB slots per lane at a time: their community totals are gathered with independent
-        // loads (a slot that is no candidate reads the vertex's own community, a valid index)
-        // before any is scored; every candidate is scored exactly as one at a time, and the
-        // argmax does not depend on the order
+        // SB candidates per lane at a time: the lane walks its table slots (stride G) in LDS
+        // until it holds SB occupied ones (its own community's weight is taken on the way),
+        // then their community totals are gathered with independent loads (an unused entry
+        // reads the vertex's own community, a valid index) before any is scored. Candidates
+        // are few next to the slots, so most lanes gather once. Every candidate is scored
+        // exactly as one at a time, and the argmax does not depend on the order.
+        // (A table in global memory, for the highest degrees, is read SB slots at a time with
+        // independent loads instead: walking it slot by slot would chain its loads.)
         constexpr int SB = 4;
-        for (uint32_t s0 = lane; s0 < tsize; s0 += (uint32_t)SB * G) {
+        uint32_t sl = lane;
+        while (true) {
             uint32_t cs[SB];
             long long wv[SB], Wv[SB], Kv[SB];
+            if (GTAB) {
 #pragma unroll
-            for (int j = 0; j < SB; j++) {
-                const uint32_t s = s0 + (uint32_t)j * G;
-                cs[j] = s < tsize ? keys[s] : EMPTY_KEY;
-                wv[j] = cs[j] != EMPTY_KEY ? vals[s] : 0;
+                for (int j = 0; j < SB; j++) {
+                    const uint32_t at = sl + (uint32_t)j * G;
+                    cs[j] = at < tsize ? keys[at] : EMPTY_KEY;
+                    wv[j] = cs[j] != EMPTY_KEY ? vals[at] : 0;
+                }
+                sl += (uint32_t)SB * G;
+#pragma unroll
+                for (int j = 0; j < SB; j++)
+                    if (cs[j] == own) {
+                        wown = wv[j];
+                        cs[j] = EMPTY_KEY;
+                    }
+                if (sl - (uint32_t)SB * G >= tsize) break;
+            } else {
+#pragma unroll
+                for (int j = 0; j < SB; j++) {
+                    cs[j] = EMPTY_KEY;
+                    wv[j] = 0;
+                    while (sl < tsize) {
+                        const uint32_t key = keys[sl];
+                        const uint32_t at = sl;
+                        sl += G;
+                        if (key == EMPTY_KEY) continue;
+                        if (key == own) {
+                            wown = vals[at];
+                            continue;
+                        }
+                        cs[j] = key;
+                        wv[j] = vals[at];
+                        break;
+                    }
+                }
+                if (cs[0] == EMPTY_KEY) break;
             }
 #pragma unroll
             for (int j = 0; j < SB; j++) {
-                const uint32_t ci = (cs[j] != EMPTY_KEY && cs[j] != own) ? cs[j] : own;
+                const uint32_t ci = cs[j] != EMPTY_KEY ? cs[j] : own;
                 // Rb reads community strengths only, CPM node weights only; the cap reads weights
                 Wv[j] = (a.objective != 0 || a.has_max) ? a.Wfx[ci] : 0;
                 Kv[j] = a.objective == 0 ? a.Kfx[ci] : 0;
@@ -392,10 +429,6 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
             for (int j = 0; j < SB; j++) {
                 const uint32_t c = cs[j];
                 if (c == EMPTY_KEY) continue;
-                if (c == own) {
-                    wown = wv[j];
-                    continue;
-                }
                 ds.c++;
                 double Wc = from_fx(Wv[j], vp.sW);
                 if (a.has_max && !(Wc + na <= a.max_w)) continue;
@@ -411,6 +444,7 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
                     better(bs, bc, gain, c);
                 }
             }
+            if ((!GTAB && cs[SB - 1] == EMPTY_KEY) || sl >= tsize) break;
         }
     }
     // group-wide reduction
@@ -442,21 +476,30 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
         ds.v++;
         ds.e += end - beg;
         if (!REFINE) {
+            // everything the rest reads, with independent loads: the own community's totals, the
+            // sizes that name the empty community, and the sizes of the singleton-swap rule
+            const uint32_t bc0 = bc;
+            const long long Kofx = a.objective == 0 ? a.Kfx[own] : 0;
+            const long long Wofx = (a.objective != 0 || a.has_max) ? a.Wfx[own] : 0;
+            const uint32_t sz_v = a.csize[v], sz_nv = a.csize[a.n + v], sz_own = a.csize[own];
+            const uint32_t sz_bc0 = a.csize[bc0 != NONE ? bc0 : own];
             long long kfx = a.objective == 0 ? to_fx(k, vp.sK) : 0;
             long long wfxv = to_fx(na, vp.sW);
-            double Kown = a.objective == 0 ? from_fx(a.Kfx[own] - kfx, vp.sK) : 0.0;
-            double Wown = (a.objective != 0 || a.has_max) ? from_fx(a.Wfx[own] - wfxv, vp.sW) : 0.0;
+            double Kown = a.objective == 0 ? from_fx(Kofx - kfx, vp.sK) : 0.0;
+            double Wown = (a.objective != 0 || a.has_max) ? from_fx(Wofx - wfxv, vp.sW) : 0.0;
             double base_q = delta_insert(a.objective, vp, from_fx(wown, vp.sw), loop, k, na, Kown, Wown);
             // the empty community (local_move.rs:93-100): this vertex's own id when no one holds
             // it, else its private id n + v (community ids span [0, 2n), so a vertex whose own
             // id names a community -- its own or another -- can still leave to an empty one)
-            const uint32_t e = a.csize[v] == 0 ? v : (a.csize[a.n + v] == 0 ? a.n + v : NONE);
+            const uint32_t e = sz_v == 0 ? v : (sz_nv == 0 ? a.n + v : NONE);
             if (e != NONE && (!a.has_max || na <= a.max_w)) {
                 double ge = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
                 better(bs, bc, ge, e);
             }
             bool accept = bc != NONE && bc != own && bs - base_q > vp.threshold;
-            if (accept && a.csize[own] == 1 && a.csize[bc] == 1 && bc > own) accept = false;
+            // the empty community, when it won, has size 0
+            const uint32_t sz_bc = bc == bc0 ? sz_bc0 : 0u;
+            if (accept && sz_own == 1 && sz_bc == 1 && bc > own) accept = false;
             if (a.trace_x) {
                 const unsigned long long slot = atomicAdd(a.trace_n, 1ull);
                 if (slot < a.trace_cap) {
@@ -1003,8 +1046,9 @@ __device__ __forceinline__ void bucket_share(const SweepArgs &A, int k, uint32_t
     hi = A.bstart[k] + (uint32_t)(L * (blockIdx.x + 1) / gridDim.x);
 }
 
-// Positions (relative to their bucket) of this workgroup's share that decide in this step,
-// stored at the start of the share's own range of work[]; s_wc[k] gets their count.
+// The vertices of this workgroup's share that decide in this step (list positions, relative to
+// their bucket, for the global-table bucket), stored at the start of the share's own range of
+// work[]; s_wc[k] gets their count.
 template <bool REFINE, bool FILTER, bool BATCH>
 __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed, uint32_t sub, uint32_t it,
                                               bool filter_on, unsigned int *s_wc) {
@@ -1100,7 +1144,8 @@ __device__ __forceinline__ void sweep_compact(const SweepArgs &A, uint64_t seed,
                 if (tq && kb[q] == k) {
                     const uint32_t below =
                         __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
-                    A.work[lo[k] + basei + below] = ip[q] - A.bstart[k];
+                    // the vertex itself; the global-table bucket needs its list position
+                    A.work[lo[k] + basei + below] = k == GTAB_BUCKET ? ip[q] - A.bstart[k] : v[q];
                 }
             }
         }
