@@ -48,6 +48,8 @@ constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
 constexpr int MOVE_SUBSWEEPS = 2;
 constexpr int MAX_MOVE_ITERS = 100;
 constexpr int MAX_REFINE_ROUNDS = 64;
+// multi-launch and graph-replayed phases: iterations launched (or replayed) per host check
+constexpr int ITER_BATCH = 4;
 
 thread_local sc_leiden_stats t_stats;
 
@@ -100,6 +102,12 @@ struct DecideArgs {
     // persistent sweep does
     const uint32_t *iter_dev;
     uint64_t iter_seed_base;
+    // multi-launch steps: this step's deciders of the bucket, compacted by k_compact (vertices;
+    // list positions for the global-table bucket), their count in device memory, and the phase's
+    // stop flag (set once it has converged: every later launch of the phase returns at once)
+    const uint32_t *sel;
+    const unsigned int *sel_len;
+    const unsigned int *stop;
     // global hash tables for the highest-degree bucket
     uint32_t *gkeys;
     long long *gvals;
@@ -123,6 +131,11 @@ struct DecideArgs {
     float *f_margin;
     float *f_own;
     uint32_t *f_nmv;
+};
+
+// First list position of every degree bucket (and the total), passed by value to kernels.
+struct BucketStarts {
+    uint32_t s[NUM_BUCKETS + 1];
 };
 
 // Parameters of one job of a batched run (one graph at one resolution, sharing a disjoint-union
@@ -587,10 +600,87 @@ __global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
     }
     StepParams sp{seed, a.sub, round, 0u, a.n_movers, nullptr, nullptr};
     DecideStats ds;
-    for (uint64_t base = (uint64_t)blockIdx.x * NG; base < a.list_len; base += (uint64_t)gridDim.x * NG)
-        decide_unit<BS, G, T, REFINE, GTAB>(a, sp, a.list, nullptr, a.list_len, base, s_keys, s_vals, r_s, r_c,
-                                            r_w, ds);
+    if (a.stop && *a.stop) return;  // workgroup-uniform
+    const uint64_t len = a.sel_len ? (uint64_t)*a.sel_len : (uint64_t)a.list_len;
+    for (uint64_t base = (uint64_t)blockIdx.x * NG; base < len; base += (uint64_t)gridDim.x * NG)
+        decide_unit<BS, G, T, REFINE, GTAB>(a, sp, a.list, a.sel, len, base, s_keys, s_vals, r_s, r_c, r_w, ds);
     flush_decide_stats(ds, s_stats, a.stats);
+}
+
+// The vertices of every degree bucket that decide in this step of the multi-launch path: active,
+// in the step's subset and not frozen (refinement: still singletons), appended to the bucket's
+// part of clist (vertices; list positions for the global-table bucket) with one global atomic per
+// bucket and workgroup; their flags are cleared (they leave the active set by deciding).
+template <bool REFINE>
+__global__ void __launch_bounds__(256) k_compact(DecideArgs a, const uint32_t *lists, BucketStarts bs,
+                                                 uint32_t *clist, unsigned int *ccount) {
+    __shared__ unsigned int sc[NUM_BUCKETS], sb[NUM_BUCKETS];
+    if (a.stop && *a.stop) return;
+    uint64_t seed = a.seed;
+    if (a.iter_dev && !REFINE) seed = mix64(a.iter_seed_base ^ (0x5851F42D4C957F2Dull * (uint64_t)(*a.iter_dev + 1)));
+    const uint32_t total = bs.s[NUM_BUCKETS];
+    for (uint32_t i0 = blockIdx.x * 256u; i0 < total; i0 += gridDim.x * 256u) {  // workgroup-uniform
+        if (threadIdx.x < NUM_BUCKETS) sc[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t i = i0 + threadIdx.x;
+        int kb = 0;
+        bool take = false;
+        uint32_t v = 0, local = 0;
+        if (i < total) {
+#pragma unroll
+            for (int k = 1; k < NUM_BUCKETS; k++) kb += i >= bs.s[k] ? 1 : 0;
+            v = lists[i];
+            const VParams vp = vparams<true>(a, v);
+            take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == a.sub;
+            if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
+            if (take) local = atomicAdd(&sc[kb], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) sb[threadIdx.x] = atomicAdd(&ccount[threadIdx.x], sc[threadIdx.x]);
+        __syncthreads();
+        if (take) {
+            clist[bs.s[kb] + sb[kb] + local] = kb == GTAB_BUCKET ? i - bs.s[kb] : v;
+            a.active[v] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Start of a multi-launch step: the compacted counts and the step's mover counters start at 0.
+__global__ void k_step_begin(const unsigned int *stop, unsigned int *ccount, unsigned int *zero, uint32_t nzero) {
+    if (*stop) return;
+    if (threadIdx.x < NUM_BUCKETS) ccount[threadIdx.x] = 0;
+    if (threadIdx.x < nzero) zero[threadIdx.x] = 0;
+}
+
+// End of an iteration (local moving: both sub-sweeps; refinement: one round). moved[0 .. nmoved)
+// count what it moved; none (or the iteration limit) sets the stop flag. ctl: [0] stop,
+// [1] iterations run, [2] their moves. The graph-replayed path's iteration index advances.
+__global__ void k_iter_end(unsigned int *ctl, const unsigned int *moved, uint32_t nmoved, uint32_t max_iters,
+                           uint32_t *iter_dev) {
+    if (ctl[0]) return;
+    unsigned int m = 0;
+    for (uint32_t i = 0; i < nmoved; i++) m += moved[i];
+    const unsigned int it = ++ctl[1];
+    ctl[2] += m;
+    if (m == 0 || it >= max_iters) ctl[0] = 1u;
+    if (iter_dev) *iter_dev += 1u;
+}
+
+// Refinement: the targeted counts this round's movers set go back to 0 for the next round.
+__global__ void k_clear_targeted(const unsigned int *stop, const uint32_t *movers, const unsigned int *count,
+                                 const uint32_t *target, uint32_t *targeted_cnt) {
+    if (*stop) return;
+    const unsigned int cnt = *count;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        targeted_cnt[target[movers[i]]] = 0;
+}
+
+// Fill that does nothing once the phase has stopped.
+__global__ void k_fill_u32_live(const unsigned int *stop, uint32_t *p, uint64_t n, uint32_t x) {
+    if (*stop) return;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = x;
 }
 
 struct ApplyArgs {
@@ -614,11 +704,13 @@ struct ApplyArgs {
     const uint32_t *claim;  // with a community-weight cap: lowest mover id per target, else null
     const uint32_t *iter_dev;  // graph-replayed rounds: the round index in device memory, else null
     uint8_t *admitted;      // local moving without a cap: 1 if the mover was admitted (join_admitted)
+    const unsigned int *stop;  // the phase's stop flag (every launch returns at once once set)
 };
 
 // Local moving: a decided mover moves if its target admits it (the lowest claim with a weight
 // cap, join_admitted without; set 0 of the joiner sums, which k_clear_joins resets afterwards).
 __global__ void k_apply_move(ApplyArgs a, DecideArgs d) {
+    if (a.stop && *a.stop) return;
     unsigned int cnt = *a.n_movers;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint64_t v = a.movers[i];
@@ -654,6 +746,7 @@ __global__ void k_apply_move(ApplyArgs a, DecideArgs d) {
 // branchy form of this test was compiled into address arithmetic that, on the tc == 0 path,
 // took the high half of a 64-bit index from a stale register.
 __global__ void k_apply_refine(ApplyArgs a) {
+    if (a.stop && *a.stop) return;
     unsigned int cnt = *a.n_movers;
     const uint32_t round = a.iter_dev ? *a.iter_dev : a.round;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -683,7 +776,9 @@ __global__ void k_apply_refine(ApplyArgs a) {
 
 // With a community-weight cap, each target community admits one joiner per sub-sweep (the
 // lowest vertex id), so the snapshot check W[c] + w_v <= cap stays true after the moves.
-__global__ void k_claim(const uint32_t *movers, const unsigned int *count, const uint32_t *target, uint32_t *claim) {
+__global__ void k_claim(const uint32_t *movers, const unsigned int *count, const uint32_t *target, uint32_t *claim,
+                        const unsigned int *stop = nullptr) {
+    if (stop && *stop) return;
     unsigned int cnt = *count;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint64_t v = movers[i];
@@ -694,7 +789,8 @@ __global__ void k_claim(const uint32_t *movers, const unsigned int *count, const
 
 // Resets the joiner sums and lowest ids of this sub-sweep's targets (set 0) for the next one.
 __global__ void k_clear_joins(const uint32_t *movers, const unsigned int *count, const uint32_t *target,
-                              long long *jsum, uint32_t *jmin) {
+                              long long *jsum, uint32_t *jmin, const unsigned int *stop = nullptr) {
+    if (stop && *stop) return;
     unsigned int cnt = *count;
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint32_t t = target[movers[i]];
@@ -712,8 +808,10 @@ __global__ void __launch_bounds__(256) k_activate(const uint32_t *list, const un
                                                   const uint32_t *memb, const uint32_t *constraint,
                                                   uint8_t *active, const uint32_t *target = nullptr,
                                                   const uint32_t *claim = nullptr,
-                                                  const uint8_t *admitted = nullptr) {
+                                                  const uint8_t *admitted = nullptr,
+                                                  const unsigned int *stop = nullptr) {
     constexpr int NG = 256 / G;
+    if (stop && *stop) return;
     unsigned int cnt = *count;
     const int lane = threadIdx.x % G;
     for (unsigned int i = blockIdx.x * NG + threadIdx.x / G; i < cnt; i += gridDim.x * NG) {
@@ -763,14 +861,28 @@ static void fill_u32(uint32_t *p, uint64_t n, uint32_t x, hipStream_t s) {
 }
 
 // One iteration's launch sequence captured once (per level) and replayed per iteration.
+// An executable graph kept across the levels of one run: a level's capture updates it in place
+// (hipGraphExecUpdate) when its launch sequence has the same shape, so a run instantiates once
+// per phase kind instead of once per level and phase.
+struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    GraphSlot() = default;
+    GraphSlot(const GraphSlot &) = delete;
+    GraphSlot &operator=(const GraphSlot &) = delete;
+    ~GraphSlot() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
 struct IterGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool owned = true;  // exec belongs to a GraphSlot otherwise
     IterGraph() = default;
     IterGraph(const IterGraph &) = delete;
     IterGraph &operator=(const IterGraph &) = delete;
     template <typename F>
-    void capture(hipStream_t s, F &&record) {
+    void capture(hipStream_t s, F &&record, GraphSlot *slot = nullptr) {
         HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
             record();
@@ -781,11 +893,28 @@ struct IterGraph {
             throw;
         }
         HIP_OK(hipStreamEndCapture(s, &graph));
+        if (slot) {
+            owned = false;
+            if (slot->exec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+                if (hipGraphExecUpdate(slot->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+                    exec = slot->exec;
+                    return;
+                }
+                (void)hipGetLastError();  // a refused update is not an error of the run
+                (void)hipGraphExecDestroy(slot->exec);
+                slot->exec = nullptr;
+            }
+            HIP_OK(hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0));
+            exec = slot->exec;
+            return;
+        }
         HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
     void launch(hipStream_t s) { HIP_OK(hipGraphLaunch(exec, s)); }
     ~IterGraph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
+        if (exec && owned) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
 };
@@ -801,9 +930,6 @@ __global__ void k_bucket_count(const uint64_t *ptrs, uint32_t n, BucketBounds bb
     if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
 }
 
-struct BucketStarts {
-    uint32_t s[NUM_BUCKETS + 1];
-};
 
 // Block-local counting, one global reservation per bucket per block, then placement.
 __global__ void k_bucket_fill(const uint64_t *ptrs, uint32_t n, BucketBounds bb, BucketStarts starts, uint32_t *cursor,
@@ -1755,6 +1881,8 @@ struct RunCtx {
     // learned filter / trace collection (device pointers; null when off) and the level index
     const struct FilterOpts *fopt = nullptr;
     uint32_t level = 0;
+    // graph-replayed sweeps: one executable graph per phase kind, updated level by level
+    GraphSlot move_graph, refine_graph;
 };
 
 // Learned-filter inference or trace collection for local moving (device pointers).
@@ -1773,28 +1901,39 @@ struct FilterOpts {
 template <int BS, int G, int T, bool REFINE, bool GTAB>
 static void launch_decide(const DecideArgs &a, hipStream_t s) {
     constexpr int NG = BS / G;
-    unsigned grid = grid_for(a.list_len, NG, 8192);
+    // over a compacted list (count on the device) the grid covers the bucket but stays within
+    // about two waves of workgroups, so a small step's launch is cheap
+    unsigned grid = grid_for(a.list_len, NG, a.sel_len ? 2048u : 8192u);
     hipLaunchKernelGGL((k_decide<BS, G, T, REFINE, GTAB>), dim3(grid), dim3(BS), 0, s, a);
     KERNEL_OK();
 }
 
 template <bool REFINE>
-static void decide_all(DecideArgs a, const Buckets &b, hipStream_t s, KernelTimer *timer) {
+static void decide_all(DecideArgs a, const Buckets &b, hipStream_t s, KernelTimer *timer,
+                       const uint32_t *clist = nullptr, const unsigned int *ccount = nullptr,
+                       bool every_bucket = false) {
+    // every_bucket: empty buckets launch too (one workgroup that finds nothing), so a captured
+    // step has the same launch sequence on every level
     for (int bk = 0; bk < NUM_BUCKETS; bk++) {
         uint32_t len = b.start[bk + 1] - b.start[bk];
-        if (!len) continue;
+        if (!len && !every_bucket) continue;
         a.list = b.lists.get() + b.start[bk];
         a.list_len = len;
+        if (clist) {
+            a.sel = clist + b.start[bk];
+            a.sel_len = ccount + bk;
+        }
         hipEvent_t ev = nullptr;
         if (timer) timer->begin(s, ev);
         switch (bk) {
-            case 0: launch_decide<256, 8, 16, REFINE, false>(a, s); break;
-            case 1: launch_decide<256, 16, 32, REFINE, false>(a, s); break;
-            case 2: launch_decide<256, 32, 64, REFINE, false>(a, s); break;
-            case 3: launch_decide<256, 64, 256, REFINE, false>(a, s); break;
-            case 4: launch_decide<256, 64, 1024, REFINE, false>(a, s); break;
-            case 5: launch_decide<64, 64, 2048, REFINE, false>(a, s); break;
-            case 6: launch_decide<256, 256, 4096, REFINE, false>(a, s); break;
+            // the persistent sweep's group shapes (narrow groups keep many vertices in flight)
+            case 0: launch_decide<256, BucketShape<0>::G, BucketShape<0>::T, REFINE, false>(a, s); break;
+            case 1: launch_decide<256, BucketShape<1>::G, BucketShape<1>::T, REFINE, false>(a, s); break;
+            case 2: launch_decide<256, BucketShape<2>::G, BucketShape<2>::T, REFINE, false>(a, s); break;
+            case 3: launch_decide<256, BucketShape<3>::G, BucketShape<3>::T, REFINE, false>(a, s); break;
+            case 4: launch_decide<256, BucketShape<4>::G, BucketShape<4>::T, REFINE, false>(a, s); break;
+            case 5: launch_decide<256, BucketShape<5>::G, BucketShape<5>::T, REFINE, false>(a, s); break;
+            case 6: launch_decide<256, BucketShape<6>::G, BucketShape<6>::T, REFINE, false>(a, s); break;
             default:
                 a.gkeys = b.gkeys.get();
                 a.gvals = b.gvals.get();
@@ -2188,9 +2327,18 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         HIP_OK(hipMemsetAsync(ls.jmin.get(), 0xFF, 2 * (size_t)n * sizeof(uint32_t), s));
         if (ls.admitted.size() < n) ls.admitted.resize(n);
     }
+    // ctl: [0] stop, [1] iterations, [2] moves, [3] unused, [4 ..] movers per sub-sweep, then the
+    // compacted counts per bucket
+    constexpr uint32_t CTL = 4 + MOVE_SUBSWEEPS + NUM_BUCKETS;
+    DevBuf<unsigned int> ctl(CTL);
+    HIP_OK(hipMemsetAsync(ctl.get(), 0, CTL * sizeof(unsigned int), s));
+    unsigned int *const mv_ctr = ctl.get() + 4, *const ccount = ctl.get() + 4 + MOVE_SUBSWEEPS;
+    DevBuf<uint32_t> clist(n);
+    BucketStarts bst;
+    for (int i = 0; i <= NUM_BUCKETS; i++) bst.s[i] = b.start[i];
+    a.stop = ctl.get();
     ApplyArgs ap{};
     ap.movers = ls.movers.get();
-    ap.n_movers = ls.counters.get();
     ap.memb = memb;
     ap.target = ls.target.get();
     ap.Kfx = ls.Kfx.get();
@@ -2203,7 +2351,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
     ap.active = ls.active.get();
     ap.claim = capped ? ls.claim.get() : nullptr;
     ap.admitted = capped ? nullptr : ls.admitted.get();
-    DevBuf<unsigned int> iter_moves(MOVE_SUBSWEEPS);
+    ap.stop = ctl.get();
     const bool use_graph = graph_sweeps();
     DevBuf<uint32_t> iter_dev;
     if (use_graph) {
@@ -2212,58 +2360,75 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
         a.iter_dev = iter_dev.get();
         a.iter_seed_base = level_seed;
     }
-    // one iteration: both sub-sweeps' decide / claim / apply / activate (sync-free, so it can
-    // be captured); with a graph the iteration index and seed come from iter_dev
+    // one iteration: both sub-sweeps' compaction / decide / claim / apply / activate, then the
+    // convergence test on the device (sync-free, so it can be captured); once the phase has
+    // stopped every launch returns at once. With a graph the iteration index and seed come
+    // from iter_dev.
     auto iteration = [&](int it) {
         if (!use_graph) a.seed = mix64(level_seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(it + 1)));
         for (uint32_t sub = 0; sub < (uint32_t)MOVE_SUBSWEEPS; sub++) {
             a.sub = sub;
-            a.n_movers = iter_moves.get() + sub;
-            ap.n_movers = iter_moves.get() + sub;
-            fill_u32(iter_moves.get() + sub, 1, 0u, s);
-            decide_all<false>(a, b, s, (ctx.timer.on && !use_graph) ? &ctx.timer : nullptr);
+            a.n_movers = mv_ctr + sub;
+            ap.n_movers = mv_ctr + sub;
+            hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ctl.get(), ccount, mv_ctr + sub, 1u);
+            KERNEL_OK();
+            hipLaunchKernelGGL(k_compact<false>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b.lists.get(), bst,
+                               clist.get(), ccount);
+            KERNEL_OK();
+            decide_all<false>(a, b, s, (ctx.timer.on && !use_graph) ? &ctx.timer : nullptr, clist.get(), ccount,
+                              use_graph);
             if (capped) {
-                fill_u32(ls.claim.get(), 2 * (uint64_t)n, 0xFFFFFFFFu, s);
-                hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), iter_moves.get() + sub,
-                                   ls.target.get(), ls.claim.get());
+                hipLaunchKernelGGL(k_fill_u32_live, dim3(grid_for(2 * (uint64_t)n, 256, 4096)), dim3(256), 0, s,
+                                   ctl.get(), ls.claim.get(), 2 * (uint64_t)n, 0xFFFFFFFFu);
+                KERNEL_OK();
+                hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), mv_ctr + sub,
+                                   ls.target.get(), ls.claim.get(), ctl.get());
                 KERNEL_OK();
             }
             hipLaunchKernelGGL(k_apply_move, dim3(1024), dim3(256), 0, s, ap, a);
             KERNEL_OK();
-            hipLaunchKernelGGL((k_activate<16, false>), dim3(1024), dim3(256), 0, s, ls.movers.get(),
-                               iter_moves.get() + sub, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)memb,
-                               (const uint32_t *)nullptr, ls.active.get(), (const uint32_t *)ls.target.get(),
+            hipLaunchKernelGGL((k_activate<16, false>), dim3(1024), dim3(256), 0, s, ls.movers.get(), mv_ctr + sub,
+                               L.ptrs.get(), L.nbrs.get(), (const uint32_t *)memb, (const uint32_t *)nullptr,
+                               ls.active.get(), (const uint32_t *)ls.target.get(),
                                (const uint32_t *)(capped ? ls.claim.get() : nullptr),
-                               (const uint8_t *)(capped ? nullptr : ls.admitted.get()));
+                               (const uint8_t *)(capped ? nullptr : ls.admitted.get()), (const unsigned int *)ctl.get());
             KERNEL_OK();
             if (!capped) {
-                hipLaunchKernelGGL(k_clear_joins, dim3(1024), dim3(256), 0, s, ls.movers.get(), iter_moves.get() + sub,
-                                   ls.target.get(), ls.jsum.get(), ls.jmin.get());
+                hipLaunchKernelGGL(k_clear_joins, dim3(1024), dim3(256), 0, s, ls.movers.get(), mv_ctr + sub,
+                                   ls.target.get(), ls.jsum.get(), ls.jmin.get(), (const unsigned int *)ctl.get());
                 KERNEL_OK();
             }
         }
-        if (use_graph) {
-            hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(1), 0, s, iter_dev.get());
-            KERNEL_OK();
-        }
+        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctl.get(), (const unsigned int *)mv_ctr,
+                           (uint32_t)MOVE_SUBSWEEPS, (uint32_t)MAX_MOVE_ITERS, use_graph ? iter_dev.get() : nullptr);
+        KERNEL_OK();
     };
+    // ITER_BATCH iterations per replay (or per host check): the host reads the stop flag once per
+    // batch, the iterations after convergence return at once
     IterGraph ig;
-    if (use_graph) ig.capture(s, [&] { iteration(0); });
-    for (int it = 0; it < MAX_MOVE_ITERS; it++) {
+    if (use_graph)
+        ig.capture(
+            s,
+            [&] {
+                for (int k = 0; k < ITER_BATCH; k++) iteration(0);
+            },
+            &ctx.move_graph);
+    for (int it = 0; it < MAX_MOVE_ITERS; it += ITER_BATCH) {
         if (use_graph)
             ig.launch(s);
         else
-            iteration(it);
-        unsigned int mv[MOVE_SUBSWEEPS];
-        to_host(mv, iter_moves.get(), MOVE_SUBSWEEPS, s);
+            for (int k = 0; k < ITER_BATCH; k++) iteration(it + k);
+        unsigned int stop = 0;
+        to_host(&stop, ctl.get(), 1, s);
         sync_stream(s);
         if (ctx.timer.on) ctx.timer.harvest();
-        uint64_t moves = 0;
-        for (int i = 0; i < MOVE_SUBSWEEPS; i++) moves += mv[i];
-        ctx.st.move_iterations++;
-        ctx.st.moves += moves;
-        if (moves == 0) break;
+        if (stop) break;
     }
+    unsigned int h[3];
+    to_host(h, ctl.get(), 3, s);
+    sync_stream(s);
+    ctx.st.move_iterations += h[1];
+    ctx.st.moves += h[2];
     harvest_stats(ls, ctx, true);
 }
 
@@ -2297,6 +2462,16 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
     KERNEL_OK();
     HIP_OK(hipMemsetAsync(ls.tround.get(), 0xFF, n * sizeof(uint32_t), s));
+    fill_u32(ls.targeted.get(), n, 0u, s);  // each round then clears what its movers set
+    // ctl: [0] stop, [1] rounds, [2] moves, [3] unused, then the compacted counts per bucket
+    constexpr uint32_t CTL = 4 + NUM_BUCKETS;
+    DevBuf<unsigned int> ctl(CTL);
+    HIP_OK(hipMemsetAsync(ctl.get(), 0, CTL * sizeof(unsigned int), s));
+    unsigned int *const ccount = ctl.get() + 4;
+    DevBuf<uint32_t> clist(n);
+    BucketStarts bst;
+    for (int i = 0; i <= NUM_BUCKETS; i++) bst.s[i] = b.start[i];
+    a.stop = ctl.get();
     ApplyArgs ap{};
     ap.movers = ls.movers.get();
     ap.n_movers = ls.counters.get();
@@ -2316,6 +2491,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     ap.n_moved = ls.counters.get() + 1;
     const bool capped = ctx.cfg->has_max_community_weight != 0;
     ap.claim = capped ? ls.claim.get() : nullptr;
+    ap.stop = ctl.get();
     const bool use_graph = graph_sweeps();
     DevBuf<uint32_t> iter_dev;
     if (use_graph) {
@@ -2329,39 +2505,57 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
             a.round = r;
             ap.round = r;
         }
-        fill_u32(ls.counters.get(), 2, 0u, s);
-        fill_u32(ls.targeted.get(), n, 0u, s);
-        decide_all<true>(a, b, s, nullptr);
+        hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ctl.get(), ccount, ls.counters.get(), 2u);
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_compact<true>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b.lists.get(), bst,
+                           clist.get(), ccount);
+        KERNEL_OK();
+        decide_all<true>(a, b, s, nullptr, clist.get(), ccount, use_graph);
         if (capped) {
-            fill_u32(ls.claim.get(), 2 * (uint64_t)n, 0xFFFFFFFFu, s);
+            hipLaunchKernelGGL(k_fill_u32_live, dim3(grid_for(2 * (uint64_t)n, 256, 4096)), dim3(256), 0, s,
+                               ctl.get(), ls.claim.get(), 2 * (uint64_t)n, 0xFFFFFFFFu);
+            KERNEL_OK();
             hipLaunchKernelGGL(k_claim, dim3(1024), dim3(256), 0, s, ls.movers.get(), ls.counters.get(),
-                               ls.target.get(), ls.claim.get());
+                               ls.target.get(), ls.claim.get(), ctl.get());
             KERNEL_OK();
         }
         hipLaunchKernelGGL(k_apply_refine, dim3(1024), dim3(256), 0, s, ap);
         KERNEL_OK();
         hipLaunchKernelGGL((k_activate<16, true>), dim3(1024), dim3(256), 0, s, ls.moved.get(),
                            ls.counters.get() + 1, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)rmemb, memb,
-                           ls.active.get());
+                           ls.active.get(), (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                           (const uint8_t *)nullptr, (const unsigned int *)ctl.get());
         KERNEL_OK();
-        if (use_graph) {
-            hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(1), 0, s, iter_dev.get());
-            KERNEL_OK();
-        }
+        hipLaunchKernelGGL(k_clear_targeted, dim3(1024), dim3(256), 0, s, (const unsigned int *)ctl.get(),
+                           ls.movers.get(), (const unsigned int *)ls.counters.get(), ls.target.get(),
+                           ls.targeted.get());
+        KERNEL_OK();
+        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctl.get(), (const unsigned int *)ls.counters.get() + 1,
+                           1u, (uint32_t)MAX_REFINE_ROUNDS, use_graph ? iter_dev.get() : nullptr);
+        KERNEL_OK();
     };
     IterGraph ig;
-    if (use_graph) ig.capture(s, [&] { round_launches(0); });
-    for (uint32_t r = 0; r < (uint32_t)MAX_REFINE_ROUNDS; r++) {
+    if (use_graph)
+        ig.capture(
+            s,
+            [&] {
+                for (int k = 0; k < ITER_BATCH; k++) round_launches(0);
+            },
+            &ctx.refine_graph);
+    for (uint32_t r = 0; r < (uint32_t)MAX_REFINE_ROUNDS; r += ITER_BATCH) {
         if (use_graph)
             ig.launch(s);
         else
-            round_launches(r);
-        unsigned int c[2];
-        to_host(c, ls.counters.get(), 2, s);
+            for (int k = 0; k < ITER_BATCH; k++) round_launches(r + k);
+        unsigned int stop = 0;
+        to_host(&stop, ctl.get(), 1, s);
         sync_stream(s);
-        ctx.st.refine_rounds++;
-        if (c[1] == 0) break;
+        if (stop) break;
     }
+    unsigned int h[3];
+    to_host(h, ctl.get(), 3, s);
+    sync_stream(s);
+    ctx.st.refine_rounds += h[1];
     harvest_stats(ls, ctx, false);
 }
 
