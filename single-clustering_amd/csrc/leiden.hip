@@ -614,34 +614,74 @@ __global__ void __launch_bounds__(BS) k_decide(DecideArgs a) {
 template <bool REFINE>
 __global__ void __launch_bounds__(256) k_compact(DecideArgs a, const uint32_t *lists, BucketStarts bs,
                                                  uint32_t *clist, unsigned int *ccount) {
+    constexpr int U = 4;  // positions per thread and pass, their loads in flight together
     __shared__ unsigned int sc[NUM_BUCKETS], sb[NUM_BUCKETS];
     if (a.stop && *a.stop) return;
     uint64_t seed = a.seed;
     if (a.iter_dev && !REFINE) seed = mix64(a.iter_seed_base ^ (0x5851F42D4C957F2Dull * (uint64_t)(*a.iter_dev + 1)));
     const uint32_t total = bs.s[NUM_BUCKETS];
-    for (uint32_t i0 = blockIdx.x * 256u; i0 < total; i0 += gridDim.x * 256u) {  // workgroup-uniform
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i0 = blockIdx.x * (256u * U); i0 < total; i0 += gridDim.x * (256u * U)) {  // workgroup-uniform
         if (threadIdx.x < NUM_BUCKETS) sc[threadIdx.x] = 0;
         __syncthreads();
-        const uint32_t i = i0 + threadIdx.x;
-        int kb = 0;
-        bool take = false;
-        uint32_t v = 0, local = 0;
-        if (i < total) {
+        uint32_t ip[U], v[U], local[U];
+        int kb[U];
+        bool take[U];
 #pragma unroll
-            for (int k = 1; k < NUM_BUCKETS; k++) kb += i >= bs.s[k] ? 1 : 0;
-            v = lists[i];
-            const VParams vp = vparams<true>(a, v);
-            take = a.active[v] == 1 && !vp.frozen && subset_of(seed, v - vp.lo, a.P) == a.sub;
-            if (REFINE) take = take && a.memb[v] == v && a.csize[v] == 1;
-            if (take) local = atomicAdd(&sc[kb], 1u);
+        for (int q = 0; q < U; q++) {
+            ip[q] = i0 + (uint32_t)q * 256u + threadIdx.x;
+            v[q] = ip[q] < total ? lists[ip[q]] : 0u;
+        }
+        uint8_t f[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) f[q] = ip[q] < total ? a.active[v[q]] : (uint8_t)0;
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            kb[q] = 0;
+#pragma unroll
+            for (int k = 1; k < NUM_BUCKETS; k++) kb[q] += ip[q] >= bs.s[k] ? 1 : 0;
+            take[q] = false;
+            if (f[q] == 1) {
+                const VParams vp = vparams<true>(a, v[q]);
+                take[q] = !vp.frozen && subset_of(seed, v[q] - vp.lo, a.P) == a.sub;
+            }
+        }
+        if (REFINE) {
+            uint32_t m[U], cs[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                m[q] = take[q] ? a.memb[v[q]] : 0u;
+                cs[q] = take[q] ? a.csize[v[q]] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) take[q] = take[q] && m[q] == v[q] && cs[q] == 1u;
+        }
+        // a wave's takes per bucket reserve their places with one LDS atomic
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            local[q] = 0;
+#pragma unroll
+            for (int k = 0; k < NUM_BUCKETS; k++) {
+                const uint64_t mk = __ballot(take[q] && kb[q] == k);
+                if (!mk) continue;
+                const int leader = __ffsll((unsigned long long)mk) - 1;
+                unsigned int basei = 0;
+                if (lane == leader) basei = atomicAdd(&sc[k], (unsigned int)__popcll(mk));
+                basei = __shfl(basei, leader);
+                if (take[q] && kb[q] == k)
+                    local[q] = basei + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+            }
         }
         __syncthreads();
         if (threadIdx.x < NUM_BUCKETS && sc[threadIdx.x]) sb[threadIdx.x] = atomicAdd(&ccount[threadIdx.x], sc[threadIdx.x]);
         __syncthreads();
-        if (take) {
-            clist[bs.s[kb] + sb[kb] + local] = kb == GTAB_BUCKET ? i - bs.s[kb] : v;
-            a.active[v] = 0;
-        }
+#pragma unroll
+        for (int q = 0; q < U; q++)
+            if (take[q]) {
+                clist[bs.s[kb[q]] + sb[kb[q]] + local[q]] = kb[q] == GTAB_BUCKET ? ip[q] - bs.s[kb[q]] : v[q];
+                a.active[v[q]] = 0;
+            }
         __syncthreads();
     }
 }
@@ -827,13 +867,53 @@ __global__ void __launch_bounds__(256) k_activate(const uint32_t *list, const un
     }
 }
 
-__global__ void k_init_totals(const uint32_t *memb, const double *strength, const double *node_w, uint32_t n,
-                              int sK, int sW, long long *Kfx, long long *Wfx, uint32_t *csize) {
+// Community totals of a membership. A level that starts from earlier communities has few, large
+// ones, whose same-address atomics would serialize: the workgroup first sums its vertices per
+// community in an LDS table (integer sums, so any order gives the same totals) and adds each
+// community's sums with one global atomic; a community without a slot within eight probes is
+// added directly.
+__global__ void __launch_bounds__(256) k_init_totals(const uint32_t *memb, const double *strength,
+                                                     const double *node_w, uint32_t n, int sK, int sW, long long *Kfx,
+                                                     long long *Wfx, uint32_t *csize) {
+    constexpr uint32_t PT = 1024;
+    __shared__ uint32_t tk[PT];
+    __shared__ long long tK[PT], tW[PT];
+    __shared__ uint32_t tN[PT];
+    for (uint32_t i = threadIdx.x; i < PT; i += blockDim.x) {
+        tk[i] = EMPTY_KEY;
+        tK[i] = 0;
+        tW[i] = 0;
+        tN[i] = 0;
+    }
+    __syncthreads();
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        uint32_t c = memb[v];
-        atomicAdd((unsigned long long *)&Kfx[c], (unsigned long long)to_fx(strength[v], sK));
-        atomicAdd((unsigned long long *)&Wfx[c], (unsigned long long)to_fx(node_w[v], sW));
-        atomicAdd(&csize[c], 1u);
+        const uint32_t c = memb[v];
+        const long long kf = to_fx(strength[v], sK), wf = to_fx(node_w[v], sW);
+        uint32_t h = hash32(c) & (PT - 1);
+        bool done = false;
+        for (int p = 0; p < 8 && !done; p++) {
+            const uint32_t prev = atomicCAS(&tk[h], EMPTY_KEY, c);
+            if (prev == EMPTY_KEY || prev == c) {
+                atomicAdd((unsigned long long *)&tK[h], (unsigned long long)kf);
+                atomicAdd((unsigned long long *)&tW[h], (unsigned long long)wf);
+                atomicAdd(&tN[h], 1u);
+                done = true;
+            }
+            h = (h + 1) & (PT - 1);
+        }
+        if (!done) {
+            atomicAdd((unsigned long long *)&Kfx[c], (unsigned long long)kf);
+            atomicAdd((unsigned long long *)&Wfx[c], (unsigned long long)wf);
+            atomicAdd(&csize[c], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < PT; i += blockDim.x) {
+        const uint32_t c = tk[i];
+        if (c == EMPTY_KEY) continue;
+        atomicAdd((unsigned long long *)&Kfx[c], (unsigned long long)tK[i]);
+        atomicAdd((unsigned long long *)&Wfx[c], (unsigned long long)tW[i]);
+        atomicAdd(&csize[c], tN[i]);
     }
 }
 
@@ -2372,7 +2452,7 @@ static void local_move(const sc_graph &L, uint32_t *memb, LevelState &ls, const 
             ap.n_movers = mv_ctr + sub;
             hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ctl.get(), ccount, mv_ctr + sub, 1u);
             KERNEL_OK();
-            hipLaunchKernelGGL(k_compact<false>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b.lists.get(), bst,
+            hipLaunchKernelGGL(k_compact<false>, dim3(grid_for(n, 1024, 4096)), dim3(256), 0, s, a, b.lists.get(), bst,
                                clist.get(), ccount);
             KERNEL_OK();
             decide_all<false>(a, b, s, (ctx.timer.on && !use_graph) ? &ctx.timer : nullptr, clist.get(), ccount,
@@ -2507,7 +2587,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
         }
         hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ctl.get(), ccount, ls.counters.get(), 2u);
         KERNEL_OK();
-        hipLaunchKernelGGL(k_compact<true>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b.lists.get(), bst,
+        hipLaunchKernelGGL(k_compact<true>, dim3(grid_for(n, 1024, 4096)), dim3(256), 0, s, a, b.lists.get(), bst,
                            clist.get(), ccount);
         KERNEL_OK();
         decide_all<true>(a, b, s, nullptr, clist.get(), ccount, use_graph);
