@@ -3016,16 +3016,33 @@ __global__ void __launch_bounds__(256) k_emit_fine_h(const uint32_t *members, co
                                                      const float *wts, const uint32_t *groups, uint32_t ng,
                                                      uint32_t *keys, uint64_t *gw) {
     constexpr int NG = 256 / G;
+    constexpr int EB = 4;  // entries per lane with their loads in flight together
     const int lane = threadIdx.x % G;
     for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
         const uint32_t v = members[j];
         const uint32_t g = sorted_groups[j];
         const uint64_t o = moff[j], lo = ptrs[v], hi = ptrs[v + 1];
-        for (uint64_t e = lo + lane; e < hi; e += G) {
-            const uint32_t h = groups[nbrs[e]];
-            const uint64_t t = o + (e - lo);
-            keys[t] = h >= g ? h : ng;
-            gw[t] = pack_gw(g, wts[e]);
+        for (uint64_t e0 = lo + lane; e0 < hi; e0 += (uint64_t)EB * G) {
+            uint32_t u[EB];
+            float w[EB];
+#pragma unroll
+            for (int q = 0; q < EB; q++) {
+                const uint64_t e = e0 + (uint64_t)q * G;
+                u[q] = e < hi ? nbrs[e] : 0u;
+                w[q] = e < hi ? wts[e] : 0.0f;
+            }
+            uint32_t h[EB];
+#pragma unroll
+            for (int q = 0; q < EB; q++) h[q] = e0 + (uint64_t)q * G < hi ? groups[u[q]] : 0u;
+#pragma unroll
+            for (int q = 0; q < EB; q++) {
+                const uint64_t e = e0 + (uint64_t)q * G;
+                if (e < hi) {
+                    const uint64_t t = o + (e - lo);
+                    keys[t] = h[q] >= g ? h[q] : ng;
+                    gw[t] = pack_gw(g, w[q]);
+                }
+            }
         }
     }
 }
@@ -3225,9 +3242,18 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     uint32_t np = 0;
     DevBuf<uint32_t> head(nnz + 1), run(nnz + 1);
     if (nnz) {
-        hipLaunchKernelGGL((k_emit_fine_h<16>), dim3(grid_for(n, 16, 2048)), dim3(256), 0, s, members.get(), sg.get(),
-                           n, moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, ng, keys.get(), gw.get());
-        KERNEL_OK();
+        // lanes per member row from the level's mean degree (about four entries per lane)
+        const double mean_deg = n ? (double)nnz / (double)n : 0.0;
+        const int eg = mean_deg <= 16.0 ? 4 : (mean_deg <= 32.0 ? 8 : (mean_deg <= 64.0 ? 16 : 32));
+        auto emit = [&](auto kern, int grp) {
+            hipLaunchKernelGGL(kern, dim3(grid_for(n, 256 / grp, 4096)), dim3(256), 0, s, members.get(), sg.get(), n,
+                               moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, ng, keys.get(), gw.get());
+            KERNEL_OK();
+        };
+        if (eg == 4) emit(k_emit_fine_h<4>, 4);
+        else if (eg == 8) emit(k_emit_fine_h<8>, 8);
+        else if (eg == 16) emit(k_emit_fine_h<16>, 16);
+        else emit(k_emit_fine_h<32>, 32);
         radix_sort_pairs(keys, gw, nnz, kbits, s);
         hipLaunchKernelGGL(k_pair_heads_h, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), gw.get(), nnz,
                            sent, head.get());
