@@ -223,8 +223,9 @@ def main():
                     help="objective for the configuration's Leiden run (default: its own, Rb)")
     ap.add_argument("--resolution", type=float, default=None,
                     help="resolution with --objective (default 1.0 for rb, 0.05 for cpm)")
-    ap.add_argument("--knn-explore", type=int, default=3,
-                    help="untimed kNN graph builds first (the tuner measures its column-split candidates)")
+    ap.add_argument("--knn-explore", type=int, default=6,
+                    help="untimed kNN graph builds first (the tuner measures each of its three column-split "
+                         "candidates twice)")
     ap.add_argument("--knn-builds", type=int, default=None,
                     help="timed kNN graph builds (default 2 above 1M points, else 10)")
     ap.add_argument("--cpu-reps", type=int, default=None,

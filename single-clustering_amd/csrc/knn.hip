@@ -153,6 +153,13 @@ __device__ __forceinline__ void list_insert_net(float (&lv)[KM], uint32_t (&li)[
     }
 }
 
+// The least f32 value at or above the double v (v may be infinite).
+__device__ __forceinline__ float f32_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return f;
+}
+
 template <int NKC> struct KnnTileShape {
     static constexpr int TN = NKC <= 2 ? 64 : 32;      // points per LDS stage
     static constexpr int BCAP = 20;                     // buffered hits per row between merges
@@ -186,7 +193,7 @@ constexpr size_t knn_tile_lds_bytes() {
 template <int KM, int BCAP>
 __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[KM], float (&thr_r)[4][4],
                                                uint32_t (&cnt4)[4], uint32_t *sCnt, float *sThr, uint2 *sBuf,
-                                               int wave, int lane, float tseed) {
+                                               int wave, int lane, float tseed, float tmargin, uint32_t kk) {
     const int rl = wave * 64 + lane;
     const int q = lane >> 4;
     if ((lane & 15) == 0) {
@@ -201,7 +208,15 @@ __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[K
         const uint2 e = p < c ? sBuf[rl * BCAP + p] : make_uint2(__float_as_uint(INFINITY), KNONE);
         list_insert_net<KM>(lv, li, __uint_as_float(e.x), e.y);
     }
-    sThr[rl] = fminf(lv[KM - 1], tseed);
+    // the threshold: the list's last value, the seed, and the row's k-th value plus the margin
+    // (2.02 times the error bound; the sum rounded up): a point at or above it is strictly
+    // farther than k points of this list (each within the bound of its value), so it cannot be
+    // among the k nearest
+    float lk = INFINITY;
+#pragma unroll
+    for (int p = 0; p < KM; p++) lk = (uint32_t)p + 1u == kk ? lv[p] : lk;
+    const float kthr = lk < INFINITY ? f32_up((double)lk + (double)tmargin) : INFINITY;
+    sThr[rl] = fminf(fminf(lv[KM - 1], tseed), kthr);
     __threadfence_block();
 #pragma unroll
     for (int rt = 0; rt < 4; rt++) {
@@ -292,7 +307,8 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
                                                   uint32_t cols_per_split, uint32_t row_blocks, uint32_t splits,
                                                   const uint32_t *blist, const uint32_t *boff, const uint32_t *bcnt,
                                                   const uint32_t *rb_order,
-                                                  const float *seed_s, uint32_t *cand_idx, float *cand_s) {
+                                                  const float *seed_s, const float *margin_s, uint32_t kk,
+                                                  uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
     constexpr int BCAP = KnnTileShape<NKC>::BCAP;
@@ -353,6 +369,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     // a row's threshold starts at its seed (s-space bound above which no k-nearest point lies,
     // pruned search) or +inf, and never rises above it
     const float tseed = seed_s ? seed_s[wrow0 + lane] : INFINITY;
+    const float tmargin = margin_s[wrow0 + lane];  // twice the row's error bound, rounded up
     sThr[tid] = tseed;
 
     // A fragments of the wave's four row tiles, in registers for the whole sweep: the query row
@@ -500,17 +517,17 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
                                                     al, af, acc1);
             bool need = knn_subtile_hits<BCAP>(acc0, c0 + sub * 16 + (lane & 15), row0, rl_base, qmask, thr_r, cnt4,
                                                sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
             if (sub + 2 < TN / 16)
                 knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
                                                         ah, al, af, acc0);
             need = knn_subtile_hits<BCAP>(acc1, c0 + (sub + 1) * 16 + (lane & 15), row0, rl_base, qmask, thr_r, cnt4,
                                           sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
         }
     }
 #undef KNN_PREFETCH
-    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed);
+    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
     // write this split's candidate list: cand[(row * splits + split) * KM + t]
     const uint32_t my_row = wrow0 + lane;
     if (my_row < n) {
@@ -1104,13 +1121,6 @@ __global__ void __launch_bounds__(256) k_group_radii(const float *X, const uint3
     }
 }
 
-// f32 value at or above the (non-negative, possibly infinite) double v.
-__device__ __forceinline__ float f32_up(double v) {
-    float f = (float)v;
-    if ((double)f < v) f = nextafterf(f, INFINITY);
-    return f;
-}
-
 // Per-group data of the group-level pruning: centres transposed (cenT[t * C + g]) and their f32
 // norms, and the radii rounded up: r64 = sqrt(r2) (1 + 1e-12) (f64), rf = r64 rounded up to f32,
 // rcf the same for the core.
@@ -1418,6 +1428,21 @@ __global__ void k_seed_thresholds(const double *seed_d2, const float *ny, uint32
     }
 }
 
+// Twice the error bound of every layout row's approximate values (2.02 eps, rounded up; +inf for
+// padding rows): the tile's k-th-value threshold adds it to the row's k-th listed value.
+__global__ void k_knn_margins(const float *ny, uint32_t rows, double eps_coef, double eps_const, float *margin) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x) {
+        const double y = (double)ny[j];
+        float out = INFINITY;
+        if (y < INFINITY) {
+            const double v = 2.02 * (eps_coef * sqrt(y) + eps_const);
+            out = (float)v;
+            if ((double)out < v) out = nextafterf(out, INFINITY);
+        }
+        margin[j] = out;
+    }
+}
+
 // Candidate lists of layout row j (per_row entries; rows layout rows) -> the point's own row,
 // columns mapped through perm, so the rescore works on the points in their own order and index
 // ties are broken on the points' own indices. Padding rows are dropped.
@@ -1671,13 +1696,13 @@ template <bool BF16, int NKC, int KM>
 static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                            uint32_t d, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
                            const uint32_t *boff, const uint32_t *bcnt, const uint32_t *rbo, const float *seed_s,
-                           uint32_t *cidx, float *cs, hipStream_t s) {
+                           const float *margin_s, uint32_t kk, uint32_t *cidx, float *cs, hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
     const uint32_t row_blocks = grid.x, splits = grid.y;
     const dim3 g1(8 * ((row_blocks * splits + 7) / 8));  // one dimension; the kernel maps it XCD-aware
     hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), g1, dim3(256), lds, s, Xh, Xl, Xf, d, n, npad, cps, row_blocks, splits, blist,
-                       boff, bcnt, rbo, seed_s, cidx, cs);
+                       boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs);
     KERNEL_OK();
 }
 
@@ -1685,11 +1710,11 @@ template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                         uint32_t d, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
                         const uint32_t *boff, const uint32_t *bcnt, const uint32_t *rbo, const float *seed_s,
-                        uint32_t *cidx, float *cs, hipStream_t s) {
+                        const float *margin_s, uint32_t kk, uint32_t *cidx, float *cs, hipStream_t s) {
     switch (KM) {
-        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, cidx, cs, s); break;
-        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, cidx, cs, s); break;
-        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, cidx, cs, s); break;
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs, s); break;
     }
 }
 
@@ -1763,7 +1788,9 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     }
     const std::string tkey = "knn_tile/rb2^" + std::to_string(bits_for(row_blocks)) + "/kc" + std::to_string(nkc) +
                              "/km" + std::to_string(K2) + (bf16 ? "/bf16" : "/fp32") + (prune ? "/pruned" : "");
-    const TuneChoice tc = tune_enabled() && cand.size() > 1 ? tune_pick(tkey, (int)cand.size(), 1) : TuneChoice{0, false};
+    // each candidate is measured twice (the minimum counts): a process's first build also pays
+    // for loading the kernels, which would bias the comparison against the first candidate
+    const TuneChoice tc = tune_enabled() && cand.size() > 1 ? tune_pick(tkey, (int)cand.size(), 2) : TuneChoice{0, false};
     splits = cand[tc.index];
     if (const char *e = std::getenv("SCGPU_KNN_SPLITS")) {  // fixed split count (measurements)
         const uint32_t f = (uint32_t)std::atoi(e);
@@ -1928,6 +1955,11 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         KERNEL_OK();
     }
     const float *sd = prune ? seed_s.get() : nullptr;
+    DevBuf<float> margin_s(rows);
+    hipLaunchKernelGGL(k_knn_margins, dim3(grid_for(rows, 256, 4096)), dim3(256), 0, s, ny.get(), rows, eps_coef,
+                       eps_const, margin_s.get());
+    KERNEL_OK();
+    const float *mg = margin_s.get();
     // candidate lists per layout row (the pruned layout has padding rows)
     const uint32_t crow = prune ? rows : n;
     DevBuf<uint32_t> cidx((size_t)crow * splits * K2);
@@ -1937,14 +1969,14 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const unsigned short *xh = Xh.get(), *xl = Xl.get();
     const float *xf = Xf.get();
     switch (nkc * 2 + (bf16 ? 0 : 1)) {
-        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
-        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, cidx.get(), cs.get(), s); break;
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
     }
     HIP_OK(hipEventRecord(e2, s));
     if (prune) {
