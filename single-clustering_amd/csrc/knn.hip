@@ -854,6 +854,143 @@ __global__ void k_kmeans_norms(const float *cen, uint32_t d, uint32_t C, float *
     }
 }
 
+// Centroids as the matrix cores' B rows for the assignment below (d + 2 <= 32): split-BF16
+// coordinates (hi in cb_h, lo in cb_l) and the f32 squared norm split into slots d (hi) and
+// d + 1 (lo) of cb_h; rows C .. Cpad - 1 are padding whose norm is +inf (never nearest).
+__global__ void k_kmeans_pack(const float *cen, uint32_t d, uint32_t C, uint32_t Cpad, unsigned short *cb_h,
+                              unsigned short *cb_l) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < Cpad; c += gridDim.x * blockDim.x) {
+        float q = 0.0f;
+        for (uint32_t t = 0; t < 32; t++) {
+            unsigned short h = 0, l = 0;
+            if (c < C && t < d) {
+                const float v = cen[(uint64_t)c * d + t];
+                q = fmaf(v, v, q);
+                h = bf16_bits_rn(v);
+                l = bf16_bits_rn(v - bf16_bits_to_f32(h));
+            }
+            cb_h[(uint64_t)c * 32 + t] = h;
+            cb_l[(uint64_t)c * 32 + t] = l;
+        }
+        if (c < C) {
+            const unsigned short qh = bf16_bits_rn(q);
+            cb_h[(uint64_t)c * 32 + d] = qh;
+            cb_h[(uint64_t)c * 32 + d + 1] = bf16_bits_rn(q - bf16_bits_to_f32(qh));
+        } else {
+            cb_h[(uint64_t)c * 32 + d] = 0x7F80u;  // +inf
+        }
+    }
+}
+
+// Nearest centroid of every point (d + 2 <= 32) by |c|^2 - 2 x.c on the matrix cores: the point
+// as split-BF16 (-2 x, 1, 1) against the centroid (c, |c|^2 hi, |c|^2 lo), three products per
+// 16 x 16 block (x_h c_h + x_h c_l + x_l c_h), FP32 accumulation; ties to the lower centroid.
+// The assignment only groups points for the pruned search (any assignment gives the same kNN
+// result; the groups' spheres are computed from the points each group holds), so the products'
+// rounding may pick a near-tied centroid. A wave holds 64 points as four row tiles; 64
+// centroids at a time pass through LDS; each lane keeps, for its column slot, the best value and
+// centroid of its 16 (row tile, row) pairs, reduced across the 16 column slots at the end.
+__global__ void __launch_bounds__(256) k_kmeans_assign_mfma(const float *X, uint32_t n, uint32_t d,
+                                                            const unsigned short *cb_h, const unsigned short *cb_l,
+                                                            uint32_t Cpad, uint32_t *assign) {
+    constexpr int STR = 40;  // staged centroid stride (bf16 elements)
+    __shared__ __attribute__((aligned(16))) unsigned short sBh[64 * STR], sBl[64 * STR];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, quarter = lane >> 4;
+    for (uint64_t r0 = (uint64_t)blockIdx.x * 256; r0 < n; r0 += (uint64_t)gridDim.x * 256) {  // workgroup-uniform
+        const uint64_t wrow0 = r0 + (uint64_t)wave * 64;
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            const uint64_t arow = wrow0 + rt * 16 + (lane & 15);
+            union {
+                bf16x8 v;
+                unsigned short u[8];
+            } h, l;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const uint32_t t = 8 * quarter + e;
+                float xv = 0.0f;
+                if (arow < n && t < d) xv = X[arow * d + t];
+                if (t < d) {
+                    const unsigned short xh = bf16_bits_rn(xv);
+                    const unsigned short xl = bf16_bits_rn(xv - bf16_bits_to_f32(xh));
+                    h.u[e] = bf16_bits_rn(-2.0f * bf16_bits_to_f32(xh));  // exact: scaling by -2
+                    l.u[e] = bf16_bits_rn(-2.0f * bf16_bits_to_f32(xl));
+                } else {
+                    h.u[e] = t < d + 2 ? (unsigned short)0x3F80u : (unsigned short)0;  // 1.0 at the norm slots
+                    l.u[e] = 0;
+                }
+            }
+            ah[rt] = h.v;
+            al[rt] = l.v;
+        }
+        float bv[4][4];
+        uint32_t bi[4][4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                bv[rt][i] = INFINITY;
+                bi[rt][i] = 0xFFFFFFFFu;
+            }
+        for (uint32_t c0 = 0; c0 < Cpad; c0 += 64) {
+            __syncthreads();
+            {
+                // 64 centroids x 32 values x 2 arrays = 512 chunks of 8 values, two per thread
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const int ch = tid + 256 * q;
+                    const int arr = ch >> 8, r = (ch >> 2) & 63, part = ch & 3;
+                    const unsigned short *src = (arr ? cb_l : cb_h) + (uint64_t)(c0 + r) * 32 + part * 8;
+                    unsigned short *dst = (arr ? sBl : sBh) + r * STR + part * 8;
+                    *reinterpret_cast<uint4 *>(dst) = *reinterpret_cast<const uint4 *>(src);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                const int off = (ct * 16 + (lane & 15)) * STR + 8 * quarter;
+                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
+                const uint32_t col = c0 + ct * 16 + (lane & 15);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) {
+                    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, acc, 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        if (acc[i] < bv[rt][i]) {  // columns arrive in increasing order: ties keep the lower
+                            bv[rt][i] = acc[i];
+                            bi[rt][i] = col;
+                        }
+                }
+            }
+        }
+        // per row: the best over the 16 column slots (lanes with the same quarter), ties to the
+        // lower centroid
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float v = bv[rt][i];
+                uint32_t c = bi[rt][i];
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1) {
+                    const float ov = __shfl_xor(v, off, 16);
+                    const uint32_t oc = (uint32_t)__shfl_xor((int)c, off, 16);
+                    if (ov < v || (ov == v && oc < c)) {
+                        v = ov;
+                        c = oc;
+                    }
+                }
+                const uint64_t row = wrow0 + rt * 16 + 4 * quarter + i;
+                if ((lane & 15) == 0 && row < n) assign[row] = c;
+            }
+    }
+}
+
 // Nearest centroid of every point by |c|^2 - 2 x.c (f32; ties to the lower centroid). The
 // centroids pass through LDS in chunks of KMC, transposed so a thread reads one value per FMA.
 __global__ void __launch_bounds__(256) k_kmeans_assign(const float *X, uint32_t n, uint32_t d, const float *cen,
@@ -1630,7 +1767,15 @@ static uint32_t knn_group_order(const float *d_X, uint32_t n, uint32_t d, DevBuf
         hipLaunchKernelGGL(k_kmeans_norms, dim3(grid_for(C, 256, 1024)), dim3(256), 0, s, cen.get(), d, C, cnorm.get());
         KERNEL_OK();
         const dim3 ag(grid_for(np, 256, 8192));
-        if (d <= 32)
+        if (d + 2 <= 32) {
+            const uint32_t Cpad = (C + 63) / 64 * 64;
+            DevBuf<unsigned short> cbh((size_t)Cpad * 32), cbl((size_t)Cpad * 32);
+            hipLaunchKernelGGL(k_kmeans_pack, dim3(grid_for(Cpad, 256, 1024)), dim3(256), 0, s, cen.get(), d, C, Cpad,
+                               cbh.get(), cbl.get());
+            KERNEL_OK();
+            hipLaunchKernelGGL(k_kmeans_assign_mfma, dim3(grid_for(np, 256, 4096)), dim3(256), 0, s, P, np, d,
+                               cbh.get(), cbl.get(), Cpad, keys.get());
+        } else if (d <= 32)
             hipLaunchKernelGGL(k_kmeans_assign_r<32>, ag, dim3(256), 0, s, P, np, d, cen.get(), cnorm.get(), C, keys.get());
         else if (d <= 56)
             hipLaunchKernelGGL(k_kmeans_assign_r<56>, ag, dim3(256), 0, s, P, np, d, cen.get(), cnorm.get(), C, keys.get());
