@@ -1537,6 +1537,188 @@ __global__ void __launch_bounds__(256) k_list_filter(const float *X, const uint3
     }
 }
 
+// The column blocks' centres as the matrix cores' B rows (d + 2 <= 32): split-BF16 cB32 (hi in
+// cbh, lo in cbl) with the f32 squared norm nb2 split into slots d (hi) and d + 1 (lo) of cbh.
+__global__ void k_block_pack(const float *cB32, const float *nb2, uint32_t ncb, uint32_t d, unsigned short *cbh,
+                             unsigned short *cbl) {
+    for (uint32_t B = blockIdx.x * blockDim.x + threadIdx.x; B < ncb; B += gridDim.x * blockDim.x) {
+        for (uint32_t t = 0; t < 32; t++) {
+            unsigned short h = 0, l = 0;
+            if (t < d) {
+                const float v = cB32[(uint64_t)B * d + t];
+                h = bf16_bits_rn(v);
+                l = bf16_bits_rn(v - bf16_bits_to_f32(h));
+            }
+            cbh[(uint64_t)B * 32 + t] = h;
+            cbl[(uint64_t)B * 32 + t] = l;
+        }
+        const float q = nb2[B];
+        const unsigned short qh = bf16_bits_rn(q);
+        cbh[(uint64_t)B * 32 + d] = qh;
+        cbh[(uint64_t)B * 32 + d + 1] = bf16_bits_rn(q - bf16_bits_to_f32(qh));
+    }
+}
+
+// The per-row list filter of k_list_filter (same decision rule) for d + 2 <= 32, with the
+// squared distances on the matrix cores: s = |cB32|^2 - 2 x.cB32 as split-BF16 products of the
+// row (-2 x, 1, 1) and the centre (cB32, |cB32|^2 hi, lo), FP32 accumulation. Its error against
+// the exact value is below e1 |x| |c| + e2 |c|^2 (the split truncation and the accumulation of
+// the 3 x 32 + 2 products, the f32 norm and its split; the same terms as the tile's bound, per
+// row and centre instead of with the largest norm). |x - cB32|^2 = |x|^2 + s is bounded below by
+// the row's f32 |x|^2 less its rounding, plus s, less that error, less the rounding of those f32
+// additions; the root's rounding, the centre's rounding and the radius are handled as in
+// k_list_filter. A wave holds 64 rows as four row tiles; 64 listed blocks at a time are staged
+// in LDS, read before any of them is written back.
+__global__ void __launch_bounds__(256) k_list_filter_mfma(const float *X, const uint32_t *layout, uint32_t d,
+                                                          uint32_t nrb, const double *seed_d2, const unsigned short *cbh,
+                                                          const unsigned short *cbl, const float *rBf, const float *nbn,
+                                                          const uint32_t *lbeg, const uint32_t *lcnt, uint32_t *blist,
+                                                          uint32_t *lkept, float e1, float e2) {
+    constexpr int STR = 40;
+    __shared__ __attribute__((aligned(16))) unsigned short sBh[64 * STR], sBl[64 * STR];
+    __shared__ float sr[64], sn[64];
+    __shared__ uint32_t sB[64];
+    __shared__ float s_lox[256], s_ex[256], s_ux[256];  // per row: |x|^2 lower bound, e1 |x|, u_x
+    __shared__ uint32_t sf[4][2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, quarter = lane >> 4;
+    constexpr float u24 = 1.0f / 16777216.0f;
+    for (uint32_t R = blockIdx.x; R < nrb; R += gridDim.x) {  // workgroup-uniform
+        // this thread's row: its norm and bound; a padding row gets u_x = -inf, so that it
+        // needs no block (x > -inf holds for the finite left side of the test)
+        bool norow;
+        {
+            const uint64_t j = (uint64_t)R * KNN_BM + tid;
+            const uint32_t y = layout[j];
+            const bool real = y != KNONE;
+            float xn2 = 0.0f;
+            for (uint32_t t = 0; real && t < d; t++) {
+                const float v = X[(uint64_t)y * d + t];
+                xn2 = fmaf(v, v, xn2);
+            }
+            const double sd = real ? seed_d2[j] : 0.0;
+            norow = real && !(sd < INFINITY);
+            // the f32 sum of d squares is within 1.01 d 2^-24 |x|^2 of |x|^2; the factor's own
+            // rounding and the product's take at most 2 more units
+            s_lox[tid] = xn2 * (1.0f - (1.01f * (float)d + 6.0f) * u24);
+            s_ex[tid] = e1 * sqrtf(xn2) * (1.0f + 4.0f * u24);
+            s_ux[tid] = real ? f32_up(sqrt(sd)) : -INFINITY;
+        }
+        const uint32_t L = lcnt[R], base = lbeg[R];
+        if (__syncthreads_or(norow)) {
+            if (tid == 0) lkept[R] = L;
+            continue;  // workgroup-uniform: a row without a bound keeps the whole list
+        }
+        // A fragments of the wave's four row tiles
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            const uint64_t j = (uint64_t)R * KNN_BM + wave * 64 + rt * 16 + (lane & 15);
+            const uint32_t y = layout[j];
+            union {
+                bf16x8 v;
+                unsigned short u[8];
+            } h, l;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const uint32_t t = 8 * quarter + e;
+                if (t < d) {
+                    const float xv = y != KNONE ? X[(uint64_t)y * d + t] : 0.0f;
+                    const unsigned short xh = bf16_bits_rn(xv);
+                    const unsigned short xl = bf16_bits_rn(xv - bf16_bits_to_f32(xh));
+                    h.u[e] = bf16_bits_rn(-2.0f * bf16_bits_to_f32(xh));  // exact: scaling by -2
+                    l.u[e] = bf16_bits_rn(-2.0f * bf16_bits_to_f32(xl));
+                } else {
+                    h.u[e] = t < d + 2 ? (unsigned short)0x3F80u : (unsigned short)0;  // 1.0 at the norm slots
+                    l.u[e] = 0;
+                }
+            }
+            ah[rt] = h.v;
+            al[rt] = l.v;
+        }
+        // this lane's 16 rows (row tile rt, value i: row 16 rt + 4 quarter + i of the wave)
+        float lox[4][4], ex[4][4], ux[4][4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = wave * 64 + rt * 16 + 4 * quarter + i;
+                lox[rt][i] = s_lox[r];
+                ex[rt][i] = s_ex[r];
+                ux[rt][i] = s_ux[r];
+            }
+        uint32_t kept = 0;  // wave 0's count (the only wave that writes)
+        for (uint32_t c0 = 0; c0 < L; c0 += 64) {
+            const uint32_t cn = min(64u, L - c0);
+            __syncthreads();
+            if (tid < 64) {
+                const uint32_t B = tid < (int)cn ? blist[base + c0 + tid] : 0u;
+                sB[tid] = B;
+                sr[tid] = tid < (int)cn ? rBf[B] : INFINITY;
+                sn[tid] = tid < (int)cn ? nbn[B] : 0.0f;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int ch = tid + 256 * q;
+                const int arr = ch >> 8, r = (ch >> 2) & 63, part = ch & 3;
+                const unsigned short *src = (arr ? cbl : cbh) + (uint64_t)sB[r] * 32 + part * 8;
+                unsigned short *dst = (arr ? sBl : sBh) + r * STR + part * 8;
+                *reinterpret_cast<uint4 *>(dst) = *reinterpret_cast<const uint4 *>(src);
+            }
+            __syncthreads();
+            uint64_t keep = 0;  // bit c: some row of this wave needs block c (wave-uniform)
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                const int col = ct * 16 + (lane & 15);
+                const int off = col * STR + 8 * quarter;
+                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
+                const float cn_ = sn[col] * (1.0f + 4.0f * u24);  // |cB32|, rounded up
+                const float e2c = e2 * cn_ * cn_ * (1.0f + 4.0f * u24);
+                const float rc = sr[col];
+                bool need = false;
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) {
+                    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, acc, 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float err = fmaf(ex[rt][i], cn_, e2c) * (1.0f + 4.0f * u24);
+                        const float sv = acc[i];
+                        // lower bound of |x - cB32|^2; the three f32 operations' rounding taken off
+                        const float lo2 = (lox[rt][i] + sv - err) - 4.0f * u24 * (lox[rt][i] + fabsf(sv) + err);
+                        need |= !(sqrtf(fmaxf(lo2, 0.0f)) * (1.0f - 1e-6f) - 1e-6f * cn_ >
+                                  (rc + ux[rt][i]) * (1.0f + 1e-6f));
+                    }
+                }
+                // the column's four quarters (lanes col, col + 16, col + 32, col + 48)
+                const uint64_t m = __ballot(need);
+                const uint32_t m16 = (uint32_t)(m | (m >> 16) | (m >> 32) | (m >> 48)) & 0xFFFFu;
+                keep |= (uint64_t)m16 << (16 * ct);
+            }
+            if (lane == 0) {
+                sf[wave][0] = (uint32_t)keep;
+                sf[wave][1] = (uint32_t)(keep >> 32);
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t lo = sf[0][0] | sf[1][0] | sf[2][0] | sf[3][0];
+                const uint32_t hi = sf[0][1] | sf[1][1] | sf[2][1] | sf[3][1];
+                const bool k = (uint32_t)lane < cn && (((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1u) != 0u);
+                const uint64_t mk = __ballot(k);
+                if (k)
+                    blist[base + kept +
+                          __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] = sB[lane];
+                kept += (uint32_t)__popcll(mk);
+            }
+        }
+        if (tid == 0) lkept[R] = kept;
+        __syncthreads();
+    }
+}
+
 // Sort keys of the longest-list-first order: the complement of the list length, and the row block.
 __global__ void k_lpt_keys(const uint32_t *bcnt, uint32_t nrb, uint32_t *keys, uint32_t *ids) {
     for (uint32_t R = blockIdx.x * blockDim.x + threadIdx.x; R < nrb; R += gridDim.x * blockDim.x) {
@@ -2042,7 +2224,20 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
             hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks, seed_d2.get(), cB32.get(),
                                rBf.get(), nbn.get(), nb2.get(), boff.get(), cnt.get(), blist.get(), bcnt.get());
         };
-        if (d <= 32) lfilt(k_list_filter<32>);
+        if (d + 2 <= 32) {
+            // the matrix-core filter: its error coefficients follow the tile's bound (see there)
+            const double u24 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -16);
+            const double acc_c = 3.03 * (3.0 * 32 + 2.0) * u24, split_c = 3.01 * u16;
+            const float e1 = (float)(2.0 * 2.0 * (split_c + acc_c) * 1.001);
+            const float e2 = (float)(2.0 * (1.01 * d * u24 + u16 + acc_c) * 1.001);
+            DevBuf<unsigned short> cbh((size_t)ncb * 32), cbl((size_t)ncb * 32);
+            hipLaunchKernelGGL(k_block_pack, dim3(grid_for(ncb, 256, 4096)), dim3(256), 0, s, cB32.get(), nb2.get(),
+                               ncb, d, cbh.get(), cbl.get());
+            KERNEL_OK();
+            hipLaunchKernelGGL(k_list_filter_mfma, kg, dim3(256), 0, s, d_X, layout.get(), d, row_blocks,
+                               seed_d2.get(), cbh.get(), cbl.get(), rBf.get(), nbn.get(), boff.get(), cnt.get(),
+                               blist.get(), bcnt.get(), e1, e2);
+        } else if (d <= 32) lfilt(k_list_filter<32>);
         else if (d <= 64) lfilt(k_list_filter<64>);
         else lfilt(k_list_filter<128>);
         KERNEL_OK();
