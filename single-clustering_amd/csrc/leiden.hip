@@ -569,6 +569,114 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
     group_sync<G, GTAB>();
 }
 
+// Refinement's first step, in which every vertex is still a singleton (memb[u] = u, size 1, totals
+// its own): every neighbour in the working community is a candidate of its own with the entry's
+// weight (a row holds each neighbour once), so no table is built and no membership is gathered.
+// Candidate set, scores and tie order are decide_unit's, so the decisions are the same. sel holds
+// the step's vertices (list positions for the global-table bucket).
+template <int BS, int G, bool GTAB, bool BATCH>
+__device__ __forceinline__ void decide_unit_single(const DecideArgs &a, const StepParams &sp, const uint32_t *blist,
+                                                   const uint32_t *sel, uint32_t len, uint64_t base, double *r_s,
+                                                   uint32_t *r_c, long long *r_w, DecideStats &ds) {
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const uint64_t idx = base + grp;
+    const bool valid = idx < len;
+    const uint32_t pos = valid ? sel[idx] : 0u;
+    const uint32_t v = valid ? (GTAB ? blist[pos] : pos) : 0u;
+    const VParams vp = vparams<BATCH>(a, v);
+    double bs = -INFINITY;
+    uint32_t bc = NONE;
+    long long wown = 0;
+    uint64_t beg = 0, end = 0;
+    if (valid) {
+        beg = a.ptrs[v];
+        end = a.ptrs[v + 1];
+        const uint32_t cv = a.constraint[v];
+        const double k = a.strength[v], na = a.node_w[v], loop = a.self_loop[v];
+        // the node leaves its singleton, which becomes empty (refine.rs:71-74)
+        const double rbase = delta_insert(a.objective, vp, 0.0, loop, k, na, 0.0, 0.0);
+        constexpr int EB = 4;
+        for (uint64_t e0 = beg + lane; e0 < end; e0 += (uint64_t)EB * G) {
+            uint32_t u[EB];
+            float w[EB];
+#pragma unroll
+            for (int j = 0; j < EB; j++) {
+                const uint64_t e = e0 + (uint64_t)j * G;
+                const bool in = e < end;
+                u[j] = in ? a.nbrs[e] : v;  // v: no entry (self-loops are the partition's)
+                w[j] = in ? a.wts[e] : 0.0f;
+            }
+            bool ok[EB];
+#pragma unroll
+            for (int j = 0; j < EB; j++) ok[j] = u[j] != v && a.constraint[u[j]] == cv;
+            long long Wv[EB], Kv[EB];
+#pragma unroll
+            for (int j = 0; j < EB; j++) {
+                const uint32_t ci = ok[j] ? u[j] : v;
+                Wv[j] = (a.objective != 0 || a.has_max) ? a.Wfx[ci] : 0;
+                Kv[j] = a.objective == 0 ? a.Kfx[ci] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < EB; j++) {
+                if (!ok[j]) continue;
+                ds.c++;
+                const double Wc = from_fx(Wv[j], vp.sW);
+                if (a.has_max && !(Wc + na <= a.max_w)) continue;
+                const double Kc = a.objective == 0 ? from_fx(Kv[j], vp.sK) : 0.0;
+                const double gain =
+                    delta_insert(a.objective, vp, from_fx(to_fx((double)w[j], vp.sw), vp.sw), loop, k, na, Kc, Wc);
+                const double rel = gain - rbase;
+                if (!(rel >= 0.0)) continue;  // candidates that do not lose quality (refine.rs:91-95)
+                const double score =
+                    vp.theta > 0.0 ? rel / vp.theta + gumbel(sp.seed, sp.round, v - vp.lo, u[j] - vp.lo) : rel;
+                better(bs, bc, score, u[j]);
+            }
+        }
+    }
+    if (G <= 64) {
+        group_reduce_wave<(G <= 64 ? G : 64)>(bs, bc, wown);
+    } else {
+        group_reduce_wave<64>(bs, bc, wown);
+        const int wave = threadIdx.x / 64;
+        if ((threadIdx.x & 63) == 0) {
+            r_s[wave] = bs;
+            r_c[wave] = bc;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            bs = r_s[0];
+            bc = r_c[0];
+            for (int w = 1; w < BS / 64; w++) better(bs, bc, r_s[w], r_c[w]);
+        }
+    }
+    bool push = false;
+    if (valid && lane == 0) {
+        a.active[v] = 0;
+        ds.v++;
+        ds.e += end - beg;
+        if (bc != NONE) {
+            a.target[v] = bc;
+            a.tround[v] = sp.round;
+            atomicAdd(&a.targeted_cnt[bc], 1u);
+            push = true;
+        }
+    }
+    const uint64_t pm = __ballot(push);
+    if (pm) {
+        const int leader = __ffsll((unsigned long long)pm) - 1;
+        unsigned int slot0 = 0;
+        if ((int)(threadIdx.x & 63) == leader) slot0 = atomicAdd(sp.n_movers, (unsigned int)__popcll(pm));
+        slot0 = __shfl(slot0, leader);
+        if (push) {
+            const unsigned int slot = slot0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+            (sp.mv_list ? sp.mv_list : a.movers)[slot] = v;
+        }
+    }
+    if (G > 64) __syncthreads();  // r_s is rewritten by the next unit
+}
+
 __device__ __forceinline__ void flush_decide_stats(const DecideStats &ds, unsigned long long *s_stats,
                                                    unsigned long long *stats) {
     if (ds.v) {
@@ -1411,6 +1519,35 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
 #undef SC_DECIDE_BUCKET
 }
 
+// sweep_decide for refinement's first step (every vertex a singleton): decide_unit_single over
+// the workgroup's share of each bucket, in the buckets' group shapes.
+template <bool BATCH>
+__device__ __forceinline__ void sweep_decide_single(const SweepArgs &A, const StepParams &sp, const unsigned int *s_wc,
+                                                    double *r_s, uint32_t *r_c, long long *r_w, DecideStats &ds) {
+#define SC_DECIDE_SINGLE(BK)                                                                                 \
+    {                                                                                                        \
+        constexpr int G = BucketShape<BK>::G, NG = SWEEP_BS / G;                                             \
+        const uint32_t len = s_wc[BK];                                                                       \
+        if (len) {                                                                                           \
+            uint32_t lo, hi;                                                                                 \
+            bucket_share(A, BK, lo, hi);                                                                     \
+            for (uint64_t base = 0; base < len; base += NG)                                                  \
+                decide_unit_single<SWEEP_BS, G, BK == GTAB_BUCKET, BATCH>(A.d, sp, A.lists + A.bstart[BK],    \
+                                                                          A.work + lo, len, base, r_s, r_c,   \
+                                                                          r_w, ds);                          \
+        }                                                                                                    \
+    }
+    SC_DECIDE_SINGLE(0)
+    SC_DECIDE_SINGLE(1)
+    SC_DECIDE_SINGLE(2)
+    SC_DECIDE_SINGLE(3)
+    SC_DECIDE_SINGLE(4)
+    SC_DECIDE_SINGLE(5)
+    SC_DECIDE_SINGLE(6)
+    SC_DECIDE_SINGLE(7)
+#undef SC_DECIDE_SINGLE
+}
+
 // A workgroup's mover totals, summed per community in LDS before they go to global memory:
 // late steps send many movers into the same few large communities, and same-address device
 // atomics serialize. AGG_SLOTS keys with K, W and size sums (integers, so the totals do not
@@ -1459,6 +1596,24 @@ __device__ __forceinline__ void agg_move(const SweepArgs &A, const AggTable &t, 
     const long long wf = (A.d.objective != 0 || A.d.has_max) ? to_fx(A.d.node_w[v], vp.sW) : 0;
     agg_add(A, t, to, kf, wf, 1);
     agg_add(A, t, from, -kf, -wf, -1);
+}
+
+// agg_move without the table: the totals of both communities by device atomics
+template <bool BATCH>
+__device__ __forceinline__ void direct_move(const SweepArgs &A, uint64_t v, uint32_t from, uint32_t to) {
+    const VParams vp = vparams<BATCH>(A.d, (uint32_t)v);
+    if (A.d.objective == 0) {
+        const long long kf = to_fx(A.d.strength[v], vp.sK);
+        atomicAdd((unsigned long long *)&A.Kfx[to], (unsigned long long)kf);
+        atomicAdd((unsigned long long *)&A.Kfx[from], (unsigned long long)-kf);
+    }
+    if (A.d.objective != 0 || A.d.has_max) {
+        const long long wf = to_fx(A.d.node_w[v], vp.sW);
+        atomicAdd((unsigned long long *)&A.Wfx[to], (unsigned long long)wf);
+        atomicAdd((unsigned long long *)&A.Wfx[from], (unsigned long long)-wf);
+    }
+    atomicAdd(&A.csize[to], 1u);
+    atomicSub(&A.csize[from], 1u);
 }
 
 __device__ __forceinline__ void agg_flush(const SweepArgs &A, const AggTable &t) {
@@ -1522,6 +1677,7 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         a.active[v] = 1;
         if (REFINE) {
             A.memb[v] = (uint32_t)v;
+            A.mstep[v] = NONE;
             a.tround[v] = NONE;
             a.targeted_cnt[v] = 0;
         } else {
@@ -1661,7 +1817,8 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
             // compact and decide the workgroup's own share of every bucket
             sweep_compact<REFINE, FILTER, BATCH>(A, seed, sub, it, filter_on, s_wc);
             StepParams sp{seed, sub, it, step, &s_mc, REFINE ? nullptr : A.mstep, mv_list, REFINE ? nullptr : mv_tgt};
-            sweep_decide<REFINE, BATCH>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
+            if (REFINE && step == 0) sweep_decide_single<BATCH>(A, sp, s_wc, r_s, r_c, r_w, ds);
+            else sweep_decide<REFINE, BATCH>(A, sp, s_wc, s_keys, s_vals, r_s, r_c, r_w, ds);
             __syncthreads();
             const unsigned int nm = s_mc;
             if (!REFINE && mv_tgt) {
@@ -1746,8 +1903,12 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 return !a.jsum || join_admitted(a, vparams<BATCH>(a, (uint32_t)v), (uint32_t)v, (uint32_t)t, step & 1u);
             };
             unsigned int applied = 0;
+            // more movers than table slots (early steps, whose targets are mostly distinct): the
+            // table would only add probes, so the totals take direct atomics (integer sums: the
+            // same totals either way)
+            const bool direct = nm > 2 * AGG_SLOTS;
             const AggTable agg{s_keys, s_vals, s_vals + AGG_SLOTS, s_vals + 2 * AGG_SLOTS};
-            agg_clear(agg);  // the decide tables are free: every wave has left the decide phase
+            if (!direct) agg_clear(agg);  // the decide tables are free: every wave has left the decide phase
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < nm; i += SWEEP_BS) {
                 const uint64_t v = mv_list[i];
@@ -1757,27 +1918,15 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 } else {
                     const uint64_t from = REFINE ? v : a.memb[v];
                     A.memb[v] = (uint32_t)t;
-                    agg_move<BATCH>(A, agg, v, (uint32_t)from, (uint32_t)t);
+                    if (REFINE) A.mstep[v] = step;  // read by the re-activation scan below
+                    if (direct) direct_move<BATCH>(A, v, (uint32_t)from, (uint32_t)t);
+                    else agg_move<BATCH>(A, agg, v, (uint32_t)from, (uint32_t)t);
                     applied++;
                     if (FILTER && it == judge_it && A.f_mark[v]) atomicAdd(&A.fstats[4], 1ull);
                 }
             }
             __syncthreads();
-            agg_flush(A, agg);
-            if (REFINE) {
-                // refinement: neighbours in the same working community (refine.rs), SWEEP_ACT_G
-                // lanes per mover; the admission test reads only what apply leaves unchanged
-                for (uint32_t i = gslot; i < nm; i += NGA) {
-                    const uint64_t v = mv_list[i];
-                    const uint64_t t = a.target[v];
-                    if (!admitted(v, t)) continue;
-                    const uint32_t mv = a.constraint[v];
-                    for (uint64_t e = a.ptrs[v] + lane; e < a.ptrs[v + 1]; e += SWEEP_ACT_G) {
-                        const uint32_t u = a.nbrs[e];
-                        if (u != v && a.constraint[u] == mv) a.active[u] = 1;
-                    }
-                }
-            }
+            if (!direct) agg_flush(A, agg);
             if (REFINE) {
                 const unsigned long long moved_here = block_sum((unsigned long long)applied);
                 if (threadIdx.x == 0) {
@@ -1786,6 +1935,25 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
                 }
             }
             if (!(ok = grid_barrier(A.bc, bst, &s_flag, A.trace, A.trace_cap))) break;
+            if (REFINE && ld_acquire_ctr(A.ccnt + step) != 0u) {
+                // refinement re-activates the singletons that have a neighbour in their working
+                // community that moved in this step (refine.rs: only singletons move); gathered
+                // from the singletons' side: after the first steps few singletons are left, and a
+                // vertex that is not a singleton never becomes one again. Each lane takes one
+                // vertex; the step's next barrier orders these flags before compaction reads them.
+                for (uint64_t u = gtid; u < n; u += gsize) {
+                    if (a.active[u] || A.memb[u] != (uint32_t)u || A.csize[u] != 1u) continue;
+                    const uint32_t cu = a.constraint[u];
+                    const uint64_t e1 = a.ptrs[u + 1];
+                    for (uint64_t e = a.ptrs[u]; e < e1; e++) {
+                        const uint32_t w = a.nbrs[e];
+                        if (w != (uint32_t)u && A.mstep[w] == step && a.constraint[w] == cu) {
+                            a.active[u] = 1;
+                            break;
+                        }
+                    }
+                }
+            }
             if (!REFINE) {
                 // local moving: after all moves landed, a mover that moved re-activates its
                 // neighbours outside its new community (local_move.rs:114-118)
