@@ -261,36 +261,48 @@ __device__ __forceinline__ void knn_subtile_mfma(const unsigned short *sBh, cons
 // Threshold test of one sub-tile's values in the MFMA layout. A hit's buffer slot is its row's
 // count plus the number of hits of lower lanes of the same row (16-lane group) in the ballot.
 // Byte i of cnt4[rt] counts the buffered hits of row 16 rt + i of this lane's quarter.
-// Returns true when some row could overflow its buffer on the next sub-tile.
+// Returns true when some row could overflow its buffer on the next sub-tile. c0s: the sub-tile's
+// first column.
 template <int BCAP>
-__device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], uint32_t col, uint32_t row0,
+__device__ __forceinline__ bool knn_subtile_hits(const f32x4 (&acc)[4], uint32_t c0s, uint32_t row0,
                                                  uint32_t rl_base, uint64_t qmask, const float (&thr_r)[4][4],
                                                  uint32_t (&cnt4)[4], uint2 *sBuf) {
-    // hits are rare once the thresholds have settled: one wave-wide test of all 16 values first
-    // (one branch per sub-tile instead of one per value), the per-value pass only when some
-    // lane has a hit
-    bool any = false;
+    // hits are rare once the thresholds have settled: the 16 compare masks (one per value of
+    // the lane's rows, wave-wide) are taken once; a value whose mask is empty costs no more
+    // vector work, and a sub-tile without any hit returns after the masks
+    uint64_t m[4][4];
+    uint64_t anym = 0;
 #pragma unroll
     for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) any |= acc[rt][i] < thr_r[rt][i];
-    if (!__ballot(any)) return false;
+        for (int i = 0; i < 4; i++) {
+            m[rt][i] = __ballot(acc[rt][i] < thr_r[rt][i]);
+            anym |= m[rt][i];
+        }
+    if (!anym) return false;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t col = c0s + (lane & 15u);
+    // a row never lists itself: only a sub-tile that meets the block's own rows can hold one
+    const bool selfp = c0s < row0 + KNN_BM && c0s + 16u > row0;
     bool need = false;
 #pragma unroll
     for (int rt = 0; rt < 4; rt++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float sv = acc[rt][i];  // |y|^2 - 2 x.y (the norm is folded into the product)
-            bool hit = sv < thr_r[rt][i];
-            if (__ballot(hit)) {
+            uint64_t mm = m[rt][i];
+            if (mm) {
                 const uint32_t rl = rl_base + rt * 16 + i;
-                hit = hit && col != row0 + rl;
-                const uint64_t gm = __ballot(hit) & qmask;
+                bool hit = acc[rt][i] < thr_r[rt][i];  // this lane's bit of mm
+                if (selfp) {
+                    hit = hit && col != row0 + rl;
+                    mm = __ballot(hit);
+                }
+                const uint64_t gm = mm & qmask;
                 const uint32_t below =
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(gm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)gm, 0u));
                 const uint32_t c = (cnt4[rt] >> (8 * i)) & 0xFFu;
                 const uint32_t add = (uint32_t)__popcll(gm);
-                if (hit) sBuf[rl * BCAP + c + below] = make_uint2(__float_as_uint(sv), col);
+                if (hit) sBuf[rl * BCAP + c + below] = make_uint2(__float_as_uint(acc[rt][i]), col);
                 cnt4[rt] += add << (8 * i);
                 need |= c + add > (uint32_t)(BCAP - 16);
             }
@@ -515,14 +527,12 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         for (int sub = 0; sub < TN / 16; sub += 2) {
             knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 1) * (16u * STRIDE) + lane_base, ah,
                                                     al, af, acc1);
-            bool need = knn_subtile_hits<BCAP>(acc0, c0 + sub * 16 + (lane & 15), row0, rl_base, qmask, thr_r, cnt4,
-                                               sBuf);
+            bool need = knn_subtile_hits<BCAP>(acc0, c0 + sub * 16, row0, rl_base, qmask, thr_r, cnt4, sBuf);
             if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
             if (sub + 2 < TN / 16)
                 knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
                                                         ah, al, af, acc0);
-            need = knn_subtile_hits<BCAP>(acc1, c0 + (sub + 1) * 16 + (lane & 15), row0, rl_base, qmask, thr_r, cnt4,
-                                          sBuf);
+            need = knn_subtile_hits<BCAP>(acc1, c0 + (sub + 1) * 16, row0, rl_base, qmask, thr_r, cnt4, sBuf);
             if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
         }
     }
