@@ -606,7 +606,6 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
             jv[h] = e < total ? cand_idx[base + e] : KNONE;
             sv[h] = jv[h] != KNONE ? cand_s[base + e] : INFINITY;
             sS[wave][e] = sv[h];
-            sI[wave][e] = jv[h];
         }
         // a split whose list is full (its last entry is a point) may hide closer points; the
         // smallest last value over those splits bounds every point not in the lists. A list that
@@ -630,11 +629,32 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
         const double eps = eps_coef * sqrt(nx64[i]) + eps_const;
         double cut = INFINITY;
         if (cnt > k) {
+            // each split's list is sorted (ascending values, empty entries last), so a candidate's
+            // rank by (value, list position) is its position in its own list plus, per other list,
+            // the entries before it: values at or below it in an earlier list, below it in a later
+            // one (a binary search of KM entries, KM a power of two)
             uint32_t rank[2] = {0u, 0u};
-            for (uint32_t e = 0; e < total; e++) {
-                const float v = sS[wave][e];
 #pragma unroll
-                for (int h = 0; h < 2; h++) rank[h] += (v < sv[h] || (v == sv[h] && e < (uint32_t)lane + 64u * h)) ? 1u : 0u;
+            for (int h = 0; h < 2; h++) {
+                const uint32_t e = (uint32_t)lane + 64u * h;
+                if (e >= total) continue;
+                const uint32_t sp = e / KM;
+                const float x = sv[h];
+                uint32_t r = e - sp * KM;
+                for (uint32_t s2 = 0; s2 < splits; s2++) {
+                    if (s2 == sp) continue;
+                    const float *L = &sS[wave][s2 * KM];
+                    const bool le = s2 < sp;
+                    uint32_t lo = 0;
+                    for (uint32_t step = KM >> 1; step > 0; step >>= 1) {
+                        const float y = L[lo + step - 1];
+                        if (le ? y <= x : y < x) lo += step;
+                    }
+                    const float y = L[lo];
+                    if (le ? y <= x : y < x) lo++;
+                    r += lo;
+                }
+                rank[h] = r;
             }
             float sk = INFINITY;
 #pragma unroll
@@ -645,17 +665,29 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
             cut = (double)sk + 2.0 * eps;
         }
 
-        // exact distances of the candidates that can still be among the k nearest
+        // exact distances of the candidates that can still be among the k nearest, packed to the
+        // front of the row's LDS arrays (the list order is no longer needed)
         double dd[2];
         bool pass[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             pass[h] = jv[h] != KNONE && (double)sv[h] <= cut;
             dd[h] = pass[h] ? exact_d2(X, d, i, jv[h]) : INFINITY;
-            sD[wave][lane + 64 * h] = dd[h];
         }
-        const uint32_t P = (uint32_t)__popcll(__ballot(pass[0])) + (uint32_t)__popcll(__ballot(pass[1]));
+        const uint64_t pm0 = __ballot(pass[0]), pm1 = __ballot(pass[1]);
+        const uint32_t P0 = (uint32_t)__popcll(pm0);
+        const uint32_t P = P0 + (uint32_t)__popcll(pm1);
         local += lane == 0 ? P : 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint64_t pm = h ? pm1 : pm0;
+            if (pass[h]) {
+                const uint32_t at = (h ? P0 : 0u) + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+                sD[wave][at] = dd[h];
+                sI[wave][at] = jv[h];
+            }
+        }
         __threadfence_block();
 
         // rank by (d2, index) among the rescored candidates; the first k are the answer
@@ -664,7 +696,7 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
         for (int h = 0; h < 2; h++) {
             uint32_t r = 0;
             if (pass[h]) {
-                for (uint32_t e = 0; e < total; e++) {
+                for (uint32_t e = 0; e < P; e++) {
                     const double v = sD[wave][e];
                     r += (v < dd[h] || (v == dd[h] && sI[wave][e] < jv[h])) ? 1u : 0u;
                 }
@@ -1182,31 +1214,59 @@ __global__ void __launch_bounds__(256) k_kmeans_update(const float *X, uint32_t 
 __global__ void __launch_bounds__(256) k_block_spheres(const float *X, const uint32_t *perm, uint32_t n, uint32_t d,
                                                        uint32_t bsz, uint32_t nblocks, double *cen, double *rad,
                                                        uint32_t *cnt) {
-    __shared__ double sc[128];
-    __shared__ double red[4];
-    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        const uint64_t j = (uint64_t)b * bsz + threadIdx.x;
-        const uint32_t src = (threadIdx.x < bsz && j < n) ? perm[j] : KNONE;  // n: layout rows here
-        const bool real = src != KNONE;
-        const double c = block_sum_f64(real ? 1.0 : 0.0, red);
-        for (uint32_t t = 0; t < d; t++) {
-            const double v = block_sum_f64(real ? (double)X[(uint64_t)src * d + t] : 0.0, red);
-            if (threadIdx.x == 0) sc[t] = c > 0.0 ? v / c : 0.0;
+    // one wave per block: the block's point ids go to LDS, lane t sums coordinate t over the
+    // points in block order (one coalesced row read per point), then each lane takes the
+    // squared distances of its points (lane, lane + 64, ...) and a butterfly takes their maximum
+    __shared__ uint32_t ssrc[4][256];
+    __shared__ double sc[4][128];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t b = blockIdx.x * 4u + (uint32_t)wave; b < nblocks; b += gridDim.x * 4u) {  // wave-uniform
+        uint32_t creal = 0;
+        for (uint32_t p = (uint32_t)lane; p < bsz; p += 64) {
+            const uint64_t j = (uint64_t)b * bsz + p;
+            const uint32_t src = j < n ? perm[j] : KNONE;  // n: layout rows here
+            ssrc[wave][p] = src;
+            creal += src != KNONE ? 1u : 0u;
         }
-        __syncthreads();
-        double r2 = 0.0;
-        if (real)
-            for (uint32_t t = 0; t < d; t++) {
-                const double df = (double)X[(uint64_t)src * d + t] - sc[t];
-                r2 += df * df;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) creal += __shfl_xor(creal, o);
+        const double c = (double)creal;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t t = (uint32_t)lane; t < d; t += 64) {
+            double v = 0.0;
+#pragma unroll 8
+            for (uint32_t p = 0; p < bsz; p++) {
+                const uint32_t src = ssrc[wave][p];
+                if (src != KNONE) v += (double)X[(uint64_t)src * d + t];
             }
-        r2 = block_max_f64(r2, red);
-        if (threadIdx.x == 0) {
-            rad[b] = c > 0.0 ? sqrt(r2) * (1.0 + 1e-12) + 1e-300 : 0.0;
-            cnt[b] = (uint32_t)c;
+            sc[wave][t] = c > 0.0 ? v / c : 0.0;
         }
-        for (uint32_t t = threadIdx.x; t < d; t += 256) cen[(uint64_t)b * d + t] = sc[t];
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double r2 = 0.0;
+        for (uint32_t p = (uint32_t)lane; p < bsz; p += 64) {
+            const uint32_t src = ssrc[wave][p];
+            if (src == KNONE) continue;
+            double q = 0.0;
+            for (uint32_t t = 0; t < d; t++) {
+                const double df = (double)X[(uint64_t)src * d + t] - sc[wave][t];
+                q += df * df;
+            }
+            r2 = fmax(r2, q);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
+        if (lane == 0) {
+            rad[b] = c > 0.0 ? sqrt(r2) * (1.0 + 1e-12) + 1e-300 : 0.0;
+            cnt[b] = creal;
+        }
+        for (uint32_t t = (uint32_t)lane; t < d; t += 64) cen[(uint64_t)b * d + t] = sc[wave][t];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  // ssrc and sc are rewritten by the next block
     }
 }
 
@@ -2152,7 +2212,7 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         const uint32_t ncb = rows / TN;
         DevBuf<double> cR((size_t)row_blocks * d), rR(row_blocks), uR(row_blocks);
         DevBuf<uint32_t> nR(row_blocks), cnt(row_blocks + 1);
-        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(row_blocks, 16384)), dim3(256), 0, s, d_X,
+        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>((row_blocks + 3) / 4, 16384)), dim3(256), 0, s, d_X,
                            layout.get(), rows, d, (uint32_t)KNN_BM, row_blocks, cR.get(), rR.get(), nR.get());
         KERNEL_OK();
         // the coordinates held per row (DMAX >= d) and the insertion network (KK >= k) are sized
@@ -2220,7 +2280,7 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
         // per-row filter of the lists against the column blocks' spheres (in place; bcnt: kept)
         DevBuf<double> cB((size_t)ncb * d), rB(ncb);
         DevBuf<uint32_t> nB(ncb);
-        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>(ncb, 16384)), dim3(256), 0, s, d_X, layout.get(),
+        hipLaunchKernelGGL(k_block_spheres, dim3(std::min<uint32_t>((ncb + 3) / 4, 16384)), dim3(256), 0, s, d_X, layout.get(),
                            rows, d, TN, ncb, cB.get(), rB.get(), nB.get());
         KERNEL_OK();
         DevBuf<float> cB32((size_t)ncb * d), rBf(ncb), nbn(ncb), nb2(ncb);
