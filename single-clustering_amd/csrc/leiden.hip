@@ -42,6 +42,7 @@
 namespace scgpu {
 
 double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, double gamma, hipStream_t s);
+double quality_singletons_device(const sc_graph &g, int mode, double gamma, hipStream_t s);
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
@@ -3167,9 +3168,9 @@ static std::unique_ptr<sc_graph> aggregate_device_wide(const sc_graph &L, const 
 // ---- aggregation with 32-bit keys --------------------------------------------------------------
 // The reference walks the groups in order and, per group g, adds the weights of its members'
 // entries (members ascending, rows ascending) towards groups h >= g (csr_network.rs:548-564).
-// Here every fine entry t (member order) is keyed by its neighbour group h (h >= g), or by the
-// sentinel ng, and carries (g, w) packed in 64 bits; a stable sort by h leaves each pair's
-// entries contiguous and in t order, with the pairs ordered by (h, g). So the pair list directly
+// Here every fine entry t toward a group h >= g (member order, rows in order) is keyed by h and
+// carries (g, w) packed in 64 bits; a stable sort by h leaves each pair's entries contiguous
+// and in t order, with the pairs ordered by (h, g). So the pair list directly
 // gives each row r its entries (g, r), g <= r, ascending in g; a second stable sort of the
 // pairs with g < h by g (carrying (h, w)) gives each row its entries (r, h), h > r, ascending in
 // h. Both sorts use 32-bit keys of bits_for(ng) bits.
@@ -3178,26 +3179,25 @@ __device__ __forceinline__ uint64_t pack_gw(uint32_t g, float w) { return ((uint
 __device__ __forceinline__ uint32_t gw_group(uint64_t x) { return (uint32_t)(x >> 32); }
 __device__ __forceinline__ float gw_weight(uint64_t x) { return __uint_as_float((uint32_t)x); }
 
+// The entries aggregation keeps are those toward groups at or above the row's own: per stored
+// entry its neighbour's group (hb[e], entry order), per row the number it keeps (ucnt[v]). G
+// lanes per row.
 template <int G>
-__global__ void __launch_bounds__(256) k_emit_fine_h(const uint32_t *members, const uint32_t *sorted_groups, uint32_t n,
-                                                     const uint64_t *moff, const uint64_t *ptrs, const uint32_t *nbrs,
-                                                     const float *wts, const uint32_t *groups, uint32_t ng,
-                                                     uint32_t *keys, uint64_t *gw) {
+__global__ void __launch_bounds__(256) k_upper_counts(const uint64_t *ptrs, const uint32_t *nbrs, const uint32_t *groups,
+                                                      uint32_t n, uint32_t *hb, uint32_t *ucnt) {
     constexpr int NG = 256 / G;
     constexpr int EB = 4;  // entries per lane with their loads in flight together
     const int lane = threadIdx.x % G;
-    for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
-        const uint32_t v = members[j];
-        const uint32_t g = sorted_groups[j];
-        const uint64_t o = moff[j], lo = ptrs[v], hi = ptrs[v + 1];
+    for (uint32_t v = blockIdx.x * NG + threadIdx.x / G; v < n; v += gridDim.x * NG) {
+        const uint32_t g = groups[v];
+        const uint64_t lo = ptrs[v], hi = ptrs[v + 1];
+        uint32_t c = 0;
         for (uint64_t e0 = lo + lane; e0 < hi; e0 += (uint64_t)EB * G) {
             uint32_t u[EB];
-            float w[EB];
 #pragma unroll
             for (int q = 0; q < EB; q++) {
                 const uint64_t e = e0 + (uint64_t)q * G;
                 u[q] = e < hi ? nbrs[e] : 0u;
-                w[q] = e < hi ? wts[e] : 0.0f;
             }
             uint32_t h[EB];
 #pragma unroll
@@ -3206,10 +3206,61 @@ __global__ void __launch_bounds__(256) k_emit_fine_h(const uint32_t *members, co
             for (int q = 0; q < EB; q++) {
                 const uint64_t e = e0 + (uint64_t)q * G;
                 if (e < hi) {
-                    const uint64_t t = o + (e - lo);
-                    keys[t] = h[q] >= g ? h[q] : ng;
-                    gw[t] = pack_gw(g, w[q]);
+                    hb[e] = h[q];
+                    c += h[q] >= g ? 1u : 0u;
                 }
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, G);
+        if (lane == 0) ucnt[v] = c;
+    }
+}
+
+__global__ void k_member_upper(const uint32_t *members, uint32_t n, const uint32_t *ucnt, uint64_t *deg) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j <= n; j += gridDim.x * blockDim.x)
+        deg[j] = j == n ? 0u : ucnt[members[j]];
+}
+
+// The kept entries of the member rows in member order (rows ascending within a member), packed:
+// entry t of the list gets key h and payload (g, w). A group of G lanes walks a row EB * G
+// entries at a time; a kept entry's place is the row's running count plus the kept entries
+// before it in the batch (compare ballots of the group's lanes).
+template <int G>
+__global__ void __launch_bounds__(256) k_emit_upper_h(const uint32_t *members, const uint32_t *sorted_groups, uint32_t n,
+                                                      const uint64_t *moff, const uint64_t *ptrs, const uint32_t *hb,
+                                                      const float *wts, uint32_t *keys, uint64_t *gw) {
+    constexpr int NG = 256 / G;
+    constexpr int EB = 4;
+    const int lane = threadIdx.x % G;
+    const int wl = threadIdx.x & 63;
+    const int gl0 = wl - lane;  // the group's first lane in the wave
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const uint64_t lower = (1ull << lane) - 1ull;  // the group's lanes before this one
+    for (uint32_t j = blockIdx.x * NG + threadIdx.x / G; j < n; j += gridDim.x * NG) {
+        const uint32_t v = members[j];
+        const uint32_t g = sorted_groups[j];
+        const uint64_t lo = ptrs[v], hi = ptrs[v + 1];
+        uint64_t t = moff[j];
+        for (uint64_t b = lo; b < hi; b += (uint64_t)EB * G) {  // uniform over the group
+            uint32_t h[EB];
+            float w[EB];
+#pragma unroll
+            for (int q = 0; q < EB; q++) {
+                const uint64_t e = b + (uint64_t)q * G + lane;
+                h[q] = e < hi ? hb[e] : 0u;
+                w[q] = e < hi ? wts[e] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < EB; q++) {
+                const bool keep = b + (uint64_t)q * G + lane < hi && h[q] >= g;
+                const uint64_t gm = (__ballot(keep) >> gl0) & gmask;
+                if (keep) {
+                    const uint64_t at = t + (uint64_t)__popcll(gm & lower);
+                    keys[at] = h[q];
+                    gw[at] = pack_gw(g, w[q]);
+                }
+                t += (uint64_t)__popcll(gm);
             }
         }
     }
@@ -3398,40 +3449,62 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
     hipLaunchKernelGGL(k_group_sums, dim3(grid_for(ng, 16, 4096)), dim3(256), 0, s, starts.get(), members.get(), ng,
                        L.node_w.get(), L.self_loop.get(), out->node_w.get(), cloops.get());
     KERNEL_OK();
-    DevBuf<uint64_t> deg(n + 1), moff(n + 1);
-    hipLaunchKernelGGL(k_member_degrees, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, members.get(), n,
-                       L.ptrs.get(), deg.get());
-    KERNEL_OK();
-    exclusive_scan_u64(deg.get(), moff.get(), n + 1, s);
     const uint32_t sent = ng;
     const int kbits = std::max(1, bits_for(ng));
-    DevBuf<uint32_t> keys(nnz);
-    DevBuf<uint64_t> gw(nnz);
+    // only the entries toward groups at or above the row's own enter the sort (the others are
+    // the mirror images the pair list restores): their neighbour groups and counts first, then
+    // the packed list in member order
+    uint64_t nk = 0;
+    DevBuf<uint64_t> moff(n + 1);
+    DevBuf<uint32_t> keys, head, run;
+    DevBuf<uint64_t> gw;
     uint32_t np = 0;
-    DevBuf<uint32_t> head(nnz + 1), run(nnz + 1);
     if (nnz) {
-        // lanes per member row from the level's mean degree (about four entries per lane)
+        // lanes per row from the level's mean degree (about four entries per lane)
         const double mean_deg = n ? (double)nnz / (double)n : 0.0;
         const int eg = mean_deg <= 16.0 ? 4 : (mean_deg <= 32.0 ? 8 : (mean_deg <= 64.0 ? 16 : 32));
-        auto emit = [&](auto kern, int grp) {
-            hipLaunchKernelGGL(kern, dim3(grid_for(n, 256 / grp, 4096)), dim3(256), 0, s, members.get(), sg.get(), n,
-                               moff.get(), L.ptrs.get(), L.nbrs.get(), L.wts.get(), groups, ng, keys.get(), gw.get());
+        DevBuf<uint32_t> hb(nnz), ucnt(n);
+        auto count = [&](auto kern, int grp) {
+            hipLaunchKernelGGL(kern, dim3(grid_for(n, 256 / grp, 4096)), dim3(256), 0, s, L.ptrs.get(), L.nbrs.get(),
+                               groups, n, hb.get(), ucnt.get());
             KERNEL_OK();
         };
-        if (eg == 4) emit(k_emit_fine_h<4>, 4);
-        else if (eg == 8) emit(k_emit_fine_h<8>, 8);
-        else if (eg == 16) emit(k_emit_fine_h<16>, 16);
-        else emit(k_emit_fine_h<32>, 32);
-        radix_sort_pairs(keys, gw, nnz, kbits, s);
-        hipLaunchKernelGGL(k_pair_heads_h, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, keys.get(), gw.get(), nnz,
-                           sent, head.get());
+        if (eg == 4) count(k_upper_counts<4>, 4);
+        else if (eg == 8) count(k_upper_counts<8>, 8);
+        else if (eg == 16) count(k_upper_counts<16>, 16);
+        else count(k_upper_counts<32>, 32);
+        DevBuf<uint64_t> deg(n + 1);
+        hipLaunchKernelGGL(k_member_upper, dim3(grid_for(n + 1, 256, 4096)), dim3(256), 0, s, members.get(), n,
+                           ucnt.get(), deg.get());
         KERNEL_OK();
-        exclusive_scan_u32(head.get(), run.get(), nnz, s);
-        uint32_t lr = 0, lh = 0;
-        to_host(&lr, run.get() + nnz - 1, 1, s);
-        to_host(&lh, head.get() + nnz - 1, 1, s);
+        exclusive_scan_u64(deg.get(), moff.get(), n + 1, s);
+        to_host(&nk, moff.get() + n, 1, s);
         sync_stream(s);
-        np = lr + lh;
+        if (nk) {
+            keys.resize(nk);
+            gw.resize(nk);
+            head.resize(nk + 1);
+            run.resize(nk + 1);
+            auto emit = [&](auto kern, int grp) {
+                hipLaunchKernelGGL(kern, dim3(grid_for(n, 256 / grp, 4096)), dim3(256), 0, s, members.get(), sg.get(),
+                                   n, moff.get(), L.ptrs.get(), hb.get(), L.wts.get(), keys.get(), gw.get());
+                KERNEL_OK();
+            };
+            if (eg == 4) emit(k_emit_upper_h<4>, 4);
+            else if (eg == 8) emit(k_emit_upper_h<8>, 8);
+            else if (eg == 16) emit(k_emit_upper_h<16>, 16);
+            else emit(k_emit_upper_h<32>, 32);
+            radix_sort_pairs(keys, gw, nk, kbits, s);
+            hipLaunchKernelGGL(k_pair_heads_h, dim3(grid_for(nk, 256, 8192)), dim3(256), 0, s, keys.get(), gw.get(),
+                               nk, sent, head.get());
+            KERNEL_OK();
+            exclusive_scan_u32(head.get(), run.get(), nk, s);
+            uint32_t lr = 0, lh = 0;
+            to_host(&lr, run.get() + nk - 1, 1, s);
+            to_host(&lh, head.get() + nk - 1, 1, s);
+            sync_stream(s);
+            np = lr + lh;
+        }
     }
     DevBuf<uint32_t> pa(np + 1), pb(np + 1), keep(np + 1), kscan(np + 1), key2(np + 1);
     DevBuf<uint64_t> hw(np + 1);
@@ -3444,8 +3517,8 @@ std::unique_ptr<sc_graph> aggregate_device(const sc_graph &L, const uint32_t *gr
         DevBuf<uint32_t> long_list(np);
         DevBuf<unsigned int> n_long(1);
         HIP_OK(hipMemsetAsync(n_long.get(), 0, sizeof(unsigned int), s));
-        hipLaunchKernelGGL(k_seg_starts_h, dim3(grid_for(nnz + 1, 256, 8192)), dim3(256), 0, s, keys.get(), sent,
-                           head.get(), run.get(), nnz, np, seg.get());
+        hipLaunchKernelGGL(k_seg_starts_h, dim3(grid_for(nk + 1, 256, 8192)), dim3(256), 0, s, keys.get(), sent,
+                           head.get(), run.get(), nk, np, seg.get());
         KERNEL_OK();
         hipLaunchKernelGGL(k_pair_sums_short_h, dim3(grid_for(np, 256, 8192)), dim3(256), 0, s, keys.get(), gw.get(),
                            seg.get(), np, cloops.get(), pa.get(), pb.get(), pw.get(), keep.get(), long_list.get(),
@@ -3922,7 +3995,8 @@ void leiden_device(const sc_graph &G0, const sc_leiden_config &cfg, uint32_t *d_
         hipLaunchKernelGGL(k_iota, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, d_labels, n0);
         KERNEL_OK();
         PhaseClock pq(ctx, s);
-        double q_prev = labels_quality(G0, d_labels, cfg, s);
+        // the starting labels are the singletons
+        double q_prev = quality_singletons_device(G0, cfg.objective == SC_RB ? 1 : 2, cfg.resolution, s);
         pq.lap(5);
         uint64_t stagnant = 0;
         double scale = std::max(std::fabs(G0.total_weight), 1.0);

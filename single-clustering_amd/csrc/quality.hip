@@ -170,6 +170,38 @@ double quality_device(const sc_graph &g, const uint64_t *d_labels, int mode, dou
     return device_sum_f64(term.get(), nl, s);
 }
 
+// Quality of the singleton partition (label v for vertex v): a vertex's only same-label entry is
+// its self-loop (rows hold each neighbour once), whose weight is self_loop[v], so the per-label
+// totals are the vertex's own values, stored without atomics. Same totals and sum as
+// quality_device on labels 0 .. n - 1.
+__global__ void k_singleton_terms(uint64_t n, const double *self_loop, const double *strength, const double *node_w,
+                                  int s_in, int sK, int sW, long long *in_fx, long long *K_fx, long long *W_fx) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const double loop = self_loop[v];
+        in_fx[v] = loop != 0.0 ? to_fx(loop, s_in) : 0;
+        K_fx[v] = to_fx(strength[v], sK);
+        W_fx[v] = to_fx(node_w[v], sW);
+    }
+}
+
+double quality_singletons_device(const sc_graph &g, int mode, double gamma, hipStream_t s) {
+    const uint64_t n = g.n;
+    if (n == 0) return 0.0;
+    if (mode == 0 && g.total_weight == 0.0) return 0.0;
+    const int s_in = fixed_scale_for(g.total_weight + 1e-300);
+    const int sK = fixed_scale_for(2.0 * g.total_weight + 1e-300);
+    const int sW = fixed_scale_for(g.total_node_weight + 1e-300);
+    DevBuf<long long> acc(3 * n);
+    hipLaunchKernelGGL(k_singleton_terms, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g.self_loop.get(),
+                       g.strength.get(), g.node_w.get(), s_in, sK, sW, acc.get(), acc.get() + n, acc.get() + 2 * n);
+    KERNEL_OK();
+    DevBuf<double> term(n);
+    hipLaunchKernelGGL(k_community_terms, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, acc.get(), acc.get() + n,
+                       acc.get() + 2 * n, n, s_in, sK, sW, mode, gamma, g.total_weight, term.get());
+    KERNEL_OK();
+    return device_sum_f64(term.get(), n, s);
+}
+
 // ---- batched runs: one quality per job -----------------------------------------------------
 __global__ void k_label_jobs(const uint64_t *labels, const uint32_t *vjob, uint64_t n, uint32_t *ljob) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
