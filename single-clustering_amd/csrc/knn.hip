@@ -187,13 +187,25 @@ constexpr size_t knn_tile_lds_bytes() {
            (size_t)KNN_BM * KnnTileShape<NKC>::BCAP * 8;
 }
 
+// f32 values as unsigned integers in the same order (negative values below positive ones), for
+// atomicMin on thresholds that may be negative.
+__device__ __forceinline__ uint32_t f32_ord(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float f32_unord(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
 // Row-owner merge: lane r folds row r's buffered hits into its sorted list, clears the buffer
 // and republishes the row's threshold; then every lane reloads the 16 thresholds it compares
-// against in the MFMA layout.
+// against in the MFMA layout. With a shared threshold (gthr_row, the row's slot shared by its
+// splits) the row publishes its own with an atomic minimum and takes the smaller of the two.
 template <int KM, int BCAP>
 __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[KM], float (&thr_r)[4][4],
                                                uint32_t (&cnt4)[4], uint32_t *sCnt, float *sThr, uint2 *sBuf,
-                                               int wave, int lane, float tseed, float tmargin, uint32_t kk) {
+                                               int wave, int lane, float tseed, float tmargin, uint32_t kk,
+                                               uint32_t *gthr_row) {
     const int rl = wave * 64 + lane;
     const int q = lane >> 4;
     if ((lane & 15) == 0) {
@@ -216,7 +228,12 @@ __device__ __forceinline__ void knn_merge_rows(float (&lv)[KM], uint32_t (&li)[K
 #pragma unroll
     for (int p = 0; p < KM; p++) lk = (uint32_t)p + 1u == kk ? lv[p] : lk;
     const float kthr = lk < INFINITY ? f32_up((double)lk + (double)tmargin) : INFINITY;
-    sThr[rl] = fminf(fminf(lv[KM - 1], tseed), kthr);
+    float t = fminf(fminf(lv[KM - 1], tseed), kthr);
+    // every split's threshold is one of these bounds (a point at or above it is strictly farther
+    // than k points of that split's list, which only improves), so the smallest over the
+    // row's splits holds for all of them
+    if (gthr_row) t = fminf(t, f32_unord(atomicMin(gthr_row, f32_ord(t))));
+    sThr[rl] = t;
     __threadfence_block();
 #pragma unroll
     for (int rt = 0; rt < 4; rt++) {
@@ -320,7 +337,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
                                                   const uint32_t *blist, const uint32_t *boff, const uint32_t *bcnt,
                                                   const uint32_t *rb_order,
                                                   const float *seed_s, const float *margin_s, uint32_t kk,
-                                                  uint32_t *cand_idx, float *cand_s) {
+                                                  uint32_t *gthr, uint32_t *cand_idx, float *cand_s) {
     constexpr int KP = 32 * NKC;
     constexpr int TN = KnnTileShape<NKC>::TN;
     constexpr int BCAP = KnnTileShape<NKC>::BCAP;
@@ -382,6 +399,7 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
     // pruned search) or +inf, and never rises above it
     const float tseed = seed_s ? seed_s[wrow0 + lane] : INFINITY;
     const float tmargin = margin_s[wrow0 + lane];  // twice the row's error bound, rounded up
+    uint32_t *const grow = gthr ? gthr + wrow0 + lane : nullptr;  // the row's shared threshold
     sThr[tid] = tseed;
 
     // A fragments of the wave's four row tiles, in registers for the whole sweep: the query row
@@ -518,6 +536,19 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         }
         if (st + 1 < ns) KNN_PREFETCH(c0n);  // lands while this stage computes
         __syncthreads();
+        if (grow && (st & 7u) == 7u) {
+            // every eighth stage the rows take what their other splits have published
+            const float gt = f32_unord(__hip_atomic_load(grow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            const int rl = wave * 64 + lane;
+            if (gt < sThr[rl]) sThr[rl] = gt;
+            __threadfence_block();
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                const f32x4 t = *reinterpret_cast<const f32x4 *>(sThr + wave * 64 + rt * 16 + 4 * quarter);
+#pragma unroll
+                for (int i = 0; i < 4; i++) thr_r[rt][i] = t[i];
+            }
+        }
 
         // two accumulator sets: the products of sub-tile t + 1 are issued before the threshold
         // test of sub-tile t, so the matrix cores and the vector unit overlap
@@ -528,16 +559,16 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
             knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 1) * (16u * STRIDE) + lane_base, ah,
                                                     al, af, acc1);
             bool need = knn_subtile_hits<BCAP>(acc0, c0 + sub * 16, row0, rl_base, qmask, thr_r, cnt4, sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk, grow);
             if (sub + 2 < TN / 16)
                 knn_subtile_mfma<BF16, NKC, STRIDE, TN>(sBh, sBf, (uint32_t)(sub + 2) * (16u * STRIDE) + lane_base,
                                                         ah, al, af, acc0);
             need = knn_subtile_hits<BCAP>(acc1, c0 + (sub + 1) * 16, row0, rl_base, qmask, thr_r, cnt4, sBuf);
-            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
+            if (__any(need)) knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk, grow);
         }
     }
 #undef KNN_PREFETCH
-    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk);
+    knn_merge_rows<KM, BCAP>(lv, li, thr_r, cnt4, sCnt, sThr, sBuf, wave, lane, tseed, tmargin, kk, grow);
     // write this split's candidate list: cand[(row * splits + split) * KM + t]
     const uint32_t my_row = wrow0 + lane;
     if (my_row < n) {
@@ -1819,6 +1850,11 @@ __global__ void k_seed_thresholds(const double *seed_d2, const float *ny, uint32
 
 // Twice the error bound of every layout row's approximate values (2.02 eps, rounded up; +inf for
 // padding rows): the tile's k-th-value threshold adds it to the row's k-th listed value.
+__global__ void k_thr_init(const float *seed_s, uint32_t rows, uint32_t *gthr) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x)
+        gthr[j] = f32_ord(seed_s ? seed_s[j] : INFINITY);
+}
+
 __global__ void k_knn_margins(const float *ny, uint32_t rows, double eps_coef, double eps_const, float *margin) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x) {
         const double y = (double)ny[j];
@@ -2093,13 +2129,14 @@ template <bool BF16, int NKC, int KM>
 static void launch_tile_km(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                            uint32_t d, uint32_t n, uint32_t npad, uint32_t cps, const uint32_t *blist,
                            const uint32_t *boff, const uint32_t *bcnt, const uint32_t *rbo, const float *seed_s,
-                           const float *margin_s, uint32_t kk, uint32_t *cidx, float *cs, hipStream_t s) {
+                           const float *margin_s, uint32_t kk, uint32_t *gthr, uint32_t *cidx, float *cs,
+                           hipStream_t s) {
     constexpr size_t lds = knn_tile_lds_bytes<BF16, NKC>();
     if (lds > 64 * 1024) throw ScError(SC_INVALID_ARGUMENT, "kNN tile does not fit in LDS for this d");
     const uint32_t row_blocks = grid.x, splits = grid.y;
     const dim3 g1(8 * ((row_blocks * splits + 7) / 8));  // one dimension; the kernel maps it XCD-aware
     hipLaunchKernelGGL((k_knn_tile<BF16, NKC, KM>), g1, dim3(256), lds, s, Xh, Xl, Xf, d, n, npad, cps, row_blocks, splits, blist,
-                       boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs);
+                       boff, bcnt, rbo, seed_s, margin_s, kk, gthr, cidx, cs);
     KERNEL_OK();
 }
 
@@ -2107,11 +2144,11 @@ template <bool BF16, int NKC>
 static void launch_tile(dim3 grid, const unsigned short *Xh, const unsigned short *Xl, const float *Xf,
                         uint32_t d, uint32_t n, uint32_t npad, uint32_t KM, uint32_t cps, const uint32_t *blist,
                         const uint32_t *boff, const uint32_t *bcnt, const uint32_t *rbo, const float *seed_s,
-                        const float *margin_s, uint32_t kk, uint32_t *cidx, float *cs, hipStream_t s) {
+                        const float *margin_s, uint32_t kk, uint32_t *gthr, uint32_t *cidx, float *cs, hipStream_t s) {
     switch (KM) {
-        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs, s); break;
-        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs, s); break;
-        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, cidx, cs, s); break;
+        case 16: launch_tile_km<BF16, NKC, 16>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, gthr, cidx, cs, s); break;
+        case 32: launch_tile_km<BF16, NKC, 32>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, gthr, cidx, cs, s); break;
+        default: launch_tile_km<BF16, NKC, 64>(grid, Xh, Xl, Xf, d, n, npad, cps, blist, boff, bcnt, rbo, seed_s, margin_s, kk, gthr, cidx, cs, s); break;
     }
 }
 
@@ -2370,6 +2407,11 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
                        eps_const, margin_s.get());
     KERNEL_OK();
     const float *mg = margin_s.get();
+    // the rows' thresholds shared by their splits (starting at the seeds)
+    DevBuf<uint32_t> gthr(rows);
+    hipLaunchKernelGGL(k_thr_init, dim3(grid_for(rows, 256, 4096)), dim3(256), 0, s, sd, rows, gthr.get());
+    KERNEL_OK();
+    uint32_t *const gt = splits > 1 ? gthr.get() : nullptr;
     // candidate lists per layout row (the pruned layout has padding rows)
     const uint32_t crow = prune ? rows : n;
     DevBuf<uint32_t> cidx((size_t)crow * splits * K2);
@@ -2379,14 +2421,14 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const unsigned short *xh = Xh.get(), *xl = Xl.get();
     const float *xf = Xf.get();
     switch (nkc * 2 + (bf16 ? 0 : 1)) {
-        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
-        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, cidx.get(), cs.get(), s); break;
+        case 2: launch_tile<true, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        case 3: launch_tile<false, 1>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        case 4: launch_tile<true, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        case 5: launch_tile<false, 2>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        case 6: launch_tile<true, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        case 7: launch_tile<false, 3>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        case 8: launch_tile<true, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
+        default: launch_tile<false, 4>(grid, xh, xl, xf, d, crow, rows, K2, cps, bl, bo, bc, ro, sd, mg, k, gt, cidx.get(), cs.get(), s); break;
     }
     HIP_OK(hipEventRecord(e2, s));
     if (prune) {
