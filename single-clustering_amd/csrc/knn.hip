@@ -1665,11 +1665,13 @@ __global__ void k_block_pack(const float *cB32, const float *nb2, uint32_t ncb, 
 // row (-2 x, 1, 1) and the centre (cB32, |cB32|^2 hi, lo), FP32 accumulation. Its error against
 // the exact value is below e1 |x| |c| + e2 |c|^2 (the split truncation and the accumulation of
 // the 3 x 32 + 2 products, the f32 norm and its split; the same terms as the tile's bound, per
-// row and centre instead of with the largest norm). |x - cB32|^2 = |x|^2 + s is bounded below by
-// the row's f32 |x|^2 less its rounding, plus s, less that error, less the rounding of those f32
-// additions; the root's rounding, the centre's rounding and the radius are handled as in
-// k_list_filter. A wave holds 64 rows as four row tiles; 64 listed blocks at a time are staged
-// in LDS, read before any of them is written back.
+// row and centre instead of with the largest norm). The test is k_list_filter's without the
+// root: block B is dropped for row x when |x - cB32| > R = rB + u_x + 2^-24 |cB| (the centre's
+// rounding), i.e. when s > T with T an f32 upper bound of R^2 + err - |x|^2 (R rounded up with a
+// 1e-6 relative margin, each positive term inflated and the subtracted |x|^2 lower bound
+// deflated by three units of rounding, which covers the two f32 additions forming T). A padding
+// row (|x|^2 bound +inf) needs no block. A wave holds 64 rows as four row tiles; 64 listed
+// blocks at a time are staged in LDS, read before any of them is written back.
 __global__ void __launch_bounds__(256) k_list_filter_mfma(const float *X, const uint32_t *layout, uint32_t d,
                                                           uint32_t nrb, const double *seed_d2, const unsigned short *cbh,
                                                           const unsigned short *cbl, const float *rBf, const float *nbn,
@@ -1700,9 +1702,11 @@ __global__ void __launch_bounds__(256) k_list_filter_mfma(const float *X, const 
             norow = real && !(sd < INFINITY);
             // the f32 sum of d squares is within 1.01 d 2^-24 |x|^2 of |x|^2; the factor's own
             // rounding and the product's take at most 2 more units
-            s_lox[tid] = xn2 * (1.0f - (1.01f * (float)d + 6.0f) * u24);
-            s_ex[tid] = e1 * sqrtf(xn2) * (1.0f + 4.0f * u24);
-            s_ux[tid] = real ? f32_up(sqrt(sd)) : -INFINITY;
+            // (then three more units off: the subtracted term of T)
+            s_lox[tid] = real ? xn2 * (1.0f - (1.01f * (float)d + 6.0f) * u24) * (1.0f - 3.0f * u24) : INFINITY;
+            // e1 |x| with the product's and the error fma's rounding covered
+            s_ex[tid] = real ? e1 * sqrtf(xn2) * (1.0f + 4.0f * u24) * (1.0f + 4.0f * u24) : 0.0f;
+            s_ux[tid] = real ? f32_up(sqrt(sd)) * (1.0f + 1e-6f) : 0.0f;
         }
         const uint32_t L = lcnt[R], base = lbeg[R];
         if (__syncthreads_or(norow)) {
@@ -1775,8 +1779,9 @@ __global__ void __launch_bounds__(256) k_list_filter_mfma(const float *X, const 
                 const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sBh + off);
                 const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sBl + off);
                 const float cn_ = sn[col] * (1.0f + 4.0f * u24);  // |cB32|, rounded up
-                const float e2c = e2 * cn_ * cn_ * (1.0f + 4.0f * u24);
-                const float rc = sr[col];
+                const float e2c = e2 * cn_ * cn_ * (1.0f + 4.0f * u24) * (1.0f + 4.0f * u24);
+                // rB and the centre's rounding, with the 1e-6 margin (u_x carries its own)
+                const float rcp = sr[col] * (1.0f + 1e-6f) + 1e-6f * cn_;
                 bool need = false;
 #pragma unroll
                 for (int rt = 0; rt < 4; rt++) {
@@ -1786,12 +1791,11 @@ __global__ void __launch_bounds__(256) k_list_filter_mfma(const float *X, const 
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, acc, 0, 0, 0);
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
-                        const float err = fmaf(ex[rt][i], cn_, e2c) * (1.0f + 4.0f * u24);
-                        const float sv = acc[i];
-                        // lower bound of |x - cB32|^2; the three f32 operations' rounding taken off
-                        const float lo2 = (lox[rt][i] + sv - err) - 4.0f * u24 * (lox[rt][i] + fabsf(sv) + err);
-                        need |= !(sqrtf(fmaxf(lo2, 0.0f)) * (1.0f - 1e-6f) - 1e-6f * cn_ >
-                                  (rc + ux[rt][i]) * (1.0f + 1e-6f));
+                        const float Rx = rcp + ux[rt][i];
+                        const float R2 = Rx * (Rx * (1.0f + 5.0f * u24));  // >= R^2 (1 + 3 units)
+                        const float err = fmaf(ex[rt][i], cn_, e2c);
+                        const float T = (R2 + err) - lox[rt][i];
+                        need |= !(acc[i] > T);
                     }
                 }
                 // the column's four quarters (lanes col, col + 16, col + 32, col + 48)
@@ -2259,7 +2263,8 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
             hipLaunchKernelGGL(kern, kg, dim3(256), 0, s, d_X, layout.get(), d, k, row_blocks, uR.get(), seed_d2.get());
         };
         const bool k16 = k <= 16;
-        if (d <= 32) kth(k16 ? k_block_kth<32, 16> : k_block_kth<32, 32>);
+        if (d <= 32 && k <= 8) kth(k_block_kth<32, 8>);
+        else if (d <= 32) kth(k16 ? k_block_kth<32, 16> : k_block_kth<32, 32>);
         else if (d <= 56) kth(k16 ? k_block_kth<56, 16> : k_block_kth<56, 32>);
         else if (d <= 64) kth(k16 ? k_block_kth<64, 16> : k_block_kth<64, 32>);
         else kth(k16 ? k_block_kth<128, 16> : k_block_kth<128, 32>);
