@@ -301,13 +301,23 @@ struct DecideStats {
 // blist for the global-table bucket), which are then already known to be active and in the
 // step's subset. base is the index (into sel, or blist) of the unit's first vertex. Every thread
 // of the block calls this.
-template <int BS, int G, int T, bool REFINE, bool GTAB, bool BATCH = true>
+// WAVE: the unit is one wave's (64 / G vertices, G <= 64): groups are numbered within the wave
+// and the tables lie in the wave's own part of s_keys / s_vals (WAVE_SLOTS entries), so the
+// waves of a workgroup can run units of different buckets at the same time.
+constexpr int WAVE_SLOTS = 1024;
+
+template <int BS, int G, int T, bool REFINE, bool GTAB, bool BATCH = true, bool WAVE = false>
 __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParams &sp, const uint32_t *blist,
                                             const uint32_t *sel, uint32_t len, uint64_t base, uint32_t *s_keys,
                                             long long *s_vals, double *r_s, uint32_t *r_c, long long *r_w,
                                             DecideStats &ds) {
+    static_assert(!WAVE || (G <= 64 && !GTAB && (64 / G) * T <= WAVE_SLOTS), "wave units need a wave's table");
     const int lane = threadIdx.x % G;
-    const int grp = threadIdx.x / G;
+    const int grp = WAVE ? (int)((threadIdx.x & 63) / G) : (int)(threadIdx.x / G);
+    if (WAVE) {
+        s_keys += (threadIdx.x >> 6) * WAVE_SLOTS;
+        s_vals += (threadIdx.x >> 6) * WAVE_SLOTS;
+    }
     const uint64_t idx = base + grp;
     bool valid = idx < len;
     // sel holds vertices, or list positions for the global-table bucket (it needs goff[pos])
@@ -1489,10 +1499,42 @@ template <bool REFINE, bool BATCH>
 __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParams &sp, const unsigned int *s_wc,
                                              uint32_t *s_keys, long long *s_vals, double *r_s, uint32_t *r_c,
                                              long long *r_w, DecideStats &ds) {
-    // bucket by bucket over the workgroup's own compacted share; waves run the units of a bucket
-    // independently (group_sync), and a bucket with another table layout starts after every
-    // wave has left the previous one
-    bool first = true;
+    static_assert((SWEEP_BS / 64) * WAVE_SLOTS <= SWEEP_TAB, "wave tables do not fit the sweep's LDS");
+    // buckets 0-4 (groups of at most a wave): wave units of every bucket in one list, dealt to the
+    // waves in turn, each wave with its own tables, so a step whose few deciders are spread over
+    // the buckets does not run the buckets one after another
+    if constexpr (!REFINE) {
+        const uint32_t wave = threadIdx.x >> 6;
+        uint32_t us[6];
+        us[0] = 0;
+#define SC_UNITS(BK) us[BK + 1] = us[BK] + (s_wc[BK] + (64 / BucketShape<BK>::G) - 1) / (64 / BucketShape<BK>::G);
+        SC_UNITS(0)
+        SC_UNITS(1)
+        SC_UNITS(2)
+        SC_UNITS(3)
+        SC_UNITS(4)
+#undef SC_UNITS
+#define SC_WAVE_UNIT(BK)                                                                                      \
+    {                                                                                                         \
+        constexpr int G = BucketShape<BK>::G, T = BucketShape<BK>::T;                                         \
+        uint32_t lo, hi;                                                                                      \
+        bucket_share(A, BK, lo, hi);                                                                          \
+        decide_unit<SWEEP_BS, G, T, REFINE, false, BATCH, true>(A.d, sp, A.lists + A.bstart[BK], A.work + lo, \
+                                                               s_wc[BK], (uint64_t)(u - us[BK]) * (64 / G),   \
+                                                               s_keys, s_vals, r_s, r_c, r_w, ds);           \
+    }
+        for (uint32_t u = wave; u < us[5]; u += SWEEP_BS / 64) {  // wave-uniform
+            if (u < us[1]) SC_WAVE_UNIT(0)
+            else if (u < us[2]) SC_WAVE_UNIT(1)
+            else if (u < us[3]) SC_WAVE_UNIT(2)
+            else if (u < us[4]) SC_WAVE_UNIT(3)
+            else SC_WAVE_UNIT(4)
+        }
+#undef SC_WAVE_UNIT
+    }
+    // buckets 5-7 (a workgroup per vertex; refinement: every bucket): bucket by bucket after
+    // every wave has left the previous one (the tables span the workgroup's LDS)
+    bool first = REFINE;
 #define SC_DECIDE_BUCKET(BK)                                                                          \
     {                                                                                                 \
         constexpr int G = BucketShape<BK>::G, T = BucketShape<BK>::T, NG = SWEEP_BS / G;               \
@@ -1509,11 +1551,13 @@ __device__ __forceinline__ void sweep_decide(const SweepArgs &A, const StepParam
                                                              len, base, s_keys, s_vals, r_s, r_c, r_w, ds); \
         }                                                                                             \
     }
-    SC_DECIDE_BUCKET(0)
-    SC_DECIDE_BUCKET(1)
-    SC_DECIDE_BUCKET(2)
-    SC_DECIDE_BUCKET(3)
-    SC_DECIDE_BUCKET(4)
+    if constexpr (REFINE) {
+        SC_DECIDE_BUCKET(0)
+        SC_DECIDE_BUCKET(1)
+        SC_DECIDE_BUCKET(2)
+        SC_DECIDE_BUCKET(3)
+        SC_DECIDE_BUCKET(4)
+    }
     SC_DECIDE_BUCKET(5)
     SC_DECIDE_BUCKET(6)
     SC_DECIDE_BUCKET(7)
