@@ -588,7 +588,8 @@ __device__ __forceinline__ void decide_unit(const DecideArgs &a, const StepParam
 template <int BS, int G, bool GTAB, bool BATCH>
 __device__ __forceinline__ void decide_unit_single(const DecideArgs &a, const StepParams &sp, const uint32_t *blist,
                                                    const uint32_t *sel, uint32_t len, uint64_t base, double *r_s,
-                                                   uint32_t *r_c, long long *r_w, DecideStats &ds) {
+                                                   uint32_t *r_c, long long *r_w, DecideStats &ds,
+                                                   const uint4 *rpk) {
     const int lane = threadIdx.x % G;
     const int grp = threadIdx.x / G;
     const uint64_t idx = base + grp;
@@ -619,14 +620,26 @@ __device__ __forceinline__ void decide_unit_single(const DecideArgs &a, const St
                 w[j] = in ? a.wts[e] : 0.0f;
             }
             bool ok[EB];
-#pragma unroll
-            for (int j = 0; j < EB; j++) ok[j] = u[j] != v && a.constraint[u[j]] == cv;
             long long Wv[EB], Kv[EB];
+            if (rpk) {
+                // a neighbour's working community and singleton total in one load
 #pragma unroll
-            for (int j = 0; j < EB; j++) {
-                const uint32_t ci = ok[j] ? u[j] : v;
-                Wv[j] = (a.objective != 0 || a.has_max) ? a.Wfx[ci] : 0;
-                Kv[j] = a.objective == 0 ? a.Kfx[ci] : 0;
+                for (int j = 0; j < EB; j++) {
+                    const uint4 q = rpk[u[j]];
+                    const long long val = (long long)(((unsigned long long)q.y << 32) | q.x);
+                    ok[j] = u[j] != v && q.z == cv;
+                    Kv[j] = a.objective == 0 ? val : 0;
+                    Wv[j] = a.objective != 0 ? val : 0;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < EB; j++) ok[j] = u[j] != v && a.constraint[u[j]] == cv;
+#pragma unroll
+                for (int j = 0; j < EB; j++) {
+                    const uint32_t ci = ok[j] ? u[j] : v;
+                    Wv[j] = (a.objective != 0 || a.has_max) ? a.Wfx[ci] : 0;
+                    Kv[j] = a.objective == 0 ? a.Kfx[ci] : 0;
+                }
             }
 #pragma unroll
             for (int j = 0; j < EB; j++) {
@@ -1239,7 +1252,11 @@ struct SweepArgs {
     unsigned int *wcnt;     // [max_steps][NUM_BUCKETS] compacted counts
     unsigned int *mcnt;     // [max_steps] deciders that chose to move
     unsigned int *ccnt;     // [max_steps] vertices that moved (refinement)
-    uint32_t *mstep;        // step stamp of accepted movers (local moving)
+    uint32_t *mstep;        // step stamp of accepted movers
+    // refinement's first step (every vertex a singleton): per vertex its own total of the
+    // objective's kind (strength for Rb, node weight for CPM; fixed point) and its working
+    // community, packed so one 16-byte load brings both; null under a weight cap
+    uint4 *rpk;
     uint32_t mv_stride;     // each workgroup's mover list: d.movers + blockIdx.x * mv_stride
     uint32_t *mv_tgt;       // local moving with joiner admission: the movers' targets, same layout
     uint32_t *claim;        // lowest mover id per capped target, or null without a weight cap
@@ -1579,7 +1596,7 @@ __device__ __forceinline__ void sweep_decide_single(const SweepArgs &A, const St
             for (uint64_t base = 0; base < len; base += NG)                                                  \
                 decide_unit_single<SWEEP_BS, G, BK == GTAB_BUCKET, BATCH>(A.d, sp, A.lists + A.bstart[BK],    \
                                                                           A.work + lo, len, base, r_s, r_c,   \
-                                                                          r_w, ds);                          \
+                                                                          r_w, ds, A.rpk);                   \
         }                                                                                                    \
     }
     SC_DECIDE_SINGLE(0)
@@ -1723,6 +1740,12 @@ __global__ void __launch_bounds__(SWEEP_BS) __attribute__((amdgpu_waves_per_eu(3
         if (REFINE) {
             A.memb[v] = (uint32_t)v;
             A.mstep[v] = NONE;
+            if (A.rpk) {
+                const VParams vp = vparams<BATCH>(a, (uint32_t)v);
+                const long long val = a.objective == 0 ? to_fx(a.strength[v], vp.sK) : to_fx(a.node_w[v], vp.sW);
+                A.rpk[v] = make_uint4((uint32_t)(unsigned long long)val, (uint32_t)((unsigned long long)val >> 32),
+                                      a.constraint[v], 0u);
+            }
             a.tround[v] = NONE;
             a.targeted_cnt[v] = 0;
         } else {
@@ -2116,6 +2139,7 @@ struct LevelState {
     DevBuf<uint8_t> f_mark;
     // persistent sweeps: compacted step lists, mover stamps, step counters + barrier + control
     DevBuf<uint32_t> work, mstep;
+    DevBuf<uint4> rpk;
     // joiner admission (local moving without a cap): sums and lowest ids per target (two sets),
     // the movers' snapshot gain margins, their targets beside the persistent mover lists, and
     // (multi-launch) whether each mover was admitted
@@ -2289,6 +2313,7 @@ static void alloc_level(LevelState &ls, uint32_t n, hipStream_t s) {
     HIP_OK(hipMemsetAsync(ls.stat.get(), 0, 4 * sizeof(unsigned long long), s));
     ls.work.resize(n);
     ls.mstep.resize(n);
+    ls.rpk.resize(n);
     ls.sweep.resize((size_t)MAX_MOVE_ITERS * MOVE_SUBSWEEPS * (NUM_BUCKETS + 2) + 8);
     ls.sweep_ctl.resize(sizeof(BarrierCtl));
     // sweep counters and barrier words start at zero; each sweep leaves them zero again
@@ -2374,6 +2399,7 @@ static void run_sweep(const DecideArgs &d, const Buckets &b, LevelState &ls, Run
     A.ctl = A.ccnt + A.max_steps;
     A.bc = reinterpret_cast<BarrierCtl *>(ls.sweep_ctl.get());
     A.mstep = ls.mstep.get();
+    A.rpk = ctx.cfg->has_max_community_weight ? nullptr : ls.rpk.get();
     A.claim = ctx.cfg->has_max_community_weight ? ls.claim.get() : nullptr;
     A.level_seed = level_seed;
     A.memb = memb_w;
