@@ -536,19 +536,6 @@ __global__ void __launch_bounds__(256, (KM == 16 && NKC <= 2) ? 2 : 1) k_knn_til
         }
         if (st + 1 < ns) KNN_PREFETCH(c0n);  // lands while this stage computes
         __syncthreads();
-        if (grow && (st & 7u) == 7u) {
-            // every eighth stage the rows take what their other splits have published
-            const float gt = f32_unord(__hip_atomic_load(grow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            const int rl = wave * 64 + lane;
-            if (gt < sThr[rl]) sThr[rl] = gt;
-            __threadfence_block();
-#pragma unroll
-            for (int rt = 0; rt < 4; rt++) {
-                const f32x4 t = *reinterpret_cast<const f32x4 *>(sThr + wave * 64 + rt * 16 + 4 * quarter);
-#pragma unroll
-                for (int i = 0; i < 4; i++) thr_r[rt][i] = t[i];
-            }
-        }
 
         // two accumulator sets: the products of sub-tile t + 1 are issued before the threshold
         // test of sub-tile t, so the matrix cores and the vector unit overlap
