@@ -2201,10 +2201,12 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     // wave of workgroups to matter (the result is exact for every split count)
     std::vector<uint32_t> cand{splits};
     if (prune) {
-        // the pruned lists vary in length between row blocks: more splits balance the work
-        // (two splits leave the certificate too little margin: rows fall back to the FP64 scan)
+        // the pruned lists vary in length between row blocks: more splits balance the work, fewer
+        // send fewer first hits and candidates (the splits share their thresholds); with two the
+        // certificate can run short of margin (rows fall back to the FP64 scan), which the tuner
+        // sees, since it times the whole search
         cand.clear();
-        for (uint32_t sp : {4u, 8u, 6u})
+        for (uint32_t sp : {4u, 2u, 3u})
             if (sp * K2 <= (uint32_t)KNN_MAXCAND) cand.push_back(sp);
         if (cand.empty()) cand.push_back(min_splits);
     } else if (row_blocks < 4 * slots) {
