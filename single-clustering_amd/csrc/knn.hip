@@ -879,6 +879,11 @@ __global__ void __launch_bounds__(64) k_knn_fallback_merge(uint32_t k, uint32_t 
 // brute-force result. Only the amount of skipped work depends on the grouping.
 constexpr int KMC = 64;  // centroids per LDS chunk of the assignment
 
+__global__ void k_fill_f64_knn(double *p, uint64_t n, double x) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = x;
+}
+
 __global__ void k_fill_u32_knn(uint32_t *p, uint32_t n, uint32_t x) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = x;
 }
@@ -2627,8 +2632,8 @@ std::unique_ptr<sc_graph> gaussian_graph_device(const uint32_t *d_idx, const dou
     g->ptrs.resize(n + 1);
     exclusive_scan_u64(reinterpret_cast<uint64_t *>(rc.get()), g->ptrs.get(), n + 1, s);
     g->node_w.resize(n);
-    std::vector<double> ones(n, 1.0);
-    to_device(g->node_w.get(), ones.data(), n, s);
+    hipLaunchKernelGGL(k_fill_f64_knn, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, g->node_w.get(), (uint64_t)n, 1.0);
+    KERNEL_OK();
     graph_finish_sorted(*g, s);
     sync_stream(s);
     return g;
