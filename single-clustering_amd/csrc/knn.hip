@@ -596,51 +596,60 @@ __device__ __forceinline__ double exact_d2(const float *X, uint32_t d, uint32_t 
     return s;
 }
 
-// One wave per row: FP64 rescore of the candidates of all splits, top-k by (d2, index), and the
-// certificate. With |s_approx - s| <= eps for every candidate (s = |y|^2 - 2 x.y), the k-th
-// smallest exact value among the candidates is at most S_k + eps, where S_k is the k-th smallest
-// approximate value; a candidate above S_k + 2 eps can therefore not be among the k nearest
-// candidates and is not rescored.
+// LW lanes per row (a wave holds 64 / LW rows; LW >= the row's candidates KM * splits, 32 or
+// 64): FP64 rescore of the candidates of all splits, top-k by (d2, index), and the certificate.
+// With |s_approx - s| <= eps for every candidate (s = |y|^2 - 2 x.y), the k-th smallest exact
+// value among the candidates is at most S_k + eps, where S_k is the k-th smallest approximate
+// value; a candidate above S_k + 2 eps can therefore not be among the k nearest candidates and is
+// not rescored. A row's ballots are masked to its lanes.
 constexpr int RS_WAVES = 4;
 
+template <int LW>
 __global__ void __launch_bounds__(64 * RS_WAVES)
 k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, uint32_t splits,
               const uint32_t *cand_idx, const float *cand_s, const double *nx64, double eps_coef, double eps_const,
               uint32_t *idx_out, double *d2_out, int fallback_all, uint32_t *fallback, unsigned int *n_fallback,
               unsigned long long *n_rescored) {
-    __shared__ float sS[RS_WAVES][KNN_MAXCAND];
-    __shared__ uint32_t sI[RS_WAVES][KNN_MAXCAND];
-    __shared__ double sD[RS_WAVES][KNN_MAXCAND];
+    constexpr int RPW = 64 / LW;          // rows per wave
+    constexpr int HH = LW == 64 ? 2 : 1;  // candidates per lane: up to KNN_MAXCAND (64 lanes), 32 (32)
+    __shared__ float sS[RS_WAVES * RPW][KNN_MAXCAND];
+    __shared__ uint32_t sI[RS_WAVES * RPW][KNN_MAXCAND];
+    __shared__ double sD[RS_WAVES * RPW][KNN_MAXCAND];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t total = KM * splits;  // <= KNN_MAXCAND
+    const int sg = lane / LW, sl = lane % LW;
+    const int slot = wave * RPW + sg;
+    const uint64_t rmask = LW == 64 ? ~0ull : (0xFFFFFFFFull << (32 * sg));  // this row's lanes
+    const uint32_t total = KM * splits;  // <= HH * LW
     unsigned long long local = 0;
-    for (uint32_t i = blockIdx.x * RS_WAVES + wave; i < n; i += gridDim.x * RS_WAVES) {
+    for (uint32_t i = (blockIdx.x * RS_WAVES + wave) * RPW + sg; i < n; i += gridDim.x * RS_WAVES * RPW) {
         const uint64_t base = (uint64_t)i * total;
-        float sv[2];
-        uint32_t jv[2];
+        float sv[HH];
+        uint32_t jv[HH];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint32_t e = lane + 64 * h;
+        for (int h = 0; h < HH; h++) {
+            const uint32_t e = sl + LW * h;
             jv[h] = e < total ? cand_idx[base + e] : KNONE;
             sv[h] = jv[h] != KNONE ? cand_s[base + e] : INFINITY;
-            sS[wave][e] = sv[h];
+            sS[slot][e] = sv[h];
         }
         // a split whose list is full (its last entry is a point) may hide closer points; the
         // smallest last value over those splits bounds every point not in the lists. A list that
-        // is not full hides only points at or above the row's seeded threshold (pruned search),
-        // which are strictly farther than its k-th nearest point
+        // is not full hides only points at or above the row's thresholds (pruned search), which
+        // are strictly farther than k points of the lists
         float last = INFINITY;
         bool full = false;
-        if ((uint32_t)lane < splits) {
-            const uint32_t jl = cand_idx[base + (uint64_t)lane * KM + KM - 1];
+        if ((uint32_t)sl < splits) {
+            const uint32_t jl = cand_idx[base + (uint64_t)sl * KM + KM - 1];
             full = jl != KNONE;
-            if (full) last = cand_s[base + (uint64_t)lane * KM + KM - 1];
+            if (full) last = cand_s[base + (uint64_t)sl * KM + KM - 1];
         }
-        const bool any_full = __any(full);
+        const bool any_full = (__ballot(full) & rmask) != 0ull;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) last = fminf(last, __shfl_xor(last, o));
+        for (int o = LW / 2; o > 0; o >>= 1) last = fminf(last, __shfl_xor(last, o, LW));
         const float amin = last;
-        const uint32_t cnt = (uint32_t)__popcll(__ballot(jv[0] != KNONE)) + (uint32_t)__popcll(__ballot(jv[1] != KNONE));
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int h = 0; h < HH; h++) cnt += (uint32_t)__popcll(__ballot(jv[h] != KNONE) & rmask);
         __threadfence_block();
 
         // S_k: the approximate value of rank k - 1 (ties by list position)
@@ -651,17 +660,18 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
             // rank by (value, list position) is its position in its own list plus, per other list,
             // the entries before it: values at or below it in an earlier list, below it in a later
             // one (a binary search of KM entries, KM a power of two)
-            uint32_t rank[2] = {0u, 0u};
+            uint32_t rank[HH];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const uint32_t e = (uint32_t)lane + 64u * h;
+            for (int h = 0; h < HH; h++) {
+                rank[h] = 0u;
+                const uint32_t e = (uint32_t)sl + LW * h;
                 if (e >= total) continue;
                 const uint32_t sp = e / KM;
                 const float x = sv[h];
                 uint32_t r = e - sp * KM;
                 for (uint32_t s2 = 0; s2 < splits; s2++) {
                     if (s2 == sp) continue;
-                    const float *L = &sS[wave][s2 * KM];
+                    const float *L = &sS[slot][s2 * KM];
                     const bool le = s2 < sp;
                     uint32_t lo = 0;
                     for (uint32_t step = KM >> 1; step > 0; step >>= 1) {
@@ -676,8 +686,8 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
             }
             float sk = INFINITY;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const uint64_t m = __ballot(jv[h] != KNONE && rank[h] == k - 1);
+            for (int h = 0; h < HH; h++) {
+                const uint64_t m = __ballot(jv[h] != KNONE && rank[h] == k - 1) & rmask;
                 if (m) sk = __shfl(sv[h], (int)__ffsll((unsigned long long)m) - 1);
             }
             cut = (double)sk + 2.0 * eps;
@@ -685,25 +695,29 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
 
         // exact distances of the candidates that can still be among the k nearest, packed to the
         // front of the row's LDS arrays (the list order is no longer needed)
-        double dd[2];
-        bool pass[2];
+        double dd[HH];
+        bool pass[HH];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < HH; h++) {
             pass[h] = jv[h] != KNONE && (double)sv[h] <= cut;
             dd[h] = pass[h] ? exact_d2(X, d, i, jv[h]) : INFINITY;
         }
-        const uint64_t pm0 = __ballot(pass[0]), pm1 = __ballot(pass[1]);
-        const uint32_t P0 = (uint32_t)__popcll(pm0);
-        const uint32_t P = P0 + (uint32_t)__popcll(pm1);
-        local += lane == 0 ? P : 0;
+        uint64_t pm[HH];
+        uint32_t P = 0, Pb[HH];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint64_t pm = h ? pm1 : pm0;
+        for (int h = 0; h < HH; h++) {
+            pm[h] = __ballot(pass[h]) & rmask;
+            Pb[h] = P;
+            P += (uint32_t)__popcll(pm[h]);
+        }
+        local += sl == 0 ? P : 0;
+#pragma unroll
+        for (int h = 0; h < HH; h++) {
             if (pass[h]) {
-                const uint32_t at = (h ? P0 : 0u) + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
-                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
-                sD[wave][at] = dd[h];
-                sI[wave][at] = jv[h];
+                const uint32_t at = Pb[h] + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm[h] >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)pm[h], 0u));
+                sD[slot][at] = dd[h];
+                sI[slot][at] = jv[h];
             }
         }
         __threadfence_block();
@@ -711,35 +725,36 @@ k_knn_rescore(const float *X, uint32_t n, uint32_t d, uint32_t k, uint32_t KM, u
         // rank by (d2, index) among the rescored candidates; the first k are the answer
         double kth = INFINITY;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < HH; h++) {
             uint32_t r = 0;
             if (pass[h]) {
                 for (uint32_t e = 0; e < P; e++) {
-                    const double v = sD[wave][e];
-                    r += (v < dd[h] || (v == dd[h] && sI[wave][e] < jv[h])) ? 1u : 0u;
+                    const double v = sD[slot][e];
+                    r += (v < dd[h] || (v == dd[h] && sI[slot][e] < jv[h])) ? 1u : 0u;
                 }
                 if (r < k) {
                     idx_out[(uint64_t)i * k + r] = jv[h];
                     d2_out[(uint64_t)i * k + r] = dd[h];
                 }
             }
-            const uint64_t m = __ballot(pass[h] && r == k - 1);
+            const uint64_t m = __ballot(pass[h] && r == k - 1) & rmask;
             if (m) kth = __shfl(dd[h], (int)__ffsll((unsigned long long)m) - 1);
         }
-        if ((uint32_t)lane >= P && (uint32_t)lane < k) {
-            idx_out[(uint64_t)i * k + lane] = KNONE;
-            d2_out[(uint64_t)i * k + lane] = INFINITY;
-        }
+        for (uint32_t q = (uint32_t)sl; q < k; q += LW)
+            if (q >= P) {
+                idx_out[(uint64_t)i * k + q] = KNONE;
+                d2_out[(uint64_t)i * k + q] = INFINITY;
+            }
         // certified when no full split can hide a point at or below the k-th distance
         bool ok = true;
         if (any_full) ok = cnt >= k && (nx64[i] + (double)amin - eps > kth);
-        if ((!ok || fallback_all) && lane == 0) {
-            const unsigned int slot = atomicAdd(n_fallback, 1u);
-            fallback[slot] = i;
+        if ((!ok || fallback_all) && sl == 0) {
+            const unsigned int fslot = atomicAdd(n_fallback, 1u);
+            fallback[fslot] = i;
         }
         __threadfence_block();  // this row's LDS reads are done before the next row's writes
     }
-    if (lane == 0 && local) atomicAdd(n_rescored, local);
+    if (sl == 0 && local) atomicAdd(n_rescored, local);
 }
 
 // Exact FP64 scan for the rows the certificate rejected: workgroup (r, w) (128 threads) scans
@@ -2451,10 +2466,15 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const int fallback_all = fa && fa[0] == '1' ? 1 : 0;
     // a few long-lived workgroups per CU walk the rows: launching one short workgroup per few rows
     // left most of the machine idle waiting for workgroup creation
-    hipLaunchKernelGGL(k_knn_rescore, dim3(std::min<uint32_t>((n + RS_WAVES - 1) / RS_WAVES, 2048)),
-                       dim3(64 * RS_WAVES), 0, s, d_X, n, d, k, K2, splits,
-                       cidx.get(), cs.get(), nx64.get(), eps_coef, eps_const, d_idx, d_d2, fallback_all, fb.get(),
-                       nfb.get(), nres.get());
+    // two rows per wave when a row's candidates fit in half a wave
+    if (K2 * splits <= 32)
+        hipLaunchKernelGGL(k_knn_rescore<32>, dim3(std::min<uint32_t>((n + 2 * RS_WAVES - 1) / (2 * RS_WAVES), 2048)),
+                           dim3(64 * RS_WAVES), 0, s, d_X, n, d, k, K2, splits, cidx.get(), cs.get(), nx64.get(),
+                           eps_coef, eps_const, d_idx, d_d2, fallback_all, fb.get(), nfb.get(), nres.get());
+    else
+        hipLaunchKernelGGL(k_knn_rescore<64>, dim3(std::min<uint32_t>((n + RS_WAVES - 1) / RS_WAVES, 2048)),
+                           dim3(64 * RS_WAVES), 0, s, d_X, n, d, k, K2, splits, cidx.get(), cs.get(), nx64.get(),
+                           eps_coef, eps_const, d_idx, d_d2, fallback_all, fb.get(), nfb.get(), nres.get());
     KERNEL_OK();
     unsigned int nf = 0;
     unsigned long long nr = 0;
