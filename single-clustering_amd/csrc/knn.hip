@@ -2603,15 +2603,32 @@ __global__ void k_dedup_heads(const uint64_t *keys, uint64_t m, uint64_t sent, u
 // symmetrize by max: equal keys carry equal weights here, the max keeps that explicit
 __global__ void k_dedup_emit(const uint64_t *keys, const float *vals, const uint32_t *head, const uint32_t *pos,
                              uint64_t m, uint32_t nb, uint32_t *nbrs, float *wts, unsigned long long *row_count) {
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
-        if (!head[e]) continue;
-        uint64_t key = keys[e];
-        float w = vals[e];
-        for (uint64_t f = e + 1; f < m && keys[f] == key; f++) w = fmaxf(w, vals[f]);
-        uint32_t o = pos[e];
-        nbrs[o] = (uint32_t)(key & ((1ull << nb) - 1));
-        wts[o] = w;
-        atomicAdd(&row_count[key >> nb], 1ull);
+    const int lane = threadIdx.x & 63;
+    for (uint64_t e0 = blockIdx.x * (uint64_t)blockDim.x; e0 < m; e0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = e0 + threadIdx.x;  // the loop bound is uniform: every lane reaches the ballots
+        const bool in = e < m;
+        const bool h = in && head[e] != 0u;
+        const uint64_t key = in ? keys[e] : ~0ull;
+        if (h) {
+            float w = vals[e];
+            for (uint64_t f = e + 1; f < m && keys[f] == key; f++) w = fmaxf(w, vals[f]);
+            const uint32_t o = pos[e];
+            nbrs[o] = (uint32_t)(key & ((1ull << nb) - 1));
+            wts[o] = w;
+        }
+        // row counts: the keys are sorted, so a wave's entries of one row are consecutive lanes;
+        // the first lane of each such run adds the run's heads with one atomic
+        const uint64_t row = key >> nb;
+        const uint64_t prow = __shfl_up(row, 1);
+        const uint64_t starts = __ballot(lane == 0 || row != prow);
+        const uint64_t heads = __ballot(h);
+        if (in && ((starts >> lane) & 1ull)) {
+            const uint64_t above = lane == 63 ? 0ull : (starts >> (lane + 1)) << (lane + 1);  // later run starts
+            const int end = above ? __ffsll((unsigned long long)above) - 1 : 64;             // this run's end
+            const uint64_t run = (end == 64 ? ~0ull : ((1ull << end) - 1ull)) & ~((1ull << lane) - 1ull);
+            const uint32_t c = (uint32_t)__popcll(heads & run);
+            if (c) atomicAdd(&row_count[row], (unsigned long long)c);
+        }
     }
 }
 
