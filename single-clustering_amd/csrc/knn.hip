@@ -98,17 +98,32 @@ __global__ void k_knn_prep(const float *X, const uint32_t *perm, const float *ny
     }
 }
 
-// One thread per padded point: f32 norm (+inf for padding points, so that they never pass a
-// candidate threshold), exact FP64 norm, and the largest norm (one atomic per wave). With perm,
-// ny follows the layout's order (ny[j] of point perm[j]) and nx64 the points' own order.
-__global__ void k_knn_norms(const float *X, const uint32_t *perm, uint32_t n, uint32_t d, uint32_t npad, float *ny,
-                            double *nx64, unsigned int *ymax_bits) {
-    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < npad; j0 += gridDim.x * blockDim.x) {
-        const uint32_t j = j0 + threadIdx.x;
-        float m = 0.0f;
+// Per padded point: f32 norm (+inf for padding points, so that they never pass a candidate
+// threshold), exact FP64 norm, and the largest norm (one atomic per wave). With perm, ny follows
+// the layout's order (ny[j] of point perm[j]) and nx64 the points' own order. One wave per 64
+// layout rows: their coordinates go to LDS with reads along each row (lane t reads coordinate
+// t), then each lane sums its own row in coordinate order (the same f32 and FP64 chains as one
+// thread per row reading its row).
+__global__ void __launch_bounds__(256) k_knn_norms(const float *X, const uint32_t *perm, uint32_t n, uint32_t d,
+                                                   uint32_t npad, float *ny, double *nx64, unsigned int *ymax_bits) {
+    extern __shared__ float sx[];  // [waves][64][d + 1]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t W = blockDim.x >> 6;
+    float *const my = sx + (size_t)wave * 64 * (d + 1);
+    for (uint32_t j0 = (blockIdx.x * W + wave) * 64; j0 < npad; j0 += gridDim.x * W * 64) {  // wave-uniform
+        const uint32_t j = j0 + lane;
         const uint32_t src = j < npad ? (perm ? perm[j] : (j < n ? j : KNONE)) : KNONE;
+#pragma unroll 4
+        for (int r = 0; r < 64; r++) {
+            const uint32_t sr = __shfl(src, r);
+            for (uint32_t t = lane; t < d; t += 64) my[r * (d + 1) + t] = sr != KNONE ? X[(uint64_t)sr * d + t] : 0.0f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float m = 0.0f;
         if (src != KNONE) {
-            const float *x = X + (uint64_t)src * d;
+            const float *x = my + lane * (d + 1);
             float s = 0.0f;
             double s64 = 0.0;
             for (uint32_t t = 0; t < d; t++) {
@@ -126,7 +141,10 @@ __global__ void k_knn_norms(const float *X, const uint32_t *perm, uint32_t n, ui
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(ymax_bits, __float_as_uint(m));
+        if (lane == 0 && m > 0.0f) atomicMax(ymax_bits, __float_as_uint(m));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  // my is rewritten by the next rows
     }
 }
 
@@ -2381,9 +2399,14 @@ void knn_device(const float *d_X, uint32_t n, uint32_t d, uint32_t k, int precis
     const uint32_t *bc = prune ? bcnt.get() : nullptr;
     const uint32_t *ro = prune ? rb_order.get() : nullptr;
     const uint64_t total = (uint64_t)rows * Kp;
-    hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, s, d_X, pm, n, d, rows, ny.get(),
-                       nx64.get(), ymax.get());
-    KERNEL_OK();
+    {
+        // waves per workgroup: as many 64-row LDS tiles as fit in 64 KB (4 up to d = 63)
+        const uint32_t nw = std::max<uint32_t>(1u, std::min<uint32_t>(4u, 65536u / (256u * (d + 1))));
+        const size_t lds = (size_t)nw * 64 * (d + 1) * sizeof(float);
+        hipLaunchKernelGGL(k_knn_norms, dim3(grid_for(rows, 64 * nw, 8192)), dim3(64 * nw), lds, s, d_X, pm, n, d,
+                           rows, ny.get(), nx64.get(), ymax.get());
+        KERNEL_OK();
+    }
     hipLaunchKernelGGL(k_knn_prep, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, d_X, pm, (const float *)ny.get(),
                        n, d, Kp, total, bf16 ? Xh.get() : nullptr, bf16 ? Xl.get() : nullptr, bf16 ? nullptr : Xf.get());
     KERNEL_OK();
