@@ -877,6 +877,7 @@ struct ApplyArgs {
     const uint32_t *iter_dev;  // graph-replayed rounds: the round index in device memory, else null
     uint8_t *admitted;      // local moving without a cap: 1 if the mover was admitted (join_admitted)
     const unsigned int *stop;  // the phase's stop flag (every launch returns at once once set)
+    uint32_t *mstep;        // refinement: the round in which a vertex moved (NONE before)
 };
 
 // Local moving: a decided mover moves if its target admits it (the lowest claim with a weight
@@ -934,6 +935,7 @@ __global__ void k_apply_refine(ApplyArgs a) {
             continue;
         }
         a.memb[v] = t;
+        if (a.mstep) a.mstep[v] = round;
         long long kf = to_fx(a.strength[v], a.sK), wf = to_fx(a.node_w[v], a.sW);
         atomicAdd((unsigned long long *)&a.Kfx[t], (unsigned long long)kf);
         atomicAdd((unsigned long long *)&a.Kfx[v], (unsigned long long)(-kf));
@@ -995,6 +997,30 @@ __global__ void __launch_bounds__(256) k_activate(const uint32_t *list, const un
             uint32_t u = nbrs[e];
             if (u == v) continue;
             if (REFINE ? (constraint[u] == mv) : (memb[u] != mv)) active[u] = 1;
+        }
+    }
+}
+
+// Refinement's re-activation, from the singletons' side (as the persistent sweep does it): an
+// inactive vertex that is still a singleton becomes active when a neighbour in its working
+// community moved in this round (stamped by k_apply_refine). One thread per vertex.
+__global__ void __launch_bounds__(256) k_refine_reactivate(uint32_t n, const uint64_t *ptrs, const uint32_t *nbrs,
+                                                           const uint32_t *memb, const uint32_t *csize,
+                                                           const uint32_t *constraint, const uint32_t *mstep,
+                                                           uint8_t *active, const uint32_t *iter_dev, uint32_t round,
+                                                           const unsigned int *stop) {
+    if (stop && *stop) return;
+    const uint32_t r = iter_dev ? *iter_dev : round;
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        if (active[u] || memb[u] != (uint32_t)u || csize[u] != 1u) continue;
+        const uint32_t cu = constraint[u];
+        const uint64_t e1 = ptrs[u + 1];
+        for (uint64_t e = ptrs[u]; e < e1; e++) {
+            const uint32_t w = nbrs[e];
+            if (w != (uint32_t)u && mstep[w] == r && constraint[w] == cu) {
+                active[u] = 1;
+                break;
+            }
         }
     }
 }
@@ -2781,6 +2807,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, ls.active.get(), (uint64_t)n, (uint8_t)1);
     KERNEL_OK();
     HIP_OK(hipMemsetAsync(ls.tround.get(), 0xFF, n * sizeof(uint32_t), s));
+    HIP_OK(hipMemsetAsync(ls.mstep.get(), 0xFF, n * sizeof(uint32_t), s));  // no vertex has moved
     fill_u32(ls.targeted.get(), n, 0u, s);  // each round then clears what its movers set
     // ctl: [0] stop, [1] rounds, [2] moves, [3] unused, then the compacted counts per bucket
     constexpr uint32_t CTL = 4 + NUM_BUCKETS;
@@ -2811,6 +2838,7 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
     const bool capped = ctx.cfg->has_max_community_weight != 0;
     ap.claim = capped ? ls.claim.get() : nullptr;
     ap.stop = ctl.get();
+    ap.mstep = ls.mstep.get();
     const bool use_graph = graph_sweeps();
     DevBuf<uint32_t> iter_dev;
     if (use_graph) {
@@ -2840,10 +2868,11 @@ static void refine(const sc_graph &L, const uint32_t *memb, uint32_t *rmemb, Lev
         }
         hipLaunchKernelGGL(k_apply_refine, dim3(1024), dim3(256), 0, s, ap);
         KERNEL_OK();
-        hipLaunchKernelGGL((k_activate<16, true>), dim3(1024), dim3(256), 0, s, ls.moved.get(),
-                           ls.counters.get() + 1, L.ptrs.get(), L.nbrs.get(), (const uint32_t *)rmemb, memb,
-                           ls.active.get(), (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                           (const uint8_t *)nullptr, (const unsigned int *)ctl.get());
+        hipLaunchKernelGGL(k_refine_reactivate, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, L.ptrs.get(),
+                           L.nbrs.get(), (const uint32_t *)rmemb, (const uint32_t *)ls.csize.get(), memb,
+                           (const uint32_t *)ls.mstep.get(), ls.active.get(),
+                           use_graph ? (const uint32_t *)iter_dev.get() : nullptr, ap.round,
+                           (const unsigned int *)ctl.get());
         KERNEL_OK();
         hipLaunchKernelGGL(k_clear_targeted, dim3(1024), dim3(256), 0, s, (const unsigned int *)ctl.get(),
                            ls.movers.get(), (const unsigned int *)ls.counters.get(), ls.target.get(),
