@@ -182,10 +182,21 @@ def test_persistent_sweeps_match_multi_launch():
     visibility; each graph is replayed once per iteration of its level."""
     x, _ = W.gaussian_mixture(40_000, 24, 20, 9.0, sd=1.0, alpha=2.0, seed=3)
     g = scgpu.knn_graph(x, 10)
-    cfgs = [scgpu.LeidenConfig(seed=42), scgpu.LeidenConfig(seed=7, objective=scgpu.ObjectiveKind.Cpm(0.05)),
-            scgpu.LeidenConfig(seed=11, max_community_weight=400.0)]
+    # the same graph with self-loops on every fifth vertex: the refinement's first step and the
+    # singleton terms read a vertex's own loop weight, so the three paths must agree there too
+    import scipy.sparse as sp
+    ptrs, nbrs, wts = g.to_csr_parts()
+    a = sp.csr_matrix((wts, nbrs, ptrs.astype(np.int64)), shape=(len(ptrs) - 1,) * 2)
+    loops = np.zeros(len(ptrs) - 1, dtype=np.float32)
+    loops[::5] = 0.75
+    a = (a + sp.diags(loops)).tocsr()
+    a.sort_indices()
+    gl = scgpu.CSRNetwork.from_csr_parts(a.indptr, a.indices, a.data.astype(np.float32))
+    runs = [(g, scgpu.LeidenConfig(seed=42)), (g, scgpu.LeidenConfig(seed=7, objective=scgpu.ObjectiveKind.Cpm(0.05))),
+            (g, scgpu.LeidenConfig(seed=11, max_community_weight=400.0)), (gl, scgpu.LeidenConfig(seed=5)),
+            (gl, scgpu.LeidenConfig(seed=9, objective=scgpu.ObjectiveKind.Cpm(0.05)))]
     try:
-        for cfg in cfgs:
+        for g, cfg in runs:
             scgpu.set_sweep_mode("persistent")
             p = [scgpu.leiden(g, cfg).labels() for _ in range(3)]
             scgpu.set_sweep_mode("multi")
