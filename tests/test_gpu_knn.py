@@ -54,22 +54,23 @@ def test_knn_pruned_exact_vs_oracle(knn_prune, precision, n, d, k, comps, spread
           f"fallback rows {st['rows_fallback']}")
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
 @pytest.mark.parametrize("d,offset", [(12, 400.0), (30, 150.0), (50, 100.0)])
-def test_knn_pruned_far_from_origin(knn_prune, d, offset):
+def test_knn_pruned_far_from_origin(knn_prune, precision, d, offset):
     # clusters a few units wide at a distance of hundreds from the origin: |x|^2 - 2 x.c + |c|^2
     # cancels to a small fraction of its terms, so the f32 and split-BF16 bounds of the block
     # filter and the tile (scaled by the norms) must hold where they are loosest
     x, _ = W.gaussian_mixture(3000, d, 20, 9.0, seed=d)
     x = (x + np.float32(offset)).astype(np.float32)
     knn_prune("1")
-    idx, d2, st = scgpu.knn(x, 9, return_stats=True)
+    idx, d2, st = scgpu.knn(x, 9, precision=precision, return_stats=True)
     knn_prune("0")
-    idx0, d20 = scgpu.knn(x, 9)
+    idx0, d20 = scgpu.knn(x, 9, precision=precision)
     assert np.array_equal(idx, idx0) and np.array_equal(d2.view(np.uint64), d20.view(np.uint64))
     rows = np.random.default_rng(2).choice(3000, 200, replace=False)
     oi, od = O.knn_bruteforce(x, 9, rows=rows)
     assert np.array_equal(idx[rows], oi) and np.array_equal(d2[rows].view(np.uint64), od.view(np.uint64))
-    print(f"offset {offset} d={d}: pairs {st['pairs_computed']:.3g}, fallback rows {st['rows_fallback']}")
+    print(f"{precision} offset {offset} d={d}: pairs {st['pairs_computed']:.3g}, fallback rows {st['rows_fallback']}")
 
 
 def test_knn_pruned_ties_duplicates_default_size():
